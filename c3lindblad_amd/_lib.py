@@ -1,0 +1,111 @@
+"""ctypes binding of ``libc3l_hip.so`` (declared in ``include/c3l_hip.h``).
+
+The library is loaded from the package directory.  There is no fallback: if
+it is missing or no GPU is visible, every device entry point raises.
+Status codes map to the reference's exception types: invalid shapes or
+indices -> ``ValueError``, allocation failure -> ``MemoryError``, HIP errors
+-> ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_NAME = "libc3l_hip.so"
+LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
+
+C3L_OK, C3L_EINVAL, C3L_EHIP, C3L_ENOMEM = 0, 1, 2, 3
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+
+SIGNATURES = {
+    "c3l_last_error": ([], ctypes.c_char_p),
+    "c3l_version": ([], ctypes.c_char_p),
+    "c3l_plan_create": ([_I32, _I32, _P, _I64, _I32, _P, _P, _P, ctypes.POINTER(_P)], ctypes.c_int),
+    "c3l_plan_destroy": ([_P], ctypes.c_int),
+    "c3l_plan_add_base": ([_P, _I32, _I64, _I32, _P, _P], ctypes.c_int),
+    "c3l_plan_add_level": ([_P, _I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "c3l_plan_add_block": ([_P, _I32, _I32, _I64, _P, _P, _I32, _P, _P, _I32, _P, _P], ctypes.c_int),
+    "c3l_plan_load": ([_P, _I32, _P, _P], ctypes.c_int),
+    "c3l_plan_apply": ([_P, _I32, _P, _P, _P, _P], ctypes.c_int),
+    "c3l_tilde_unpack": ([_P, _P, _P, _P, _P, _I64, _P], ctypes.c_int),
+    "c3l_tilde_pack": ([_P, _P, _P, _P, _I64, _P], ctypes.c_int),
+    "c3l_rmultidot": ([_P, _I64, _I32, _P, _I64, _P, _P, _P], ctypes.c_int),
+    "c3l_rmultiaxpy": ([_P, _I64, _I32, _P, _P, _I64, _P], ctypes.c_int),
+    "c3l_zmultidot": ([_P, _I64, _I32, _P, _I64, _I32, _P, _P, _P], ctypes.c_int),
+    "c3l_zmultiaxpy": ([_P, _I64, _I32, _P, _P, _I64, _P], ctypes.c_int),
+    "c3l_reduction_chunks": ([_I64], _I64),
+    "c3l_shifted_triangular_green": ([_P, _I32, _P, _P, _P, _P, _I32, _P, _P, _P], ctypes.c_int),
+}
+
+_lib = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Load and type the shared library (no GPU needed to load)."""
+
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP extension first (python -c 'import __graft_entry__ as g; g.build()')"
+            )
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = lib
+    return _lib
+
+
+def check(status: int) -> None:
+    if status == C3L_OK:
+        return
+    message = load_library().c3l_last_error().decode()
+    if status == C3L_EINVAL:
+        raise ValueError(message)
+    if status == C3L_ENOMEM:
+        raise MemoryError(message)
+    raise RuntimeError(message)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load_library(), name)(*args))
+
+
+def host_ptr(array: np.ndarray) -> int:
+    if not array.flags.c_contiguous:
+        raise ValueError("host array must be contiguous")
+    return array.ctypes.data
+
+
+def require_device():
+    """Return torch after checking that a GPU is visible (the product path has no CPU fallback)."""
+
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("the c3lindblad_amd device path needs a visible MI355X GPU (no CPU fallback)")
+    load_library()
+    return torch
+
+
+def dptr(tensor) -> int:
+    if not tensor.is_cuda or not tensor.is_contiguous():
+        raise ValueError("device tensor must be a contiguous GPU tensor")
+    return tensor.data_ptr()
+
+
+def stream_ptr(torch) -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def env_flag(name: str) -> bool:
+    return os.environ.get(name, "") not in ("", "0")

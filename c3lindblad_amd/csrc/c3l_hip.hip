@@ -1,0 +1,638 @@
+// MI355X (gfx950) kernels and C ABI for the C3L coupled-cluster hot path.
+//
+// FP64 path.  Data layout in HBM:
+//   * Newton/GMRES vectors: real tilde coordinates, one contiguous double array.
+//   * Amplitudes / EOM vectors: complex (double2) in the reference's canonical
+//     string order (PH rank, then mask value).
+//   * Plan workspace: one double2 array holding every padded atom tensor
+//     (n^rank entries, slot-major, antisymmetric within a group) and the padded
+//     base blocks scattered from the current input vector.
+//   * Krylov bases: basis vectors stored contiguously, column k at V + k*ld.
+//
+// Every kernel is bandwidth/latency bound (GEMV-like contractions, gathers,
+// outer-product style wedges); none is GEMM shaped at FP64, so no MFMA here.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/c3l_hip.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define C3L_HIP_CHECK(expr)                                                          \
+  do {                                                                                \
+    hipError_t err__ = (expr);                                                        \
+    if (err__ != hipSuccess) {                                                        \
+      return fail(C3L_EHIP, std::string(#expr) + ": " + hipGetErrorString(err__));    \
+    }                                                                                 \
+  } while (0)
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+constexpr int kThreads = 256;
+
+int64_t grid_for(int64_t count, int threads = kThreads) {
+  int64_t blocks = (count + threads - 1) / threads;
+  if (blocks < 1) blocks = 1;
+  if (blocks > 1 << 20) blocks = 1 << 20;
+  return blocks;
+}
+
+// ------------------------------------------------------------------ plan kernels
+
+__global__ void scatter_padded_kernel(const double2* __restrict__ v, double2* __restrict__ pad,
+                                      const int64_t* __restrict__ src, const int8_t* __restrict__ sign,
+                                      int64_t count) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = src[e];
+    double2 x = make_double2(0.0, 0.0);
+    if (s >= 0) {
+      x = v[s];
+      if (sign[e] < 0) {
+        x.x = -x.x;
+        x.y = -x.y;
+      }
+    }
+    pad[e] = x;
+  }
+}
+
+struct ContractJob {
+  int64_t out_offset;
+  int64_t par_offset;
+  int32_t par_rank;
+  int32_t slot;
+  int32_t vec;
+  int32_t sign;
+};
+
+// grid.y = job; threads cover the n^(R-1) outputs of that job.
+__global__ void contract_kernel(double2* __restrict__ ws, const double2* __restrict__ vectors,
+                                const ContractJob* __restrict__ jobs, int n) {
+  const ContractJob job = jobs[blockIdx.y];
+  const int r = job.par_rank - 1;
+  int64_t out_size = 1;
+  for (int k = 0; k < r; ++k) out_size *= n;
+  int64_t inner = 1;
+  for (int k = 0; k < job.par_rank - 1 - job.slot; ++k) inner *= n;
+  const double2* __restrict__ par = ws + job.par_offset;
+  const double2* __restrict__ vec = vectors + (int64_t)job.vec * n;
+  double2* __restrict__ out = ws + job.out_offset;
+  for (int64_t J = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; J < out_size;
+       J += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t hi = J / inner;
+    const int64_t lo = J - hi * inner;
+    const int64_t base = hi * n * inner + lo;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int q = 0; q < n; ++q) acc = cadd(acc, cmul(vec[q], par[base + (int64_t)q * inner]));
+    if (job.sign < 0) {
+      acc.x = -acc.x;
+      acc.y = -acc.y;
+    }
+    out[J] = acc;
+  }
+}
+
+// One thread per canonical output coordinate; the entry loop is uniform across
+// the wavefront so coefficient/factor metadata come through the scalar cache.
+__global__ void wedge_block_kernel(const double2* __restrict__ ws, double2* __restrict__ out,
+                                   const int32_t* __restrict__ coord, const int64_t* __restrict__ dest,
+                                   int64_t count, int deg, int n, int n_entries,
+                                   const double2* __restrict__ coeff, const int32_t* __restrict__ fstart,
+                                   const int64_t* __restrict__ foffset, const int32_t* __restrict__ fcode,
+                                   const double2* __restrict__ diag, const double2* __restrict__ x) {
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  int idx[4] = {0, 0, 0, 0};
+  for (int k = 0; k < deg; ++k) idx[k] = coord[e * deg + k];
+  double2 acc = make_double2(0.0, 0.0);
+  for (int t = 0; t < n_entries; ++t) {
+    double2 prod = coeff[t];
+    const int f1 = fstart[t + 1];
+    for (int f = fstart[t]; f < f1; ++f) {
+      const int code = fcode[f];
+      const int rank = code & 3;
+      int64_t flat = 0;
+      for (int s = 0; s < rank; ++s) flat = flat * n + idx[(code >> (2 + 2 * s)) & 3];
+      prod = cmul(prod, ws[foffset[f] + flat]);
+    }
+    acc = cadd(acc, prod);
+  }
+  const int64_t d = dest[e];
+  if (diag != nullptr) acc = cadd(acc, cmul(diag[d], x[d]));
+  out[d] = acc;
+}
+
+__global__ void fill_const_kernel(double2* __restrict__ ws, const double2* __restrict__ vectors,
+                                  const int64_t* __restrict__ offset, const int32_t* __restrict__ vec, int n) {
+  const int k = blockIdx.x;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) ws[offset[k] + q] = vectors[(int64_t)vec[k] * n + q];
+}
+
+// ------------------------------------------------------------------ tilde maps
+
+__global__ void tilde_unpack_kernel(const double* __restrict__ x, double2* __restrict__ t,
+                                    const int64_t* __restrict__ k0, const int64_t* __restrict__ k1,
+                                    const double4* __restrict__ ab, int64_t dim) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
+    const double4 c = ab[i];
+    const double a = x[k0[i]];
+    const double b = x[k1[i]];
+    t[i] = make_double2(c.x * a + c.z * b, c.y * a + c.w * b);
+  }
+}
+
+__global__ void tilde_pack_kernel(const double2* __restrict__ t, double* __restrict__ x,
+                                  const int64_t* __restrict__ src, const double2* __restrict__ w, int64_t real_dim) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < real_dim;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = t[src[c]];
+    const double2 s = w[c];
+    x[c] = s.x * v.x - s.y * v.y;
+  }
+}
+
+// ------------------------------------------------------------------ Krylov kernels
+
+constexpr int kChunkRows = 8192;
+
+int64_t chunks_for(int64_t dim) { return (dim + kChunkRows - 1) / kChunkRows; }
+
+struct cplx {
+  double x, y;
+};
+__device__ __forceinline__ cplx zero_of(cplx) { return cplx{0.0, 0.0}; }
+__device__ __forceinline__ double zero_of(double) { return 0.0; }
+__device__ __forceinline__ cplx add_of(cplx a, cplx b) { return cplx{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ double add_of(double a, double b) { return a + b; }
+__device__ __forceinline__ cplx shfl_down_of(cplx v, int off) {
+  return cplx{__shfl_down(v.x, off, 64), __shfl_down(v.y, off, 64)};
+}
+__device__ __forceinline__ double shfl_down_of(double v, int off) { return __shfl_down(v, off, 64); }
+
+template <typename T>
+__device__ T block_reduce(T v, T* shared) {
+  // 64-wide wavefront shuffle reduction, then across the 4 waves of the block.
+  for (int off = 32; off > 0; off >>= 1) v = add_of(v, shfl_down_of(v, off));
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) shared[wave] = v;
+  __syncthreads();
+  T total = zero_of(v);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) total = add_of(total, shared[k]);
+  }
+  return total;
+}
+
+// grid = (chunks, ncols): partial[k*chunks + c] = sum over rows of chunk c.
+__global__ void rmultidot_partial_kernel(const double* __restrict__ V, int64_t ld, const double* __restrict__ w,
+                                         int64_t dim, double* __restrict__ partial, int64_t chunks) {
+  __shared__ double shared[kThreads / 64];
+  const int k = blockIdx.y;
+  const int64_t begin = blockIdx.x * (int64_t)kChunkRows;
+  const int64_t end = begin + kChunkRows < dim ? begin + kChunkRows : dim;
+  const double* __restrict__ col = V + (int64_t)k * ld;
+  double acc = 0.0;
+  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) acc += col[i] * w[i];
+  const double total = block_reduce(acc, shared);
+  if (threadIdx.x == 0) partial[(int64_t)k * chunks + blockIdx.x] = total;
+}
+
+__global__ void rfinish_kernel(const double* __restrict__ partial, int64_t chunks, double* __restrict__ out) {
+  __shared__ double shared[kThreads / 64];
+  const int k = blockIdx.x;
+  double acc = 0.0;
+  for (int64_t c = threadIdx.x; c < chunks; c += blockDim.x) acc += partial[(int64_t)k * chunks + c];
+  const double total = block_reduce(acc, shared);
+  if (threadIdx.x == 0) out[k] = total;
+}
+
+__global__ void rmultiaxpy_kernel(const double* __restrict__ V, int64_t ld, int ncols,
+                                  const double* __restrict__ alpha, double* __restrict__ w, int64_t dim) {
+  extern __shared__ double coef[];
+  for (int k = threadIdx.x; k < ncols; k += blockDim.x) coef[k] = alpha[k];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = w[i];
+    for (int k = 0; k < ncols; ++k) acc += coef[k] * V[(int64_t)k * ld + i];
+    w[i] = acc;
+  }
+}
+
+__global__ void zmultidot_partial_kernel(const double2* __restrict__ V, int64_t ld, const double2* __restrict__ w,
+                                         int64_t dim, int conjugate, double2* __restrict__ partial, int64_t chunks) {
+  __shared__ cplx shared[kThreads / 64];
+  const int k = blockIdx.y;
+  const int64_t begin = blockIdx.x * (int64_t)kChunkRows;
+  const int64_t end = begin + kChunkRows < dim ? begin + kChunkRows : dim;
+  const double2* __restrict__ col = V + (int64_t)k * ld;
+  const double s = conjugate ? -1.0 : 1.0;
+  cplx acc{0.0, 0.0};
+  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+    const double2 a = col[i];
+    const double2 b = w[i];
+    acc.x += a.x * b.x - s * a.y * b.y;
+    acc.y += a.x * b.y + s * a.y * b.x;
+  }
+  const cplx total = block_reduce(acc, shared);
+  if (threadIdx.x == 0) partial[(int64_t)k * chunks + blockIdx.x] = make_double2(total.x, total.y);
+}
+
+__global__ void zfinish_kernel(const double2* __restrict__ partial, int64_t chunks, double2* __restrict__ out) {
+  __shared__ cplx shared[kThreads / 64];
+  const int k = blockIdx.x;
+  cplx acc{0.0, 0.0};
+  for (int64_t c = threadIdx.x; c < chunks; c += blockDim.x) {
+    const double2 p = partial[(int64_t)k * chunks + c];
+    acc.x += p.x;
+    acc.y += p.y;
+  }
+  const cplx total = block_reduce(acc, shared);
+  if (threadIdx.x == 0) out[k] = make_double2(total.x, total.y);
+}
+
+__global__ void zmultiaxpy_kernel(const double2* __restrict__ V, int64_t ld, int ncols,
+                                  const double2* __restrict__ alpha, double2* __restrict__ w, int64_t dim) {
+  extern __shared__ double2 zcoef[];
+  for (int k = threadIdx.x; k < ncols; k += blockDim.x) zcoef[k] = alpha[k];
+  __syncthreads();
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = w[i];
+    for (int k = 0; k < ncols; ++k) acc = cadd(acc, cmul(zcoef[k], V[(int64_t)k * ld + i]));
+    w[i] = acc;
+  }
+}
+
+// One 64-lane workgroup per shift: column-oriented back substitution on the
+// upper-triangular Schur factor held in L2, right-hand side in LDS.
+__global__ void shifted_triangular_kernel(const double2* __restrict__ T, int m, const double2* __restrict__ r,
+                                          const double2* __restrict__ w, const double2* __restrict__ q,
+                                          const double2* __restrict__ shifts, double2* __restrict__ green,
+                                          double2* __restrict__ last) {
+  extern __shared__ double2 b[];
+  const int s = blockIdx.x;
+  const double2 z = shifts[s];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) b[i] = r[i];
+  __syncthreads();
+  for (int k = m - 1; k >= 0; --k) {
+    const double2 d = T[(int64_t)k * m + k];
+    const double2 den = make_double2(z.x - d.x, z.y - d.y);
+    const double inv = 1.0 / (den.x * den.x + den.y * den.y);
+    const double2 bk = b[k];
+    const double2 yk = make_double2((bk.x * den.x + bk.y * den.y) * inv, (bk.y * den.x - bk.x * den.y) * inv);
+    __syncthreads();
+    if (threadIdx.x == 0) b[k] = yk;
+    for (int i = threadIdx.x; i < k; i += blockDim.x) b[i] = cadd(b[i], cmul(T[(int64_t)k * m + i], yk));
+    __syncthreads();
+  }
+  cplx g{0.0, 0.0}, l{0.0, 0.0};
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const double2 y = b[i];
+    const double2 wi = w[i];
+    const double2 qi = q[i];
+    g.x += wi.x * y.x - wi.y * y.y;
+    g.y += wi.x * y.y + wi.y * y.x;
+    l.x += qi.x * y.x - qi.y * y.y;
+    l.y += qi.x * y.y + qi.y * y.x;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    g = add_of(g, shfl_down_of(g, off));
+    l = add_of(l, shfl_down_of(l, off));
+  }
+  if (threadIdx.x == 0) {
+    green[s] = make_double2(g.x, g.y);
+    last[s] = make_double2(l.x, l.y);
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ plan object
+
+struct PlanBase {
+  int32_t kind;
+  int64_t offset;
+  int64_t count;
+  int64_t* src = nullptr;
+  int8_t* sign = nullptr;
+};
+
+struct PlanLevel {
+  int32_t kind;
+  int32_t n_jobs;
+  int64_t max_out;
+  ContractJob* jobs = nullptr;
+};
+
+struct PlanBlock {
+  int32_t program;
+  int32_t deg;
+  int64_t count;
+  int32_t n_entries;
+  int32_t* coord = nullptr;
+  int64_t* dest = nullptr;
+  double2* coeff = nullptr;
+  int32_t* fstart = nullptr;
+  int64_t* foffset = nullptr;
+  int32_t* fcode = nullptr;
+};
+
+struct c3l_plan {
+  int32_t n = 0;
+  int32_t n_vectors = 0;
+  int64_t workspace = 0;
+  double2* ws = nullptr;
+  double2* vectors = nullptr;
+  std::vector<PlanBase> bases;
+  std::vector<PlanLevel> levels;
+  std::vector<PlanBlock> blocks;
+  std::vector<void*> owned;
+};
+
+namespace {
+
+template <typename T>
+int upload(c3l_plan* plan, T** dst, const T* src, int64_t count) {
+  *dst = nullptr;
+  if (count <= 0) return C3L_OK;
+  void* ptr = nullptr;
+  if (hipMalloc(&ptr, sizeof(T) * (size_t)count) != hipSuccess) return fail(C3L_ENOMEM, "hipMalloc failed in plan upload");
+  plan->owned.push_back(ptr);
+  C3L_HIP_CHECK(hipMemcpy(ptr, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice));
+  *dst = static_cast<T*>(ptr);
+  return C3L_OK;
+}
+
+int64_t ipow(int64_t n, int r) {
+  int64_t v = 1;
+  for (int k = 0; k < r; ++k) v *= n;
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* c3l_last_error(void) { return g_error.c_str(); }
+
+const char* c3l_version(void) { return "c3l_hip 0.1 gfx950 fp64"; }
+
+int c3l_plan_create(int32_t n, int32_t n_vectors, const double* vectors, int64_t workspace, int32_t n_const,
+                    const int32_t* const_atom, const int64_t* const_offset, const int32_t* const_vec,
+                    c3l_plan** out) {
+  (void)const_atom;
+  if (out == nullptr || n <= 0 || n_vectors < 0 || workspace < 0 || n_const < 0)
+    return fail(C3L_EINVAL, "c3l_plan_create: invalid argument");
+  for (int32_t k = 0; k < n_const; ++k) {
+    if (const_vec[k] < 0 || const_vec[k] >= n_vectors || const_offset[k] < 0 || const_offset[k] + n > workspace)
+      return fail(C3L_EINVAL, "c3l_plan_create: constant atom out of range");
+  }
+  c3l_plan* plan = new c3l_plan();
+  plan->n = n;
+  plan->n_vectors = n_vectors;
+  plan->workspace = workspace;
+  void* ws = nullptr;
+  if (hipMalloc(&ws, sizeof(double2) * (size_t)(workspace > 0 ? workspace : 1)) != hipSuccess) {
+    delete plan;
+    return fail(C3L_ENOMEM, "c3l_plan_create: workspace allocation failed");
+  }
+  plan->owned.push_back(ws);
+  plan->ws = static_cast<double2*>(ws);
+  int rc = upload(plan, &plan->vectors, reinterpret_cast<const double2*>(vectors), (int64_t)n_vectors * n);
+  int64_t* d_off = nullptr;
+  int32_t* d_vec = nullptr;
+  if (rc == C3L_OK) rc = upload(plan, &d_off, const_offset, n_const);
+  if (rc == C3L_OK) rc = upload(plan, &d_vec, const_vec, n_const);
+  if (rc == C3L_OK && hipMemset(plan->ws, 0, sizeof(double2) * (size_t)(workspace > 0 ? workspace : 1)) != hipSuccess)
+    rc = fail(C3L_EHIP, "c3l_plan_create: workspace clear failed");
+  if (rc == C3L_OK && n_const > 0) {
+    hipLaunchKernelGGL(fill_const_kernel, dim3(n_const), dim3(64), 0, 0, plan->ws, plan->vectors, d_off, d_vec, n);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+      rc = fail(C3L_EHIP, "c3l_plan_create: constant atom fill failed");
+  }
+  if (rc != C3L_OK) {
+    c3l_plan_destroy(plan);
+    return rc;
+  }
+  *out = plan;
+  return C3L_OK;
+}
+
+int c3l_plan_destroy(c3l_plan* plan) {
+  if (plan == nullptr) return C3L_OK;
+  for (void* p : plan->owned) (void)hipFree(p);
+  delete plan;
+  return C3L_OK;
+}
+
+int c3l_plan_add_base(c3l_plan* plan, int32_t kind, int64_t offset, int32_t deg, const int64_t* src,
+                      const int8_t* sign) {
+  if (plan == nullptr || deg < 0 || deg > 4 || offset < 0) return fail(C3L_EINVAL, "c3l_plan_add_base: invalid argument");
+  PlanBase b;
+  b.kind = kind;
+  b.offset = offset;
+  b.count = ipow(plan->n, deg);
+  if (offset + b.count > plan->workspace) return fail(C3L_EINVAL, "c3l_plan_add_base: block exceeds workspace");
+  int rc = upload(plan, &b.src, src, b.count);
+  if (rc == C3L_OK) rc = upload(plan, &b.sign, sign, b.count);
+  if (rc != C3L_OK) return rc;
+  plan->bases.push_back(b);
+  return C3L_OK;
+}
+
+int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64_t* out_offset,
+                       const int64_t* par_offset, const int32_t* par_rank, const int32_t* slot, const int32_t* vec,
+                       const int32_t* sign) {
+  if (plan == nullptr || n_jobs < 0) return fail(C3L_EINVAL, "c3l_plan_add_level: invalid argument");
+  std::vector<ContractJob> jobs((size_t)n_jobs);
+  int64_t max_out = 1;
+  for (int32_t j = 0; j < n_jobs; ++j) {
+    if (par_rank[j] < 1 || par_rank[j] > 4 || slot[j] < 0 || slot[j] >= par_rank[j] || vec[j] < 0 ||
+        vec[j] >= plan->n_vectors)
+      return fail(C3L_EINVAL, "c3l_plan_add_level: job out of range");
+    const int64_t outsz = ipow(plan->n, par_rank[j] - 1);
+    if (out_offset[j] < 0 || out_offset[j] + outsz > plan->workspace || par_offset[j] < 0 ||
+        par_offset[j] + ipow(plan->n, par_rank[j]) > plan->workspace)
+      return fail(C3L_EINVAL, "c3l_plan_add_level: job exceeds workspace");
+    jobs[(size_t)j] = ContractJob{out_offset[j], par_offset[j], par_rank[j], slot[j], vec[j], sign[j]};
+    if (outsz > max_out) max_out = outsz;
+  }
+  PlanLevel level;
+  level.kind = kind;
+  level.n_jobs = n_jobs;
+  level.max_out = max_out;
+  int rc = upload(plan, &level.jobs, jobs.data(), n_jobs);
+  if (rc != C3L_OK) return rc;
+  plan->levels.push_back(level);
+  return C3L_OK;
+}
+
+int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
+                       const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
+                       int32_t n_factors, const int64_t* foffset, const int32_t* fcode) {
+  if (plan == nullptr || deg < 0 || deg > 4 || count < 0 || n_entries < 0 || n_factors < 0)
+    return fail(C3L_EINVAL, "c3l_plan_add_block: invalid argument");
+  if (n_entries > 0 && fstart[n_entries] != n_factors) return fail(C3L_EINVAL, "c3l_plan_add_block: fstart mismatch");
+  for (int32_t f = 0; f < n_factors; ++f) {
+    const int rank = fcode[f] & 3;
+    if (foffset[f] < 0 || foffset[f] + ipow(plan->n, rank) > plan->workspace)
+      return fail(C3L_EINVAL, "c3l_plan_add_block: factor exceeds workspace");
+    for (int s = 0; s < rank; ++s)
+      if (((fcode[f] >> (2 + 2 * s)) & 3) >= deg) return fail(C3L_EINVAL, "c3l_plan_add_block: factor position out of range");
+  }
+  for (int64_t k = 0; k < count * deg; ++k)
+    if (coord[k] < 0 || coord[k] >= plan->n) return fail(C3L_EINVAL, "c3l_plan_add_block: coordinate out of range");
+  PlanBlock b;
+  b.program = program;
+  b.deg = deg;
+  b.count = count;
+  b.n_entries = n_entries;
+  int rc = upload(plan, &b.coord, coord, count * deg);
+  if (rc == C3L_OK) rc = upload(plan, &b.dest, dest, count);
+  if (rc == C3L_OK) rc = upload(plan, &b.coeff, reinterpret_cast<const double2*>(coeff), n_entries);
+  if (rc == C3L_OK) rc = upload(plan, &b.fstart, fstart, n_entries + 1);
+  if (rc == C3L_OK) rc = upload(plan, &b.foffset, foffset, n_factors);
+  if (rc == C3L_OK) rc = upload(plan, &b.fcode, fcode, n_factors);
+  if (rc != C3L_OK) return rc;
+  plan->blocks.push_back(b);
+  return C3L_OK;
+}
+
+int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
+  if (plan == nullptr || v == nullptr) return fail(C3L_EINVAL, "c3l_plan_load: invalid argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const double2* vv = reinterpret_cast<const double2*>(v);
+  for (const PlanBase& b : plan->bases) {
+    if (b.kind != kind) continue;
+    hipLaunchKernelGGL(scatter_padded_kernel, dim3(grid_for(b.count)), dim3(kThreads), 0, s, vv, plan->ws + b.offset,
+                       b.src, b.sign, b.count);
+    C3L_HIP_CHECK(hipGetLastError());
+  }
+  for (const PlanLevel& l : plan->levels) {
+    if (l.kind != kind || l.n_jobs == 0) continue;
+    hipLaunchKernelGGL(contract_kernel, dim3(grid_for(l.max_out), l.n_jobs), dim3(kThreads), 0, s, plan->ws,
+                       plan->vectors, l.jobs, plan->n);
+    C3L_HIP_CHECK(hipGetLastError());
+  }
+  return C3L_OK;
+}
+
+int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out, void* stream) {
+  if (plan == nullptr || out == nullptr || ((diag == nullptr) != (x == nullptr)))
+    return fail(C3L_EINVAL, "c3l_plan_apply: invalid argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (const PlanBlock& b : plan->blocks) {
+    if (b.program != program || b.count == 0) continue;
+    hipLaunchKernelGGL(wedge_block_kernel, dim3(grid_for(b.count)), dim3(kThreads), 0, s, plan->ws,
+                       reinterpret_cast<double2*>(out), b.coord, b.dest, b.count, b.deg, plan->n, b.n_entries, b.coeff,
+                       b.fstart, b.foffset, b.fcode, reinterpret_cast<const double2*>(diag),
+                       reinterpret_cast<const double2*>(x));
+    C3L_HIP_CHECK(hipGetLastError());
+  }
+  return C3L_OK;
+}
+
+int c3l_tilde_unpack(const double* x, double* t, const int64_t* k0, const int64_t* k1, const double* ab, int64_t dim,
+                     void* stream) {
+  if (dim < 0 || (dim > 0 && (x == nullptr || t == nullptr || k0 == nullptr || k1 == nullptr || ab == nullptr)))
+    return fail(C3L_EINVAL, "c3l_tilde_unpack: invalid argument");
+  if (dim == 0) return C3L_OK;
+  hipLaunchKernelGGL(tilde_unpack_kernel, dim3(grid_for(dim)), dim3(kThreads), 0, static_cast<hipStream_t>(stream), x,
+                     reinterpret_cast<double2*>(t), k0, k1, reinterpret_cast<const double4*>(ab), dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_tilde_pack(const double* t, double* x, const int64_t* src, const double* w, int64_t real_dim, void* stream) {
+  if (real_dim < 0 || (real_dim > 0 && (x == nullptr || t == nullptr || src == nullptr || w == nullptr)))
+    return fail(C3L_EINVAL, "c3l_tilde_pack: invalid argument");
+  if (real_dim == 0) return C3L_OK;
+  hipLaunchKernelGGL(tilde_pack_kernel, dim3(grid_for(real_dim)), dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     reinterpret_cast<const double2*>(t), x, src, reinterpret_cast<const double2*>(w), real_dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int64_t c3l_reduction_chunks(int64_t dim) { return chunks_for(dim); }
+
+int c3l_rmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim, double* scratch,
+                  double* out, void* stream) {
+  if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || w == nullptr || scratch == nullptr || out == nullptr)
+    return fail(C3L_EINVAL, "c3l_rmultidot: invalid argument");
+  const int64_t chunks = chunks_for(dim);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(rmultidot_partial_kernel, dim3(chunks, ncols), dim3(kThreads), 0, s, V, ld, w, dim, scratch, chunks);
+  C3L_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(rfinish_kernel, dim3(ncols), dim3(kThreads), 0, s, scratch, chunks, out);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double* w, int64_t dim,
+                   void* stream) {
+  if (ncols <= 0 || dim <= 0 || ld < dim || ncols > 8192 || V == nullptr || alpha == nullptr || w == nullptr)
+    return fail(C3L_EINVAL, "c3l_rmultiaxpy: invalid argument");
+  hipLaunchKernelGGL(rmultiaxpy_kernel, dim3(grid_for(dim)), dim3(kThreads), sizeof(double) * ncols,
+                     static_cast<hipStream_t>(stream), V, ld, ncols, alpha, w, dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_zmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim, int32_t conjugate,
+                  double* scratch, double* out, void* stream) {
+  if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || w == nullptr || scratch == nullptr || out == nullptr)
+    return fail(C3L_EINVAL, "c3l_zmultidot: invalid argument");
+  const int64_t chunks = chunks_for(dim);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(zmultidot_partial_kernel, dim3(chunks, ncols), dim3(kThreads), 0, s,
+                     reinterpret_cast<const double2*>(V), ld, reinterpret_cast<const double2*>(w), dim, conjugate,
+                     reinterpret_cast<double2*>(scratch), chunks);
+  C3L_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(zfinish_kernel, dim3(ncols), dim3(kThreads), 0, s, reinterpret_cast<const double2*>(scratch), chunks,
+                     reinterpret_cast<double2*>(out));
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_zmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double* w, int64_t dim,
+                   void* stream) {
+  if (ncols <= 0 || dim <= 0 || ld < dim || ncols > 4096 || V == nullptr || alpha == nullptr || w == nullptr)
+    return fail(C3L_EINVAL, "c3l_zmultiaxpy: invalid argument");
+  hipLaunchKernelGGL(zmultiaxpy_kernel, dim3(grid_for(dim)), dim3(kThreads), sizeof(double2) * ncols,
+                     static_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(V), ld, ncols,
+                     reinterpret_cast<const double2*>(alpha), reinterpret_cast<double2*>(w), dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_shifted_triangular_green(const double* T, int32_t m, const double* r, const double* w, const double* q,
+                                 const double* shifts, int32_t n_shifts, double* green, double* last, void* stream) {
+  if (m <= 0 || m > 4096 || n_shifts <= 0 || T == nullptr || r == nullptr || w == nullptr || q == nullptr ||
+      shifts == nullptr || green == nullptr || last == nullptr)
+    return fail(C3L_EINVAL, "c3l_shifted_triangular_green: invalid argument");
+  hipLaunchKernelGGL(shifted_triangular_kernel, dim3(n_shifts), dim3(64), sizeof(double2) * m,
+                     static_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(T), m,
+                     reinterpret_cast<const double2*>(r), reinterpret_cast<const double2*>(w),
+                     reinterpret_cast<const double2*>(q), reinterpret_cast<const double2*>(shifts),
+                     reinterpret_cast<double2*>(green), reinterpret_cast<double2*>(last));
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+}  // extern "C"

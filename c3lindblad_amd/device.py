@@ -1,0 +1,185 @@
+"""Device-side objects: uploaded contraction plans and tilde coordinate maps.
+
+Python/torch own every vector; a ``DevicePlan`` owns the library plan handle
+(index maps, coefficients, atom workspace) and frees it on ``close``/GC.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import itertools
+
+import numpy as np
+
+from . import _lib
+from .plan_arrays import KINDS, PROGRAMS, LoweredPlan
+
+_tokens = itertools.count(1)
+
+
+def _c(array: np.ndarray, dtype) -> np.ndarray:
+    return np.ascontiguousarray(array, dtype=dtype)
+
+
+class DevicePlan:
+    """A ``LoweredPlan`` uploaded to the GPU (``c3l_plan_*``)."""
+
+    def __init__(self, lowered: LoweredPlan):
+        self.torch = _lib.require_device()
+        self.lowered = lowered
+        self.loaded: dict[int, int] = {}
+        handle = ctypes.c_void_p()
+        vectors = _c(lowered.vectors, np.complex128)
+        const_offset = _c(lowered.const_offset, np.int64)
+        const_vec = _c(lowered.const_vec, np.int32)
+        _lib.call(
+            "c3l_plan_create",
+            lowered.n,
+            vectors.shape[0],
+            _lib.host_ptr(vectors) if vectors.size else None,
+            lowered.workspace,
+            const_vec.size,
+            None,
+            _lib.host_ptr(const_offset) if const_offset.size else None,
+            _lib.host_ptr(const_vec) if const_vec.size else None,
+            ctypes.byref(handle),
+        )
+        self.handle = handle
+        for kind, offset, deg, src, sign in lowered.bases:
+            s = _c(src, np.int64)
+            g = _c(sign, np.int8)
+            _lib.call("c3l_plan_add_base", handle, kind, int(offset), deg, _lib.host_ptr(s), _lib.host_ptr(g))
+        for kind, out_off, par_off, par_rank, slot, vec, sign in lowered.levels:
+            arrays = [
+                _c(out_off, np.int64),
+                _c(par_off, np.int64),
+                _c(par_rank, np.int32),
+                _c(slot, np.int32),
+                _c(vec, np.int32),
+                _c(sign, np.int32),
+            ]
+            _lib.call("c3l_plan_add_level", handle, kind, arrays[0].size, *[_lib.host_ptr(a) for a in arrays])
+        for program, deg, coord, dest, coeff, fstart, foff, fcode in lowered.blocks:
+            coord = _c(coord, np.int32)
+            dest = _c(dest, np.int64)
+            coeff = _c(coeff, np.complex128)
+            fstart = _c(fstart, np.int32)
+            foff = _c(foff, np.int64)
+            fcode = _c(fcode, np.int32)
+            _lib.call(
+                "c3l_plan_add_block",
+                handle,
+                program,
+                deg,
+                dest.size,
+                _lib.host_ptr(coord) if coord.size else None,
+                _lib.host_ptr(dest),
+                coeff.size,
+                _lib.host_ptr(coeff) if coeff.size else None,
+                _lib.host_ptr(fstart),
+                foff.size,
+                _lib.host_ptr(foff) if foff.size else None,
+                _lib.host_ptr(fcode) if fcode.size else None,
+            )
+
+    def load(self, kind: str, vector) -> int:
+        """Scatter a complex device vector into the ``kind`` bases and build its atoms."""
+
+        _lib.call("c3l_plan_load", self.handle, KINDS[kind], _lib.dptr(vector), _lib.stream_ptr(self.torch))
+        token = next(_tokens)
+        self.loaded[KINDS[kind]] = token
+        return token
+
+    def apply(self, program: str, out, diag=None, x=None) -> None:
+        _lib.call(
+            "c3l_plan_apply",
+            self.handle,
+            PROGRAMS[program],
+            None if diag is None else _lib.dptr(diag),
+            None if x is None else _lib.dptr(x),
+            _lib.dptr(out),
+            _lib.stream_ptr(self.torch),
+        )
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _lib.load_library().c3l_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceTilde:
+    """Tilde orbit maps on the device: real coordinates <-> complex amplitudes."""
+
+    def __init__(self, tilde):
+        torch = _lib.require_device()
+        self.torch = torch
+        dev = torch.device("cuda")
+        self.dim = tilde.dimension
+        self.real_dim = tilde.real_size
+        ab = np.stack([tilde.alpha.real, tilde.alpha.imag, tilde.beta.real, tilde.beta.imag], axis=1)
+        self.ab = torch.from_numpy(np.ascontiguousarray(ab)).to(dev)
+        self.k0 = torch.from_numpy(tilde.k0.astype(np.int64)).to(dev)
+        self.k1 = torch.from_numpy(tilde.k1.astype(np.int64)).to(dev)
+        self.src = torch.from_numpy(tilde.pack_src.astype(np.int64)).to(dev)
+        self.w = torch.from_numpy(np.ascontiguousarray(tilde.pack_w)).to(dev)
+
+    def unpack(self, x, t) -> None:
+        _lib.call(
+            "c3l_tilde_unpack",
+            _lib.dptr(x),
+            _lib.dptr(t),
+            _lib.dptr(self.k0),
+            _lib.dptr(self.k1),
+            _lib.dptr(self.ab),
+            self.dim,
+            _lib.stream_ptr(self.torch),
+        )
+
+    def pack(self, t, x) -> None:
+        _lib.call(
+            "c3l_tilde_pack",
+            _lib.dptr(t),
+            _lib.dptr(x),
+            _lib.dptr(self.src),
+            _lib.dptr(self.w),
+            self.real_dim,
+            _lib.stream_ptr(self.torch),
+        )
+
+
+class Reducer:
+    """Scratch-backed multi-dot and multi-axpy over contiguous Krylov bases."""
+
+    def __init__(self, dim: int, max_cols: int, complex_: bool):
+        torch = _lib.require_device()
+        self.torch = torch
+        self.dim = int(dim)
+        self.complex = complex_
+        chunks = int(_lib.load_library().c3l_reduction_chunks(self.dim))
+        dtype = torch.complex128 if complex_ else torch.float64
+        self.scratch = torch.empty(chunks * max_cols, dtype=dtype, device="cuda")
+        self.coef = torch.empty(max_cols, dtype=dtype, device="cuda")
+
+    def dot(self, V, ncols: int, w, out, conjugate: bool = True) -> None:
+        ld = V.stride(0)
+        if self.complex:
+            _lib.call(
+                "c3l_zmultidot", _lib.dptr(V), ld, ncols, _lib.dptr(w), self.dim, 1 if conjugate else 0,
+                _lib.dptr(self.scratch), _lib.dptr(out), _lib.stream_ptr(self.torch),
+            )
+        else:
+            _lib.call(
+                "c3l_rmultidot", _lib.dptr(V), ld, ncols, _lib.dptr(w), self.dim,
+                _lib.dptr(self.scratch), _lib.dptr(out), _lib.stream_ptr(self.torch),
+            )
+
+    def axpy(self, V, ncols: int, alpha, w) -> None:
+        ld = V.stride(0)
+        name = "c3l_zmultiaxpy" if self.complex else "c3l_rmultiaxpy"
+        _lib.call(name, _lib.dptr(V), ld, ncols, _lib.dptr(alpha), _lib.dptr(w), self.dim, _lib.stream_ptr(self.torch))

@@ -1,0 +1,186 @@
+"""Flat arrays of a compiled contraction plan, in the exact form the C ABI takes.
+
+``lower_plan`` places every atom and every padded base block in one complex
+workspace and converts the symbolic plan of ``algebra.emit_plan`` into:
+
+* ``bases``  -- ``(kind, offset, deg, src, sign)``: padded block element ``e`` is
+  ``sign[e] * v[src[e]]`` of the loaded complex vector (``src < 0`` -> 0).
+  The padding enumerates every within-group permutation of a canonical
+  coordinate with its parity (the padded-tensor convention of
+  ``component_tensor.py:302-337``);
+* ``levels`` -- contraction jobs per kind and depth;
+* ``blocks`` -- per program and output block: coordinates, destinations and
+  the wedge entries (``fcode = rank | pos0<<2 | pos1<<4 | pos2<<6``).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from itertools import permutations
+
+import numpy as np
+
+from .algebra import Plan, key_slots
+from .lowering import BlockKey, CoordinateSet
+
+KINDS = {"T": 0, "D": 1, "X": 2}
+PROGRAMS = {"residual": 0, "reference": 1, "jvp": 2, "sigma": 3}
+
+
+@dataclass
+class LoweredPlan:
+    n: int
+    workspace: int
+    vectors: np.ndarray
+    const_offset: np.ndarray
+    const_vec: np.ndarray
+    bases: list = field(default_factory=list)
+    levels: list = field(default_factory=list)
+    blocks: list = field(default_factory=list)
+    n_terms: dict = field(default_factory=dict)
+
+    def entries(self, program: str) -> int:
+        pid = PROGRAMS[program]
+        return int(sum(b[4].size for b in self.blocks if b[0] == pid))
+
+
+def padded_scatter(coords: CoordinateSet, key: BlockKey) -> tuple[np.ndarray, np.ndarray]:
+    """Index/sign maps from a vector to the padded antisymmetric tensor of one block."""
+
+    n = coords.n
+    block = coords.block(key)
+    deg = sum(key)
+    size = n**deg
+    src = np.full(size, -1, dtype=np.int64)
+    sign = np.zeros(size, dtype=np.int8)
+    if block is None or block.positions.size == 0:
+        return src, sign
+    slots = key_slots(key)
+    groups = sorted(set(slots))
+    group_slots = [[s for s, g in enumerate(slots) if g == grp] for grp in groups]
+    idx = block.indices
+    strides = n ** np.arange(deg - 1, -1, -1, dtype=np.int64)
+    per_group_perms = []
+    for gs in group_slots:
+        perms = []
+        for perm in permutations(range(len(gs))):
+            inv = sum(1 for a in range(len(perm)) for b in range(a + 1, len(perm)) if perm[a] > perm[b])
+            perms.append(([gs[p] for p in perm], -1 if inv % 2 else 1))
+        per_group_perms.append(perms)
+
+    def rec(level, mapping, parity):
+        if level == len(group_slots):
+            cols = np.empty((idx.shape[0], deg), dtype=np.int64)
+            for target, source in mapping.items():
+                cols[:, target] = idx[:, source]
+            flat = cols @ strides
+            src[flat] = block.positions
+            sign[flat] = parity
+            return
+        gs = group_slots[level]
+        for perm, s in per_group_perms[level]:
+            m = dict(mapping)
+            for target, source in zip(gs, perm):
+                m[target] = source
+            rec(level + 1, m, parity * s)
+
+    rec(0, {}, 1)
+    return src, sign
+
+
+def lower_plan(
+    plan: Plan,
+    base_sets: dict[str, tuple[CoordinateSet, tuple[BlockKey, ...]]],
+    out_sets: dict[str, CoordinateSet],
+) -> LoweredPlan:
+    """Lay out a symbolic plan for execution.
+
+    ``base_sets[kind]`` is the coordinate set and ordered block keys of the
+    vector loaded for atoms of that kind; ``out_sets[program]`` the
+    coordinate set of the program's output vector.
+    """
+
+    n = plan.n
+    cursor = plan.workspace
+    base_offset: dict[tuple[str, int], int] = {}
+    bases = []
+    for kind, (coords, keys) in base_sets.items():
+        for b, key in enumerate(keys):
+            src, sign = padded_scatter(coords, key)
+            base_offset[(kind, b)] = cursor
+            bases.append((KINDS[kind], cursor, sum(key), src, sign))
+            cursor += src.size
+    const_offset = []
+    const_vec = []
+    for i, atom in enumerate(plan.atoms):
+        if atom.base == "U":
+            const_offset.append(plan.atom_offsets[i])
+            const_vec.append(atom.ident)
+    levels = []
+    for kind, per_depth in plan.jobs.items():
+        for jobs in per_depth:
+            if jobs.size == 0:
+                continue
+            out_off = plan.atom_offsets[jobs[:, 0]]
+            par_off = np.empty(jobs.shape[0], dtype=np.int64)
+            par_rank = np.empty(jobs.shape[0], dtype=np.int32)
+            for j, (a, pid, _, _, _) in enumerate(jobs):
+                if pid >= 0:
+                    par_off[j] = plan.atom_offsets[pid]
+                else:
+                    par_off[j] = base_offset[(kind, -1 - int(pid))]
+                par_rank[j] = plan.atoms[a].rank + 1
+            levels.append(
+                (
+                    KINDS[kind],
+                    out_off.astype(np.int64),
+                    par_off,
+                    par_rank,
+                    jobs[:, 2].astype(np.int32),
+                    jobs[:, 3].astype(np.int32),
+                    jobs[:, 4].astype(np.int32),
+                )
+            )
+    blocks = []
+    for name, programs in plan.programs.items():
+        coords = out_sets[name]
+        for bp in programs:
+            deg = sum(bp.key)
+            if deg == 0:
+                coord = np.zeros((1, 0), dtype=np.int32)
+                dest = np.zeros(1, dtype=np.int64)
+            else:
+                block = coords.block(bp.key)
+                if block is None or block.positions.size == 0:
+                    continue
+                coord = block.indices.astype(np.int32)
+                dest = block.positions.astype(np.int64)
+            ranks = np.array([plan.atoms[a].rank for a in bp.factor_atom], dtype=np.int32)
+            fcode = ranks.copy()
+            for s in range(3):
+                pos = np.where(bp.factor_positions[:, s] >= 0, bp.factor_positions[:, s], 0)
+                fcode |= (pos.astype(np.int32) << (2 + 2 * s)) * (s < ranks)
+            foffset = plan.atom_offsets[bp.factor_atom] if bp.factor_atom.size else np.zeros(0, dtype=np.int64)
+            blocks.append(
+                (
+                    PROGRAMS[name],
+                    deg,
+                    np.ascontiguousarray(coord),
+                    dest,
+                    bp.coefficients,
+                    bp.factor_start,
+                    foffset.astype(np.int64),
+                    fcode.astype(np.int32),
+                )
+            )
+    return LoweredPlan(
+        n=n,
+        workspace=int(cursor),
+        vectors=plan.vectors,
+        const_offset=np.asarray(const_offset, dtype=np.int64),
+        const_vec=np.asarray(const_vec, dtype=np.int32),
+        bases=bases,
+        levels=levels,
+        blocks=blocks,
+        n_terms=dict(plan.n_terms),
+    )

@@ -1,0 +1,198 @@
+"""CCSD steady-state residual and Jacobian-vector products on the GPU.
+
+Drop-in for ``ComponentStaticProblem.compile``, ``component_static_residual``,
+``component_static_linearization``, ``component_static_linearized_jvp`` and
+``component_static_jvp`` (``src/c3l/cc/component_tensor.py:1276-1298,
+1371-1549,1552-1657,1822-1868,1871-1930``).
+
+``R(T) = -i (Λ∘T + sum_products [exp(-T) prod F exp(T)|0>]_{retained})`` with
+``Λ_μ = sum_{p in μ} λ_p``; the rank-0 part is the ``reference`` scalar.
+
+Compile (host, once per problem): the term compiler expands all products,
+differentiates them for the tangent, and the plan is uploaded.  Each action
+is then: scatter the input into padded base blocks, contract the atoms
+level by level, evaluate every output block at its canonical coordinates.
+The linearisation keeps the ``T`` atoms resident in the plan workspace, so a
+JVP only rebuilds the tangent atoms.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from time import perf_counter
+
+import numpy as np
+
+from . import _lib
+from .algebra import TermCompiler, emit_plan, tangent_terms
+from .lowering import STATIC_KEYS, FactorizedCCProblem
+from .plan_arrays import LoweredPlan, lower_plan
+
+
+@dataclass
+class ComponentStaticAction:
+    retained: np.ndarray
+    reference: complex
+    wall_seconds: float
+
+
+@dataclass
+class ComponentStaticProblem:
+    """Compiled static CCSD problem (host plan + lazily created device state)."""
+
+    oracle: FactorizedCCProblem
+    lowered: LoweredPlan
+    diagonal: np.ndarray  # -i Λ_μ
+    compile_seconds: float
+    _device: "StaticDevice | None" = field(default=None, repr=False)
+
+    @classmethod
+    def compile(cls, problem: FactorizedCCProblem, *, fuse_products: bool = False) -> "ComponentStaticProblem":
+        started = perf_counter()
+        compiler = TermCompiler(problem.model.n_sites, problem.products)
+        residual = compiler.static_terms(4)
+        jvp = tangent_terms(residual)
+        plan = emit_plan(
+            {
+                "residual": ({k: -1j * v for k, v in residual.items()}, STATIC_KEYS),
+                "reference": ({k: -1j * v for k, v in residual.items()}, ((0, 0, 0, 0),)),
+                "jvp": ({k: -1j * v for k, v in jvp.items()}, STATIC_KEYS),
+            },
+            problem.model.n_sites,
+            compiler.table.array(),
+        )
+        coords = problem.coordinates
+        lowered = lower_plan(
+            plan,
+            {"T": (coords, STATIC_KEYS), "D": (coords, STATIC_KEYS)},
+            {"residual": coords, "reference": coords, "jvp": coords},
+        )
+        diagonal = -1j * problem.rapidity_sums()
+        return cls(problem, lowered, diagonal, perf_counter() - started)
+
+    @property
+    def coordinates(self):
+        return self.oracle.coordinates
+
+    @property
+    def device(self) -> "StaticDevice":
+        if self._device is None:
+            self._device = StaticDevice(self)
+        return self._device
+
+
+class StaticDevice:
+    """GPU buffers and actions of one compiled static problem."""
+
+    def __init__(self, problem: ComponentStaticProblem):
+        from .device import DevicePlan, DeviceTilde
+
+        torch = _lib.require_device()
+        self.torch = torch
+        self.problem = problem
+        self.plan = DevicePlan(problem.lowered)
+        self.tilde = DeviceTilde(problem.oracle.tilde)
+        dim = problem.oracle.num_amplitudes
+        self.dim = dim
+        self.real_dim = problem.oracle.num_variables
+        dev = torch.device("cuda")
+        self.diag = torch.from_numpy(np.ascontiguousarray(problem.diagonal)).to(dev)
+        self.t = torch.zeros(dim, dtype=torch.complex128, device=dev)
+        self.dt = torch.zeros(dim, dtype=torch.complex128, device=dev)
+        self.out = torch.zeros(dim, dtype=torch.complex128, device=dev)
+        self.ref = torch.zeros(1, dtype=torch.complex128, device=dev)
+        self.t_token = 0
+
+    # complex-vector actions ------------------------------------------------
+    def load_amplitudes(self, t) -> None:
+        if t.data_ptr() != self.t.data_ptr():
+            self.t.copy_(t)
+        self.t_token = self.plan.load("T", self.t)
+
+    def residual_complex(self, out=None):
+        """Residual of the currently loaded amplitudes into ``out`` (complex)."""
+
+        out = self.out if out is None else out
+        self.plan.apply("residual", out, self.diag, self.t)
+        self.plan.apply("reference", self.ref)
+        return out
+
+    def jvp_complex(self, dt, out=None):
+        out = self.out if out is None else out
+        if dt.data_ptr() != self.dt.data_ptr():
+            self.dt.copy_(dt)
+        self.plan.load("D", self.dt)
+        self.plan.apply("jvp", out, self.diag, self.dt)
+        return out
+
+    # real tilde-coordinate actions -----------------------------------------
+    def residual_real(self, x, r) -> None:
+        self.tilde.unpack(x, self.t)
+        self.load_amplitudes(self.t)
+        self.residual_complex(self.out)
+        self.tilde.pack(self.out, r)
+
+    def jvp_real(self, dx, jdx) -> None:
+        self.tilde.unpack(dx, self.dt)
+        self.plan.load("D", self.dt)
+        self.plan.apply("jvp", self.out, self.diag, self.dt)
+        self.tilde.pack(self.out, jdx)
+
+
+def _to_device(array, torch):
+    return torch.from_numpy(np.ascontiguousarray(array, dtype=np.complex128)).to("cuda")
+
+
+def _validate(problem: ComponentStaticProblem, values, name: str) -> np.ndarray:
+    a = np.asarray(values, dtype=np.complex128)
+    if a.shape != (problem.oracle.num_amplitudes,) or not np.all(np.isfinite(a)):
+        raise ValueError(f"{name} must be finite and match the CCSD basis")
+    return a
+
+
+def component_static_residual(problem: ComponentStaticProblem, amplitudes) -> ComponentStaticAction:
+    started = perf_counter()
+    t = _validate(problem, amplitudes, "amplitudes")
+    dev = problem.device
+    torch = dev.torch
+    dev.load_amplitudes(_to_device(t, torch))
+    out = dev.residual_complex()
+    retained = out.cpu().numpy().copy()
+    reference = complex(dev.ref.cpu().numpy()[0])
+    return ComponentStaticAction(retained, reference, perf_counter() - started)
+
+
+@dataclass
+class ComponentStaticLinearization:
+    """Primal state of one Newton step; its ``T`` atoms stay on the device."""
+
+    problem: ComponentStaticProblem
+    amplitudes: np.ndarray
+    residual: ComponentStaticAction
+    token: int
+
+
+def component_static_linearization(problem: ComponentStaticProblem, amplitudes) -> ComponentStaticLinearization:
+    action = component_static_residual(problem, amplitudes)
+    return ComponentStaticLinearization(problem, _validate(problem, amplitudes, "amplitudes"), action, problem.device.t_token)
+
+
+def _ensure_loaded(context: ComponentStaticLinearization) -> None:
+    dev = context.problem.device
+    if dev.t_token != context.token:
+        dev.load_amplitudes(_to_device(context.amplitudes, dev.torch))
+        context.token = dev.t_token
+
+
+def component_static_linearized_jvp(context: ComponentStaticLinearization, tangent) -> ComponentStaticAction:
+    started = perf_counter()
+    dt = _validate(context.problem, tangent, "tangent")
+    _ensure_loaded(context)
+    dev = context.problem.device
+    out = dev.jvp_complex(_to_device(dt, dev.torch))
+    return ComponentStaticAction(out.cpu().numpy().copy(), 0.0j, perf_counter() - started)
+
+
+def component_static_jvp(problem: ComponentStaticProblem, amplitudes, direction) -> ComponentStaticAction:
+    context = component_static_linearization(problem, amplitudes)
+    return component_static_linearized_jvp(context, direction)

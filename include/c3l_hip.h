@@ -1,0 +1,130 @@
+/*
+ * c3l_hip.h -- C ABI of the MI355X (gfx950) coupled-cluster hot path of C3L.
+ *
+ * All vectors are device pointers borrowed for the duration of a
+ * stream-ordered call; complex numbers are interleaved (re, im) doubles.
+ * Plan handles own their device copies of index maps, coefficients and
+ * atom workspaces and are released by c3l_plan_destroy.  Every entry point
+ * returns 0 on success or a C3L_E* code; c3l_last_error() gives a
+ * thread-local message.  Streams are passed as void* (a hipStream_t).
+ *
+ * Reference interfaces replaced (file:line in haoen2021/C3Lindblad):
+ *   c3l_plan_*          ComponentStaticProblem.compile / component_static_residual /
+ *                       component_static_linearization / component_static_linearized_jvp
+ *                       (src/c3l/cc/component_tensor.py:1276,1822,1371,1552) and
+ *                       ComponentEOMProblem.compile / component_eom_context /
+ *                       component_eom_sigma (component_tensor.py:2012,2234,2288);
+ *                       the JAX twins Engine.residual/jvp (cc/_static_engine.py:88-107)
+ *                       and JaxFullS135ResponseOperator._device_action (cc/s135_jax.py:941-977).
+ *   c3l_tilde_unpack/pack  TildeMap.unpack/pack (src/c3l/cc/symmetry.py:67-107);
+ *                       JAX unpack/pack (cc/_static_engine.py:119-126).
+ *   c3l_rmultidot / c3l_rmultiaxpy    GMRES CGS2 inside solve_component_ccsd_jax
+ *                       (cc/_static_gmres.py:12-66).
+ *   c3l_zmultidot / c3l_zmultiaxpy    Arnoldi CGS2 (src/c3l/green/krylov.py:361-370;
+ *                       green/resident_arnoldi.py:82-114).
+ *   c3l_shifted_triangular_green      per-shift reduced resolvent
+ *                       (green/krylov.py:66-93,257-281; cc/component_jax.py:998-1009).
+ */
+#ifndef C3L_HIP_H
+#define C3L_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  C3L_OK = 0,
+  C3L_EINVAL = 1,   /* bad shape, index or argument */
+  C3L_EHIP = 2,     /* HIP runtime error */
+  C3L_ENOMEM = 3,   /* device allocation failed */
+};
+
+typedef struct c3l_plan c3l_plan;
+
+/* Last error message of the calling thread ("" when none). */
+const char* c3l_last_error(void);
+/* Library version string. */
+const char* c3l_version(void);
+
+/* ---- contraction plans (residual, JVP, EOM sigma) ---------------------------------
+ * A plan owns one complex workspace of `workspace` elements holding every atom
+ * (padded tensor of n^rank entries at atom_offset[a]) plus the padded base blocks
+ * added with c3l_plan_add_base.  `vectors` is the [n_vectors x n] complex table of
+ * creator/annihilator vectors.  Constant creator atoms are filled at creation:
+ * atom const_atom[k] <- vectors[const_vec[k]].
+ */
+int c3l_plan_create(int32_t n, int32_t n_vectors, const double* vectors, int64_t workspace,
+                    int32_t n_const, const int32_t* const_atom, const int64_t* const_offset,
+                    const int32_t* const_vec, c3l_plan** out);
+int c3l_plan_destroy(c3l_plan* plan);
+
+/* Register padded base block `block` of atom kind `kind` (0=T, 1=D, 2=X): padded
+ * element e (of n^deg) equals sign[e] * v[src[e]] of the loaded complex vector v,
+ * or 0 when src[e] < 0.  The block lives at workspace offset `offset`. */
+int c3l_plan_add_base(c3l_plan* plan, int32_t kind, int64_t offset, int32_t deg, const int64_t* src,
+                      const int8_t* sign);
+
+/* Register one contraction level of `kind`: job j builds atom at out_offset[j]
+ * (rank par_rank[j]-1) = sign[j] * sum_q vectors[vec[j]][q] * parent[.., q@slot[j], ..]
+ * with the parent at par_offset[j] (rank par_rank[j]).  Levels run in the order added. */
+int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64_t* out_offset,
+                       const int64_t* par_offset, const int32_t* par_rank, const int32_t* slot,
+                       const int32_t* vec, const int32_t* sign);
+
+/* Register one output block of program `program`: `count` canonical coordinates of
+ * degree `deg` (coord[count*deg] local indices, dest[count] output positions) and
+ * `n_entries` wedge entries: entry t = coeff[t] * prod_{f in [fstart[t], fstart[t+1])}
+ * ws[foffset[f] + flat_f], flat_f = sum_s coord[pos_s] n^(rank-1-s) with
+ * fcode[f] = rank | pos0<<2 | pos1<<4 | pos2<<6. */
+int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
+                       const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
+                       int32_t n_factors, const int64_t* foffset, const int32_t* fcode);
+
+/* Scatter the complex device vector v into the base blocks of `kind` and run that
+ * kind's contraction levels (atoms of that kind are then current). */
+int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream);
+
+/* out[dest] = diag[dest]*x[dest] + sum of the program's entries, for every block of
+ * `program` (out positions not covered by a block are left untouched).  diag and x
+ * may be NULL (no diagonal term). */
+int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out,
+                   void* stream);
+
+/* ---- tilde real coordinates -------------------------------------------------------- */
+/* t[i] = ab[4i..4i+1] * x[k0[i]] + ab[4i+2..4i+3] * x[k1[i]]  (complex * real) */
+int c3l_tilde_unpack(const double* x, double* t, const int64_t* k0, const int64_t* k1, const double* ab,
+                     int64_t dim, void* stream);
+/* x[c] = Re(w[c] * t[src[c]]) */
+int c3l_tilde_pack(const double* t, double* x, const int64_t* src, const double* w, int64_t real_dim,
+                   void* stream);
+
+/* ---- Krylov building blocks --------------------------------------------------------- */
+/* Real: out[k] = sum_i V[k*ld + i] * w[i], k < ncols (partial sums in scratch of
+ * at least ncols*c3l_reduction_chunks(dim) doubles). */
+int c3l_rmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim, double* scratch,
+                  double* out, void* stream);
+/* Real: w[i] += sum_k alpha[k] * V[k*ld + i] */
+int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double* w, int64_t dim,
+                   void* stream);
+/* Complex: out[k] = sum_i conj?(V[k*ld+i]) * w[i] (conj when conjugate != 0). */
+int c3l_zmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim, int32_t conjugate,
+                  double* scratch, double* out, void* stream);
+/* Complex: w[i] += sum_k alpha[k] * V[k*ld + i] */
+int c3l_zmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double* w, int64_t dim,
+                   void* stream);
+/* Number of row chunks used by the multidot reductions for a given dimension. */
+int64_t c3l_reduction_chunks(int64_t dim);
+
+/* For every shift z_s: y = (z_s I - T)^{-1} r with T upper triangular (m x m,
+ * column-major complex), green[s] = sum_k w[k] y[k], last[s] = sum_k q[k] y[k]. */
+int c3l_shifted_triangular_green(const double* T, int32_t m, const double* r, const double* w, const double* q,
+                                 const double* shifts, int32_t n_shifts, double* green, double* last,
+                                 void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* C3L_HIP_H */

@@ -1,0 +1,275 @@
+"""Bit-mask polynomial oracle -- TEST INFRASTRUCTURE ONLY (see ``oracle/__init__``).
+
+A polynomial is ``{mask: coefficient}`` over canonical creator strings
+``bar P_p1 bar P_p2 ...`` with ``p1 < p2 < ...`` (bit ``p`` of the integer
+mask).  Functions follow the reference line by line in meaning:
+
+* ``derivatives`` / ``linear_shifts``   -- ``factorized.py:598-633``
+* ``wedge_sign`` / ``apply_shift``      -- ``factorized.py:636-668``
+* ``apply_transformed_linear``          -- ``factorized.py:671-702``
+* ``product_action`` / ``residual``     -- ``factorized.py:709-795``
+* ``product_action_jvp`` / ``jvp``      -- ``factorized.py:798-889``
+* ``sigma``  -- ``component_tensor.py:2116-2302``: ``Λ∘X + P_13[K̄ X|0> - X ∧ K̄|0>]``
+  with ``K̄ = exp(-T) K exp(T)`` and no ``-i`` (``eom.py:79``).
+* ``source`` / ``sink``                 -- ``component_tensor.py:2780-2858``.
+
+Pins: against the sparse Fock-space brute force (``oracle/fock.py``) on
+random amplitudes/inputs for n = 1, 2, 3 (``tests/test_oracle_pins.py``),
+and end to end against dense Lindblad ED at n = 1 where CCSD and S13 are
+exact (``tests/test_physics_pins.py``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+Polynomial = dict
+
+
+def _add(poly: dict, mask: int, value: complex) -> None:
+    if value == 0.0:
+        return
+    updated = poly.get(mask, 0.0j) + value
+    if updated == 0.0:
+        poly.pop(mask, None)
+    else:
+        poly[mask] = updated
+
+
+def mask_of(modes) -> int:
+    m = 0
+    for p in modes:
+        m |= 1 << int(p)
+    return m
+
+
+def coordinate_masks(coords) -> list[int]:
+    """Python-int masks of a ``CoordinateSet`` in vector order."""
+
+    return [mask_of(row[row >= 0]) for row in coords.modes]
+
+
+def derivatives(masks, amplitudes, num_modes: int) -> list[dict]:
+    """Coefficients of ``[P_p, T]`` in canonical order, per mode ``p``."""
+
+    out = [dict() for _ in range(num_modes)]
+    for mask, amp in zip(masks, amplitudes):
+        occ = mask
+        while occ:
+            bit = occ & -occ
+            p = bit.bit_length() - 1
+            sign = -1.0 if (mask & (bit - 1)).bit_count() % 2 else 1.0
+            out[p][mask ^ bit] = sign * complex(amp)
+            occ ^= bit
+    return out
+
+
+def linear_shift(components, derivs) -> dict:
+    """``sum_p f_P[p] [P_p, T]`` of one linear form."""
+
+    poly: dict = {}
+    for kind, mode, coefficient in components:
+        if kind != "annihilate":
+            continue
+        for mask, value in derivs[mode].items():
+            _add(poly, mask, coefficient * value)
+    return poly
+
+
+def wedge_sign(left: int, right: int) -> float:
+    parity = 0
+    occ = left
+    while occ:
+        bit = occ & -occ
+        parity ^= (right & (bit - 1)).bit_count() & 1
+        occ ^= bit
+    return -1.0 if parity else 1.0
+
+
+def apply_shift(shift: dict, state: dict, rank_limit: int) -> dict:
+    out: dict = {}
+    for lm, lv in shift.items():
+        for rm, rv in state.items():
+            if lm & rm:
+                continue
+            target = lm | rm
+            if target.bit_count() > rank_limit:
+                continue
+            _add(out, target, lv * rv * wedge_sign(lm, rm))
+    return out
+
+
+def apply_transformed_linear(components, shift: dict, state: dict, rank_limit: int) -> dict:
+    """Apply ``F + [F, T]`` to a sparse rank-bounded polynomial."""
+
+    out: dict = {}
+    for mask, amp in state.items():
+        for kind, mode, coefficient in components:
+            bit = 1 << mode
+            occupied = bool(mask & bit)
+            if kind == "annihilate":
+                if not occupied:
+                    continue
+                target = mask ^ bit
+            else:
+                if occupied:
+                    continue
+                target = mask | bit
+            if target.bit_count() > rank_limit:
+                continue
+            sign = -1.0 if (mask & (bit - 1)).bit_count() % 2 else 1.0
+            _add(out, target, amp * coefficient * sign)
+    for mask, value in apply_shift(shift, state, rank_limit).items():
+        _add(out, mask, value)
+    return out
+
+
+def _shifts(products, derivs) -> dict:
+    shifts = {}
+    for record in products:
+        for f in record.factors:
+            if f.components not in shifts:
+                shifts[f.components] = linear_shift(f.components, derivs)
+    return shifts
+
+
+def product_action(record, shifts, max_rank: int, initial: dict | None = None) -> dict:
+    state = {0: record.coefficient} if initial is None else {m: record.coefficient * v for m, v in initial.items()}
+    count = len(record.factors)
+    for step, f in enumerate(reversed(record.factors)):
+        remaining = count - step - 1
+        state = apply_transformed_linear(f.components, shifts[f.components], state, max_rank + remaining)
+    return state
+
+
+def _diagonal(masks, rapidities, values) -> np.ndarray:
+    out = np.empty(len(masks), dtype=np.complex128)
+    for i, (mask, v) in enumerate(zip(masks, values)):
+        lam = 0.0j
+        occ = mask
+        while occ:
+            bit = occ & -occ
+            lam += rapidities[bit.bit_length() - 1]
+            occ ^= bit
+        out[i] = lam * v
+    return out
+
+
+def residual(problem, amplitudes, masks=None) -> tuple[np.ndarray, complex]:
+    """``R = -i (Λ∘t + P[sum_products exp(-T) prod F exp(T)|0>])`` and the rank-0 part."""
+
+    masks = coordinate_masks(problem.coordinates) if masks is None else masks
+    index = {m: i for i, m in enumerate(masks)}
+    t = np.asarray(amplitudes, dtype=np.complex128)
+    derivs = derivatives(masks, t, problem.num_modes)
+    shifts = _shifts(problem.products, derivs)
+    out = _diagonal(masks, problem.rapidities, t)
+    reference = 0.0j
+    for record in problem.products:
+        result = product_action(record, shifts, 4)
+        reference += result.get(0, 0.0j)
+        for mask, value in result.items():
+            i = index.get(mask)
+            if i is not None:
+                out[i] += value
+    return -1j * out, -1j * reference
+
+
+def product_action_jvp(record, shifts, dshifts, max_rank: int) -> dict:
+    state = {0: record.coefficient}
+    tangent: dict = {}
+    count = len(record.factors)
+    for step, f in enumerate(reversed(record.factors)):
+        remaining = count - step - 1
+        limit = max_rank + remaining
+        from_state = apply_shift(dshifts[f.components], state, limit)
+        tangent = apply_transformed_linear(f.components, shifts[f.components], tangent, limit)
+        for mask, value in from_state.items():
+            _add(tangent, mask, value)
+        state = apply_transformed_linear(f.components, shifts[f.components], state, limit)
+    return tangent
+
+
+def jvp(problem, amplitudes, direction, masks=None) -> np.ndarray:
+    masks = coordinate_masks(problem.coordinates) if masks is None else masks
+    index = {m: i for i, m in enumerate(masks)}
+    t = np.asarray(amplitudes, dtype=np.complex128)
+    dt = np.asarray(direction, dtype=np.complex128)
+    shifts = _shifts(problem.products, derivatives(masks, t, problem.num_modes))
+    dshifts = _shifts(problem.products, derivatives(masks, dt, problem.num_modes))
+    out = _diagonal(masks, problem.rapidities, dt)
+    for record in problem.products:
+        for mask, value in product_action_jvp(record, shifts, dshifts, 4).items():
+            i = index.get(mask)
+            if i is not None:
+                out[i] += value
+    return -1j * out
+
+
+def vacuum_polynomial(problem, amplitudes, masks, max_rank: int = 4) -> dict:
+    """``sum_products exp(-T) prod F exp(T)|0>`` through ``max_rank``."""
+
+    t = np.asarray(amplitudes, dtype=np.complex128)
+    shifts = _shifts(problem.products, derivatives(masks, t, problem.num_modes))
+    vac: dict = {}
+    for record in problem.products:
+        for mask, value in product_action(record, shifts, max_rank).items():
+            _add(vac, mask, value)
+    return vac
+
+
+def sigma(problem, amplitudes, response_masks, x, static_masks=None) -> np.ndarray:
+    """EOM-CCSD sigma on one charge block: ``Λ∘X + P[K̄ X|0> - X ∧ K̄|0>]``."""
+
+    static_masks = coordinate_masks(problem.coordinates) if static_masks is None else static_masks
+    index = {m: i for i, m in enumerate(response_masks)}
+    t = np.asarray(amplitudes, dtype=np.complex128)
+    x = np.asarray(x, dtype=np.complex128)
+    shifts = _shifts(problem.products, derivatives(static_masks, t, problem.num_modes))
+    xpoly = {m: complex(v) for m, v in zip(response_masks, x) if v != 0.0}
+    out = _diagonal(response_masks, problem.rapidities, x)
+    for record in problem.products:
+        for mask, value in product_action(record, shifts, 3, initial=xpoly).items():
+            i = index.get(mask)
+            if i is not None:
+                out[i] += value
+    vac = vacuum_polynomial(problem, t, static_masks, max_rank=2)
+    for xm, xv in xpoly.items():
+        for vm, vv in vac.items():
+            if xm & vm:
+                continue
+            i = index.get(xm | vm)
+            if i is not None:
+                out[i] -= xv * vv * wedge_sign(xm, vm)
+    return out
+
+
+def source(problem, amplitudes, response_masks, components, static_masks=None) -> np.ndarray:
+    """``pack(F_creator + ι_F T)`` on one charge block."""
+
+    static_masks = coordinate_masks(problem.coordinates) if static_masks is None else static_masks
+    index = {m: i for i, m in enumerate(response_masks)}
+    t = np.asarray(amplitudes, dtype=np.complex128)
+    shift = linear_shift(components, derivatives(static_masks, t, problem.num_modes))
+    out = np.zeros(len(response_masks), dtype=np.complex128)
+    for kind, mode, coefficient in components:
+        if kind == "create":
+            i = index.get(1 << mode)
+            if i is not None:
+                out[i] += coefficient
+    for mask, value in shift.items():
+        i = index.get(mask)
+        if i is not None:
+            out[i] += value
+    return out
+
+
+def sink(response_masks, components) -> np.ndarray:
+    """Rank-one annihilator coefficients of the sink operator."""
+
+    out = np.zeros(len(response_masks), dtype=np.complex128)
+    coeff = {mode: c for kind, mode, c in components if kind == "annihilate"}
+    for i, mask in enumerate(response_masks):
+        if mask.bit_count() == 1:
+            out[i] = coeff.get(mask.bit_length() - 1, 0.0)
+    return out

@@ -1,0 +1,63 @@
+"""NumPy executor of lowered contraction plans -- TEST INFRASTRUCTURE / CPU BASELINE ONLY.
+
+Runs exactly the arrays the device library receives (``c3lindblad_amd.plan_arrays``):
+padded base scatter, contraction levels with ``np.tensordot``, and the wedge
+entries evaluated over all canonical output coordinates at once.  It is the
+CPU restatement of the component algorithm of ``src/c3l/cc/component_tensor.py``
+timed beside the GPU (``bench.py`` ``cpu_baseline``), and the second checker
+of the device kernels at sizes the polynomial oracle cannot reach.
+
+Pins: equals ``oracle.polynomial`` residual/JVP/sigma for n = 1..4 random
+inputs (``tests/test_oracle_pins.py``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+class PortPlan:
+    def __init__(self, lowered):
+        self.lp = lowered
+        self.n = lowered.n
+        self.ws = np.zeros(max(lowered.workspace, 1), dtype=np.complex128)
+        for off, vec in zip(lowered.const_offset, lowered.const_vec):
+            self.ws[off : off + self.n] = lowered.vectors[vec]
+
+    def load(self, kind: int, v: np.ndarray) -> None:
+        n = self.n
+        v = np.asarray(v, dtype=np.complex128)
+        for k, off, deg, src, sign in self.lp.bases:
+            if k != kind:
+                continue
+            vals = np.where(src >= 0, v[np.maximum(src, 0)], 0.0) * sign
+            self.ws[off : off + src.size] = vals
+        for level in self.lp.levels:
+            k, out_off, par_off, par_rank, slot, vec, sign = level
+            if k != kind:
+                continue
+            for o, p, r, s, vi, sg in zip(out_off, par_off, par_rank, slot, vec, sign):
+                parent = self.ws[p : p + n**r].reshape((n,) * r)
+                out = np.tensordot(parent, self.lp.vectors[vi], axes=([s], [0]))
+                self.ws[o : o + n ** (r - 1)] = sg * np.ravel(out)
+
+    def apply(self, program: int, out: np.ndarray, diag=None, x=None) -> np.ndarray:
+        n = self.n
+        for pid, deg, coord, dest, coeff, fstart, foff, fcode in self.lp.blocks:
+            if pid != program:
+                continue
+            acc = np.zeros(dest.size, dtype=np.complex128)
+            for t in range(coeff.size):
+                prod = np.full(dest.size, coeff[t], dtype=np.complex128)
+                for f in range(fstart[t], fstart[t + 1]):
+                    code = int(fcode[f])
+                    rank = code & 3
+                    flat = np.zeros(dest.size, dtype=np.int64)
+                    for s in range(rank):
+                        flat = flat * n + coord[:, (code >> (2 + 2 * s)) & 3]
+                    prod *= self.ws[foff[f] + flat]
+                acc += prod
+            if diag is not None:
+                acc += diag[dest] * x[dest]
+            out[dest] = acc
+        return out
