@@ -167,6 +167,35 @@ __global__ void tilde_pack_kernel(const double2* __restrict__ t, double* __restr
   }
 }
 
+// y[c] = Re(w[c] * (alpha x[k0] + beta x[k1])[src[c]] * inv_diag[src[c]]): unpack,
+// divide by the rapidity diagonal and pack in one pass over the real coordinates.
+__global__ void tilde_precondition_kernel(const double* __restrict__ x, double* __restrict__ y,
+                                          const int64_t* __restrict__ k0, const int64_t* __restrict__ k1,
+                                          const double4* __restrict__ ab, const int64_t* __restrict__ src,
+                                          const double2* __restrict__ w, const double2* __restrict__ inv_diag,
+                                          int64_t real_dim) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < real_dim;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = src[c];
+    const double4 co = ab[i];
+    const double a = x[k0[i]];
+    const double b = x[k1[i]];
+    const double2 t = make_double2(co.x * a + co.z * b, co.y * a + co.w * b);
+    const double2 v = cmul(cmul(t, inv_diag[i]), w[c]);
+    y[c] = v.x;
+  }
+}
+
+__global__ void rscale_kernel(const double* __restrict__ x, double* __restrict__ y, double a, int64_t dim) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
+}
+
+__global__ void raxpby_kernel(double a, const double* __restrict__ x, double b, double* __restrict__ y, int64_t dim) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+
 // ------------------------------------------------------------------ Krylov kernels
 
 constexpr int kChunkRows = 8192;
@@ -224,9 +253,9 @@ __global__ void rfinish_kernel(const double* __restrict__ partial, int64_t chunk
 }
 
 __global__ void rmultiaxpy_kernel(const double* __restrict__ V, int64_t ld, int ncols,
-                                  const double* __restrict__ alpha, double* __restrict__ w, int64_t dim) {
+                                  const double* __restrict__ alpha, double scale, double* __restrict__ w, int64_t dim) {
   extern __shared__ double coef[];
-  for (int k = threadIdx.x; k < ncols; k += blockDim.x) coef[k] = alpha[k];
+  for (int k = threadIdx.x; k < ncols; k += blockDim.x) coef[k] = scale * alpha[k];
   __syncthreads();
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
     double acc = w[i];
@@ -268,9 +297,11 @@ __global__ void zfinish_kernel(const double2* __restrict__ partial, int64_t chun
 }
 
 __global__ void zmultiaxpy_kernel(const double2* __restrict__ V, int64_t ld, int ncols,
-                                  const double2* __restrict__ alpha, double2* __restrict__ w, int64_t dim) {
+                                  const double2* __restrict__ alpha, double scale, double2* __restrict__ w,
+                                  int64_t dim) {
   extern __shared__ double2 zcoef[];
-  for (int k = threadIdx.x; k < ncols; k += blockDim.x) zcoef[k] = alpha[k];
+  for (int k = threadIdx.x; k < ncols; k += blockDim.x)
+    zcoef[k] = make_double2(scale * alpha[k].x, scale * alpha[k].y);
   __syncthreads();
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
     double2 acc = w[i];
@@ -584,12 +615,45 @@ int c3l_rmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, i
   return C3L_OK;
 }
 
-int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double* w, int64_t dim,
-                   void* stream) {
-  if (ncols <= 0 || dim <= 0 || ld < dim || ncols > 8192 || V == nullptr || alpha == nullptr || w == nullptr)
+int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double scale, double* w,
+                   int64_t dim, void* stream) {
+  if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || alpha == nullptr || w == nullptr)
     return fail(C3L_EINVAL, "c3l_rmultiaxpy: invalid argument");
-  hipLaunchKernelGGL(rmultiaxpy_kernel, dim3(grid_for(dim)), dim3(kThreads), sizeof(double) * ncols,
-                     static_cast<hipStream_t>(stream), V, ld, ncols, alpha, w, dim);
+  constexpr int kCols = 2048;  // coefficients staged in LDS per launch
+  for (int32_t k0 = 0; k0 < ncols; k0 += kCols) {
+    const int32_t cols = ncols - k0 < kCols ? ncols - k0 : kCols;
+    hipLaunchKernelGGL(rmultiaxpy_kernel, dim3(grid_for(dim)), dim3(kThreads), sizeof(double) * cols,
+                       static_cast<hipStream_t>(stream), V + (int64_t)k0 * ld, ld, cols, alpha + k0, scale, w, dim);
+    C3L_HIP_CHECK(hipGetLastError());
+  }
+  return C3L_OK;
+}
+
+int c3l_tilde_precondition(const double* x, double* y, const int64_t* k0, const int64_t* k1, const double* ab,
+                           const int64_t* src, const double* w, const double* inv_diag, int64_t real_dim,
+                           void* stream) {
+  if (real_dim <= 0 || x == nullptr || y == nullptr || k0 == nullptr || k1 == nullptr || ab == nullptr ||
+      src == nullptr || w == nullptr || inv_diag == nullptr || x == y)
+    return fail(C3L_EINVAL, "c3l_tilde_precondition: invalid argument");
+  hipLaunchKernelGGL(tilde_precondition_kernel, dim3(grid_for(real_dim)), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), x, y, k0, k1, reinterpret_cast<const double4*>(ab), src,
+                     reinterpret_cast<const double2*>(w), reinterpret_cast<const double2*>(inv_diag), real_dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_rscale(const double* x, double* y, double a, int64_t dim, void* stream) {
+  if (dim <= 0 || x == nullptr || y == nullptr) return fail(C3L_EINVAL, "c3l_rscale: invalid argument");
+  hipLaunchKernelGGL(rscale_kernel, dim3(grid_for(dim)), dim3(kThreads), 0, static_cast<hipStream_t>(stream), x, y, a,
+                     dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_raxpby(double a, const double* x, double b, double* y, int64_t dim, void* stream) {
+  if (dim <= 0 || x == nullptr || y == nullptr) return fail(C3L_EINVAL, "c3l_raxpby: invalid argument");
+  hipLaunchKernelGGL(raxpby_kernel, dim3(grid_for(dim)), dim3(kThreads), 0, static_cast<hipStream_t>(stream), a, x, b,
+                     y, dim);
   C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;
 }
@@ -610,14 +674,20 @@ int c3l_zmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, i
   return C3L_OK;
 }
 
-int c3l_zmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double* w, int64_t dim,
-                   void* stream) {
-  if (ncols <= 0 || dim <= 0 || ld < dim || ncols > 4096 || V == nullptr || alpha == nullptr || w == nullptr)
+int c3l_zmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double scale, double* w,
+                   int64_t dim, void* stream) {
+  if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || alpha == nullptr || w == nullptr)
     return fail(C3L_EINVAL, "c3l_zmultiaxpy: invalid argument");
-  hipLaunchKernelGGL(zmultiaxpy_kernel, dim3(grid_for(dim)), dim3(kThreads), sizeof(double2) * ncols,
-                     static_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(V), ld, ncols,
-                     reinterpret_cast<const double2*>(alpha), reinterpret_cast<double2*>(w), dim);
-  C3L_HIP_CHECK(hipGetLastError());
+  constexpr int kCols = 1024;  // coefficients staged in LDS per launch
+  const double2* Vz = reinterpret_cast<const double2*>(V);
+  const double2* az = reinterpret_cast<const double2*>(alpha);
+  for (int32_t k0 = 0; k0 < ncols; k0 += kCols) {
+    const int32_t cols = ncols - k0 < kCols ? ncols - k0 : kCols;
+    hipLaunchKernelGGL(zmultiaxpy_kernel, dim3(grid_for(dim)), dim3(kThreads), sizeof(double2) * cols,
+                       static_cast<hipStream_t>(stream), Vz + (int64_t)k0 * ld, ld, cols, az + k0, scale,
+                       reinterpret_cast<double2*>(w), dim);
+    C3L_HIP_CHECK(hipGetLastError());
+  }
   return C3L_OK;
 }
 

@@ -141,6 +141,23 @@ class DeviceTilde:
             _lib.stream_ptr(self.torch),
         )
 
+    def precondition(self, x, y, inv_diag) -> None:
+        """``y = pack(unpack(x) * inv_diag)`` in one kernel."""
+
+        _lib.call(
+            "c3l_tilde_precondition",
+            _lib.dptr(x),
+            _lib.dptr(y),
+            _lib.dptr(self.k0),
+            _lib.dptr(self.k1),
+            _lib.dptr(self.ab),
+            _lib.dptr(self.src),
+            _lib.dptr(self.w),
+            _lib.dptr(inv_diag),
+            self.real_dim,
+            _lib.stream_ptr(self.torch),
+        )
+
     def pack(self, t, x) -> None:
         _lib.call(
             "c3l_tilde_pack",
@@ -179,7 +196,18 @@ class Reducer:
                 _lib.dptr(self.scratch), _lib.dptr(out), _lib.stream_ptr(self.torch),
             )
 
-    def axpy(self, V, ncols: int, alpha, w) -> None:
+    def axpy(self, V, ncols: int, alpha, w, scale: float = 1.0) -> None:
+        """``w += scale * sum_k alpha[k] V[k]``."""
+
         ld = V.stride(0)
         name = "c3l_zmultiaxpy" if self.complex else "c3l_rmultiaxpy"
-        _lib.call(name, _lib.dptr(V), ld, ncols, _lib.dptr(alpha), _lib.dptr(w), self.dim, _lib.stream_ptr(self.torch))
+        _lib.call(name, _lib.dptr(V), ld, ncols, _lib.dptr(alpha), float(scale), _lib.dptr(w), self.dim,
+                  _lib.stream_ptr(self.torch))
+
+
+def rscale(x, y, a: float, torch) -> None:
+    _lib.call("c3l_rscale", _lib.dptr(x), _lib.dptr(y), float(a), x.numel(), _lib.stream_ptr(torch))
+
+
+def raxpby(a: float, x, b: float, y, torch) -> None:
+    _lib.call("c3l_raxpby", float(a), _lib.dptr(x), float(b), _lib.dptr(y), x.numel(), _lib.stream_ptr(torch))

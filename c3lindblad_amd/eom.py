@@ -77,6 +77,17 @@ class ComponentEOMProblem:
     def dimension(self) -> int:
         return self.coordinates.dimension
 
+    def device_plan(self):
+        """The uploaded sigma plan, shared by every context of this problem."""
+
+        plan = getattr(self, "_plan", None)
+        if plan is None:
+            from .device import DevicePlan
+
+            plan = DevicePlan(self.lowered)
+            self._plan = plan
+        return plan
+
 
 @dataclass
 class ComponentEOMContext:
@@ -101,20 +112,22 @@ def component_eom_context(problem: ComponentEOMProblem, amplitudes) -> Component
 
 
 class EOMDevice:
-    def __init__(self, context: ComponentEOMContext):
-        from .device import DevicePlan
+    """Device state of one context: the problem's plan with this context's amplitude atoms."""
 
+    def __init__(self, context: ComponentEOMContext):
         torch = _lib.require_device()
         self.torch = torch
-        self.plan = DevicePlan(context.problem.lowered)
+        self.plan = context.problem.device_plan()
         self.t = torch.from_numpy(np.ascontiguousarray(context.amplitudes)).to("cuda")
-        self.plan.load("T", self.t)
+        self.token = self.plan.load("T", self.t)
         self.diag = torch.from_numpy(np.ascontiguousarray(context.problem.diagonal)).to("cuda")
         self.dim = context.problem.dimension
 
     def sigma(self, x, out) -> None:
         """``out = σ(x)`` for contiguous complex device vectors of the block dimension."""
 
+        if self.plan.loaded.get(0) != self.token:
+            self.token = self.plan.load("T", self.t)
         self.plan.load("X", x)
         self.plan.apply("sigma", out, self.diag, x)
 

@@ -1,0 +1,88 @@
+"""One complete CCSD + EOM G^R(ω) solve of a model: the unit of the solves/sec metric.
+
+``prepare`` does the host set-up once (full HF, normal modes, lowering, plan
+compilation for the static problem and each requested charge block).
+``SolvePipeline.solve`` is one solve on the GPU: Newton from the zero guess to
+``||pack R|| <= tolerance``, an EOM context per charge, and the addition and
+removal resolvents on every frequency.  Nothing computed by one solve is
+reused by the next apart from the compiled plans.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from time import perf_counter
+
+import numpy as np
+
+from .eom import ComponentEOMProblem, ComponentSigmaOperator, component_eom_context, component_eom_sources, eom_ccsd_problem
+from .lowering import factorized_cc_problem
+from .response import component_retarded_green
+from .solver import NewtonKrylovPolicy, solve_component_ccsd
+from .static import ComponentStaticProblem
+
+
+@dataclass
+class SolveResult:
+    green: dict  # charge -> complex G^R on the grid
+    packed_residual_norm: float
+    newton_iterations: int
+    jvp_evaluations: int
+    residual_evaluations: int
+    krylov_dimensions: dict
+    max_shift_residual: float
+    seconds: dict = field(default_factory=dict)
+    amplitudes: np.ndarray | None = None
+
+
+@dataclass
+class SolvePipeline:
+    static: ComponentStaticProblem
+    eom: dict  # charge -> ComponentEOMProblem
+    omega: np.ndarray
+    eta: float
+    setup_seconds: float
+    policy: NewtonKrylovPolicy
+    rtol: float = 1e-10
+
+    @classmethod
+    def prepare(cls, model, omega, eta=0.0, charges=(1, -1), policy=None, rtol=1e-10):
+        started = perf_counter()
+        problem = factorized_cc_problem(model)
+        static = ComponentStaticProblem.compile(problem)
+        eom = eom_ccsd_problem(problem)
+        blocks = {q: ComponentEOMProblem.compile(eom, static, q) for q in charges}
+        static.device  # upload
+        for b in blocks.values():
+            b.device_plan()
+        return cls(static, blocks, np.asarray(omega, dtype=np.float64), float(eta), perf_counter() - started,
+                   policy or NewtonKrylovPolicy(), rtol)
+
+    def solve(self) -> SolveResult:
+        t0 = perf_counter()
+        sol = solve_component_ccsd(self.static, policy=self.policy)
+        if not sol.converged:
+            raise RuntimeError(f"CCSD did not converge (packed residual {sol.packed_residual_norm:.2e})")
+        t1 = perf_counter()
+        green = {}
+        dims = {}
+        worst = 0.0
+        for q, block in self.eom.items():
+            ctx = component_eom_context(block, sol.amplitudes)
+            sources = component_eom_sources(ctx)
+            result = component_retarded_green(ComponentSigmaOperator(ctx), sources, self.omega, self.eta, rtol=self.rtol)
+            green[q] = result.retarded
+            dims[q] = tuple(p.dimension if p is not None else 0 for p in (result.addition_projection, result.removal_projection))
+            worst = max(worst, float(np.max(result.addition_relative_residuals)), float(np.max(result.removal_relative_residuals)))
+        t2 = perf_counter()
+        return SolveResult(
+            green=green,
+            packed_residual_norm=sol.packed_residual_norm,
+            newton_iterations=sol.iterations,
+            jvp_evaluations=sol.jvp_evaluations,
+            residual_evaluations=sol.residual_evaluations,
+            krylov_dimensions=dims,
+            max_shift_residual=worst,
+            seconds={"ccsd": t1 - t0, "eom_green": t2 - t1},
+            amplitudes=sol.amplitudes,
+        )

@@ -1,0 +1,315 @@
+"""Inexact Newton--Krylov CCSD solver with every vector on the GPU.
+
+Drop-in for ``solve_component_ccsd`` (``src/c3l/cc/component_solver.py:103-295``)
+with ``NewtonKrylovPolicy`` (``cc/newton_krylov.py:35-123``).  Same control
+flow: real tilde coordinates, rapidity (diagonal) preconditioner with the
+1e-12 regularisation (``newton_krylov.py:158-192``), restarted GMRES with the
+legacy or Eisenstat--Walker forcing term (``newton_krylov.py:209-237``),
+Armijo backtracking (``component_solver.py:224-256``), final packed residual
+check.  GMRES follows SciPy's left-preconditioned restarted scheme
+(preconditioned inner tolerance ``||M b|| * rtol`` and a true-residual test
+at every restart) with two-pass classical Gram--Schmidt on the device; only
+the Hessenberg column (``j+2`` numbers) crosses to the host per iteration for
+the Givens update.
+"""
+
+from __future__ import annotations
+
+import resource
+from dataclasses import dataclass, field
+from time import perf_counter
+
+import numpy as np
+
+from . import _lib
+from .device import Reducer, raxpby, rscale
+from .static import ComponentStaticAction, ComponentStaticProblem, component_static_residual
+
+
+@dataclass(frozen=True)
+class NewtonKrylovPolicy:
+    tolerance: float = 1.0e-10
+    maximum_newton_iterations: int = 20
+    gmres_restart: int = 40
+    gmres_maximum_restart_cycles: int = 12
+    minimum_gmres_relative_tolerance: float = 1.0e-8
+    maximum_gmres_relative_tolerance: float = 1.0e-3
+    maximum_line_search_steps: int = 14
+    armijo: float = 1.0e-4
+    forcing_strategy: str = "legacy"
+    initial_gmres_relative_tolerance: float = 1.0e-2
+    eisenstat_walker_gamma: float = 0.9
+    eisenstat_walker_exponent: float = 1.5
+    gmres_absolute_tolerance_factor: float = 0.1
+
+    def __post_init__(self) -> None:
+        positive = (
+            self.tolerance,
+            self.minimum_gmres_relative_tolerance,
+            self.maximum_gmres_relative_tolerance,
+            self.armijo,
+            self.initial_gmres_relative_tolerance,
+        )
+        if not all(np.isfinite(v) and v > 0.0 for v in positive):
+            raise ValueError("Newton--Krylov tolerances must be finite and positive")
+        if self.minimum_gmres_relative_tolerance > self.maximum_gmres_relative_tolerance:
+            raise ValueError("minimum GMRES tolerance exceeds the maximum")
+        if self.forcing_strategy not in ("legacy", "eisenstat_walker"):
+            raise ValueError("unknown inexact-Newton forcing strategy")
+        for v in (self.maximum_newton_iterations, self.gmres_restart, self.gmres_maximum_restart_cycles, self.maximum_line_search_steps):
+            if isinstance(v, bool) or not isinstance(v, int) or v <= 0:
+                raise ValueError("Newton--Krylov iteration limits must be positive integers")
+
+
+def forcing_tolerance(norm: float, policy: NewtonKrylovPolicy, previous_norm=None, previous_tolerance=None) -> float:
+    if policy.forcing_strategy == "legacy":
+        value = 0.1 * max(norm, policy.tolerance)
+    elif previous_norm is None:
+        value = policy.initial_gmres_relative_tolerance
+    else:
+        ratio = norm / max(previous_norm, np.finfo(float).tiny)
+        value = policy.eisenstat_walker_gamma * ratio**policy.eisenstat_walker_exponent
+        if previous_tolerance is not None:
+            value = max(value, policy.eisenstat_walker_gamma * previous_tolerance**policy.eisenstat_walker_exponent)
+    return float(min(policy.maximum_gmres_relative_tolerance, max(policy.minimum_gmres_relative_tolerance, value)))
+
+
+@dataclass(frozen=True)
+class NewtonKrylovStep:
+    iteration: int
+    residual_norm: float
+    gmres_relative_tolerance: float
+    gmres_info: int
+    gmres_iterations: int
+    true_linear_relative_residual: float
+    step_norm: float
+    damping: float
+    accepted_residual_norm: float
+
+
+@dataclass
+class ComponentCCSolution:
+    problem: ComponentStaticProblem
+    variables: np.ndarray
+    amplitudes: np.ndarray
+    residual: ComponentStaticAction
+    residual_norm: float
+    packed_residual_norm: float
+    reference_residual: float
+    converged: bool
+    iterations: int
+    residual_evaluations: int
+    jvp_evaluations: int
+    history: tuple
+    wall_seconds: float
+    peak_rss_kib: int
+    device_seconds: dict = field(default_factory=dict)
+
+
+class _DeviceNewton:
+    """Device buffers of one Newton--GMRES run."""
+
+    def __init__(self, problem: ComponentStaticProblem, restart: int):
+        self.torch = torch = _lib.require_device()
+        self.dev = problem.device
+        D = problem.oracle.num_variables
+        self.D = D
+        f64 = torch.float64
+        self.V = torch.zeros(restart + 1, D, dtype=f64, device="cuda")
+        self.w = torch.empty(D, dtype=f64, device="cuda")
+        self.tmp = torch.empty(D, dtype=f64, device="cuda")
+        self.r = torch.empty(D, dtype=f64, device="cuda")
+        self.trial = torch.empty(D, dtype=f64, device="cuda")
+        self.step = torch.empty(D, dtype=f64, device="cuda")
+        self.xtrial = torch.empty(D, dtype=f64, device="cuda")
+        self.coef = torch.zeros(2 * (restart + 2) + 2, dtype=f64, device="cuda")
+        self.host = torch.zeros(2 * (restart + 2) + 2, dtype=f64).pin_memory()
+        self.reducer = Reducer(D, restart + 2, complex_=False)
+        diag = -1j * problem.oracle.rapidity_sums()
+        mags = np.abs(diag)
+        threshold = 1.0e-12 * max(float(np.max(mags, initial=0.0)), 1.0)
+        small = mags < threshold
+        diag[small] = threshold * np.exp(1j * np.angle(diag[small] + 1e-300))
+        self.inv_diag = torch.from_numpy(1.0 / diag).to("cuda")
+        self.jvp_calls = 0
+        self.residual_calls = 0
+
+    def norm(self, v) -> float:
+        out = self.coef[:1]
+        self.reducer.dot(v.view(1, -1), 1, v, out)
+        return float(np.sqrt(max(out.item(), 0.0)))
+
+    def residual(self, x, r) -> float:
+        self.residual_calls += 1
+        self.dev.residual_real(x, r)
+        return self.norm(r)
+
+    def apply_A(self, v, out) -> None:
+        self.jvp_calls += 1
+        self.dev.jvp_real(v, out)
+
+    def precondition(self, v, out) -> None:
+        self.dev.tilde.precondition(v, out, self.inv_diag)
+
+    def gmres(self, b, x, rtol: float, restart: int, cycles: int):
+        """Left-preconditioned restarted GMRES for ``J x = b`` (x starts at 0)."""
+
+        torch = self.torch
+        x.zero_()
+        bnrm = self.norm(b)
+        if bnrm == 0.0:
+            return 0, 0, 0.0
+        self.precondition(b, self.tmp)
+        ptol = self.norm(self.tmp) * rtol
+        atol = rtol * bnrm
+        iterations = 0
+        info = 1
+        V = self.V
+        for cycle in range(cycles):
+            if cycle == 0:
+                self.w.copy_(b)
+            else:
+                self.apply_A(x, self.w)
+                raxpby(1.0, b, -1.0, self.w, torch)  # w = b - A x
+                if self.norm(self.w) < atol:
+                    info = 0
+                    break
+            self.precondition(self.w, V[0])
+            beta = self.norm(V[0])
+            rscale(V[0], V[0], 1.0 / beta, torch)
+            H = np.zeros((restart + 1, restart))
+            givens = np.zeros((restart, 2))
+            S = np.zeros(restart + 1)
+            S[0] = beta
+            used = 0
+            done = False
+            for j in range(restart):
+                self.apply_A(V[j], self.tmp)
+                self.precondition(self.tmp, self.w)
+                c1 = self.coef[: j + 1]
+                c2 = self.coef[restart + 1 : restart + 2 + j]
+                s = self.coef[2 * restart + 3 : 2 * restart + 4]
+                self.reducer.dot(V, j + 1, self.w, c1)
+                self.reducer.axpy(V, j + 1, c1, self.w, -1.0)
+                self.reducer.dot(V, j + 1, self.w, c2)
+                self.reducer.axpy(V, j + 1, c2, self.w, -1.0)
+                self.reducer.dot(self.w.view(1, -1), 1, self.w, s)
+                self.host.copy_(self.coef)
+                h = self.host.numpy()
+                col = h[: j + 1] + h[restart + 1 : restart + 2 + j]
+                hn = float(np.sqrt(max(h[2 * restart + 3], 0.0)))
+                H[: j + 1, j] = col
+                H[j + 1, j] = hn
+                if hn > 0.0:
+                    rscale(self.w, V[j + 1], 1.0 / hn, torch)
+                # Givens update of the least-squares problem
+                for k in range(j):
+                    c, sn = givens[k]
+                    a, bb = H[k, j], H[k + 1, j]
+                    H[k, j], H[k + 1, j] = c * a + sn * bb, -sn * a + c * bb
+                a, bb = H[j, j], H[j + 1, j]
+                rr = np.hypot(a, bb)
+                c, sn = (1.0, 0.0) if rr == 0.0 else (a / rr, bb / rr)
+                givens[j] = (c, sn)
+                H[j, j], H[j + 1, j] = rr, 0.0
+                S[j + 1] = -sn * S[j]
+                S[j] = c * S[j]
+                iterations += 1
+                used = j + 1
+                if abs(S[j + 1]) <= ptol or hn == 0.0:
+                    done = True
+                    break
+            y = np.zeros(used)
+            for k in range(used - 1, -1, -1):
+                y[k] = (S[k] - H[k, k + 1 : used] @ y[k + 1 :]) / H[k, k]
+            self.coef[:used].copy_(torch.from_numpy(y))
+            self.reducer.axpy(V, used, self.coef[:used], x, 1.0)
+            if done and cycle == cycles - 1:
+                info = 0
+        # true relative residual of the returned step
+        self.apply_A(x, self.w)
+        raxpby(1.0, b, -1.0, self.w, torch)
+        true = self.norm(self.w) / bnrm
+        if true <= rtol * (1.0 + 1e-8):
+            info = 0
+        return info, iterations, true
+
+
+def solve_component_ccsd(
+    problem: ComponentStaticProblem,
+    initial_amplitudes=None,
+    *,
+    policy: NewtonKrylovPolicy | None = None,
+    accepted_step_callback=None,
+) -> ComponentCCSolution:
+    """Solve the tilde-real CCSD equations to ``||pack R|| <= policy.tolerance`` on the GPU."""
+
+    controls = NewtonKrylovPolicy() if policy is None else policy
+    oracle = problem.oracle
+    started = perf_counter()
+    state = _DeviceNewton(problem, controls.gmres_restart)
+    torch = state.torch
+    if initial_amplitudes is None:
+        x0 = np.zeros(oracle.num_variables)
+    else:
+        initial = np.asarray(initial_amplitudes, dtype=np.complex128)
+        if initial.shape != (oracle.num_amplitudes,) or not np.all(np.isfinite(initial)):
+            raise ValueError("initial amplitudes do not match the component CCSD problem")
+        x0 = oracle.tilde.pack(initial, check=False)
+    x = torch.from_numpy(np.ascontiguousarray(x0)).to("cuda")
+    history = []
+    converged = False
+    previous_norm = None
+    previous_tol = None
+    for iteration in range(controls.maximum_newton_iterations + 1):
+        norm = state.residual(x, state.r)
+        if not np.isfinite(norm):
+            break
+        if norm <= controls.tolerance:
+            converged = True
+            break
+        if iteration == controls.maximum_newton_iterations:
+            break
+        tol = forcing_tolerance(norm, controls, previous_norm, previous_tol)
+        rscale(state.r, state.trial, -1.0, torch)  # b = -r
+        info, its, true = state.gmres(state.trial, state.step, tol, controls.gmres_restart, controls.gmres_maximum_restart_cycles)
+        damping = 1.0
+        accepted = False
+        accepted_norm = np.inf
+        for _ in range(controls.maximum_line_search_steps):
+            state.xtrial.copy_(x)
+            raxpby(damping, state.step, 1.0, state.xtrial, torch)
+            accepted_norm = state.residual(state.xtrial, state.trial)
+            if accepted_norm <= (1.0 - controls.armijo * damping) * norm:
+                x.copy_(state.xtrial)
+                accepted = True
+                if accepted_step_callback is not None:
+                    accepted_step_callback(iteration, oracle.tilde.unpack(x.cpu().numpy()), accepted_norm)
+                break
+            damping *= 0.5
+        history.append(
+            NewtonKrylovStep(iteration, norm, tol, info, its, true, state.norm(state.step), damping, accepted_norm)
+        )
+        if not accepted:
+            break
+        previous_norm, previous_tol = norm, tol
+    xh = x.cpu().numpy()
+    amplitudes = oracle.tilde.unpack(xh)
+    final = component_static_residual(problem, amplitudes)
+    packed = float(np.linalg.norm(oracle.tilde.pack(final.retained, check=False)))
+    return ComponentCCSolution(
+        problem=problem,
+        variables=xh,
+        amplitudes=amplitudes,
+        residual=final,
+        residual_norm=float(np.linalg.norm(final.retained)),
+        packed_residual_norm=packed,
+        reference_residual=float(abs(final.reference)),
+        converged=bool(converged and packed <= controls.tolerance),
+        iterations=len(history),
+        residual_evaluations=state.residual_calls + 1,
+        jvp_evaluations=state.jvp_calls,
+        history=tuple(history),
+        wall_seconds=float(perf_counter() - started),
+        peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
+    )

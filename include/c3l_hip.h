@@ -105,15 +105,24 @@ int c3l_tilde_pack(const double* t, double* x, const int64_t* src, const double*
  * at least ncols*c3l_reduction_chunks(dim) doubles). */
 int c3l_rmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim, double* scratch,
                   double* out, void* stream);
-/* Real: w[i] += sum_k alpha[k] * V[k*ld + i] */
-int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double* w, int64_t dim,
-                   void* stream);
+/* Real: w[i] += scale * sum_k alpha[k] * V[k*ld + i] */
+int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double scale, double* w,
+                   int64_t dim, void* stream);
 /* Complex: out[k] = sum_i conj?(V[k*ld+i]) * w[i] (conj when conjugate != 0). */
 int c3l_zmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim, int32_t conjugate,
                   double* scratch, double* out, void* stream);
-/* Complex: w[i] += sum_k alpha[k] * V[k*ld + i] */
-int c3l_zmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double* w, int64_t dim,
-                   void* stream);
+/* Complex: w[i] += scale * sum_k alpha[k] * V[k*ld + i] */
+int c3l_zmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double scale, double* w,
+                   int64_t dim, void* stream);
+/* Rapidity preconditioner in tilde coordinates: y = pack(unpack(x) * inv_diag), fused
+ * (rapidity_diagonal/_preconditioner, src/c3l/cc/newton_krylov.py:137-192). x != y. */
+int c3l_tilde_precondition(const double* x, double* y, const int64_t* k0, const int64_t* k1, const double* ab,
+                           const int64_t* src, const double* w, const double* inv_diag, int64_t real_dim,
+                           void* stream);
+/* y = a * x (x may equal y) */
+int c3l_rscale(const double* x, double* y, double a, int64_t dim, void* stream);
+/* y = a * x + b * y */
+int c3l_raxpby(double a, const double* x, double b, double* y, int64_t dim, void* stream);
 /* Number of row chunks used by the multidot reductions for a given dimension. */
 int64_t c3l_reduction_chunks(int64_t dim);
 

@@ -1,0 +1,245 @@
+"""Device (HIP) parity against the CPU oracle.
+
+Bar: FP64 results within 1e-12 relative (max-norm, relative to the output
+scale) of the polynomial oracle for residual, JVP and sigma at small sizes and
+of the NumPy plan port at larger sizes; Newton root with packed residual
+<= 1e-10; G^R within 1e-8 relative of the FP64 oracle.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.conftest import random_model
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _static(n, U, seed):
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.static import ComponentStaticProblem
+
+    p = factorized_cc_problem(random_model(n, U, seed))
+    return p, ComponentStaticProblem.compile(p)
+
+
+@pytest.mark.parametrize("n,U,seed", [(1, 1.3, 1), (2, 0.9, 2), (3, 1.2, 3)])
+def test_residual_and_jvp_match_polynomial_oracle(torch_gpu, n, U, seed):
+    from c3lindblad_amd.static import component_static_jvp, component_static_residual
+    from oracle import polynomial as poly
+
+    print(f"case residual/jvp n={n} U={U} seed={seed}", flush=True)
+    p, sp = _static(n, U, seed)
+    rng = np.random.default_rng(11)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    dt = p.tilde.unpack(rng.normal(size=p.num_variables))
+    R, ref = poly.residual(p, t)
+    J = poly.jvp(p, t, dt)
+    got = component_static_residual(sp, t)
+    assert _rel(got.retained, R) < RTOL
+    assert abs(got.reference - ref) < RTOL * max(1.0, abs(ref))
+    assert _rel(component_static_jvp(sp, t, dt).retained, J) < RTOL
+
+
+def test_residual_and_jvp_match_port_n7(torch_gpu):
+    from c3lindblad_amd.static import component_static_jvp, component_static_residual
+    from oracle.port import PortPlan
+
+    print("case residual/jvp port n=7", flush=True)
+    p, sp = _static(7, 2.0, 5)
+    rng = np.random.default_rng(12)
+    t = p.tilde.unpack(0.02 * rng.normal(size=p.num_variables))
+    dt = p.tilde.unpack(rng.normal(size=p.num_variables))
+    port = PortPlan(sp.lowered)
+    port.load(0, t)
+    R = port.apply(0, np.zeros(p.num_amplitudes, complex), sp.diagonal, t)
+    port.load(1, dt)
+    J = port.apply(2, np.zeros(p.num_amplitudes, complex), sp.diagonal, dt)
+    assert _rel(component_static_residual(sp, t).retained, R) < RTOL
+    assert _rel(component_static_jvp(sp, t, dt).retained, J) < RTOL
+
+
+@pytest.mark.parametrize("n,charge", [(2, 1), (2, -1), (3, 1), (3, -1)])
+def test_sigma_matches_polynomial_oracle(torch_gpu, n, charge):
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, component_eom_sigma, eom_ccsd_problem
+    from oracle import polynomial as poly
+
+    print(f"case sigma n={n} charge={charge}", flush=True)
+    p, sp = _static(n, 1.1, 20 + n)
+    rng = np.random.default_rng(13)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, charge)
+    ctx = component_eom_context(ep, t)
+    x = rng.normal(size=ep.dimension) + 1j * rng.normal(size=ep.dimension)
+    want = poly.sigma(p, t, poly.coordinate_masks(ep.coordinates), x)
+    assert _rel(component_eom_sigma(ctx, x).retained, want) < RTOL
+
+
+def test_sigma_matches_port_n8(torch_gpu):
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, component_eom_sigma, eom_ccsd_problem
+    from oracle.port import PortPlan
+
+    print("case sigma port n=8", flush=True)
+    p, sp = _static(8, 3.0, 9)
+    rng = np.random.default_rng(14)
+    t = p.tilde.unpack(0.02 * rng.normal(size=p.num_variables))
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    ctx = component_eom_context(ep, t)
+    x = rng.normal(size=ep.dimension) + 1j * rng.normal(size=ep.dimension)
+    port = PortPlan(ep.lowered)
+    port.load(0, t)
+    port.load(2, x)
+    want = port.apply(3, np.zeros(ep.dimension, complex), ep.diagonal, x)
+    assert _rel(component_eom_sigma(ctx, x).retained, want) < RTOL
+
+
+def test_tilde_round_trip_and_krylov_kernels(torch_gpu):
+    import torch
+
+    from c3lindblad_amd.device import DeviceTilde, Reducer
+
+    print("case tilde/krylov kernels", flush=True)
+    p, _ = _static(4, 1.0, 4)
+    tl = DeviceTilde(p.tilde)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=p.num_variables)
+    xd = torch.from_numpy(x).cuda()
+    td = torch.empty(p.num_amplitudes, dtype=torch.complex128, device="cuda")
+    tl.unpack(xd, td)
+    assert np.max(np.abs(td.cpu().numpy() - p.tilde.unpack(x))) < 1e-15
+    back = torch.empty_like(xd)
+    tl.pack(td, back)
+    assert np.max(np.abs(back.cpu().numpy() - x)) < 1e-14
+    # complex multi-dot / multi-axpy over a ragged dimension (not a chunk multiple)
+    dim, cols = 20011, 37
+    V = rng.normal(size=(cols, dim)) + 1j * rng.normal(size=(cols, dim))
+    w = rng.normal(size=dim) + 1j * rng.normal(size=dim)
+    Vd, wd = torch.from_numpy(V).cuda(), torch.from_numpy(w).cuda()
+    red = Reducer(dim, cols, complex_=True)
+    out = torch.empty(cols, dtype=torch.complex128, device="cuda")
+    red.dot(Vd, cols, wd, out)
+    assert _rel(out.cpu().numpy(), V.conj() @ w) < 1e-13
+    red.dot(Vd, cols, wd, out, conjugate=False)
+    assert _rel(out.cpu().numpy(), V @ w) < 1e-13
+    alpha = rng.normal(size=cols) + 1j * rng.normal(size=cols)
+    red.axpy(Vd, cols, torch.from_numpy(alpha).cuda(), wd, -0.5)
+    assert _rel(wd.cpu().numpy(), w - 0.5 * alpha @ V) < 1e-13
+    rr = Reducer(dim, cols, complex_=False)
+    Vr, wr = torch.from_numpy(V.real.copy()).cuda(), torch.from_numpy(w.real.copy()).cuda()
+    outr = torch.empty(cols, dtype=torch.float64, device="cuda")
+    rr.dot(Vr, cols, wr, outr)
+    assert _rel(outr.cpu().numpy(), V.real @ w.real) < 1e-13
+
+
+def test_shifted_triangular_green_kernel(torch_gpu):
+    import scipy.linalg
+
+    from c3lindblad_amd.response import shifted_green
+
+    print("case shifted triangular", flush=True)
+    rng = np.random.default_rng(6)
+    m = 97
+    H = np.triu(rng.normal(size=(m, m)) + 1j * rng.normal(size=(m, m)), -1)
+    T, Q = scipy.linalg.schur(H, output="complex")
+    r = rng.normal(size=m) + 1j * rng.normal(size=m)
+    w = rng.normal(size=m) + 1j * rng.normal(size=m)
+    q = rng.normal(size=m) + 1j * rng.normal(size=m)
+    z = np.linspace(-3, 3, 301) + 0.5j
+    g, last = shifted_green(T, r, w, q, z)
+    for k in (0, 150, 300):
+        y = np.linalg.solve(z[k] * np.eye(m) - T, r)
+        assert abs(g[k] - w @ y) < 1e-11 * max(1, abs(w @ y))
+        assert abs(last[k] - q @ y) < 1e-11 * max(1, abs(q @ y))
+
+
+def test_newton_on_device_matches_cpu_oracle_root(torch_gpu):
+    from c3lindblad_amd.configs import readme_model
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.solver import solve_component_ccsd
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    print("case newton readme model", flush=True)
+    p = factorized_cc_problem(readme_model())
+    sp = ComponentStaticProblem.compile(p)
+    sol = solve_component_ccsd(sp)
+    assert sol.converged and sol.packed_residual_norm <= 1e-10
+    masks = poly.coordinate_masks(p.coordinates)
+    _, t_cpu, packed, _, _ = solvers.solve_ccsd(
+        p.tilde, p.rapidity_sums(), lambda t: poly.residual(p, t, masks)[0], lambda t: (lambda d: poly.jvp(p, t, d, masks))
+    )
+    assert packed <= 1e-10
+    # both roots satisfy the same equations; they agree to the conditioning of the Jacobian
+    assert _rel(sol.amplitudes, t_cpu) < 1e-8
+    R, _ = poly.residual(p, sol.amplitudes, masks)
+    assert np.linalg.norm(p.tilde.pack(R, check=False)) <= 1e-10
+
+
+def test_readme_green_function_matches_dense_oracle(torch_gpu):
+    from c3lindblad_amd.configs import readme_grid, readme_model
+    from c3lindblad_amd.eom import (
+        ComponentEOMProblem,
+        ComponentSigmaOperator,
+        component_eom_context,
+        component_eom_sources,
+        eom_ccsd_problem,
+    )
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.response import component_retarded_green
+    from c3lindblad_amd.solver import solve_component_ccsd
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    print("case readme G^R", flush=True)
+    p = factorized_cc_problem(readme_model())
+    sp = ComponentStaticProblem.compile(p)
+    sol = solve_component_ccsd(sp)
+    omega = readme_grid()
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    ctx = component_eom_context(ep, sol.amplitudes)
+    sources = component_eom_sources(ctx)
+    result = component_retarded_green(ComponentSigmaOperator(ctx), sources, omega, 0.0)
+    masks = poly.coordinate_masks(ep.coordinates)
+    smasks = poly.coordinate_masks(p.coordinates)
+    A = solvers.dense_matrix(lambda v: poly.sigma(p, sol.amplitudes, masks, v, smasks), ep.dimension)
+    add, rem = solvers.retarded_green_dense(A, sources.addition, sources.removal, sources.sink, omega, 0.0)
+    assert _rel(result.retarded, add + rem) < 1e-8
+    assert np.max(result.addition_relative_residuals) <= 1e-9
+
+
+def test_single_site_green_function_matches_lindblad_ed(torch_gpu):
+    from c3lindblad_amd import (
+        ComponentEOMProblem,
+        ComponentSigmaOperator,
+        ComponentStaticProblem,
+        FermionicLindbladModel,
+        component_eom_context,
+        component_eom_sources,
+        component_retarded_green,
+        eom_ccsd_problem,
+        factorized_cc_problem,
+        solve_component_ccsd,
+    )
+    from oracle import fock
+
+    print("case single site vs ED", flush=True)
+    model = FermionicLindbladModel(E=[[-0.45]], gamma_loss=[[0.35]], gamma_gain=[[0.15]], U=1.3)
+    p = factorized_cc_problem(model)
+    sp = ComponentStaticProblem.compile(p)
+    sol = solve_component_ccsd(sp)
+    assert sol.converged
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    ctx = component_eom_context(ep, sol.amplitudes)
+    omega = np.linspace(-3.0, 3.0, 31)
+    result = component_retarded_green(ComponentSigmaOperator(ctx), component_eom_sources(ctx), omega, 0.0)
+    want = fock.exact_retarded_green(model, omega, 0.0)
+    assert _rel(result.retarded, want) < 1e-8

@@ -1,0 +1,129 @@
+"""Pins of the CPU oracle (no GPU).
+
+* polynomial oracle residual / sigma == sparse Fock-space brute force of
+  ``exp(-T) K exp(T)`` (every sign from literal Jordan--Wigner products);
+* polynomial JVP == central finite difference of the polynomial residual
+  (``tests/cc/test_charge_s13.py:125-135`` uses the same check);
+* compiled plans executed by the NumPy port == polynomial oracle (pins the
+  term compiler, the plan layout and the padded-tensor conventions);
+* the residual is tilde-even (``t_partner = η conj(t)``), so packing loses
+  nothing (``src/c3l/cc/factorized.py:16-19``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.conftest import random_model
+
+
+def _setup(n, U, seed):
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.static import ComponentStaticProblem
+
+    p = factorized_cc_problem(random_model(n, U, seed))
+    return p, ComponentStaticProblem.compile(p)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("n,U,seed", [(1, 1.3, 1), (2, 0.9, 2), (3, 1.2, 3)])
+def test_polynomial_residual_equals_fock_brute_force(n, U, seed):
+    from oracle import polynomial as poly
+    from oracle.fock import PHFockSpace
+
+    p, _ = _setup(n, U, seed)
+    rng = np.random.default_rng(seed)
+    t = p.tilde.unpack(0.1 * rng.normal(size=p.num_variables))
+    R, _ = poly.residual(p, t)
+    assert _rel(PHFockSpace(p).residual(p.coordinates, t), R) < 1e-13
+
+
+@pytest.mark.parametrize("n,charge", [(1, 1), (2, 1), (2, -1)])
+def test_polynomial_sigma_equals_fock_brute_force(n, charge):
+    from c3lindblad_amd.lowering import response_coordinates
+    from oracle import polynomial as poly
+    from oracle.fock import PHFockSpace
+
+    p, _ = _setup(n, 0.8, 30 + n)
+    rng = np.random.default_rng(n)
+    t = p.tilde.unpack(0.1 * rng.normal(size=p.num_variables))
+    rc = response_coordinates(n, charge)
+    x = rng.normal(size=rc.dimension) + 1j * rng.normal(size=rc.dimension)
+    want = PHFockSpace(p).sigma(p.coordinates, t, rc, x)
+    assert _rel(poly.sigma(p, t, poly.coordinate_masks(rc), x), want) < 1e-13
+
+
+def test_polynomial_jvp_equals_finite_difference():
+    from oracle import polynomial as poly
+
+    p, _ = _setup(2, 1.5, 8)
+    rng = np.random.default_rng(3)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    d = p.tilde.unpack(rng.normal(size=p.num_variables))
+    h = 1e-5
+    fd = (poly.residual(p, t + h * d)[0] - poly.residual(p, t - h * d)[0]) / (2 * h)
+    assert np.max(np.abs(poly.jvp(p, t, d) - fd)) < 1e-9
+
+
+@pytest.mark.parametrize("n,U,seed", [(1, 1.3, 1), (2, 0.9, 2), (3, 1.2, 3), (4, 2.5, 4)])
+def test_compiled_plan_equals_polynomial_oracle(n, U, seed):
+    from oracle import polynomial as poly
+    from oracle.port import PortPlan
+
+    p, sp = _setup(n, U, seed)
+    rng = np.random.default_rng(seed + 100)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    dt = p.tilde.unpack(rng.normal(size=p.num_variables))
+    R, ref = poly.residual(p, t)
+    port = PortPlan(sp.lowered)
+    port.load(0, t)
+    got = port.apply(0, np.zeros(p.num_amplitudes, complex), sp.diagonal, t)
+    assert _rel(got, R) < 1e-13
+    got_ref = port.apply(1, np.zeros(1, complex))
+    assert abs(got_ref[0] - ref) < 1e-13 * max(1.0, abs(ref))
+    port.load(1, dt)
+    assert _rel(port.apply(2, np.zeros(p.num_amplitudes, complex), sp.diagonal, dt), poly.jvp(p, t, dt)) < 1e-13
+    assert p.tilde.defect(R) < 1e-13 * max(1.0, np.max(np.abs(R)))
+
+
+@pytest.mark.parametrize("n,charge", [(2, 1), (3, -1), (4, 1)])
+def test_compiled_sigma_and_sources_equal_polynomial_oracle(n, charge):
+    from c3lindblad_amd.eom import ComponentEOMProblem, _sink, _source, eom_ccsd_problem
+    from c3lindblad_amd.lowering import physical_mode_operator
+    from oracle import polynomial as poly
+    from oracle.port import PortPlan
+
+    p, sp = _setup(n, 1.7, 40 + n)
+    rng = np.random.default_rng(n + 7)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, charge)
+    masks = poly.coordinate_masks(ep.coordinates)
+    x = rng.normal(size=ep.dimension) + 1j * rng.normal(size=ep.dimension)
+    port = PortPlan(ep.lowered)
+    port.load(0, t)
+    port.load(2, x)
+    got = port.apply(3, np.zeros(ep.dimension, complex), ep.diagonal, x)
+    assert _rel(got, poly.sigma(p, t, masks, x)) < 1e-13
+    for channel in ("c", "c_dagger", "tilde_c", "tilde_c_dagger"):
+        comps = physical_mode_operator(p.reference, 0, 0, channel).components
+        want = poly.source(p, t, masks, comps)
+        assert np.max(np.abs(_source(ep, t, comps) - want)) < 1e-14 * max(1.0, np.max(np.abs(want)))
+        assert np.array_equal(_sink(ep, comps), poly.sink(masks, comps))
+
+
+def test_empty_and_degenerate_inputs_are_rejected():
+    from oracle import polynomial as poly
+
+    p, sp = _setup(2, 1.0, 2)
+    t = np.zeros(p.num_amplitudes, complex)
+    R, ref = poly.residual(p, t)
+    # the bare residual at T = 0 is the vacuum projection of K; it must be finite
+    assert np.all(np.isfinite(R)) and np.isfinite(ref)
+    with pytest.raises(ValueError):
+        p.tilde.unpack(np.zeros(p.num_variables + 1))
+    with pytest.raises(ValueError):
+        p.tilde.pack(np.full(p.num_amplitudes, np.nan))

@@ -1,0 +1,103 @@
+"""Physics pins of the whole lowering + CC chain on the CPU oracle (no GPU).
+
+* The PH-mode generator ``K = sum λ P̄P + sum products`` built from the
+  lowered problem has, on the charge-neutral sector, exactly ``i`` times the
+  spectrum of the physical Lindbladian on number-diagonal operators
+  (``tests/test_general_interaction.py:259-360`` reconstructs the generator
+  the same way).  This pins full HF, normal modes, mode operators, products
+  and counterterms at once.
+* n = 1: CCSD and S13 span their complete sectors, so ``G^R`` equals dense
+  Lindblad ED (``tests/cc/test_charge_s13.py:93-122``) and the single-mode
+  quadratic result ``1/(ω - E + i(Γl+Γg))`` (``tests/cc/test_charge_small_sectors.py:17-38``).
+* U = 0: the HF reference is exact, the residual at T = 0 vanishes and
+  ``G^R = [z - F + i(Γl+Γg)]^{-1}`` (``tests/test_general_interaction.py:645-724``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from scipy.optimize import linear_sum_assignment
+
+from tests.conftest import random_model
+
+
+@pytest.mark.parametrize("n,U,seed", [(1, 1.3, 1), (2, 0.9, 2), (2, 2.0, 5)])
+def test_lowered_generator_spectrum_equals_lindbladian(n, U, seed):
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from oracle import fock
+
+    model = random_model(n, U, seed)
+    k = fock.ph_neutral_spectrum(factorized_cc_problem(model))
+    l = fock.lindblad_neutral_spectrum(model)
+    assert k.size == l.size
+    cost = np.abs(k[:, None] - 1j * l[None, :])
+    r, c = linear_sum_assignment(cost)
+    assert cost[r, c].max() < 1e-12 * max(1.0, np.abs(k).max())
+
+
+def _oracle_green(model, omega, eta, charge=1):
+    from c3lindblad_amd.eom import ComponentEOMProblem, _sink, _source, eom_ccsd_problem
+    from c3lindblad_amd.lowering import factorized_cc_problem, physical_mode_operator
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    p = factorized_cc_problem(model)
+    masks = poly.coordinate_masks(p.coordinates)
+    _, t, packed, _, _ = solvers.solve_ccsd(
+        p.tilde, p.rapidity_sums(), lambda a: poly.residual(p, a, masks)[0], lambda a: (lambda d: poly.jvp(p, a, d, masks))
+    )
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), ComponentStaticProblem.compile(p), charge)
+    rmasks = poly.coordinate_masks(ep.coordinates)
+    A = solvers.dense_matrix(lambda v: poly.sigma(p, t, rmasks, v, masks), ep.dimension)
+    ref = p.reference
+    add = _source(ep, t, physical_mode_operator(ref, 0, 0, "c_dagger").components)
+    rem = -1j * _source(ep, t, physical_mode_operator(ref, 0, 0, "tilde_c").components)
+    sink = _sink(ep, physical_mode_operator(ref, 0, 0, "c").components)
+    a, r = solvers.retarded_green_dense(A, add, rem, sink, omega, eta)
+    return packed, a + r, p
+
+
+def test_single_site_interacting_green_function_equals_lindblad_ed():
+    from c3lindblad_amd.model import FermionicLindbladModel
+    from oracle import fock
+
+    model = FermionicLindbladModel(E=[[-0.45]], gamma_loss=[[0.35]], gamma_gain=[[0.15]], U=1.3)
+    omega = np.linspace(-3.0, 3.0, 25)
+    packed, G, _ = _oracle_green(model, omega, 0.0)
+    assert packed <= 1e-10
+    want = fock.exact_retarded_green(model, omega, 0.0)
+    assert np.max(np.abs(G - want)) < 1e-12 * np.max(np.abs(want))
+
+
+def test_single_mode_quadratic_closed_form():
+    from c3lindblad_amd.model import FermionicLindbladModel
+
+    model = FermionicLindbladModel(E=[[0.3]], gamma_loss=[[0.8]], gamma_gain=[[0.2]], U=0.0)
+    omega = np.array([-0.4, 0.1, 0.7])
+    _, G, _ = _oracle_green(model, omega, 0.1)
+    np.testing.assert_allclose(G, 1 / (omega - 0.3 + 1.1j), atol=1e-12)
+
+
+def test_noninteracting_three_site_green_function_equals_quadratic_resolvent():
+    from c3lindblad_amd.meanfield import quadratic_retarded_green
+
+    model = random_model(3, 0.0, 17)
+    omega = np.linspace(-2.0, 2.0, 9)
+    packed, G, p = _oracle_green(model, omega, 0.0)
+    assert packed <= 1e-12
+    want = quadratic_retarded_green(model.E, model.gamma_loss, model.gamma_gain, omega)[:, 0, 0]
+    assert np.max(np.abs(G - want)) < 1e-11 * np.max(np.abs(want))
+
+
+def test_single_site_occupation_and_exact_covariance_agree_with_hf():
+    """At n = 1 the steady state is Gaussian: HF gamma equals the ED covariance."""
+
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.model import FermionicLindbladModel
+    from oracle import fock
+
+    model = FermionicLindbladModel(E=[[-0.45]], gamma_loss=[[0.35]], gamma_gain=[[0.15]], U=1.3)
+    p = factorized_cc_problem(model)
+    assert np.max(np.abs(p.reference.gamma - fock.exact_covariance(model))) < 1e-12
