@@ -1,0 +1,207 @@
+"""Benchmark: CCSD + EOM G^R(ω) solves/sec on one MI355X (FP64 GPU path).
+
+Workload (default ``siam24``): single-impurity Anderson model with 24 Lindblad
+pseudomodes (n = 25 sites, U = 4, eps_d = -2, seeded), CCSD steady state to
+``||pack R|| <= 1e-10``, then charge +1 and -1 EOM-CCSD (S1+S3) retarded Green
+functions on 4001 frequencies in [-8, 8] with eta = 0.  One step is one full
+solve from the zero initial guess; host set-up (full HF, normal modes, plan
+compilation and upload) happens once before timing and is reported as
+``config.setup_seconds``.  ``--config readme`` runs the 3-site README model.
+
+Prints one JSON line (rank 0).  ``roofline`` is for the kernel with the
+largest total time among the instrumented ones, timed with HIP events on its
+launch stream during the timed steps; ``cpu_baseline`` times the NumPy CPU
+port of the same plans on a bounded sample (see ``sample``).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _algorithmic_block_bytes(lowered, block_index: int) -> float:
+    """Compulsory bytes of one wedge-block launch: outputs, coordinates, diagonal terms, atoms read."""
+
+    program, deg, coord, dest, coeff, fstart, foff, fcode = lowered.blocks[block_index]
+    n = lowered.n
+    count = dest.size
+    atoms = {}
+    for off, code in zip(foff, fcode):
+        atoms[int(off)] = n ** (int(code) & 3)
+    diag = 32 if program in (0, 2, 3) else 0  # diagonal and input value per coordinate
+    return float(count * (16 + 8 + 4 * deg + diag) + 16 * sum(atoms.values()))
+
+
+def _cpu_baseline(pipe, result, seconds_budget: float = 30.0) -> dict:
+    """Time the NumPy port (single thread) on one residual, one JVP and one sigma per charge."""
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle.port import PortPlan
+
+    sp = pipe.static
+    p = sp.oracle
+    t = result.amplitudes
+    rng = np.random.default_rng(0)
+    dt = p.tilde.unpack(rng.normal(size=p.num_variables))
+    port = PortPlan(sp.lowered)
+    t0 = time.perf_counter()
+    port.load(0, t)
+    port.apply(0, np.zeros(p.num_amplitudes, complex), sp.diagonal, t)
+    t1 = time.perf_counter()
+    port.load(1, dt)
+    port.apply(2, np.zeros(p.num_amplitudes, complex), sp.diagonal, dt)
+    t2 = time.perf_counter()
+    sigma = {}
+    for q, block in pipe.eom.items():
+        pp = PortPlan(block.lowered)
+        pp.load(0, t)
+        x = rng.normal(size=block.dimension) + 1j * rng.normal(size=block.dimension)
+        s0 = time.perf_counter()
+        pp.load(2, x)
+        pp.apply(3, np.zeros(block.dimension, complex), block.diagonal, x)
+        sigma[q] = time.perf_counter() - s0
+    t_res, t_jvp = t1 - t0, t2 - t1
+    n_sigma = sum(sum(d) for d in result.krylov_dimensions.values())
+    per_sigma = float(np.mean(list(sigma.values())))
+    solve_seconds = result.residual_evaluations * t_res + result.jvp_evaluations * t_jvp + n_sigma * per_sigma
+    return {
+        "value": 1.0 / solve_seconds,
+        "unit": "solves/sec",
+        "cores": 1,
+        "kind": "port",
+        "sample": (
+            f"NumPy port of the same plans on this host: 1 residual ({t_res:.2f}s), 1 JVP ({t_jvp:.2f}s), "
+            f"1 sigma per charge ({per_sigma:.3f}s) at n={p.model.n_sites}; scaled by one GPU solve's counts "
+            f"({result.residual_evaluations} residuals, {result.jvp_evaluations} JVPs, {n_sigma} sigmas); "
+            "Krylov orthogonalisation and shifted solves excluded (optimistic for the CPU)"
+        ),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="siam24", choices=("siam24", "readme"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(
+            "bench.py runs the single-GPU path only: this build has no multi-GPU mode "
+            "(the solve is one instance on one MI355X); run with --gpus 1",
+            file=sys.stderr,
+        )
+        return 2
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import torch
+
+    from c3lindblad_amd.configs import config2_grid, config2_model, readme_grid, readme_model
+    from c3lindblad_amd.device import KernelTimer
+    from c3lindblad_amd.pipeline import SolvePipeline
+
+    if args.config == "siam24":
+        model, grid, charges = config2_model(), config2_grid(), (1, -1)
+        workload = "siam_24_pseudomodes_U4_ccsd_eom_s13_gr_4001w"
+    else:
+        model, grid, charges = readme_model(), readme_grid(), (1,)
+        workload = "readme_3site_U1.2_ccsd_eom_s13_gr_81w"
+    pipe = SolvePipeline.prepare(model, grid, 0.0, charges=charges)
+    print(f"[bench] setup {pipe.setup_seconds:.2f}s n={model.n_sites} D={pipe.static.oracle.num_amplitudes}", file=sys.stderr, flush=True)
+    for k in range(args.warmup):
+        r = pipe.solve()
+        print(f"[bench] warmup {k} {r.seconds}", file=sys.stderr, flush=True)
+    plans = [pipe.static.device.plan] + [b.device_plan() for b in pipe.eom.values()]
+    for plan in plans:
+        plan.set_timing(True)
+    KernelTimer.enabled = True
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    results = []
+    for k in range(args.steps):
+        results.append(pipe.solve())
+        print(f"[bench] step {k} {results[-1].seconds}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    KernelTimer.enabled = False
+    kernels = {name: d for name, d in KernelTimer.summary().items()}
+    for plan in plans:
+        for i, st in enumerate(plan.block_stats()):
+            if st["launches"] == 0:
+                continue
+            name = f"wedge_block[{st['program']},deg{st['deg']},{st['count']}]"
+            d = kernels.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0.0})
+            d["ms"] += st["ms"]
+            d["launches"] += st["launches"]
+            d["bytes"] += st["launches"] * _algorithmic_block_bytes(plan.lowered, i)
+        plan.set_timing(False)
+    dominant = max(kernels, key=lambda k: kernels[k]["ms"])
+    dk = kernels[dominant]
+    achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
+    last = results[-1]
+    spread = max(
+        float(np.max(np.abs(r.green[q] - last.green[q])) / np.max(np.abs(last.green[q]))) for r in results for q in last.green
+    )
+    line = {
+        "metric": "CCSD+EOM G^R(ω) solves/sec, 1 MI355X: mixed+batched+learned vs FP64 @1e-8",
+        "value": args.steps / elapsed,
+        "unit": "solves/sec",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": workload,
+            "variant": "fp64_gpu",
+            "n_sites": model.n_sites,
+            "ccsd_amplitudes": pipe.static.oracle.num_amplitudes,
+            "eom_dimension_per_charge": {str(q): b.dimension for q, b in pipe.eom.items()},
+            "omega_points": int(grid.size),
+            "eta": 0.0,
+            "charges": list(charges),
+            "setup_seconds": pipe.setup_seconds,
+            "newton_iterations": last.newton_iterations,
+            "jvp_evaluations": last.jvp_evaluations,
+            "residual_evaluations": last.residual_evaluations,
+            "krylov_dimensions": {str(q): list(v) for q, v in last.krylov_dimensions.items()},
+            "packed_residual_norm": last.packed_residual_norm,
+            "max_shift_relative_residual": last.max_shift_residual,
+            "gr_max_rel_spread_between_steps": spread,
+            "seconds_per_phase": last.seconds,
+        },
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_us": 1e3 * dk["ms"] / dk["launches"],
+            "launches": dk["launches"],
+            "share_of_step": dk["ms"] * 1e-3 / elapsed,
+        },
+        "kernels": {k: {"ms": round(v["ms"], 3), "launches": v["launches"]} for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["ms"])[:8]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = _cpu_baseline(pipe, last)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -34,6 +34,9 @@ SIGNATURES = {
     "c3l_plan_add_block": ([_P, _I32, _I32, _I64, _P, _P, _I32, _P, _P, _I32, _P, _P], ctypes.c_int),
     "c3l_plan_load": ([_P, _I32, _P, _P], ctypes.c_int),
     "c3l_plan_apply": ([_P, _I32, _P, _P, _P, _P], ctypes.c_int),
+    "c3l_plan_set_timing": ([_P, _I32], ctypes.c_int),
+    "c3l_plan_block_count": ([_P], _I32),
+    "c3l_plan_block_info": ([_P, _I32, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "c3l_tilde_unpack": ([_P, _P, _P, _P, _P, _I64, _P], ctypes.c_int),
     "c3l_tilde_pack": ([_P, _P, _P, _P, _I64, _P], ctypes.c_int),
     "c3l_rmultidot": ([_P, _I64, _I32, _P, _I64, _P, _P, _P], ctypes.c_int),
@@ -48,6 +51,15 @@ SIGNATURES = {
 }
 
 _lib = None
+
+# Placement of every device tensor of the package.
+DEVICE = "cuda"
+
+
+def pinned(tensor):
+    """Page-locked host staging buffer for device-to-host scalar transfers."""
+
+    return tensor.pin_memory() if DEVICE == "cuda" else tensor
 
 
 def load_library() -> ctypes.CDLL:

@@ -384,6 +384,12 @@ struct PlanBlock {
   int32_t* fcode = nullptr;
 };
 
+struct PendingTiming {
+  size_t block;
+  hipEvent_t start;
+  hipEvent_t stop;
+};
+
 struct c3l_plan {
   int32_t n = 0;
   int32_t n_vectors = 0;
@@ -394,6 +400,11 @@ struct c3l_plan {
   std::vector<PlanLevel> levels;
   std::vector<PlanBlock> blocks;
   std::vector<void*> owned;
+  // optional per-block kernel timing with HIP events
+  bool timing = false;
+  std::vector<PendingTiming> pending;
+  std::vector<double> block_ms;
+  std::vector<int64_t> block_launches;
 };
 
 namespace {
@@ -467,10 +478,56 @@ int c3l_plan_create(int32_t n, int32_t n_vectors, const double* vectors, int64_t
 
 int c3l_plan_destroy(c3l_plan* plan) {
   if (plan == nullptr) return C3L_OK;
+  for (PendingTiming& p : plan->pending) {
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
   for (void* p : plan->owned) (void)hipFree(p);
   delete plan;
   return C3L_OK;
 }
+
+int c3l_plan_set_timing(c3l_plan* plan, int32_t enable) {
+  if (plan == nullptr) return fail(C3L_EINVAL, "c3l_plan_set_timing: invalid argument");
+  plan->timing = enable != 0;
+  plan->block_ms.assign(plan->blocks.size(), 0.0);
+  plan->block_launches.assign(plan->blocks.size(), 0);
+  return C3L_OK;
+}
+
+int c3l_plan_block_info(c3l_plan* plan, int32_t index, int32_t* program, int32_t* deg, int64_t* count,
+                        int32_t* n_entries, int32_t* n_factors, double* ms_total, int64_t* launches) {
+  if (plan == nullptr || index < 0 || index >= (int32_t)plan->blocks.size())
+    return fail(C3L_EINVAL, "c3l_plan_block_info: invalid block index");
+  // resolve every recorded timing (waits for the recorded work)
+  for (PendingTiming& p : plan->pending) {
+    C3L_HIP_CHECK(hipEventSynchronize(p.stop));
+    float ms = 0.f;
+    C3L_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+    if (p.block < plan->block_ms.size()) {
+      plan->block_ms[p.block] += ms;
+      plan->block_launches[p.block] += 1;
+    }
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  plan->pending.clear();
+  const PlanBlock& b = plan->blocks[(size_t)index];
+  if (program) *program = b.program;
+  if (deg) *deg = b.deg;
+  if (count) *count = b.count;
+  if (n_entries) *n_entries = b.n_entries;
+  if (n_factors) {
+    int32_t nf = 0;
+    if (b.n_entries > 0) C3L_HIP_CHECK(hipMemcpy(&nf, b.fstart + b.n_entries, sizeof(int32_t), hipMemcpyDeviceToHost));
+    *n_factors = nf;
+  }
+  if (ms_total) *ms_total = index < (int32_t)plan->block_ms.size() ? plan->block_ms[(size_t)index] : 0.0;
+  if (launches) *launches = index < (int32_t)plan->block_launches.size() ? plan->block_launches[(size_t)index] : 0;
+  return C3L_OK;
+}
+
+int32_t c3l_plan_block_count(c3l_plan* plan) { return plan == nullptr ? 0 : (int32_t)plan->blocks.size(); }
 
 int c3l_plan_add_base(c3l_plan* plan, int32_t kind, int64_t offset, int32_t deg, const int64_t* src,
                       const int8_t* sign) {
@@ -568,13 +625,24 @@ int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const do
   if (plan == nullptr || out == nullptr || ((diag == nullptr) != (x == nullptr)))
     return fail(C3L_EINVAL, "c3l_plan_apply: invalid argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  for (const PlanBlock& b : plan->blocks) {
+  for (size_t i = 0; i < plan->blocks.size(); ++i) {
+    const PlanBlock& b = plan->blocks[i];
     if (b.program != program || b.count == 0) continue;
+    PendingTiming t{i, nullptr, nullptr};
+    if (plan->timing) {
+      C3L_HIP_CHECK(hipEventCreate(&t.start));
+      C3L_HIP_CHECK(hipEventCreate(&t.stop));
+      C3L_HIP_CHECK(hipEventRecord(t.start, s));
+    }
     hipLaunchKernelGGL(wedge_block_kernel, dim3(grid_for(b.count)), dim3(kThreads), 0, s, plan->ws,
                        reinterpret_cast<double2*>(out), b.coord, b.dest, b.count, b.deg, plan->n, b.n_entries, b.coeff,
                        b.fstart, b.foffset, b.fcode, reinterpret_cast<const double2*>(diag),
                        reinterpret_cast<const double2*>(x));
     C3L_HIP_CHECK(hipGetLastError());
+    if (plan->timing) {
+      C3L_HIP_CHECK(hipEventRecord(t.stop, s));
+      plan->pending.push_back(t);
+    }
   }
   return C3L_OK;
 }

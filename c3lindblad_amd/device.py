@@ -101,6 +101,27 @@ class DevicePlan:
             _lib.stream_ptr(self.torch),
         )
 
+    def set_timing(self, enable: bool) -> None:
+        _lib.call("c3l_plan_set_timing", self.handle, 1 if enable else 0)
+
+    def block_stats(self) -> list[dict]:
+        """Per output block: program, degree, coordinates, entries, factors, kernel ms, launches."""
+
+        lib = _lib.load_library()
+        out = []
+        for i in range(lib.c3l_plan_block_count(self.handle)):
+            program, deg, n_entries, n_factors = (ctypes.c_int32() for _ in range(4))
+            count, launches = ctypes.c_int64(), ctypes.c_int64()
+            ms = ctypes.c_double()
+            _lib.call(
+                "c3l_plan_block_info", self.handle, i, ctypes.byref(program), ctypes.byref(deg), ctypes.byref(count),
+                ctypes.byref(n_entries), ctypes.byref(n_factors), ctypes.byref(ms), ctypes.byref(launches),
+            )
+            names = {v: k for k, v in PROGRAMS.items()}
+            out.append(dict(program=names[program.value], deg=deg.value, count=count.value, entries=n_entries.value,
+                            factors=n_factors.value, ms=ms.value, launches=launches.value))
+        return out
+
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:
             _lib.load_library().c3l_plan_destroy(self.handle)
@@ -119,7 +140,7 @@ class DeviceTilde:
     def __init__(self, tilde):
         torch = _lib.require_device()
         self.torch = torch
-        dev = torch.device("cuda")
+        dev = torch.device(_lib.DEVICE)
         self.dim = tilde.dimension
         self.real_dim = tilde.real_size
         ab = np.stack([tilde.alpha.real, tilde.alpha.imag, tilde.beta.real, tilde.beta.imag], axis=1)
@@ -170,6 +191,47 @@ class DeviceTilde:
         )
 
 
+class KernelTimer:
+    """Optional HIP-event timing of Krylov kernels on the launching (current) stream."""
+
+    enabled = False
+    _pending: list = []
+
+    @classmethod
+    def start(cls):
+        if not cls.enabled:
+            return None
+        import torch
+
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return ev
+
+    @classmethod
+    def stop(cls, name: str, start, nbytes: float) -> None:
+        if start is None:
+            return
+        import torch
+
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        cls._pending.append((name, start, ev, nbytes))
+
+    @classmethod
+    def summary(cls) -> dict:
+        """``{name: {"ms": total, "launches": k, "bytes": total algorithmic bytes}}``."""
+
+        out: dict = {}
+        for name, a, b, nbytes in cls._pending:
+            b.synchronize()
+            d = out.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0.0})
+            d["ms"] += a.elapsed_time(b)
+            d["launches"] += 1
+            d["bytes"] += nbytes
+        cls._pending = []
+        return out
+
+
 class Reducer:
     """Scratch-backed multi-dot and multi-axpy over contiguous Krylov bases."""
 
@@ -180,11 +242,12 @@ class Reducer:
         self.complex = complex_
         chunks = int(_lib.load_library().c3l_reduction_chunks(self.dim))
         dtype = torch.complex128 if complex_ else torch.float64
-        self.scratch = torch.empty(chunks * max_cols, dtype=dtype, device="cuda")
-        self.coef = torch.empty(max_cols, dtype=dtype, device="cuda")
+        self.scratch = torch.empty(chunks * max_cols, dtype=dtype, device=_lib.DEVICE)
+        self.coef = torch.empty(max_cols, dtype=dtype, device=_lib.DEVICE)
 
     def dot(self, V, ncols: int, w, out, conjugate: bool = True) -> None:
         ld = V.stride(0)
+        ev = KernelTimer.start()
         if self.complex:
             _lib.call(
                 "c3l_zmultidot", _lib.dptr(V), ld, ncols, _lib.dptr(w), self.dim, 1 if conjugate else 0,
@@ -195,14 +258,19 @@ class Reducer:
                 "c3l_rmultidot", _lib.dptr(V), ld, ncols, _lib.dptr(w), self.dim,
                 _lib.dptr(self.scratch), _lib.dptr(out), _lib.stream_ptr(self.torch),
             )
+        elem = 16 if self.complex else 8
+        KernelTimer.stop("zmultidot" if self.complex else "rmultidot", ev, float((ncols + 1) * self.dim * elem))
 
     def axpy(self, V, ncols: int, alpha, w, scale: float = 1.0) -> None:
         """``w += scale * sum_k alpha[k] V[k]``."""
 
         ld = V.stride(0)
         name = "c3l_zmultiaxpy" if self.complex else "c3l_rmultiaxpy"
+        ev = KernelTimer.start()
         _lib.call(name, _lib.dptr(V), ld, ncols, _lib.dptr(alpha), float(scale), _lib.dptr(w), self.dim,
                   _lib.stream_ptr(self.torch))
+        elem = 16 if self.complex else 8
+        KernelTimer.stop(name[4:], ev, float((ncols + 2) * self.dim * elem))
 
 
 def rscale(x, y, a: float, torch) -> None:

@@ -118,9 +118,9 @@ class EOMDevice:
         torch = _lib.require_device()
         self.torch = torch
         self.plan = context.problem.device_plan()
-        self.t = torch.from_numpy(np.ascontiguousarray(context.amplitudes)).to("cuda")
+        self.t = torch.from_numpy(np.ascontiguousarray(context.amplitudes)).to(_lib.DEVICE)
         self.token = self.plan.load("T", self.t)
-        self.diag = torch.from_numpy(np.ascontiguousarray(context.problem.diagonal)).to("cuda")
+        self.diag = torch.from_numpy(np.ascontiguousarray(context.problem.diagonal)).to(_lib.DEVICE)
         self.dim = context.problem.dimension
 
     def sigma(self, x, out) -> None:
@@ -144,7 +144,7 @@ def component_eom_sigma(context: ComponentEOMContext, vector) -> ComponentEOMAct
     if x.shape != (context.problem.dimension,) or not np.all(np.isfinite(x)):
         raise ValueError("EOM vector has the wrong shape or is nonfinite")
     dev = context.device
-    xd = dev.torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+    xd = dev.torch.from_numpy(np.ascontiguousarray(x)).to(_lib.DEVICE)
     out = dev.torch.empty_like(xd)
     dev.sigma(xd, out)
     return ComponentEOMAction(out.cpu().numpy(), perf_counter() - started)

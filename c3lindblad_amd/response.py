@@ -62,8 +62,8 @@ class DeviceArnoldi:
         r = Q.conj().T[:, 0] * self.source_norm
         P = self.basis.shape[1]
         red = Reducer(P, m, complex_=True)
-        sink_d = torch.from_numpy(np.ascontiguousarray(sink, dtype=np.complex128)).to("cuda")
-        sv = torch.empty(m, dtype=torch.complex128, device="cuda")
+        sink_d = torch.from_numpy(np.ascontiguousarray(sink, dtype=np.complex128)).to(_lib.DEVICE)
+        sv = torch.empty(m, dtype=torch.complex128, device=_lib.DEVICE)
         red.dot(self.basis, m, sink_d, sv, conjugate=False)
         w = sv.cpu().numpy() @ Q
         q = Q[m - 1, :]
@@ -78,16 +78,21 @@ def shifted_green(T, r, w, q, z):
 
     torch = _lib.require_device()
     m = T.shape[0]
-    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).to("cuda")
-    Tc = dev(np.asfortranarray(T).ravel(order="F"))
-    green = torch.empty(z.size, dtype=torch.complex128, device="cuda")
-    last = torch.empty(z.size, dtype=torch.complex128, device="cuda")
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).to(_lib.DEVICE)
+
+    # every operand is held by a name until the results are copied back
+    Tc, rd, wd, qd, zd = dev(np.asfortranarray(T).ravel(order="F")), dev(r), dev(w), dev(q), dev(z)
+    green = torch.empty(z.size, dtype=torch.complex128, device=_lib.DEVICE)
+    last = torch.empty(z.size, dtype=torch.complex128, device=_lib.DEVICE)
     _lib.call(
         "c3l_shifted_triangular_green",
-        _lib.dptr(Tc), m, _lib.dptr(dev(r)), _lib.dptr(dev(w)), _lib.dptr(dev(q)), _lib.dptr(dev(z)), z.size,
+        _lib.dptr(Tc), m, _lib.dptr(rd), _lib.dptr(wd), _lib.dptr(qd), _lib.dptr(zd), z.size,
         _lib.dptr(green), _lib.dptr(last), _lib.stream_ptr(torch),
     )
-    return green.cpu().numpy(), last.cpu().numpy()
+    out = green.cpu().numpy(), last.cpu().numpy()
+    del Tc, rd, wd, qd, zd
+    return out
 
 
 def _estimated_residuals(H: np.ndarray, shifts: np.ndarray) -> np.ndarray:
@@ -134,13 +139,13 @@ def arnoldi_resolvent(
         raise ValueError("rhs must be nonzero")
     started = perf_counter()
     m_max = maximum_dimension
-    V = torch.zeros(m_max + 1, n, dtype=torch.complex128, device="cuda")
+    V = torch.zeros(m_max + 1, n, dtype=torch.complex128, device=_lib.DEVICE)
     V[0].copy_(torch.from_numpy(b / beta))
     H = np.zeros((m_max + 1, m_max), dtype=np.complex128)
     red = Reducer(n, m_max + 1, complex_=True)
-    coef = torch.zeros(2 * (m_max + 1) + 1, dtype=torch.complex128, device="cuda")
-    host = torch.zeros(2 * (m_max + 1) + 1, dtype=torch.complex128).pin_memory()
-    w = torch.empty(n, dtype=torch.complex128, device="cuda")
+    coef = torch.zeros(2 * (m_max + 1) + 1, dtype=torch.complex128, device=_lib.DEVICE)
+    host = _lib.pinned(torch.zeros(2 * (m_max + 1) + 1, dtype=torch.complex128))
+    w = torch.empty(n, dtype=torch.complex128, device=_lib.DEVICE)
     checks, values = [], []
     actual = m_max
     breakdown = False

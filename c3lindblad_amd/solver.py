@@ -115,22 +115,22 @@ class _DeviceNewton:
         D = problem.oracle.num_variables
         self.D = D
         f64 = torch.float64
-        self.V = torch.zeros(restart + 1, D, dtype=f64, device="cuda")
-        self.w = torch.empty(D, dtype=f64, device="cuda")
-        self.tmp = torch.empty(D, dtype=f64, device="cuda")
-        self.r = torch.empty(D, dtype=f64, device="cuda")
-        self.trial = torch.empty(D, dtype=f64, device="cuda")
-        self.step = torch.empty(D, dtype=f64, device="cuda")
-        self.xtrial = torch.empty(D, dtype=f64, device="cuda")
-        self.coef = torch.zeros(2 * (restart + 2) + 2, dtype=f64, device="cuda")
-        self.host = torch.zeros(2 * (restart + 2) + 2, dtype=f64).pin_memory()
+        self.V = torch.zeros(restart + 1, D, dtype=f64, device=_lib.DEVICE)
+        self.w = torch.empty(D, dtype=f64, device=_lib.DEVICE)
+        self.tmp = torch.empty(D, dtype=f64, device=_lib.DEVICE)
+        self.r = torch.empty(D, dtype=f64, device=_lib.DEVICE)
+        self.trial = torch.empty(D, dtype=f64, device=_lib.DEVICE)
+        self.step = torch.empty(D, dtype=f64, device=_lib.DEVICE)
+        self.xtrial = torch.empty(D, dtype=f64, device=_lib.DEVICE)
+        self.coef = torch.zeros(2 * (restart + 2) + 2, dtype=f64, device=_lib.DEVICE)
+        self.host = _lib.pinned(torch.zeros(2 * (restart + 2) + 2, dtype=f64))
         self.reducer = Reducer(D, restart + 2, complex_=False)
         diag = -1j * problem.oracle.rapidity_sums()
         mags = np.abs(diag)
         threshold = 1.0e-12 * max(float(np.max(mags, initial=0.0)), 1.0)
         small = mags < threshold
         diag[small] = threshold * np.exp(1j * np.angle(diag[small] + 1e-300))
-        self.inv_diag = torch.from_numpy(1.0 / diag).to("cuda")
+        self.inv_diag = torch.from_numpy(1.0 / diag).to(_lib.DEVICE)
         self.jvp_calls = 0
         self.residual_calls = 0
 
@@ -256,7 +256,7 @@ def solve_component_ccsd(
         if initial.shape != (oracle.num_amplitudes,) or not np.all(np.isfinite(initial)):
             raise ValueError("initial amplitudes do not match the component CCSD problem")
         x0 = oracle.tilde.pack(initial, check=False)
-    x = torch.from_numpy(np.ascontiguousarray(x0)).to("cuda")
+    x = torch.from_numpy(np.ascontiguousarray(x0)).to(_lib.DEVICE)
     history = []
     converged = False
     previous_norm = None

@@ -95,7 +95,7 @@ class StaticDevice:
         dim = problem.oracle.num_amplitudes
         self.dim = dim
         self.real_dim = problem.oracle.num_variables
-        dev = torch.device("cuda")
+        dev = torch.device(_lib.DEVICE)
         self.diag = torch.from_numpy(np.ascontiguousarray(problem.diagonal)).to(dev)
         self.t = torch.zeros(dim, dtype=torch.complex128, device=dev)
         self.dt = torch.zeros(dim, dtype=torch.complex128, device=dev)
@@ -140,7 +140,7 @@ class StaticDevice:
 
 
 def _to_device(array, torch):
-    return torch.from_numpy(np.ascontiguousarray(array, dtype=np.complex128)).to("cuda")
+    return torch.from_numpy(np.ascontiguousarray(array, dtype=np.complex128)).to(_lib.DEVICE)
 
 
 def _validate(problem: ComponentStaticProblem, values, name: str) -> np.ndarray:

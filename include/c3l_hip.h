@@ -92,6 +92,15 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream);
 int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out,
                    void* stream);
 
+/* Measurement hooks (no counterpart in the reference, which times with perf_counter,
+ * cc/component_jax.py:53-65): when enabled, every block launch of c3l_plan_apply is
+ * bracketed by HIP events on its stream; c3l_plan_block_info waits for them and
+ * returns per-block metadata with the accumulated kernel time. */
+int c3l_plan_set_timing(c3l_plan* plan, int32_t enable);
+int32_t c3l_plan_block_count(c3l_plan* plan);
+int c3l_plan_block_info(c3l_plan* plan, int32_t index, int32_t* program, int32_t* deg, int64_t* count,
+                        int32_t* n_entries, int32_t* n_factors, double* ms_total, int64_t* launches);
+
 /* ---- tilde real coordinates -------------------------------------------------------- */
 /* t[i] = ab[4i..4i+1] * x[k0[i]] + ab[4i+2..4i+3] * x[k1[i]]  (complex * real) */
 int c3l_tilde_unpack(const double* x, double* t, const int64_t* k0, const int64_t* k1, const double* ab,
