@@ -48,7 +48,31 @@ SIGNATURES = {
     "c3l_raxpby": ([ctypes.c_double, _P, ctypes.c_double, _P, _I64, _P], ctypes.c_int),
     "c3l_reduction_chunks": ([_I64], _I64),
     "c3l_shifted_triangular_green": ([_P, _I32, _P, _P, _P, _P, _I32, _P, _P, _P], ctypes.c_int),
+    "c3l_gmres": (
+        [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I32, _I32, _P, _P, _P, _P, _P],
+        ctypes.c_int,
+    ),
+    "c3l_arnoldi": (
+        [_P, _P, _P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, _I32, ctypes.c_double,
+         _P, _P, _P, _P],
+        ctypes.c_int,
+    ),
 }
+
+
+class TildeMaps(ctypes.Structure):
+    """``c3l_tilde_maps`` of ``include/c3l_hip.h``."""
+
+    _fields_ = [
+        ("dim", ctypes.c_int64),
+        ("real_dim", ctypes.c_int64),
+        ("k0", ctypes.c_void_p),
+        ("k1", ctypes.c_void_p),
+        ("ab", ctypes.c_void_p),
+        ("src", ctypes.c_void_p),
+        ("w", ctypes.c_void_p),
+        ("inv_diag", ctypes.c_void_p),
+    ]
 
 _lib = None
 

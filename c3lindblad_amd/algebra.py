@@ -44,7 +44,9 @@ import numpy as np
 from .lowering import STATIC_KEYS, BlockKey, FactorizedModeProduct, LinearModeOperator
 
 N_GROUPS = 4
-BASE_ORDER = {"U": 0, "T": 1, "D": 2, "X": 3}
+# U: constant creator vector, T: amplitude block, C: creation tensor (a merged
+# sum of wedges of U/T atoms), D: tangent block, X: EOM input block.
+BASE_ORDER = {"U": 0, "T": 1, "C": 2, "D": 3, "X": 4}
 
 
 def key_slots(key: BlockKey) -> tuple[int, ...]:
@@ -259,6 +261,41 @@ def tangent_terms(terms: dict) -> dict:
     return {k: v for k, v in out.items() if v != 0.0}
 
 
+def factor_linear_terms(terms: dict, linear_base: str) -> tuple[dict, list[tuple[BlockKey, dict]]]:
+    """Rewrite terms linear in one ``linear_base`` atom as ``C_g ∧ L_g``.
+
+    Every term contains exactly one atom ``L`` of ``linear_base``; moving it
+    to the right end costs ``(-1)^(rank L * rank of the atoms after it)``.
+    Terms are grouped by ``(L, block key of the rest)``; the rest of each
+    group is summed into one creation tensor ``C_g`` (a new ``C`` atom),
+    which depends only on the fixed atoms and is materialised once per
+    linearisation point.  Returns the two-factor terms and, per creation
+    atom, its key and the terms that define it.
+    """
+
+    groups: dict = defaultdict(lambda: defaultdict(complex))
+    for atoms, coefficient in terms.items():
+        where = [i for i, a in enumerate(atoms) if a.base == linear_base]
+        if len(where) != 1:
+            raise ValueError("a linear term must contain exactly one input atom")
+        j = where[0]
+        lin = atoms[j]
+        rest = atoms[:j] + atoms[j + 1 :]
+        after = sum(a.rank for a in atoms[j + 1 :])
+        sign = -1 if (lin.rank * after) % 2 else 1
+        groups[(lin, term_key(rest))][rest] += sign * coefficient
+    factored: dict = {}
+    creations: list[tuple[BlockKey, dict]] = []
+    for (lin, key), rest_terms in groups.items():
+        rest_terms = {k: v for k, v in rest_terms.items() if v != 0.0}
+        if not rest_terms:
+            continue
+        c = Atom("C", len(creations), key_slots(key))
+        creations.append((key, rest_terms))
+        factored[(c, lin)] = 1.0 + 0.0j
+    return factored, creations
+
+
 def term_key(atoms: Term) -> BlockKey:
     counts = [0] * N_GROUPS
     for a in atoms:
@@ -329,13 +366,20 @@ def _shuffles(atoms: Term, key: BlockKey) -> list[tuple[int, list[list[int]]]]:
 
 @dataclass
 class BlockProgram:
-    """Entries evaluating one output block at its canonical coordinates."""
+    """Entries evaluating one output block at its canonical coordinates.
+
+    ``target_atom >= 0`` marks a creation program: it evaluates the padded
+    tensor of that ``C`` atom at every index tuple of ``key`` (the shuffle
+    formula is valid for unsorted and repeated indices, giving the full
+    antisymmetric tensor) and writes it into the atom workspace.
+    """
 
     key: BlockKey
     coefficients: np.ndarray  # complex [E]
     factor_start: np.ndarray  # int32 [E+1]
     factor_atom: np.ndarray  # int32 [F] atom id
-    factor_positions: np.ndarray  # int32 [F, 3] output positions per slot (-1 unused)
+    factor_positions: np.ndarray  # int32 [F, 4] output positions per slot (-1 unused)
+    target_atom: int = -1
 
 
 @dataclass
@@ -362,11 +406,50 @@ class Plan:
         return int(sum(b.coefficients.size for b in self.programs[name]))
 
 
-def emit_plan(actions: dict[str, tuple[dict, tuple[BlockKey, ...]]], n: int, vectors: np.ndarray) -> Plan:
-    """Lower merged term dictionaries (one per action) into flat arrays."""
+def _block_program(term_list, key, index, target_atom=-1) -> BlockProgram:
+    coefficients, starts, f_atom, f_pos = [], [0], [], []
+    for term_atoms, coefficient in term_list:
+        for sign, positions in _shuffles(term_atoms, key):
+            coefficients.append(sign * coefficient)
+            for a, ps in zip(term_atoms, positions):
+                f_atom.append(index[a])
+                f_pos.append(ps + [-1] * (4 - len(ps)))
+            starts.append(len(f_atom))
+    return BlockProgram(
+        key,
+        np.asarray(coefficients, dtype=np.complex128),
+        np.asarray(starts, dtype=np.int32),
+        np.asarray(f_atom, dtype=np.int32),
+        np.asarray(f_pos, dtype=np.int32).reshape(-1, 4),
+        target_atom,
+    )
+
+
+def emit_plan(
+    actions: dict[str, tuple[dict, tuple[BlockKey, ...]]],
+    n: int,
+    vectors: np.ndarray,
+    creations: list[tuple[BlockKey, dict]] | None = None,
+) -> Plan:
+    """Lower merged term dictionaries (one per action) into flat arrays.
+
+    ``creations[i]`` defines ``C`` atom ``i`` (its block key and the terms it
+    sums); it becomes the ``creation`` program, run after the fixed atoms are
+    loaded and before any program that reads ``C`` atoms.
+    """
+
+    creations = creations or []
+    needed: dict[Atom, None] = {}
+
+    def need(a: Atom) -> None:
+        chain = a
+        needed[chain] = None
+        while chain.contractions:
+            chain = chain.parent()
+            if chain.contractions:
+                needed[chain] = None
 
     by_action: dict[str, dict[BlockKey, list[tuple[Term, complex]]]] = {}
-    needed: dict[Atom, None] = {}
     for name, (terms, out_keys) in actions.items():
         by_key: dict[BlockKey, list[tuple[Term, complex]]] = defaultdict(list)
         for atoms, coefficient in terms.items():
@@ -374,13 +457,12 @@ def emit_plan(actions: dict[str, tuple[dict, tuple[BlockKey, ...]]], n: int, vec
             if k in out_keys:
                 by_key[k].append((atoms, coefficient))
                 for a in atoms:
-                    chain = a
-                    needed[chain] = None
-                    while chain.contractions:
-                        chain = chain.parent()
-                        if chain.contractions:
-                            needed[chain] = None
+                    need(a)
         by_action[name] = by_key
+    for key, rest_terms in creations:
+        for atoms in rest_terms:
+            for a in atoms:
+                need(a)
     atoms = sorted(needed, key=lambda a: (len(a.contractions), a.sort_key()))
     index = {a: i for i, a in enumerate(atoms)}
     offsets = np.zeros(len(atoms), dtype=np.int64)
@@ -408,27 +490,13 @@ def emit_plan(actions: dict[str, tuple[dict, tuple[BlockKey, ...]]], n: int, vec
     programs: dict[str, list[BlockProgram]] = {}
     n_terms: dict[str, int] = {}
     for name, (terms, out_keys) in actions.items():
+        programs[name] = [_block_program(by_action[name][k], k, index) for k in out_keys]
+        n_terms[name] = sum(len(by_action[name][k]) for k in out_keys)
+    if creations:
         blocks = []
-        count = 0
-        for k in out_keys:
-            coefficients, starts, f_atom, f_pos = [], [0], [], []
-            for term_atoms, coefficient in by_action[name][k]:
-                count += 1
-                for sign, positions in _shuffles(term_atoms, k):
-                    coefficients.append(sign * coefficient)
-                    for a, ps in zip(term_atoms, positions):
-                        f_atom.append(index[a])
-                        f_pos.append(ps + [-1] * (3 - len(ps)))
-                    starts.append(len(f_atom))
-            blocks.append(
-                BlockProgram(
-                    k,
-                    np.asarray(coefficients, dtype=np.complex128),
-                    np.asarray(starts, dtype=np.int32),
-                    np.asarray(f_atom, dtype=np.int32),
-                    np.asarray(f_pos, dtype=np.int32).reshape(-1, 3),
-                )
-            )
-        programs[name] = blocks
-        n_terms[name] = count
+        for i, (key, rest_terms) in enumerate(creations):
+            target = index[Atom("C", i, key_slots(key))]
+            blocks.append(_block_program(list(rest_terms.items()), key, index, target))
+        programs["creation"] = blocks
+        n_terms["creation"] = sum(len(t) for _, t in creations)
     return Plan(n, atoms, offsets, int(cursor), jobs, programs, vectors, n_terms)

@@ -7,7 +7,7 @@ import subprocess
 from pathlib import Path
 
 PACKAGE = Path(__file__).resolve().parent
-SOURCE = PACKAGE / "csrc" / "c3l_hip.hip"
+SOURCES = sorted((PACKAGE / "csrc").glob("*.hip"))
 TARGET = PACKAGE / "libc3l_hip.so"
 
 HIPCC_FLAGS = [
@@ -27,7 +27,7 @@ HIPCC_FLAGS = [
 def build(verbose: bool = False) -> Path:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = TARGET.with_suffix(".so.tmp")
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", str(tmp), str(SOURCE)]
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", str(tmp), *[str(s) for s in SOURCES]]
     result = subprocess.run(cmd, capture_output=True, text=True)
     if result.returncode != 0:
         raise RuntimeError(f"hipcc failed ({result.returncode}):\n{result.stderr}")

@@ -126,9 +126,9 @@ __global__ void wedge_block_kernel(const double2* __restrict__ ws, double2* __re
     const int f1 = fstart[t + 1];
     for (int f = fstart[t]; f < f1; ++f) {
       const int code = fcode[f];
-      const int rank = code & 3;
+      const int rank = code & 7;
       int64_t flat = 0;
-      for (int s = 0; s < rank; ++s) flat = flat * n + idx[(code >> (2 + 2 * s)) & 3];
+      for (int s = 0; s < rank; ++s) flat = flat * n + idx[(code >> (3 + 2 * s)) & 3];
       prod = cmul(prod, ws[foffset[f] + flat]);
     }
     acc = cadd(acc, prod);
@@ -376,6 +376,7 @@ struct PlanBlock {
   int32_t deg;
   int64_t count;
   int32_t n_entries;
+  int64_t max_dest = -1;
   int32_t* coord = nullptr;
   int64_t* dest = nullptr;
   double2* coeff = nullptr;
@@ -432,6 +433,8 @@ int64_t ipow(int64_t n, int r) {
 extern "C" {
 
 const char* c3l_last_error(void) { return g_error.c_str(); }
+
+int c3l_internal_fail(int code, const char* message) { return fail(code, message ? message : ""); }
 
 const char* c3l_version(void) { return "c3l_hip 0.1 gfx950 fp64"; }
 
@@ -578,11 +581,16 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
     return fail(C3L_EINVAL, "c3l_plan_add_block: invalid argument");
   if (n_entries > 0 && fstart[n_entries] != n_factors) return fail(C3L_EINVAL, "c3l_plan_add_block: fstart mismatch");
   for (int32_t f = 0; f < n_factors; ++f) {
-    const int rank = fcode[f] & 3;
-    if (foffset[f] < 0 || foffset[f] + ipow(plan->n, rank) > plan->workspace)
+    const int rank = fcode[f] & 7;
+    if (rank > 4 || foffset[f] < 0 || foffset[f] + ipow(plan->n, rank) > plan->workspace)
       return fail(C3L_EINVAL, "c3l_plan_add_block: factor exceeds workspace");
     for (int s = 0; s < rank; ++s)
-      if (((fcode[f] >> (2 + 2 * s)) & 3) >= deg) return fail(C3L_EINVAL, "c3l_plan_add_block: factor position out of range");
+      if (((fcode[f] >> (3 + 2 * s)) & 3) >= deg) return fail(C3L_EINVAL, "c3l_plan_add_block: factor position out of range");
+  }
+  int64_t max_dest = -1;
+  for (int64_t k = 0; k < count; ++k) {
+    if (dest[k] < 0) return fail(C3L_EINVAL, "c3l_plan_add_block: negative destination");
+    if (dest[k] > max_dest) max_dest = dest[k];
   }
   for (int64_t k = 0; k < count * deg; ++k)
     if (coord[k] < 0 || coord[k] >= plan->n) return fail(C3L_EINVAL, "c3l_plan_add_block: coordinate out of range");
@@ -591,6 +599,7 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
   b.deg = deg;
   b.count = count;
   b.n_entries = n_entries;
+  b.max_dest = max_dest;
   int rc = upload(plan, &b.coord, coord, count * deg);
   if (rc == C3L_OK) rc = upload(plan, &b.dest, dest, count);
   if (rc == C3L_OK) rc = upload(plan, &b.coeff, reinterpret_cast<const double2*>(coeff), n_entries);
@@ -622,12 +631,16 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
 }
 
 int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out, void* stream) {
-  if (plan == nullptr || out == nullptr || ((diag == nullptr) != (x == nullptr)))
+  if (plan == nullptr || ((diag == nullptr) != (x == nullptr)) || (out == nullptr && diag != nullptr))
     return fail(C3L_EINVAL, "c3l_plan_apply: invalid argument");
+  const bool internal = out == nullptr;  // creation programs write into the workspace
+  if (internal) out = reinterpret_cast<double*>(plan->ws);
   hipStream_t s = static_cast<hipStream_t>(stream);
   for (size_t i = 0; i < plan->blocks.size(); ++i) {
     const PlanBlock& b = plan->blocks[i];
     if (b.program != program || b.count == 0) continue;
+    if (internal && b.max_dest >= plan->workspace)
+      return fail(C3L_EINVAL, "c3l_plan_apply: workspace program writes outside the workspace");
     PendingTiming t{i, nullptr, nullptr};
     if (plan->timing) {
       C3L_HIP_CHECK(hipEventCreate(&t.start));

@@ -86,20 +86,28 @@ class DevicePlan:
         """Scatter a complex device vector into the ``kind`` bases and build its atoms."""
 
         _lib.call("c3l_plan_load", self.handle, KINDS[kind], _lib.dptr(vector), _lib.stream_ptr(self.torch))
+        if kind == "T" and self.has_creation:
+            self.apply("creation", None)  # creation tensors depend on the amplitudes only
         token = next(_tokens)
         self.loaded[KINDS[kind]] = token
         return token
 
     def apply(self, program: str, out, diag=None, x=None) -> None:
+        """Run a program into ``out``; ``out=None`` runs a workspace (creation) program."""
+
         _lib.call(
             "c3l_plan_apply",
             self.handle,
             PROGRAMS[program],
             None if diag is None else _lib.dptr(diag),
             None if x is None else _lib.dptr(x),
-            _lib.dptr(out),
+            None if out is None else _lib.dptr(out),
             _lib.stream_ptr(self.torch),
         )
+
+    @property
+    def has_creation(self) -> bool:
+        return any(b[0] == PROGRAMS["creation"] for b in self.lowered.blocks)
 
     def set_timing(self, enable: bool) -> None:
         _lib.call("c3l_plan_set_timing", self.handle, 1 if enable else 0)

@@ -22,7 +22,7 @@ from time import perf_counter
 import numpy as np
 
 from . import _lib
-from .algebra import TermCompiler, emit_plan
+from .algebra import TermCompiler, emit_plan, factor_linear_terms, term_key
 from .lowering import STATIC_KEYS, CoordinateSet, FactorizedCCProblem, physical_mode_operator, response_coordinates, response_keys
 from .plan_arrays import LoweredPlan, lower_plan
 from .static import ComponentStaticProblem
@@ -67,8 +67,9 @@ class ComponentEOMProblem:
         coords = response_coordinates(n, charge)
         keys = tuple(b.key for b in coords.blocks)
         compiler = TermCompiler(n, problem.cc.products)
-        terms = compiler.response_terms(keys, 3)
-        plan = emit_plan({"sigma": (terms, keys)}, n, compiler.table.array())
+        terms = {k: v for k, v in compiler.response_terms(keys, 3).items() if term_key(k) in keys}
+        factored, creations = factor_linear_terms(terms, "X")
+        plan = emit_plan({"sigma": (factored, keys)}, n, compiler.table.array(), creations)
         lowered = lower_plan(plan, {"T": (static.oracle.coordinates, STATIC_KEYS), "X": (coords, keys)}, {"sigma": coords})
         diagonal = problem.cc.rapidity_sums(coords)
         return cls(problem, static, charge, coords, keys, lowered, diagonal)

@@ -10,7 +10,9 @@ workspace and converts the symbolic plan of ``algebra.emit_plan`` into:
   ``component_tensor.py:302-337``);
 * ``levels`` -- contraction jobs per kind and depth;
 * ``blocks`` -- per program and output block: coordinates, destinations and
-  the wedge entries (``fcode = rank | pos0<<2 | pos1<<4 | pos2<<6``).
+  the wedge entries (``fcode = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9``);
+  creation programs write every padded index tuple of a ``C`` atom into the
+  workspace (destinations are workspace offsets).
 """
 
 from __future__ import annotations
@@ -24,7 +26,17 @@ from .algebra import Plan, key_slots
 from .lowering import BlockKey, CoordinateSet
 
 KINDS = {"T": 0, "D": 1, "X": 2}
-PROGRAMS = {"residual": 0, "reference": 1, "jvp": 2, "sigma": 3}
+PROGRAMS = {"residual": 0, "reference": 1, "jvp": 2, "sigma": 3, "creation": 4}
+
+
+def encode_factors(ranks: np.ndarray, positions: np.ndarray) -> np.ndarray:
+    """``fcode = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9`` (rank <= 4, positions < 4)."""
+
+    code = ranks.astype(np.int32).copy()
+    for s in range(positions.shape[1] if positions.size else 0):
+        pos = np.where(positions[:, s] >= 0, positions[:, s], 0).astype(np.int32)
+        code |= (pos << (3 + 2 * s)) * (s < ranks)
+    return code
 
 
 @dataclass
@@ -143,10 +155,14 @@ def lower_plan(
             )
     blocks = []
     for name, programs in plan.programs.items():
-        coords = out_sets[name]
+        coords = out_sets.get(name)
         for bp in programs:
             deg = sum(bp.key)
-            if deg == 0:
+            if bp.target_atom >= 0:
+                # creation tensor: every index tuple of its padded storage, written in place
+                coord = np.indices((n,) * deg).reshape(deg, -1).T.astype(np.int32) if deg else np.zeros((1, 0), np.int32)
+                dest = plan.atom_offsets[bp.target_atom] + np.arange(n**deg, dtype=np.int64)
+            elif deg == 0:
                 coord = np.zeros((1, 0), dtype=np.int32)
                 dest = np.zeros(1, dtype=np.int64)
             else:
@@ -155,11 +171,8 @@ def lower_plan(
                     continue
                 coord = block.indices.astype(np.int32)
                 dest = block.positions.astype(np.int64)
-            ranks = np.array([plan.atoms[a].rank for a in bp.factor_atom], dtype=np.int32)
-            fcode = ranks.copy()
-            for s in range(3):
-                pos = np.where(bp.factor_positions[:, s] >= 0, bp.factor_positions[:, s], 0)
-                fcode |= (pos.astype(np.int32) << (2 + 2 * s)) * (s < ranks)
+            fcode = encode_factors(np.array([plan.atoms[a].rank for a in bp.factor_atom], dtype=np.int32),
+                                   bp.factor_positions)
             foffset = plan.atom_offsets[bp.factor_atom] if bp.factor_atom.size else np.zeros(0, dtype=np.int64)
             blocks.append(
                 (

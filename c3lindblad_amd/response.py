@@ -22,6 +22,7 @@ stated tolerance.
 
 from __future__ import annotations
 
+import ctypes
 import resource
 from dataclasses import dataclass
 from time import perf_counter
@@ -95,20 +96,6 @@ def shifted_green(T, r, w, q, z):
     return out
 
 
-def _estimated_residuals(H: np.ndarray, shifts: np.ndarray) -> np.ndarray:
-    """``prod_k |h_{k+1,k}| / prod_k |z - λ_k|`` in log space (relative residual)."""
-
-    m = H.shape[1]
-    lam = scipy.linalg.eigvals(H[:m, :m], check_finite=False)
-    logh = float(np.sum(np.log(np.abs(np.diag(H, -1)[:m]))))
-    logdet = np.sum(np.log(np.abs(shifts[:, None] - lam[None, :])), axis=1)
-    return np.exp(logh - logdet)
-
-
-def _next_check(actual: int, minimum: int, interval: int) -> int:
-    return max(minimum, actual + max(interval, actual // 8))
-
-
 def arnoldi_resolvent(
     operator: ComponentSigmaOperator,
     rhs,
@@ -120,12 +107,17 @@ def arnoldi_resolvent(
     check_interval: int = 8,
     breakdown_tolerance: float = 1.0e-14,
 ) -> DeviceArnoldi:
-    """Build one shift-invariant Krylov space for ``(zI - σ)^{-1} b`` on the GPU."""
+    """Build one shift-invariant Krylov space for ``(zI - σ)^{-1} b`` with the native driver.
+
+    The exact FOM residual of every shift is tracked on the device at each
+    step, so ``check_interval`` only bounds how late a converged basis may
+    stop (the driver tests every step).
+    """
 
     torch = _lib.require_device()
     n = int(operator.dimension)
     b = np.asarray(rhs, dtype=np.complex128)
-    z = np.asarray(shifts, dtype=np.complex128)
+    z = np.ascontiguousarray(np.asarray(shifts, dtype=np.complex128))
     if b.shape != (n,) or not np.all(np.isfinite(b)):
         raise ValueError("rhs must be a finite vector matching the operator")
     if z.ndim != 1 or z.size == 0 or not np.all(np.isfinite(z)):
@@ -139,58 +131,44 @@ def arnoldi_resolvent(
         raise ValueError("rhs must be nonzero")
     started = perf_counter()
     m_max = maximum_dimension
-    V = torch.zeros(m_max + 1, n, dtype=torch.complex128, device=_lib.DEVICE)
+    dev = _lib.DEVICE
+    c128, f64 = torch.complex128, torch.float64
+    V = torch.zeros(m_max + 1, n, dtype=c128, device=dev)
     V[0].copy_(torch.from_numpy(b / beta))
-    H = np.zeros((m_max + 1, m_max), dtype=np.complex128)
-    red = Reducer(n, m_max + 1, complex_=True)
-    coef = torch.zeros(2 * (m_max + 1) + 1, dtype=torch.complex128, device=_lib.DEVICE)
-    host = _lib.pinned(torch.zeros(2 * (m_max + 1) + 1, dtype=torch.complex128))
-    w = torch.empty(n, dtype=torch.complex128, device=_lib.DEVICE)
-    checks, values = [], []
-    actual = m_max
-    breakdown = False
-    calls = 0
-    next_check = minimum_dimension
-    for j in range(m_max):
-        operator.matvec_device(V[j], w)
-        calls += 1
-        c1 = coef[: j + 1]
-        c2 = coef[m_max + 1 : m_max + 2 + j]
-        s = coef[2 * (m_max + 1) : 2 * (m_max + 1) + 1]
-        red.dot(V, j + 1, w, c1)
-        red.axpy(V, j + 1, c1, w, -1.0)
-        red.dot(V, j + 1, w, c2)
-        red.axpy(V, j + 1, c2, w, -1.0)
-        red.dot(w.view(1, -1), 1, w, s)
-        host.copy_(coef)
-        h = host.numpy()
-        H[: j + 1, j] = h[: j + 1] + h[m_max + 1 : m_max + 2 + j]
-        hn = float(np.sqrt(max(h[2 * (m_max + 1)].real, 0.0)))
-        if not np.isfinite(hn) or not np.all(np.isfinite(H[: j + 1, j])):
-            raise FloatingPointError("nonfinite Arnoldi recurrence")
-        H[j + 1, j] = hn
-        actual = j + 1
-        if hn <= breakdown_tolerance:
-            breakdown = True
-            break
-        _lib.call("c3l_rscale", _lib.dptr(w), _lib.dptr(V[j + 1]), 1.0 / hn, 2 * n, _lib.stream_ptr(torch))
-        if actual >= next_check or actual == m_max:
-            res = float(np.max(_estimated_residuals(H[: actual + 1, :actual], z)))
-            checks.append(actual)
-            values.append(res)
-            if res <= residual_tolerance:
-                break
-            next_check = _next_check(actual, minimum_dimension, check_interval)
+    H = torch.zeros(m_max, m_max + 1, dtype=c128, device=dev)  # row j = column j of H
+    zs = torch.from_numpy(z).to(dev)
+    rot_c = torch.zeros(m_max * z.size, dtype=f64, device=dev)
+    rot_s = torch.zeros(m_max * z.size, dtype=c128, device=dev)
+    rho = torch.zeros(z.size, dtype=f64, device=dev)
+    resid = torch.zeros(z.size, dtype=f64, device=dev)
+    chunks = int(_lib.load_library().c3l_reduction_chunks(n))
+    scratch = torch.zeros(2 * n + 2 * chunks * (m_max + 1), dtype=f64, device=dev)
+    coef = torch.zeros(2 * (m_max + 1) + 1, dtype=c128, device=dev)
+    status = torch.zeros(2, dtype=f64, device=dev)
+    m_out, broke = ctypes.c_int32(), ctypes.c_int32()
+    worst = ctypes.c_double()
+    plan = operator.context.device
+    if plan.plan.loaded.get(0) != plan.token:
+        plan.token = plan.plan.load("T", plan.t)
+    _lib.call(
+        "c3l_arnoldi", plan.plan.handle, _lib.dptr(plan.diag), _lib.dptr(V), _lib.dptr(H), n, m_max, _lib.dptr(zs), z.size,
+        _lib.dptr(rot_c), _lib.dptr(rot_s), _lib.dptr(rho), _lib.dptr(resid), _lib.dptr(scratch), _lib.dptr(coef),
+        _lib.dptr(status), float(residual_tolerance), int(minimum_dimension), float(breakdown_tolerance),
+        ctypes.byref(m_out), ctypes.byref(broke), ctypes.byref(worst), _lib.stream_ptr(torch),
+    )
+    m = m_out.value
+    operator.action_calls += m
+    hess = H[:m, : m + 1].cpu().numpy().T.copy()
     return DeviceArnoldi(
-        basis=V[: actual + 1],
-        hessenberg=H[: actual + 1, :actual].copy(),
+        basis=V[: m + 1],
+        hessenberg=hess,
         source_norm=beta,
-        breakdown=breakdown,
-        check_dimensions=tuple(checks),
-        check_maximum_relative_residuals=tuple(values),
-        converged=bool(breakdown or (values and values[-1] <= residual_tolerance)),
+        breakdown=bool(broke.value),
+        check_dimensions=(m,),
+        check_maximum_relative_residuals=(float(worst.value),),
+        converged=bool(broke.value or worst.value <= residual_tolerance),
         construction_wall_seconds=perf_counter() - started,
-        action_calls=calls,
+        action_calls=m,
     )
 
 

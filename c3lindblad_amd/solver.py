@@ -15,6 +15,7 @@ the Givens update.
 
 from __future__ import annotations
 
+import ctypes
 import resource
 from dataclasses import dataclass, field
 from time import perf_counter
@@ -152,87 +153,24 @@ class _DeviceNewton:
         self.dev.tilde.precondition(v, out, self.inv_diag)
 
     def gmres(self, b, x, rtol: float, restart: int, cycles: int):
-        """Left-preconditioned restarted GMRES for ``J x = b`` (x starts at 0)."""
+        """Left-preconditioned restarted GMRES for ``J x = b`` in the native driver (``c3l_gmres``)."""
 
-        torch = self.torch
-        x.zero_()
-        bnrm = self.norm(b)
-        if bnrm == 0.0:
-            return 0, 0, 0.0
-        self.precondition(b, self.tmp)
-        ptol = self.norm(self.tmp) * rtol
-        atol = rtol * bnrm
-        iterations = 0
-        info = 1
-        V = self.V
-        for cycle in range(cycles):
-            if cycle == 0:
-                self.w.copy_(b)
-            else:
-                self.apply_A(x, self.w)
-                raxpby(1.0, b, -1.0, self.w, torch)  # w = b - A x
-                if self.norm(self.w) < atol:
-                    info = 0
-                    break
-            self.precondition(self.w, V[0])
-            beta = self.norm(V[0])
-            rscale(V[0], V[0], 1.0 / beta, torch)
-            H = np.zeros((restart + 1, restart))
-            givens = np.zeros((restart, 2))
-            S = np.zeros(restart + 1)
-            S[0] = beta
-            used = 0
-            done = False
-            for j in range(restart):
-                self.apply_A(V[j], self.tmp)
-                self.precondition(self.tmp, self.w)
-                c1 = self.coef[: j + 1]
-                c2 = self.coef[restart + 1 : restart + 2 + j]
-                s = self.coef[2 * restart + 3 : 2 * restart + 4]
-                self.reducer.dot(V, j + 1, self.w, c1)
-                self.reducer.axpy(V, j + 1, c1, self.w, -1.0)
-                self.reducer.dot(V, j + 1, self.w, c2)
-                self.reducer.axpy(V, j + 1, c2, self.w, -1.0)
-                self.reducer.dot(self.w.view(1, -1), 1, self.w, s)
-                self.host.copy_(self.coef)
-                h = self.host.numpy()
-                col = h[: j + 1] + h[restart + 1 : restart + 2 + j]
-                hn = float(np.sqrt(max(h[2 * restart + 3], 0.0)))
-                H[: j + 1, j] = col
-                H[j + 1, j] = hn
-                if hn > 0.0:
-                    rscale(self.w, V[j + 1], 1.0 / hn, torch)
-                # Givens update of the least-squares problem
-                for k in range(j):
-                    c, sn = givens[k]
-                    a, bb = H[k, j], H[k + 1, j]
-                    H[k, j], H[k + 1, j] = c * a + sn * bb, -sn * a + c * bb
-                a, bb = H[j, j], H[j + 1, j]
-                rr = np.hypot(a, bb)
-                c, sn = (1.0, 0.0) if rr == 0.0 else (a / rr, bb / rr)
-                givens[j] = (c, sn)
-                H[j, j], H[j + 1, j] = rr, 0.0
-                S[j + 1] = -sn * S[j]
-                S[j] = c * S[j]
-                iterations += 1
-                used = j + 1
-                if abs(S[j + 1]) <= ptol or hn == 0.0:
-                    done = True
-                    break
-            y = np.zeros(used)
-            for k in range(used - 1, -1, -1):
-                y[k] = (S[k] - H[k, k + 1 : used] @ y[k + 1 :]) / H[k, k]
-            self.coef[:used].copy_(torch.from_numpy(y))
-            self.reducer.axpy(V, used, self.coef[:used], x, 1.0)
-            if done and cycle == cycles - 1:
-                info = 0
-        # true relative residual of the returned step
-        self.apply_A(x, self.w)
-        raxpby(1.0, b, -1.0, self.w, torch)
-        true = self.norm(self.w) / bnrm
-        if true <= rtol * (1.0 + 1e-8):
-            info = 0
-        return info, iterations, true
+        tl = self.dev.tilde
+        maps = _lib.TildeMaps(
+            tl.dim, tl.real_dim, _lib.dptr(tl.k0), _lib.dptr(tl.k1), _lib.dptr(tl.ab), _lib.dptr(tl.src),
+            _lib.dptr(tl.w), _lib.dptr(self.inv_diag),
+        )
+        info, iterations, calls = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        true = ctypes.c_double()
+        _lib.call(
+            "c3l_gmres", self.dev.plan.handle, ctypes.byref(maps), _lib.dptr(self.dev.diag), _lib.dptr(self.dev.dt),
+            _lib.dptr(self.dev.out), _lib.dptr(b), _lib.dptr(x), _lib.dptr(self.V), _lib.dptr(self.w),
+            _lib.dptr(self.tmp), _lib.dptr(self.reducer.scratch), _lib.dptr(self.coef), float(rtol), int(restart),
+            int(cycles), ctypes.byref(info), ctypes.byref(iterations), ctypes.byref(true), ctypes.byref(calls),
+            _lib.stream_ptr(self.torch),
+        )
+        self.jvp_calls += calls.value
+        return info.value, iterations.value, true.value
 
 
 def solve_component_ccsd(

@@ -24,7 +24,7 @@ from time import perf_counter
 import numpy as np
 
 from . import _lib
-from .algebra import TermCompiler, emit_plan, tangent_terms
+from .algebra import TermCompiler, emit_plan, factor_linear_terms, tangent_terms, term_key
 from .lowering import STATIC_KEYS, FactorizedCCProblem
 from .plan_arrays import LoweredPlan, lower_plan
 
@@ -51,15 +51,17 @@ class ComponentStaticProblem:
         started = perf_counter()
         compiler = TermCompiler(problem.model.n_sites, problem.products)
         residual = compiler.static_terms(4)
-        jvp = tangent_terms(residual)
+        jvp = {k: -1j * v for k, v in tangent_terms(residual).items() if term_key(k) in STATIC_KEYS}
+        jvp_factored, creations = factor_linear_terms(jvp, "D")
         plan = emit_plan(
             {
                 "residual": ({k: -1j * v for k, v in residual.items()}, STATIC_KEYS),
                 "reference": ({k: -1j * v for k, v in residual.items()}, ((0, 0, 0, 0),)),
-                "jvp": ({k: -1j * v for k, v in jvp.items()}, STATIC_KEYS),
+                "jvp": (jvp_factored, STATIC_KEYS),
             },
             problem.model.n_sites,
             compiler.table.array(),
+            creations,
         )
         coords = problem.coordinates
         lowered = lower_plan(

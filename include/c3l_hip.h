@@ -73,11 +73,11 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
                        const int64_t* par_offset, const int32_t* par_rank, const int32_t* slot,
                        const int32_t* vec, const int32_t* sign);
 
-/* Register one output block of program `program`: `count` canonical coordinates of
- * degree `deg` (coord[count*deg] local indices, dest[count] output positions) and
- * `n_entries` wedge entries: entry t = coeff[t] * prod_{f in [fstart[t], fstart[t+1])}
- * ws[foffset[f] + flat_f], flat_f = sum_s coord[pos_s] n^(rank-1-s) with
- * fcode[f] = rank | pos0<<2 | pos1<<4 | pos2<<6. */
+/* Register one output block of program `program`: `count` coordinates of degree `deg`
+ * (coord[count*deg] local indices, dest[count] output positions) and `n_entries` wedge
+ * entries: entry t = coeff[t] * prod_{f in [fstart[t], fstart[t+1])} ws[foffset[f] + flat_f],
+ * flat_f = sum_s coord[pos_s] n^(rank-1-s) with
+ * fcode[f] = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9 (rank <= 4). */
 int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
                        const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
                        int32_t n_factors, const int64_t* foffset, const int32_t* fcode);
@@ -88,7 +88,8 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream);
 
 /* out[dest] = diag[dest]*x[dest] + sum of the program's entries, for every block of
  * `program` (out positions not covered by a block are left untouched).  diag and x
- * may be NULL (no diagonal term). */
+ * may be NULL (no diagonal term).  out == NULL runs a workspace program (creation
+ * tensors): destinations are workspace offsets, diag and x must be NULL. */
 int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out,
                    void* stream);
 
@@ -140,6 +141,45 @@ int64_t c3l_reduction_chunks(int64_t dim);
 int c3l_shifted_triangular_green(const double* T, int32_t m, const double* r, const double* w, const double* q,
                                  const double* shifts, int32_t n_shifts, double* green, double* last,
                                  void* stream);
+
+/* ---- native Krylov drivers ----------------------------------------------------------- */
+
+/* Tilde coordinate maps and rapidity preconditioner of one static problem (device). */
+typedef struct c3l_tilde_maps {
+  int64_t dim;             /* complex amplitudes */
+  int64_t real_dim;        /* independent real coordinates */
+  const int64_t* k0;       /* unpack gathers [dim] */
+  const int64_t* k1;
+  const double* ab;        /* [dim][4] alpha re, im, beta re, im */
+  const int64_t* src;      /* pack gather [real_dim] */
+  const double* w;         /* pack weights, complex [real_dim] */
+  const double* inv_diag;  /* 1 / (-i Lambda), complex [dim] */
+} c3l_tilde_maps;
+
+/* Newton correction J x = b in real tilde coordinates (replaces the scipy gmres call of
+ * solve_component_ccsd, src/c3l/cc/component_solver.py:181-191, and the JAX cycle
+ * get_cycle, cc/_static_gmres.py:12-66).  Left-preconditioned GMRES(restart), at most
+ * `cycles` restarts, inner tolerance ||M b|| * rtol, true residual test per restart.
+ * The plan's T atoms must be loaded (linearisation point).  Workspace: t_work, out_work
+ * complex [dim]; V [(restart+1) x real_dim]; w, tmp [real_dim]; scratch >= chunks*(restart+2)
+ * doubles; coef >= 2*(restart+1)+1 doubles. x is overwritten with the step. */
+int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
+              const double* b, double* x, double* V, double* w, double* tmp, double* scratch, double* coef,
+              double rtol, int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations, double* true_rel,
+              int32_t* jvp_calls, void* stream);
+
+/* Shift-invariant Arnoldi on the sigma program (replaces arnoldi_resolvent,
+ * src/c3l/green/krylov.py:284-417, and the device loop of
+ * component_retarded_matrix_green_arnoldi_jax, cc/component_jax.py:924-957).  V [(m_max+1) x dim]
+ * complex with V[0] = b/beta on entry; H complex column-major (m_max+1) x m_max; rot_c
+ * [m_max*n_shifts] doubles, rot_s [m_max*n_shifts] complex, rho, resid [n_shifts] doubles;
+ * scratch >= 2*dim + 2*chunks*(m_max+1) doubles; coef >= 4*(m_max+1)+2 doubles; status >= 2
+ * doubles.  Stops at the first m >= min_dim with every shift's relative residual <= tolerance,
+ * at a breakdown, or at m_max. */
+int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
+                const double* shifts, int32_t n_shifts, double* rot_c, double* rot_s, double* rho, double* resid,
+                double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
+                double breakdown_tol, int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
 
 #ifdef __cplusplus
 }

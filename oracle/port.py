@@ -40,9 +40,14 @@ class PortPlan:
                 parent = self.ws[p : p + n**r].reshape((n,) * r)
                 out = np.tensordot(parent, self.lp.vectors[vi], axes=([s], [0]))
                 self.ws[o : o + n ** (r - 1)] = sg * np.ravel(out)
+        if kind == 0:  # amplitudes changed: rebuild the creation tensors that depend on them
+            self.apply(4, None)
 
-    def apply(self, program: int, out: np.ndarray, diag=None, x=None) -> np.ndarray:
+    def apply(self, program: int, out: np.ndarray | None, diag=None, x=None) -> np.ndarray:
+        """Run one program; ``out=None`` writes a workspace (creation) program in place."""
+
         n = self.n
+        target = self.ws if out is None else out
         for pid, deg, coord, dest, coeff, fstart, foff, fcode in self.lp.blocks:
             if pid != program:
                 continue
@@ -51,13 +56,13 @@ class PortPlan:
                 prod = np.full(dest.size, coeff[t], dtype=np.complex128)
                 for f in range(fstart[t], fstart[t + 1]):
                     code = int(fcode[f])
-                    rank = code & 3
+                    rank = code & 7
                     flat = np.zeros(dest.size, dtype=np.int64)
                     for s in range(rank):
-                        flat = flat * n + coord[:, (code >> (2 + 2 * s)) & 3]
+                        flat = flat * n + coord[:, (code >> (3 + 2 * s)) & 3]
                     prod *= self.ws[foff[f] + flat]
                 acc += prod
             if diag is not None:
                 acc += diag[dest] * x[dest]
-            out[dest] = acc
-        return out
+            target[dest] = acc
+        return target
