@@ -8,10 +8,11 @@
 // tolerance ||M b|| * rtol, true residual test at every restart).
 //
 // Arnoldi: two-pass CGS on the device; the Hessenberg matrix stays on the
-// device; for every shift z a Givens QR of (z I - H) is advanced by one
-// column per step (one thread per shift), giving the exact FOM residual
-// |h_{m+1,m} e_m^T (z - H_m)^{-1} beta e_1| / beta = rho_GMRES(z) / c_m(z)
-// at O(m) cost per shift and step.  One 16-byte transfer per step carries
+// device; every shift z runs GMRES on the shared basis: its Givens QR of
+// (z I - H) advances by one column per step (one wave per shift, O(m) work)
+// and the sink projection of its solution is updated incrementally, so the
+// Green function and the exact residual of every shift are available at every
+// step without any dense factorisation.  One 16-byte transfer per step carries
 // the new sub-diagonal entry and the worst shift residual to the host.
 
 #include <hip/hip_runtime.h>
@@ -69,54 +70,124 @@ __global__ void normalize_kernel(const double2* __restrict__ w, double2* __restr
     v[i] = make_double2(a * w[i].x, a * w[i].y);
 }
 
-// One thread per shift: advance the Givens QR of (z I - H) by column j.
-// rot_c [m_max][n_shifts] (real cosines), rot_s [m_max][n_shifts] (complex sines),
-// rho [n_shifts] running product of |s_k|, resid [n_shifts] FOM relative residual.
-__global__ void shifted_givens_kernel(const double2* __restrict__ H, int j, int m_max,
-                                      const double2* __restrict__ shifts, int n_shifts, double* __restrict__ rot_c,
-                                      double2* __restrict__ rot_s, double* __restrict__ rho, double* __restrict__ resid) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n_shifts) return;
+// ---------------------------------------------------------------------------
+// Multi-shift GMRES on one shared Arnoldi basis, one 64-lane wave per shift.
+//
+// For shift z the (j+2)-entry column j of (z I~ - H) is a_k = [k == j] z - H[k, j].
+// The earlier Givens rotations act on it as the affine recurrence
+//   x_{k+1} = -conj(s_k) x_k + c_k a_{k+1},   R[k, j] = c_k x_k + s_k a_{k+1},
+// which the wave evaluates 64 rotations at a time with a shuffle scan of affine
+// maps.  The solution is tracked only through its sink projection:
+//   pi_j = (sink.v_j - sum_{k<j} R[k, j] pi_k) / R[j, j],  G += gamma_j pi_j,
+// where gamma_j is the rotated right-hand side entry (GMRES with incremental
+// solution update, x_m = x_{m-1} + gamma_m p_m, p = V R^{-1}).  The GMRES
+// residual |gamma~| / beta is exact for every shift at every step.
+// Per-shift state (row s, length m_max): rot_c, rot_s, pi.
+__device__ __forceinline__ double2 shfl_up2(double2 v, int off) {
+  return make_double2(__shfl_up(v.x, off, 64), __shfl_up(v.y, off, 64));
+}
+__device__ __forceinline__ double2 shfl2(double2 v, int lane) {
+  return make_double2(__shfl(v.x, lane, 64), __shfl(v.y, lane, 64));
+}
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+__global__ void multishift_gmres_kernel(const double2* __restrict__ H, int j, int m_max,
+                                        const double2* __restrict__ shifts, int n_shifts,
+                                        const double2* __restrict__ sink_v, double beta, double* __restrict__ rot_c,
+                                        double2* __restrict__ rot_s, double2* __restrict__ pi,
+                                        double2* __restrict__ gamma, double2* __restrict__ green,
+                                        double* __restrict__ resid) {
+  const int lane = threadIdx.x & 63;
+  const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (s >= n_shifts) return;  // whole wave exits together
   const double2 z = shifts[s];
   const double2* col = H + (int64_t)j * (m_max + 1);
-  double2 x = make_double2(-col[0].x, -col[0].y);
-  if (j == 0) {
-    x.x += z.x;
-    x.y += z.y;
-  }
-  for (int k = 0; k < j; ++k) {
-    double2 y = make_double2(-col[k + 1].x, -col[k + 1].y);
-    if (k + 1 == j) {
-      y.x += z.x;
-      y.y += z.y;
+  const int64_t row = (int64_t)s * m_max;
+  // running x at the start of the current block, and sum_k R[k, j] pi_k
+  double2 x = make_double2(-col[0].x + (j == 0 ? z.x : 0.0), -col[0].y + (j == 0 ? z.y : 0.0));
+  double2 acc = make_double2(0.0, 0.0);
+  for (int base = 0; base < j; base += 64) {
+    const int k = base + lane;
+    double2 A = make_double2(1.0, 0.0), B = make_double2(0.0, 0.0);  // identity map
+    double c = 1.0;
+    double2 sn = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0), pk = make_double2(0.0, 0.0);
+    if (k < j) {
+      c = rot_c[row + k];
+      sn = rot_s[row + k];
+      pk = pi[row + k];
+      a1 = make_double2(-col[k + 1].x + (k + 1 == j ? z.x : 0.0), -col[k + 1].y + (k + 1 == j ? z.y : 0.0));
+      A = make_double2(-sn.x, sn.y);  // -conj(s)
+      B = make_double2(c * a1.x, c * a1.y);
     }
-    const double c = rot_c[(int64_t)k * n_shifts + s];
-    const double2 sn = rot_s[(int64_t)k * n_shifts + s];
-    // x_new (discarded) = c x + sn y ; y_new = -conj(sn) x + c y
-    const double2 cs = make_double2(sn.x, -sn.y);
-    const double2 t = zmul(cs, x);
-    x = make_double2(c * y.x - t.x, c * y.y - t.y);
+    // inclusive scan: lane l holds map_l o ... o map_base
+    for (int off = 1; off < 64; off <<= 1) {
+      const double2 Ap = shfl_up2(A, off);
+      const double2 Bp = shfl_up2(B, off);
+      if (lane >= off) {
+        B = zadd(zmul(A, Bp), B);
+        A = zmul(A, Ap);
+      }
+    }
+    // exclusive prefix for this lane: previous lane's inclusive map (identity for lane 0)
+    double2 Ae = shfl_up2(A, 1), Be = shfl_up2(B, 1);
+    if (lane == 0) {
+      Ae = make_double2(1.0, 0.0);
+      Be = make_double2(0.0, 0.0);
+    }
+    const double2 xk = zadd(zmul(Ae, x), Be);
+    if (k < j) {
+      const double2 Rkj = zadd(make_double2(c * xk.x, c * xk.y), zmul(sn, a1));
+      acc = zadd(acc, zmul(Rkj, pk));
+    }
+    // advance x past the whole block (last active lane's inclusive map)
+    const int last = (j - base) < 64 ? (j - base - 1) : 63;
+    const double2 Al = shfl2(A, last), Bl = shfl2(B, last);
+    x = zadd(zmul(Al, x), Bl);
   }
-  const double2 b = make_double2(-col[j + 1].x, -col[j + 1].y);
-  const double ax = hypot(x.x, x.y);
-  const double bx = hypot(b.x, b.y);
-  double c;
-  double2 sn;
-  if (ax == 0.0) {
-    c = 0.0;
-    sn = make_double2(1.0, 0.0);
-  } else {
-    const double nrm = hypot(ax, bx);
-    c = ax / nrm;
-    // sn = (x/|x|) conj(b) / nrm
-    const double2 u = make_double2(x.x / ax, x.y / ax);
-    sn = zmul(u, make_double2(b.x / nrm, -b.y / nrm));
+  for (int off = 32; off > 0; off >>= 1) {
+    acc.x += __shfl_down(acc.x, off, 64);
+    acc.y += __shfl_down(acc.y, off, 64);
   }
-  rot_c[(int64_t)j * n_shifts + s] = c;
-  rot_s[(int64_t)j * n_shifts + s] = sn;
-  const double r = rho[s] * hypot(sn.x, sn.y);
-  rho[s] = r;
-  resid[s] = c > 0.0 ? r / c : INFINITY;
+  if (lane == 0) {
+    const double2 b = make_double2(-col[j + 1].x, -col[j + 1].y);
+    const double ax = hypot(x.x, x.y);
+    const double bx = hypot(b.x, b.y);
+    double c;
+    double2 sn, r;
+    if (ax == 0.0) {
+      c = 0.0;
+      sn = make_double2(1.0, 0.0);
+      r = b;
+    } else {
+      const double nrm = hypot(ax, bx);
+      const double2 u = make_double2(x.x / ax, x.y / ax);
+      c = ax / nrm;
+      sn = zmul(u, make_double2(b.x / nrm, -b.y / nrm));
+      r = make_double2(u.x * nrm, u.y * nrm);
+    }
+    rot_c[row + j] = c;
+    rot_s[row + j] = sn;
+    // pi_j = (sink.v_j - acc) / R[j, j]
+    const double2 num = make_double2(sink_v[0].x - acc.x, sink_v[0].y - acc.y);
+    const double den = r.x * r.x + r.y * r.y;
+    const double2 p = den > 0.0 ? make_double2((num.x * r.x + num.y * r.y) / den, (num.y * r.x - num.x * r.y) / den)
+                                : make_double2(0.0, 0.0);
+    pi[row + j] = p;
+    const double2 g = gamma[s];  // residual entry gamma~ before this rotation
+    const double2 gj = make_double2(c * g.x, c * g.y);
+    const double2 gn = zmul(make_double2(-sn.x, sn.y), g);
+    gamma[s] = gn;
+    green[s] = zadd(green[s], zmul(gj, p));
+    resid[s] = hypot(gn.x, gn.y) / beta;
+  }
+}
+
+__global__ void init_shift_state_kernel(double2* __restrict__ gamma, double2* __restrict__ green, double beta,
+                                        int n_shifts) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_shifts; s += gridDim.x * blockDim.x) {
+    gamma[s] = make_double2(beta, 0.0);
+    green[s] = make_double2(0.0, 0.0);
+  }
 }
 
 __global__ void fill_kernel(double* __restrict__ a, double v, int64_t n) {
@@ -301,12 +372,14 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
 }
 
 int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
-                const double* shifts, int32_t n_shifts, double* rot_c, double* rot_s, double* rho, double* resid,
-                double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
-                double breakdown_tol, int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream) {
+                const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
+                double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
+                double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
+                int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream) {
   if (plan == nullptr || V == nullptr || H == nullptr || dim <= 0 || m_max <= 0 || n_shifts <= 0 || shifts == nullptr ||
-      rot_c == nullptr || rot_s == nullptr || rho == nullptr || resid == nullptr || scratch == nullptr ||
-      coef == nullptr || status == nullptr || m_out == nullptr || breakdown_out == nullptr || worst_out == nullptr)
+      sink == nullptr || !(beta > 0.0) || rot_c == nullptr || rot_s == nullptr || pi == nullptr || gamma == nullptr ||
+      green == nullptr || resid == nullptr || scratch == nullptr || coef == nullptr || status == nullptr ||
+      m_out == nullptr || breakdown_out == nullptr || worst_out == nullptr)
     return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   double2* Vz = reinterpret_cast<double2*>(V);
@@ -314,30 +387,39 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
   double2* cz = reinterpret_cast<double2*>(coef);
   double2* w = reinterpret_cast<double2*>(scratch);  // first dim complex entries of scratch hold w
   double* red = scratch + 2 * dim;                   // reduction partials after w
-  hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s, rho, 1.0, n_shifts);
+  double* c1 = coef;                                  // complex [m_max+1]
+  double* c2 = coef + 2 * (m_max + 1);                // complex [m_max+1]
+  double* sq = coef + 4 * (m_max + 1);                // complex [1]
+  double* sink_v = coef + 4 * (m_max + 1) + 2;        // complex [1]
+  hipLaunchKernelGGL(init_shift_state_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s,
+                     reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), beta, n_shifts);
   D_CHECK(hipGetLastError());
   double host[2] = {0.0, 0.0};
   int32_t m = 0;
   int32_t broke = 0;
   double worst = INFINITY;
+  const int waves_per_block = 4;
+  const int64_t shift_blocks = (n_shifts + waves_per_block - 1) / waves_per_block;
   for (int32_t j = 0; j < m_max; ++j) {
     double* vj = V + (int64_t)2 * j * dim;
-    D_CALL(c3l_plan_load(plan, 2, vj, stream));
-    D_CALL(c3l_plan_apply(plan, 3, diag, vj, reinterpret_cast<double*>(w), stream));
     double* wd = reinterpret_cast<double*>(w);
-    D_CALL(c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, coef, stream));
-    D_CALL(c3l_zmultiaxpy(V, dim, j + 1, coef, -1.0, wd, dim, stream));
-    D_CALL(c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, coef + 2 * (m_max + 1), stream));
-    D_CALL(c3l_zmultiaxpy(V, dim, j + 1, coef + 2 * (m_max + 1), -1.0, wd, dim, stream));
-    D_CALL(c3l_zmultidot(wd, dim, 1, wd, dim, 1, red, coef + 4 * (m_max + 1), stream));
+    D_CALL(c3l_plan_load(plan, 2, vj, stream));
+    D_CALL(c3l_plan_apply(plan, 3, diag, vj, wd, stream));
+    D_CALL(c3l_zmultidot(sink, dim, 1, vj, dim, 0, red, sink_v, stream));
+    D_CALL(c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c1, stream));
+    D_CALL(c3l_zmultiaxpy(V, dim, j + 1, c1, -1.0, wd, dim, stream));
+    D_CALL(c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c2, stream));
+    D_CALL(c3l_zmultiaxpy(V, dim, j + 1, c2, -1.0, wd, dim, stream));
+    D_CALL(c3l_zmultidot(wd, dim, 1, wd, dim, 1, red, sq, stream));
     hipLaunchKernelGGL(finish_column_kernel, dim3(1), dim3(256), 0, s, cz, j, m_max, Hz, status);
     D_CHECK(hipGetLastError());
     hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(dim)), dim3(256), 0, s, w, Vz + (int64_t)(j + 1) * dim, status,
                        breakdown_tol, dim);
     D_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(shifted_givens_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s, Hz, j, m_max,
-                       reinterpret_cast<const double2*>(shifts), n_shifts, rot_c, reinterpret_cast<double2*>(rot_s), rho,
-                       resid);
+    hipLaunchKernelGGL(multishift_gmres_kernel, dim3(shift_blocks), dim3(64 * waves_per_block), 0, s, Hz, j, m_max,
+                       reinterpret_cast<const double2*>(shifts), n_shifts, reinterpret_cast<const double2*>(sink_v), beta,
+                       rot_c, reinterpret_cast<double2*>(rot_s), reinterpret_cast<double2*>(pi),
+                       reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), resid);
     D_CHECK(hipGetLastError());
     hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, s, resid, n_shifts, status + 1);
     D_CHECK(hipGetLastError());

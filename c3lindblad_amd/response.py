@@ -1,23 +1,18 @@
-"""Shift-invariant Arnoldi resolvent and retarded Green functions on the GPU.
+"""Shift-invariant Krylov resolvent and retarded Green functions on the GPU.
 
 Drop-in for ``component_retarded_green`` (``src/c3l/cc/component_response.py:180-245``)
 and ``arnoldi_resolvent`` (``src/c3l/green/krylov.py:284-417``).
 
-One Krylov basis per source serves every frequency:
-``(z - σ)^{-1} b ≈ V_m (z - H_m)^{-1} β e_1`` with the exact Arnoldi-relation
-residual ``|h_{m+1,m} e_m^T (z - H_m)^{-1} β e_1| / β`` for every shift
-(``krylov.py:257-281``).  The basis lives in HBM (``V[k]`` contiguous); every
-step is one sigma matvec plus two passes of classical Gram--Schmidt done by
-the multi-dot/multi-axpy kernels, with one ``j+2``-number transfer to the host
-for the Hessenberg column.  Convergence is tested on a geometric schedule of
-dimensions; the test uses ``|e_m^T (z-H)^{-1} e_1| = prod h_{k+1,k} / |det(z-H)|``
-(eigenvalues only).  The final projection uses the Schur form
-``H = Q T Q^H`` and the batched per-shift triangular-solve kernel, which also
-returns the exact residual of every shift.
-
-The reference's per-frequency GMRES loop (``component_response.py:180-245``)
-solves the same linear systems; at convergence both give ``G^R`` to the
-stated tolerance.
+One Arnoldi basis per source serves every frequency, because the Krylov space
+of ``(z - σ)`` does not depend on ``z``.  The native driver (``c3l_arnoldi``)
+builds the basis with two-pass classical Gram--Schmidt and, for every shift at
+once, runs GMRES on it: a Givens QR of ``(z I - H)`` advanced one column per
+step and the sink projection ``G(z) = sink·x_m(z)`` updated incrementally.
+Every shift's true relative residual ``||b - (z - σ) x_m(z)|| / ||b||`` is
+exact at every step, so the basis stops growing exactly when the worst shift
+meets the tolerance.  The reference solves the same systems with one restarted
+GMRES per frequency (``component_response.py:72-128``); both converge to the
+same ``G^R`` within the stated tolerance.
 """
 
 from __future__ import annotations
@@ -31,20 +26,25 @@ import numpy as np
 import scipy.linalg
 
 from . import _lib
-from .device import Reducer, rscale
+from .device import Reducer
 from .eom import ComponentEOMSources, ComponentSigmaOperator
 
 
 @dataclass
 class DeviceArnoldi:
-    """Krylov projection ``σ V_m = V_{m+1} H`` with the basis kept on the GPU."""
+    """Krylov projection ``σ V_m = V_{m+1} H`` with the basis kept on the GPU.
+
+    ``green``/``relative_residuals`` hold the multi-shift GMRES results for
+    the sink and shifts the basis was built with.
+    """
 
     basis: object  # torch [m+1, P] complex128 (rows are basis vectors)
     hessenberg: np.ndarray  # (m+1, m)
     source_norm: float
     breakdown: bool
-    check_dimensions: tuple
-    check_maximum_relative_residuals: tuple
+    shifts: np.ndarray
+    green: np.ndarray
+    relative_residuals: np.ndarray
     converged: bool
     construction_wall_seconds: float
     action_calls: int
@@ -54,22 +54,19 @@ class DeviceArnoldi:
         return int(self.hessenberg.shape[1])
 
     def project(self, sink, shifts):
-        """Return ``(green[s], relative_residual[s])`` for all shifts via the Schur form."""
+        """``sink·V_m (z - H_m)^{-1} β e_1`` for other sinks/shifts via the Schur form (host O(m^3))."""
 
         torch = _lib.require_device()
         m = self.dimension
-        H = self.hessenberg[:m, :m]
-        T, Q = scipy.linalg.schur(H, output="complex")
+        T, Q = scipy.linalg.schur(self.hessenberg[:m, :m], output="complex")
         r = Q.conj().T[:, 0] * self.source_norm
-        P = self.basis.shape[1]
-        red = Reducer(P, m, complex_=True)
+        red = Reducer(self.basis.shape[1], m, complex_=True)
         sink_d = torch.from_numpy(np.ascontiguousarray(sink, dtype=np.complex128)).to(_lib.DEVICE)
         sv = torch.empty(m, dtype=torch.complex128, device=_lib.DEVICE)
         red.dot(self.basis, m, sink_d, sv, conjugate=False)
         w = sv.cpu().numpy() @ Q
-        q = Q[m - 1, :]
         z = np.ascontiguousarray(np.asarray(shifts, dtype=np.complex128))
-        green, last = shifted_green(T, r, w, q, z)
+        green, last = shifted_green(T, r, w, Q[m - 1, :], z)
         residual = np.abs(self.hessenberg[m, m - 1] * last) / max(self.source_norm, np.finfo(float).tiny)
         return green, residual
 
@@ -79,6 +76,7 @@ def shifted_green(T, r, w, q, z):
 
     torch = _lib.require_device()
     m = T.shape[0]
+
     def dev(a):
         return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).to(_lib.DEVICE)
 
@@ -106,12 +104,13 @@ def arnoldi_resolvent(
     minimum_dimension: int = 16,
     check_interval: int = 8,
     breakdown_tolerance: float = 1.0e-14,
+    sink=None,
 ) -> DeviceArnoldi:
     """Build one shift-invariant Krylov space for ``(zI - σ)^{-1} b`` with the native driver.
 
-    The exact FOM residual of every shift is tracked on the device at each
-    step, so ``check_interval`` only bounds how late a converged basis may
-    stop (the driver tests every step).
+    Every shift's exact residual is tested at every step (``check_interval``
+    is accepted for signature compatibility).  With ``sink`` the returned
+    ``green`` holds ``sink·x(z)`` for every shift.
     """
 
     torch = _lib.require_device()
@@ -126,6 +125,9 @@ def arnoldi_resolvent(
         raise ValueError("maximum_dimension must lie in [1, operator.dimension]")
     if not (np.isfinite(residual_tolerance) and residual_tolerance > 0.0):
         raise ValueError("residual_tolerance must be finite and positive")
+    sink_h = np.zeros(n, dtype=np.complex128) if sink is None else np.asarray(sink, dtype=np.complex128)
+    if sink_h.shape != (n,) or not np.all(np.isfinite(sink_h)):
+        raise ValueError("sink must be a finite vector matching the operator")
     beta = float(np.linalg.norm(b))
     if beta == 0.0:
         raise ValueError("rhs must be nonzero")
@@ -133,40 +135,46 @@ def arnoldi_resolvent(
     m_max = maximum_dimension
     dev = _lib.DEVICE
     c128, f64 = torch.complex128, torch.float64
+    ns = z.size
     V = torch.zeros(m_max + 1, n, dtype=c128, device=dev)
     V[0].copy_(torch.from_numpy(b / beta))
     H = torch.zeros(m_max, m_max + 1, dtype=c128, device=dev)  # row j = column j of H
     zs = torch.from_numpy(z).to(dev)
-    rot_c = torch.zeros(m_max * z.size, dtype=f64, device=dev)
-    rot_s = torch.zeros(m_max * z.size, dtype=c128, device=dev)
-    rho = torch.zeros(z.size, dtype=f64, device=dev)
-    resid = torch.zeros(z.size, dtype=f64, device=dev)
+    sink_d = torch.from_numpy(np.ascontiguousarray(sink_h)).to(dev)
+    rot_c = torch.zeros(ns * m_max, dtype=f64, device=dev)
+    rot_s = torch.zeros(ns * m_max, dtype=c128, device=dev)
+    pi = torch.zeros(ns * m_max, dtype=c128, device=dev)
+    gamma = torch.zeros(ns, dtype=c128, device=dev)
+    green = torch.zeros(ns, dtype=c128, device=dev)
+    resid = torch.zeros(ns, dtype=f64, device=dev)
     chunks = int(_lib.load_library().c3l_reduction_chunks(n))
     scratch = torch.zeros(2 * n + 2 * chunks * (m_max + 1), dtype=f64, device=dev)
-    coef = torch.zeros(2 * (m_max + 1) + 1, dtype=c128, device=dev)
+    coef = torch.zeros(2 * (m_max + 1) + 2, dtype=c128, device=dev)
     status = torch.zeros(2, dtype=f64, device=dev)
     m_out, broke = ctypes.c_int32(), ctypes.c_int32()
     worst = ctypes.c_double()
-    plan = operator.context.device
-    if plan.plan.loaded.get(0) != plan.token:
-        plan.token = plan.plan.load("T", plan.t)
+    state = operator.context.device
+    if state.plan.loaded.get(0) != state.token:
+        state.token = state.plan.load("T", state.t)
     _lib.call(
-        "c3l_arnoldi", plan.plan.handle, _lib.dptr(plan.diag), _lib.dptr(V), _lib.dptr(H), n, m_max, _lib.dptr(zs), z.size,
-        _lib.dptr(rot_c), _lib.dptr(rot_s), _lib.dptr(rho), _lib.dptr(resid), _lib.dptr(scratch), _lib.dptr(coef),
-        _lib.dptr(status), float(residual_tolerance), int(minimum_dimension), float(breakdown_tolerance),
-        ctypes.byref(m_out), ctypes.byref(broke), ctypes.byref(worst), _lib.stream_ptr(torch),
+        "c3l_arnoldi", state.plan.handle, _lib.dptr(state.diag), _lib.dptr(V), _lib.dptr(H), n, m_max, _lib.dptr(zs), ns,
+        _lib.dptr(sink_d), beta, _lib.dptr(rot_c), _lib.dptr(rot_s), _lib.dptr(pi), _lib.dptr(gamma), _lib.dptr(green),
+        _lib.dptr(resid), _lib.dptr(scratch), _lib.dptr(coef), _lib.dptr(status), float(residual_tolerance),
+        int(minimum_dimension), float(breakdown_tolerance), ctypes.byref(m_out), ctypes.byref(broke), ctypes.byref(worst),
+        _lib.stream_ptr(torch),
     )
     m = m_out.value
     operator.action_calls += m
-    hess = H[:m, : m + 1].cpu().numpy().T.copy()
+    residuals = resid.cpu().numpy()
     return DeviceArnoldi(
         basis=V[: m + 1],
-        hessenberg=hess,
+        hessenberg=H[:m, : m + 1].cpu().numpy().T.copy(),
         source_norm=beta,
         breakdown=bool(broke.value),
-        check_dimensions=(m,),
-        check_maximum_relative_residuals=(float(worst.value),),
-        converged=bool(broke.value or worst.value <= residual_tolerance),
+        shifts=z,
+        green=green.cpu().numpy(),
+        relative_residuals=residuals,
+        converged=bool(broke.value or np.max(residuals) <= residual_tolerance),
         construction_wall_seconds=perf_counter() - started,
         action_calls=m,
     )
@@ -179,8 +187,8 @@ class ComponentRetardedGreen:
     addition: np.ndarray
     removal: np.ndarray
     retarded: np.ndarray
-    addition_projection: DeviceArnoldi
-    removal_projection: DeviceArnoldi
+    addition_projection: DeviceArnoldi | None
+    removal_projection: DeviceArnoldi | None
     addition_relative_residuals: np.ndarray
     removal_relative_residuals: np.ndarray
     wall_seconds: float
@@ -203,7 +211,8 @@ def component_retarded_green(
     ``restart``/``maxiter`` are accepted for signature compatibility; the
     Krylov dimension is capped by ``maximum_dimension`` (default
     ``min(P, restart*maxiter, 2048)``).  Raises ``RuntimeError`` when a
-    shift misses ``max(10 rtol, 1e-12)`` like the reference's shifted solves.
+    shift misses ``max(10 rtol, 1e-12)`` like the reference's shifted solves
+    (``component_response.py:227-228``).
     """
 
     started = perf_counter()
@@ -216,13 +225,13 @@ def component_retarded_green(
         if not np.any(src):
             out[name] = (np.zeros(z.size, dtype=np.complex128), None, np.zeros(z.size))
             continue
-        proj = arnoldi_resolvent(operator, src, z, maximum_dimension=cap, residual_tolerance=rtol)
-        green, residual = proj.project(sources.sink, z)
-        if np.max(residual) > limit:
+        proj = arnoldi_resolvent(operator, src, z, maximum_dimension=cap, residual_tolerance=rtol, sink=sources.sink)
+        if np.max(proj.relative_residuals) > limit:
             raise RuntimeError(
-                f"component EOM Arnoldi resolvent did not converge ({name}: max residual {np.max(residual):.2e} at m={proj.dimension})"
+                f"component EOM resolvent did not converge ({name}: max residual "
+                f"{np.max(proj.relative_residuals):.2e} at m={proj.dimension})"
             )
-        out[name] = (green, proj, residual)
+        out[name] = (proj.green, proj, proj.relative_residuals)
     addition, removal = out["addition"][0], out["removal"][0]
     return ComponentRetardedGreen(
         omega=frequency,

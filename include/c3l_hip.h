@@ -168,18 +168,21 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
               double rtol, int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations, double* true_rel,
               int32_t* jvp_calls, void* stream);
 
-/* Shift-invariant Arnoldi on the sigma program (replaces arnoldi_resolvent,
- * src/c3l/green/krylov.py:284-417, and the device loop of
- * component_retarded_matrix_green_arnoldi_jax, cc/component_jax.py:924-957).  V [(m_max+1) x dim]
- * complex with V[0] = b/beta on entry; H complex column-major (m_max+1) x m_max; rot_c
- * [m_max*n_shifts] doubles, rot_s [m_max*n_shifts] complex, rho, resid [n_shifts] doubles;
- * scratch >= 2*dim + 2*chunks*(m_max+1) doubles; coef >= 4*(m_max+1)+2 doubles; status >= 2
- * doubles.  Stops at the first m >= min_dim with every shift's relative residual <= tolerance,
- * at a breakdown, or at m_max. */
+/* Shift-invariant Krylov resolvent on the sigma program (replaces the per-frequency GMRES
+ * loop of component_retarded_green, src/c3l/cc/component_response.py:180-245, and the Arnoldi
+ * resolvent arnoldi_resolvent, src/c3l/green/krylov.py:284-417).  One Arnoldi basis (two-pass
+ * CGS) serves every shift; each shift runs GMRES on it with its sink projection
+ * green[s] = sink . x_m(z_s) updated incrementally.  V [(m_max+1) x dim] complex with
+ * V[0] = b/beta on entry; H complex, column j at H + j*(m_max+1); rot_c [n_shifts*m_max]
+ * doubles, rot_s and pi [n_shifts*m_max] complex, gamma and green [n_shifts] complex, resid
+ * [n_shifts] doubles (relative GMRES residual); scratch >= 2*dim + 2*chunks*(m_max+1) doubles;
+ * coef >= 4*(m_max+1)+4 doubles; status >= 2 doubles.  Stops at the first m >= min_dim with
+ * every shift's residual <= tolerance, at a breakdown, or at m_max. */
 int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
-                const double* shifts, int32_t n_shifts, double* rot_c, double* rot_s, double* rho, double* resid,
-                double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
-                double breakdown_tol, int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
+                const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
+                double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
+                double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
+                int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
 
 #ifdef __cplusplus
 }
