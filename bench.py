@@ -86,6 +86,31 @@ def _cpu_baseline(pipe, result, seconds_budget: float = 30.0) -> dict:
     }
 
 
+def _jvp_probe(jvps: int) -> int:
+    """Counter workload: siam24 residual, ``jvps`` JVPs, then a 256 MiB copy for calibration."""
+
+    import torch
+
+    from c3lindblad_amd.configs import config2_model
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.static import ComponentStaticProblem
+
+    p = factorized_cc_problem(config2_model())
+    dev = ComponentStaticProblem.compile(p).device
+    rng = np.random.default_rng(0)
+    x = torch.from_numpy(0.01 * rng.normal(size=p.num_variables)).cuda()
+    r = torch.empty_like(x)
+    dev.residual_real(x, r)
+    dx = torch.from_numpy(rng.normal(size=p.num_variables)).cuda()
+    for _ in range(jvps):
+        dev.jvp_real(dx, r)
+    a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    torch.empty_like(a).copy_(a)
+    torch.cuda.synchronize()
+    print("jvp probe done", flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +118,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="siam24", choices=("siam24", "readme"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--jvp-probe", type=int, default=0, metavar="K",
+                    help="profiling workload only: one residual and K JVPs of siam24, then a 256 MiB copy")
     args = ap.parse_args()
+    if args.jvp_probe:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        return _jvp_probe(args.jvp_probe)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(
