@@ -78,29 +78,33 @@ class ComponentEOMProblem:
     def dimension(self) -> int:
         return self.coordinates.dimension
 
-    def device_plan(self):
-        """The uploaded sigma plan, shared by every context of this problem."""
+    def device_plan(self, slot: int = 0):
+        """Uploaded sigma plan number ``slot``; independent slots can run concurrently."""
 
-        plan = getattr(self, "_plan", None)
-        if plan is None:
+        plans = self.__dict__.setdefault("_plans", {})
+        if slot not in plans:
             from .device import DevicePlan
 
-            plan = DevicePlan(self.lowered)
-            self._plan = plan
-        return plan
+            plans[slot] = DevicePlan(self.lowered)
+        return plans[slot]
 
 
 @dataclass
 class ComponentEOMContext:
     problem: ComponentEOMProblem
     amplitudes: np.ndarray
-    _device: "EOMDevice | None" = field(default=None, repr=False)
+    _devices: dict = field(default_factory=dict, repr=False)
 
     @property
     def device(self) -> "EOMDevice":
-        if self._device is None:
-            self._device = EOMDevice(self)
-        return self._device
+        return self.device_for(0)
+
+    def device_for(self, slot: int) -> "EOMDevice":
+        """Device state on plan slot ``slot`` (amplitude atoms loaded on the current stream)."""
+
+        if slot not in self._devices:
+            self._devices[slot] = EOMDevice(self, slot)
+        return self._devices[slot]
 
 
 def component_eom_context(problem: ComponentEOMProblem, amplitudes) -> ComponentEOMContext:
@@ -108,17 +112,17 @@ def component_eom_context(problem: ComponentEOMProblem, amplitudes) -> Component
     if t.shape != (problem.static.oracle.num_amplitudes,) or not np.all(np.isfinite(t)):
         raise ValueError("amplitudes must be finite and match the CCSD basis")
     context = ComponentEOMContext(problem, t)
-    context.device  # upload the plan and make the amplitude atoms resident
+    context.device  # make the amplitude atoms resident on plan slot 0
     return context
 
 
 class EOMDevice:
     """Device state of one context: the problem's plan with this context's amplitude atoms."""
 
-    def __init__(self, context: ComponentEOMContext):
+    def __init__(self, context: ComponentEOMContext, slot: int = 0):
         torch = _lib.require_device()
         self.torch = torch
-        self.plan = context.problem.device_plan()
+        self.plan = context.problem.device_plan(slot)
         self.t = torch.from_numpy(np.ascontiguousarray(context.amplitudes)).to(_lib.DEVICE)
         self.token = self.plan.load("T", self.t)
         self.diag = torch.from_numpy(np.ascontiguousarray(context.problem.diagonal)).to(_lib.DEVICE)
@@ -158,12 +162,22 @@ class ComponentSigmaOperator:
     applies it to device tensors without host traffic.
     """
 
-    def __init__(self, context: ComponentEOMContext):
+    def __init__(self, context: ComponentEOMContext, *, slot: int = 0):
         self.context = context
+        self.slot = slot
         self.dimension = context.problem.dimension
         self.diagonal = context.problem.diagonal.copy()
         self.action_calls = 0
         self.action_wall_seconds = 0.0
+
+    @property
+    def device(self) -> "EOMDevice":
+        return self.context.device_for(self.slot)
+
+    def on_slot(self, slot: int) -> "ComponentSigmaOperator":
+        """Same operator bound to another plan slot (for concurrent resolvents)."""
+
+        return ComponentSigmaOperator(self.context, slot=slot)
 
     def matvec(self, vector) -> np.ndarray:
         started = perf_counter()
@@ -173,7 +187,7 @@ class ComponentSigmaOperator:
         return np.asarray(result, dtype=np.complex128)
 
     def matvec_device(self, x, out) -> None:
-        self.context.device.sigma(x, out)
+        self.device.sigma(x, out)
         self.action_calls += 1
 
 

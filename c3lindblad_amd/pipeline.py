@@ -17,7 +17,7 @@ import numpy as np
 
 from .eom import ComponentEOMProblem, ComponentSigmaOperator, component_eom_context, component_eom_sources, eom_ccsd_problem
 from .lowering import factorized_cc_problem
-from .response import component_retarded_green
+from .response import retarded_green_blocks
 from .solver import NewtonKrylovPolicy, solve_component_ccsd
 from .static import ComponentStaticProblem
 
@@ -44,6 +44,7 @@ class SolvePipeline:
     setup_seconds: float
     policy: NewtonKrylovPolicy
     rtol: float = 1e-10
+    concurrent: bool = True
 
     @classmethod
     def prepare(cls, model, omega, eta=0.0, charges=(1, -1), policy=None, rtol=1e-10):
@@ -54,7 +55,8 @@ class SolvePipeline:
         blocks = {q: ComponentEOMProblem.compile(eom, static, q) for q in charges}
         static.device  # upload
         for b in blocks.values():
-            b.device_plan()
+            b.device_plan(0)
+            b.device_plan(1)
         return cls(static, blocks, np.asarray(omega, dtype=np.float64), float(eta), perf_counter() - started,
                    policy or NewtonKrylovPolicy(), rtol)
 
@@ -67,10 +69,13 @@ class SolvePipeline:
         green = {}
         dims = {}
         worst = 0.0
-        for q, block in self.eom.items():
-            ctx = component_eom_context(block, sol.amplitudes)
-            sources = component_eom_sources(ctx)
-            result = component_retarded_green(ComponentSigmaOperator(ctx), sources, self.omega, self.eta, rtol=self.rtol)
+        charges = list(self.eom)
+        pairs = []
+        for q in charges:
+            ctx = component_eom_context(self.eom[q], sol.amplitudes)
+            pairs.append((ComponentSigmaOperator(ctx), component_eom_sources(ctx)))
+        results = retarded_green_blocks(pairs, self.omega, self.eta, rtol=self.rtol, concurrent=self.concurrent)
+        for q, result in zip(charges, results):
             green[q] = result.retarded
             dims[q] = tuple(p.dimension if p is not None else 0 for p in (result.addition_projection, result.removal_projection))
             worst = max(worst, float(np.max(result.addition_relative_residuals)), float(np.max(result.removal_relative_residuals)))

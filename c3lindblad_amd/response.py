@@ -153,7 +153,7 @@ def arnoldi_resolvent(
     status = torch.zeros(2, dtype=f64, device=dev)
     m_out, broke = ctypes.c_int32(), ctypes.c_int32()
     worst = ctypes.c_double()
-    state = operator.context.device
+    state = operator.device
     if state.plan.loaded.get(0) != state.token:
         state.token = state.plan.load("T", state.t)
     _lib.call(
@@ -195,6 +195,90 @@ class ComponentRetardedGreen:
     peak_rss_kib: int
 
 
+def run_concurrently(tasks):
+    """Run independent device tasks on their own default-priority streams, one host thread each.
+
+    The native drivers release the GIL, so the tasks overlap on the GPU.  Every
+    stream first waits for the caller's stream; the caller's stream waits for
+    all of them before returning.
+    """
+
+    torch = _lib.require_device()
+    if len(tasks) <= 1 or _lib.DEVICE != "cuda":
+        return [task() for task in tasks]
+    from concurrent.futures import ThreadPoolExecutor
+
+    main = torch.cuda.current_stream()
+    streams = [torch.cuda.Stream() for _ in tasks]
+    for stream in streams:
+        stream.wait_stream(main)
+
+    def run(i):
+        with torch.cuda.stream(streams[i]):
+            return tasks[i]()
+
+    with ThreadPoolExecutor(max_workers=len(tasks)) as pool:
+        results = list(pool.map(run, range(len(tasks))))
+    for stream in streams:
+        main.wait_stream(stream)
+    return results
+
+
+def _channel(operator, source, sink, z, cap, rtol, limit, name):
+    if not np.any(source):
+        return np.zeros(z.size, dtype=np.complex128), None, np.zeros(z.size)
+    proj = arnoldi_resolvent(operator, source, z, maximum_dimension=cap, residual_tolerance=rtol, sink=sink)
+    if np.max(proj.relative_residuals) > limit:
+        raise RuntimeError(
+            f"component EOM resolvent did not converge ({name}: max residual "
+            f"{np.max(proj.relative_residuals):.2e} at m={proj.dimension})"
+        )
+    return proj.green, proj, proj.relative_residuals
+
+
+def retarded_green_blocks(pairs, omega, eta: float, *, rtol: float = 1.0e-10, maximum_dimension: int | None = None,
+                          concurrent: bool = True) -> list["ComponentRetardedGreen"]:
+    """``component_retarded_green`` for several ``(operator, sources)`` charge blocks at once.
+
+    The addition and removal resolvents of every block run concurrently (plan
+    slots 0 and 1 of each block, one stream each) when ``concurrent``.
+    """
+
+    started = perf_counter()
+    frequency = np.asarray(omega, dtype=np.float64)
+    z = frequency + 1j * float(eta)
+    limit = max(10.0 * rtol, 1.0e-12)
+    tasks = []
+    for operator, sources in pairs:
+        cap = maximum_dimension or min(operator.dimension, 2048)
+        for slot, (name, src) in enumerate((("addition", sources.addition), ("removal", sources.removal))):
+            op = operator.on_slot(slot) if concurrent else operator
+            tasks.append(lambda op=op, src=src, snk=sources.sink, cap=cap, name=name:
+                         _channel(op, src, snk, z, cap, rtol, limit, name))
+    results = run_concurrently(tasks) if concurrent else [t() for t in tasks]
+    out = []
+    for b, (operator, _) in enumerate(pairs):
+        (ga, pa, ra), (gr, pr, rr) = results[2 * b], results[2 * b + 1]
+        if concurrent:  # slot operators counted their own matvecs
+            operator.action_calls += sum(p.action_calls for p in (pa, pr) if p is not None)
+        out.append(
+            ComponentRetardedGreen(
+                omega=frequency,
+                eta=float(eta),
+                addition=ga,
+                removal=gr,
+                retarded=ga + gr,
+                addition_projection=pa,
+                removal_projection=pr,
+                addition_relative_residuals=ra,
+                removal_relative_residuals=rr,
+                wall_seconds=perf_counter() - started,
+                peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
+            )
+        )
+    return out
+
+
 def component_retarded_green(
     operator: ComponentSigmaOperator,
     sources: ComponentEOMSources,
@@ -205,6 +289,7 @@ def component_retarded_green(
     restart: int = 40,
     maxiter: int = 200,
     maximum_dimension: int | None = None,
+    concurrent: bool = True,
 ) -> ComponentRetardedGreen:
     """Addition + removal contributions of ``G^R(ω)`` for one charge block.
 
@@ -215,34 +300,6 @@ def component_retarded_green(
     (``component_response.py:227-228``).
     """
 
-    started = perf_counter()
-    frequency = np.asarray(omega, dtype=np.float64)
-    z = frequency + 1j * float(eta)
     cap = maximum_dimension or min(operator.dimension, restart * maxiter, 2048)
-    limit = max(10.0 * rtol, 1.0e-12)
-    out = {}
-    for name, src in (("addition", sources.addition), ("removal", sources.removal)):
-        if not np.any(src):
-            out[name] = (np.zeros(z.size, dtype=np.complex128), None, np.zeros(z.size))
-            continue
-        proj = arnoldi_resolvent(operator, src, z, maximum_dimension=cap, residual_tolerance=rtol, sink=sources.sink)
-        if np.max(proj.relative_residuals) > limit:
-            raise RuntimeError(
-                f"component EOM resolvent did not converge ({name}: max residual "
-                f"{np.max(proj.relative_residuals):.2e} at m={proj.dimension})"
-            )
-        out[name] = (proj.green, proj, proj.relative_residuals)
-    addition, removal = out["addition"][0], out["removal"][0]
-    return ComponentRetardedGreen(
-        omega=frequency,
-        eta=float(eta),
-        addition=addition,
-        removal=removal,
-        retarded=addition + removal,
-        addition_projection=out["addition"][1],
-        removal_projection=out["removal"][1],
-        addition_relative_residuals=out["addition"][2],
-        removal_relative_residuals=out["removal"][2],
-        wall_seconds=perf_counter() - started,
-        peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
-    )
+    return retarded_green_blocks([(operator, sources)], omega, eta, rtol=rtol, maximum_dimension=cap,
+                                 concurrent=concurrent)[0]
