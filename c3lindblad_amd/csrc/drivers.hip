@@ -4,8 +4,9 @@
 //
 // GMRES: left-preconditioned restarted GMRES with two-pass classical
 // Gram-Schmidt on the device; the (j+2)-number Hessenberg column is copied to
-// the host once per iteration for the Givens update (SciPy semantics: inner
-// tolerance ||M b|| * rtol, true residual test at every restart).
+// the host once per iteration for the Givens update (SciPy's tolerance control:
+// target max(atol, rtol ||b||) on the true residual, checked after every restart,
+// with an adaptively rescaled inner tolerance on the preconditioned residual).
 //
 // Arnoldi: two-pass CGS on the device; the Hessenberg matrix stays on the
 // device; every shift z runs GMRES on the shared basis: its Givens QR of
@@ -17,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -259,11 +261,11 @@ extern "C" {
 
 int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
               const double* b, double* x, double* V, double* w, double* tmp, double* scratch, double* coef,
-              double rtol, int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations, double* true_rel,
-              int32_t* jvp_calls, void* stream) {
+              double rtol, double atol_in, int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations,
+              double* true_rel, int32_t* jvp_calls, void* stream) {
   if (plan == nullptr || tm == nullptr || b == nullptr || x == nullptr || V == nullptr || w == nullptr ||
       tmp == nullptr || scratch == nullptr || coef == nullptr || restart <= 0 || cycles <= 0 || tm->real_dim <= 0 ||
-      !(rtol > 0.0) || info == nullptr || iterations == nullptr || true_rel == nullptr)
+      !(rtol > 0.0) || !(atol_in >= 0.0) || info == nullptr || iterations == nullptr || true_rel == nullptr)
     return dfail(C3L_EINVAL, "c3l_gmres: invalid argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int64_t rd = tm->real_dim;
@@ -282,56 +284,64 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   D_CALL(g.precondition(b, tmp));
   double mbnrm = 0.0;
   D_CALL(g.norm(tmp, &mbnrm));
-  const double ptol = mbnrm * rtol;
-  const double atol = rtol * bnrm;
+  // SciPy's tolerance control: the outer target is max(atol, rtol ||b||) on the true
+  // residual; the inner test acts on the preconditioned residual with a tolerance
+  // rescaled after every restart (tightened when the inner test passed but the true
+  // residual did not, relaxed when the cycle ran out).
+  const double eps = 2.220446049250313e-16;
+  const double atol = std::max(atol_in, rtol * bnrm);
+  double ptol_factor = 1.0;
+  double ptol = mbnrm * std::min(ptol_factor, atol / bnrm);
+  if (bnrm < atol) {  // x = 0 already meets an absolute target
+    D_CHECK(hipMemcpyAsync(w, b, sizeof(double) * rd, hipMemcpyDeviceToDevice, s));
+    *info = 0;
+    *true_rel = 1.0;
+    return C3L_OK;
+  }
   const int R = restart;
-  std::vector<double> H((size_t)(R + 1) * R), cs(R), sn(R), S(R + 1), y(R), col(2 * (R + 1) + 1);
+  std::vector<double> H((size_t)(R + 1) * R), cs(R), sn(R), S(R + 1), y(R), col(2 * (R + 1) + 2);
   auto h = [&](int i, int j) -> double& { return H[(size_t)i * R + j]; };
+  D_CHECK(hipMemcpyAsync(w, b, sizeof(double) * rd, hipMemcpyDeviceToDevice, s));  // r = b - A 0
+  double rn = bnrm;
   for (int32_t cycle = 0; cycle < cycles; ++cycle) {
-    if (cycle == 0) {
-      D_CHECK(hipMemcpyAsync(w, b, sizeof(double) * rd, hipMemcpyDeviceToDevice, s));
-    } else {
-      D_CALL(g.apply(x, w));
-      D_CALL(c3l_raxpby(1.0, b, -1.0, w, rd, stream));
-      double rn = 0.0;
-      D_CALL(g.norm(w, &rn));
-      if (rn < atol) {
-        *info = 0;
-        break;
-      }
-    }
     D_CALL(g.precondition(w, V));
     double beta = 0.0;
     D_CALL(g.norm(V, &beta));
-    if (beta == 0.0) {
-      *info = 0;
-      break;
-    }
+    if (!(beta > 0.0) || !std::isfinite(beta)) return dfail(C3L_EINVAL, "c3l_gmres: singular preconditioned residual");
     D_CALL(c3l_rscale(V, V, 1.0 / beta, rd, stream));
     std::fill(H.begin(), H.end(), 0.0);
     std::fill(S.begin(), S.end(), 0.0);
     S[0] = beta;
     int used = 0;
-    bool done = false;
+    bool breakdown = false;
+    double presid = 0.0;
     for (int j = 0; j < R; ++j) {
       D_CALL(g.apply(V + (int64_t)j * rd, tmp));
       D_CALL(g.precondition(tmp, w));
       double* c1 = coef;
       double* c2 = coef + (R + 1);
       double* sq = coef + 2 * (R + 1);
+      D_CALL(c3l_rmultidot(w, rd, 1, w, rd, scratch, sq + 1, stream));  // ||w|| before orthogonalisation
       D_CALL(c3l_rmultidot(V, rd, j + 1, w, rd, scratch, c1, stream));
       D_CALL(c3l_rmultiaxpy(V, rd, j + 1, c1, -1.0, w, rd, stream));
       D_CALL(c3l_rmultidot(V, rd, j + 1, w, rd, scratch, c2, stream));
       D_CALL(c3l_rmultiaxpy(V, rd, j + 1, c2, -1.0, w, rd, stream));
       D_CALL(c3l_rmultidot(w, rd, 1, w, rd, scratch, sq, stream));
-      D_CHECK(hipMemcpyAsync(col.data(), coef, sizeof(double) * (2 * (R + 1) + 1), hipMemcpyDeviceToHost, s));
+      D_CHECK(hipMemcpyAsync(col.data(), coef, sizeof(double) * (2 * (R + 1) + 2), hipMemcpyDeviceToHost, s));
       D_CHECK(hipStreamSynchronize(s));
       for (int k = 0; k <= j; ++k) h(k, j) = col[k] + col[R + 1 + k];
       const double sqv = col[2 * (R + 1)];
+      const double h0v = col[2 * (R + 1) + 1];
       const double hn = std::sqrt(sqv > 0.0 ? sqv : 0.0);
+      const double h0 = std::sqrt(h0v > 0.0 ? h0v : 0.0);
       if (!std::isfinite(hn)) return dfail(C3L_EINVAL, "c3l_gmres: nonfinite Krylov vector");
-      h(j + 1, j) = hn;
-      if (hn > 0.0) D_CALL(c3l_rscale(w, V + (int64_t)(j + 1) * rd, 1.0 / hn, rd, stream));
+      if (hn <= eps * h0) {  // exact solution indicator
+        h(j + 1, j) = 0.0;
+        breakdown = true;
+      } else {
+        h(j + 1, j) = hn;
+        D_CALL(c3l_rscale(w, V + (int64_t)(j + 1) * rd, 1.0 / hn, rd, stream));
+      }
       for (int k = 0; k < j; ++k) {
         const double a = h(k, j), bb = h(k + 1, j);
         h(k, j) = cs[k] * a + sn[k] * bb;
@@ -345,29 +355,28 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
       h(j + 1, j) = 0.0;
       S[j + 1] = -sn[j] * S[j];
       S[j] = cs[j] * S[j];
+      presid = std::fabs(S[j + 1]);
       ++*iterations;
       used = j + 1;
-      if (std::fabs(S[j + 1]) <= ptol || hn == 0.0) {
-        done = true;
-        break;
-      }
+      if (presid <= ptol || breakdown) break;
     }
+    if (h(used - 1, used - 1) == 0.0) S[used - 1] = 0.0;
     for (int k = used - 1; k >= 0; --k) {
       double acc = S[k];
       for (int l = k + 1; l < used; ++l) acc -= h(k, l) * y[l];
-      y[k] = acc / h(k, k);
+      y[k] = h(k, k) == 0.0 ? 0.0 : acc / h(k, k);
     }
     D_CHECK(hipMemcpyAsync(coef, y.data(), sizeof(double) * used, hipMemcpyHostToDevice, s));
     D_CALL(c3l_rmultiaxpy(V, rd, used, coef, 1.0, x, rd, stream));
-    D_CHECK(hipStreamSynchronize(s));  // y is a host stack buffer reused by the next cycle
-    (void)done;
+    D_CALL(g.apply(x, w));
+    D_CALL(c3l_raxpby(1.0, b, -1.0, w, rd, stream));
+    D_CALL(g.norm(w, &rn));  // synchronises: y may be reused by the next cycle
+    if (rn <= atol || breakdown) break;
+    ptol_factor = presid <= ptol ? std::max(eps, 0.25 * ptol_factor) : std::min(1.0, 1.5 * ptol_factor);
+    ptol = presid * std::min(ptol_factor, atol / rn);
   }
-  D_CALL(g.apply(x, w));
-  D_CALL(c3l_raxpby(1.0, b, -1.0, w, rd, stream));
-  double rn = 0.0;
-  D_CALL(g.norm(w, &rn));
   *true_rel = rn / bnrm;
-  if (*true_rel <= rtol * (1.0 + 1e-8)) *info = 0;
+  *info = rn <= atol ? 0 : cycles;
   return C3L_OK;
 }
 

@@ -4,11 +4,12 @@ Drop-in for ``solve_component_ccsd`` (``src/c3l/cc/component_solver.py:103-295``
 with ``NewtonKrylovPolicy`` (``cc/newton_krylov.py:35-123``).  Same control
 flow: real tilde coordinates, rapidity (diagonal) preconditioner with the
 1e-12 regularisation (``newton_krylov.py:158-192``), restarted GMRES with the
-legacy or Eisenstat--Walker forcing term (``newton_krylov.py:209-237``),
-Armijo backtracking (``component_solver.py:224-256``), final packed residual
-check.  GMRES follows SciPy's left-preconditioned restarted scheme
-(preconditioned inner tolerance ``||M b|| * rtol`` and a true-residual test
-at every restart) with two-pass classical Gram--Schmidt on the device; only
+legacy or Eisenstat--Walker forcing term (``newton_krylov.py:209-237``) and,
+for the latter, the absolute GMRES floor and the inexact-Newton acceptance
+test (``component_solver.py:77-100,174-220``), Armijo backtracking (``component_solver.py:224-256``), final packed residual
+check.  GMRES follows SciPy's left-preconditioned restarted scheme (target
+``max(atol, rtol ||b||)`` on the true residual after every restart, adaptive
+inner tolerance on the preconditioned residual) with two-pass classical Gram--Schmidt on the device; only
 the Hessenberg column (``j+2`` numbers) crosses to the host per iteration for
 the Givens update.
 """
@@ -152,8 +153,26 @@ class _DeviceNewton:
     def precondition(self, v, out) -> None:
         self.dev.tilde.precondition(v, out, self.inv_diag)
 
-    def gmres(self, b, x, rtol: float, restart: int, cycles: int):
-        """Left-preconditioned restarted GMRES for ``J x = b`` in the native driver (``c3l_gmres``)."""
+    def acceptable_step(self, norm: float, true_rel: float, rtol: float, atol: float) -> bool:
+        """Inexact-Newton acceptance (``_acceptable_inexact_newton_step``, ``cc/component_solver.py:77-100``).
+
+        The linear defect ``J s + r`` is ``-w`` (GMRES leaves ``w = b - J s`` with ``b = -r``), so
+        ``||J s + r|| = true_rel * ||r||`` and the merit slope ``r . J s = -||r||^2 - r . w``.
+        """
+
+        out = self.coef[:1]
+        self.reducer.dot(self.r.view(1, -1), 1, self.w, out)
+        slope = -norm * norm - float(out.item())
+        defect = true_rel * norm
+        target = max(rtol * norm, atol)
+        return bool(np.isfinite(defect) and defect <= (1.0 + np.sqrt(np.finfo(float).eps)) * target
+                    and np.isfinite(slope) and slope < 0.0)
+
+    def gmres(self, b, x, rtol: float, atol: float, restart: int, cycles: int):
+        """Left-preconditioned restarted GMRES for ``J x = b`` in the native driver (``c3l_gmres``).
+
+        On return ``self.w`` holds the true linear residual ``b - J x``.
+        """
 
         tl = self.dev.tilde
         maps = _lib.TildeMaps(
@@ -165,7 +184,7 @@ class _DeviceNewton:
         _lib.call(
             "c3l_gmres", self.dev.plan.handle, ctypes.byref(maps), _lib.dptr(self.dev.diag), _lib.dptr(self.dev.dt),
             _lib.dptr(self.dev.out), _lib.dptr(b), _lib.dptr(x), _lib.dptr(self.V), _lib.dptr(self.w),
-            _lib.dptr(self.tmp), _lib.dptr(self.reducer.scratch), _lib.dptr(self.coef), float(rtol), int(restart),
+            _lib.dptr(self.tmp), _lib.dptr(self.reducer.scratch), _lib.dptr(self.coef), float(rtol), float(atol), int(restart),
             int(cycles), ctypes.byref(info), ctypes.byref(iterations), ctypes.byref(true), ctypes.byref(calls),
             _lib.stream_ptr(self.torch),
         )
@@ -209,8 +228,14 @@ def solve_component_ccsd(
         if iteration == controls.maximum_newton_iterations:
             break
         tol = forcing_tolerance(norm, controls, previous_norm, previous_tol)
+        atol = 0.0 if controls.forcing_strategy == "legacy" else controls.gmres_absolute_tolerance_factor * controls.tolerance
         rscale(state.r, state.trial, -1.0, torch)  # b = -r
-        info, its, true = state.gmres(state.trial, state.step, tol, controls.gmres_restart, controls.gmres_maximum_restart_cycles)
+        info, its, true = state.gmres(
+            state.trial, state.step, tol, atol, controls.gmres_restart, controls.gmres_maximum_restart_cycles
+        )
+        if controls.forcing_strategy == "eisenstat_walker" and not state.acceptable_step(norm, true, tol, atol):
+            history.append(NewtonKrylovStep(iteration, norm, tol, info, its, true, state.norm(state.step), 0.0, norm))
+            break
         damping = 1.0
         accepted = False
         accepted_norm = np.inf

@@ -159,14 +159,17 @@ typedef struct c3l_tilde_maps {
 /* Newton correction J x = b in real tilde coordinates (replaces the scipy gmres call of
  * solve_component_ccsd, src/c3l/cc/component_solver.py:181-191, and the JAX cycle
  * get_cycle, cc/_static_gmres.py:12-66).  Left-preconditioned GMRES(restart), at most
- * `cycles` restarts, inner tolerance ||M b|| * rtol, true residual test per restart.
+ * `cycles` restarts, target max(atol, rtol ||b||) on the true residual, checked after every
+ * restart; the inner test on the preconditioned residual follows scipy.sparse.linalg.gmres
+ * (initial ||M b|| min(1, target/||b||), rescaled after each restart).  info = 0 when the
+ * target is met, else `cycles`.  On return w holds the true residual b - J x.
  * The plan's T atoms must be loaded (linearisation point).  Workspace: t_work, out_work
  * complex [dim]; V [(restart+1) x real_dim]; w, tmp [real_dim]; scratch >= chunks*(restart+2)
- * doubles; coef >= 2*(restart+1)+1 doubles. x is overwritten with the step. */
+ * doubles; coef >= 2*(restart+1)+2 doubles. x is overwritten with the step. */
 int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
               const double* b, double* x, double* V, double* w, double* tmp, double* scratch, double* coef,
-              double rtol, int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations, double* true_rel,
-              int32_t* jvp_calls, void* stream);
+              double rtol, double atol, int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations,
+              double* true_rel, int32_t* jvp_calls, void* stream);
 
 /* Shift-invariant Krylov resolvent on the sigma program (replaces the per-frequency GMRES
  * loop of component_retarded_green, src/c3l/cc/component_response.py:180-245, and the Arnoldi

@@ -243,3 +243,63 @@ def test_single_site_green_function_matches_lindblad_ed(torch_gpu):
     result = component_retarded_green(ComponentSigmaOperator(ctx), component_eom_sources(ctx), omega, 0.0)
     want = fock.exact_retarded_green(model, omega, 0.0)
     assert _rel(result.retarded, want) < 1e-8
+
+
+@pytest.mark.parametrize("restart,cycles,rtol,atol", [(40, 12, 1e-6, 0.0), (5, 8, 1e-9, 0.0), (6, 10, 1e-3, 1e-7)])
+def test_native_gmres_matches_scipy(torch_gpu, restart, cycles, rtol, atol):
+    """c3l_gmres against scipy.sparse.linalg.gmres on the oracle Jacobian: same inner
+    iteration count, same convergence flag, same step to rounding."""
+
+    from scipy.sparse.linalg import LinearOperator, gmres
+
+    from c3lindblad_amd.solver import _DeviceNewton
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    print(f"case gmres restart={restart} cycles={cycles} rtol={rtol} atol={atol}", flush=True)
+    p, sp = _static(3, 1.2, 3)
+    torch = torch_gpu
+    rng = np.random.default_rng(7)
+    D = p.tilde.real_size
+    x = 0.05 * rng.normal(size=D)
+    state = _DeviceNewton(sp, restart)
+    xd = torch.from_numpy(x).cuda()
+    norm = state.residual(xd, state.r)
+    b = (-state.r).contiguous()
+    step = torch.empty_like(b)
+    info, its, true = state.gmres(b, step, rtol, atol, restart, cycles)
+    w = state.w.cpu().numpy()
+
+    masks = poly.coordinate_masks(p.coordinates)
+    t = p.tilde.unpack(x)
+    r = p.tilde.pack(poly.residual(p, t, masks)[0], check=False)
+    assert _rel(-b.cpu().numpy(), r) < RTOL
+    diag = solvers.preconditioner_diagonal(p.rapidity_sums())
+    A = LinearOperator((D, D), matvec=lambda v: p.tilde.pack(poly.jvp(p, t, p.tilde.unpack(v), masks), check=False),
+                       dtype=np.float64)
+    M = LinearOperator((D, D), matvec=lambda v: p.tilde.pack(p.tilde.unpack(v) / diag, check=False), dtype=np.float64)
+    seen = []
+    want, want_info = gmres(A, -r, M=M, rtol=rtol, atol=atol, restart=restart, maxiter=cycles,
+                            callback=lambda v: seen.append(v), callback_type="pr_norm")
+    assert its == len(seen)
+    assert (info == 0) == (want_info == 0)
+    step_err = _rel(step.cpu().numpy(), want)
+    assert step_err < 1e-7
+    # w is the true linear residual b - J step (a cancellation: compare on the scale of b)
+    w_err = float(np.max(np.abs(w - (-r - A.matvec(step.cpu().numpy())))) / np.max(np.abs(r)))
+    assert w_err < 1e-12
+    assert abs(true - np.linalg.norm(w) / norm) <= 1e-12
+    assert (info == 0) == (true * norm <= max(rtol * norm, atol))
+
+
+def test_eisenstat_walker_newton_reaches_legacy_root(torch_gpu):
+    from c3lindblad_amd.solver import NewtonKrylovPolicy, solve_component_ccsd
+
+    print("case newton eisenstat-walker", flush=True)
+    p, sp = _static(3, 1.2, 3)
+    legacy = solve_component_ccsd(sp)
+    ew = solve_component_ccsd(sp, policy=NewtonKrylovPolicy(forcing_strategy="eisenstat_walker"))
+    assert legacy.converged and ew.converged
+    assert ew.packed_residual_norm <= 1e-10
+    assert all(step.damping > 0.0 for step in ew.history)
+    assert _rel(ew.amplitudes, legacy.amplitudes) < 1e-8
