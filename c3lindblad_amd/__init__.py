@@ -34,9 +34,16 @@ from .eom import (
     eom_ccsd_problem,
 )
 from .response import arnoldi_resolvent, component_retarded_green
+from .observables import (
+    ComponentObservableProblem,
+    contact_currents,
+    factorized_ccsd_covariance,
+    two_body_expectations,
+)
 
 __all__ = [
     "ComponentEOMProblem",
+    "ComponentObservableProblem",
     "ComponentSigmaOperator",
     "ComponentStaticProblem",
     "FactorizedCCProblem",
@@ -48,6 +55,8 @@ __all__ = [
     "component_eom_context",
     "component_eom_sigma",
     "component_eom_sources",
+    "contact_currents",
+    "factorized_ccsd_covariance",
     "component_retarded_green",
     "component_static_jvp",
     "component_static_linearization",
@@ -57,4 +66,5 @@ __all__ = [
     "eom_ccsd_problem",
     "factorized_cc_problem",
     "solve_component_ccsd",
+    "two_body_expectations",
 ]

@@ -372,6 +372,8 @@ class BlockProgram:
     tensor of that ``C`` atom at every index tuple of ``key`` (the shuffle
     formula is valid for unsorted and repeated indices, giving the full
     antisymmetric tensor) and writes it into the atom workspace.
+    ``scalar_index >= 0`` marks one degree-0 entry of an observable program,
+    written to that position of its output vector.
     """
 
     key: BlockKey
@@ -380,6 +382,7 @@ class BlockProgram:
     factor_atom: np.ndarray  # int32 [F] atom id
     factor_positions: np.ndarray  # int32 [F, 4] output positions per slot (-1 unused)
     target_atom: int = -1
+    scalar_index: int = -1
 
 
 @dataclass
@@ -406,7 +409,7 @@ class Plan:
         return int(sum(b.coefficients.size for b in self.programs[name]))
 
 
-def _block_program(term_list, key, index, target_atom=-1) -> BlockProgram:
+def _block_program(term_list, key, index, target_atom=-1, scalar_index=-1) -> BlockProgram:
     coefficients, starts, f_atom, f_pos = [], [0], [], []
     for term_atoms, coefficient in term_list:
         for sign, positions in _shuffles(term_atoms, key):
@@ -422,6 +425,7 @@ def _block_program(term_list, key, index, target_atom=-1) -> BlockProgram:
         np.asarray(f_atom, dtype=np.int32),
         np.asarray(f_pos, dtype=np.int32).reshape(-1, 4),
         target_atom,
+        scalar_index,
     )
 
 
@@ -430,15 +434,19 @@ def emit_plan(
     n: int,
     vectors: np.ndarray,
     creations: list[tuple[BlockKey, dict]] | None = None,
+    scalars: list[dict] | None = None,
 ) -> Plan:
     """Lower merged term dictionaries (one per action) into flat arrays.
 
     ``creations[i]`` defines ``C`` atom ``i`` (its block key and the terms it
     sums); it becomes the ``creation`` program, run after the fixed atoms are
-    loaded and before any program that reads ``C`` atoms.
+    loaded and before any program that reads ``C`` atoms.  ``scalars[i]``
+    holds the degree-0 terms of observable ``i``; together they form the
+    ``observable`` program, whose output ``i`` is that sum.
     """
 
     creations = creations or []
+    scalars = scalars or []
     needed: dict[Atom, None] = {}
 
     def need(a: Atom) -> None:
@@ -461,6 +469,10 @@ def emit_plan(
         by_action[name] = by_key
     for key, rest_terms in creations:
         for atoms in rest_terms:
+            for a in atoms:
+                need(a)
+    for terms in scalars:
+        for atoms in terms:
             for a in atoms:
                 need(a)
     atoms = sorted(needed, key=lambda a: (len(a.contractions), a.sort_key()))
@@ -499,4 +511,11 @@ def emit_plan(
             blocks.append(_block_program(list(rest_terms.items()), key, index, target))
         programs["creation"] = blocks
         n_terms["creation"] = sum(len(t) for _, t in creations)
+    if scalars:
+        zero = (0, 0, 0, 0)
+        programs["observable"] = [
+            _block_program([(k, v) for k, v in terms.items() if term_key(k) == zero], zero, index, scalar_index=i)
+            for i, terms in enumerate(scalars)
+        ]
+        n_terms["observable"] = sum(len(t) for t in scalars)
     return Plan(n, atoms, offsets, int(cursor), jobs, programs, vectors, n_terms)

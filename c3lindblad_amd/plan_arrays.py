@@ -26,7 +26,7 @@ from .algebra import Plan, key_slots
 from .lowering import BlockKey, CoordinateSet
 
 KINDS = {"T": 0, "D": 1, "X": 2}
-PROGRAMS = {"residual": 0, "reference": 1, "jvp": 2, "sigma": 3, "creation": 4}
+PROGRAMS = {"residual": 0, "reference": 1, "jvp": 2, "sigma": 3, "creation": 4, "observable": 5}
 
 
 def encode_factors(ranks: np.ndarray, positions: np.ndarray) -> np.ndarray:
@@ -164,7 +164,7 @@ def lower_plan(
                 dest = plan.atom_offsets[bp.target_atom] + np.arange(n**deg, dtype=np.int64)
             elif deg == 0:
                 coord = np.zeros((1, 0), dtype=np.int32)
-                dest = np.zeros(1, dtype=np.int64)
+                dest = np.full(1, max(bp.scalar_index, 0), dtype=np.int64)
             else:
                 block = coords.block(bp.key)
                 if block is None or block.positions.size == 0:

@@ -111,6 +111,15 @@ class PHFockSpace:
         state = self.transformed(T, self.vacuum())
         return -1j * state[self.masks(coords.modes)]
 
+    def expectation(self, coords, amplitudes, factors) -> complex:
+        """``<0| F_1 ... F_k exp(T) |0>`` with the complete exponential (no rank prefix)."""
+
+        T = self.cluster(coords.modes, amplitudes)
+        state = self._exp_apply(T, self.vacuum(), 1.0, self.M // 2 + 2)
+        for components in reversed(factors):
+            state = self.linear(components) @ state
+        return complex(state[0])
+
     def sigma(self, static_coords, amplitudes, response_coords, x) -> np.ndarray:
         T = self.cluster(static_coords.modes, amplitudes)
         X = self.cluster(response_coords.modes, x)

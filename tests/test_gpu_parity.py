@@ -303,3 +303,45 @@ def test_eisenstat_walker_newton_reaches_legacy_root(torch_gpu):
     assert ew.packed_residual_norm <= 1e-10
     assert all(step.damping > 0.0 for step in ew.history)
     assert _rel(ew.amplitudes, legacy.amplitudes) < 1e-8
+
+
+@pytest.mark.parametrize("n,U,seed", [(2, 0.9, 2), (3, 1.2, 3)])
+def test_device_observables_match_oracle(torch_gpu, n, U, seed):
+    from c3lindblad_amd.observables import factorized_ccsd_covariance, two_body_expectations
+    from oracle import polynomial as poly
+
+    print(f"case observables n={n}", flush=True)
+    p, sp = _static(n, U, seed)
+    rng = np.random.default_rng(seed + 30)
+    t = p.tilde.unpack(0.1 * rng.normal(size=p.num_variables))
+    got = factorized_ccsd_covariance(sp, t)
+    assert _rel(got, poly.covariance(p, t)) < RTOL
+    N = p.model.n_spin_orbitals
+    quartets = rng.integers(0, N, size=(8, 4))
+    got4 = two_body_expectations(sp, t, quartets)
+    from c3lindblad_amd.lowering import physical_mode_operator
+
+    ref = p.reference
+    op = {(o, d): physical_mode_operator(ref, *ref.model.orbital_site_spin(o), "c_dagger" if d else "c").components
+          for o in range(N) for d in (0, 1)}
+    prefix = poly.low_rank_prefix(poly.coordinate_masks(p.coordinates), t)
+    want4 = np.array([poly.expectation(prefix, (op[a, 1], op[b, 0], op[c, 1], op[d, 0])) for a, b, c, d in quartets])
+    assert np.max(np.abs(got4 - want4)) < RTOL * max(1.0, np.max(np.abs(want4)))
+
+
+def test_single_site_device_occupations_equal_lindblad_ed(torch_gpu):
+    from c3lindblad_amd import (
+        ComponentStaticProblem,
+        FermionicLindbladModel,
+        factorized_cc_problem,
+        factorized_ccsd_covariance,
+        solve_component_ccsd,
+    )
+    from oracle import fock
+
+    print("case single site occupations vs ED", flush=True)
+    model = FermionicLindbladModel(E=[[-0.45]], gamma_loss=[[0.35]], gamma_gain=[[0.15]], U=1.3)
+    sp = ComponentStaticProblem.compile(factorized_cc_problem(model))
+    sol = solve_component_ccsd(sp)
+    got = factorized_ccsd_covariance(sp, sol.amplitudes)
+    assert np.max(np.abs(got - fock.exact_covariance(model))) < 1e-10

@@ -101,3 +101,49 @@ def test_single_site_occupation_and_exact_covariance_agree_with_hf():
     model = FermionicLindbladModel(E=[[-0.45]], gamma_loss=[[0.35]], gamma_gain=[[0.15]], U=1.3)
     p = factorized_cc_problem(model)
     assert np.max(np.abs(p.reference.gamma - fock.exact_covariance(model))) < 1e-12
+
+
+def test_single_site_ccsd_occupations_equal_lindblad_ed():
+    """n = 1: CCSD is exact, so its covariance and double occupancy equal ED."""
+
+    from c3lindblad_amd.lowering import factorized_cc_problem, physical_mode_operator
+    from c3lindblad_amd.model import FermionicLindbladModel
+    from oracle import fock
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    model = FermionicLindbladModel(E=[[-0.45]], gamma_loss=[[0.35]], gamma_gain=[[0.15]], U=1.3)
+    p = factorized_cc_problem(model)
+    masks = poly.coordinate_masks(p.coordinates)
+    _, t, packed, _, _ = solvers.solve_ccsd(
+        p.tilde, p.rapidity_sums(), lambda a: poly.residual(p, a, masks)[0], lambda a: (lambda d: poly.jvp(p, a, d, masks))
+    )
+    assert packed <= 1e-10
+    got = poly.covariance(p, t, masks)
+    want = fock.exact_covariance(model)
+    assert np.max(np.abs(got - want)) < 1e-11
+    rho, _, _, c = fock.lindblad_generators(model)
+    ref = p.reference
+    op = {(o, d): physical_mode_operator(ref, *ref.model.orbital_site_spin(o), "c_dagger" if d else "c").components
+          for o in range(2) for d in (0, 1)}
+    double = poly.expectation(poly.low_rank_prefix(masks, t), (op[0, 1], op[0, 0], op[1, 1], op[1, 0]))
+    n_up, n_dn = c[0].conj().T @ c[0], c[1].conj().T @ c[1]
+    assert abs(double - np.trace(n_up @ n_dn @ rho)) < 1e-11
+
+
+def test_contact_currents_conserve_charge_on_the_exact_steady_state():
+    """Particle balance on the exact steady state (ED covariance): the bath's
+    current into the impurity equals the bath's dissipative flux and cancels
+    the impurity's own flux."""
+
+    from c3lindblad_amd.model import FermionicLindbladModel
+    from c3lindblad_amd.observables import contact_currents
+    from oracle import fock
+
+    E = np.array([[-0.3, 0.4], [0.4, 0.8]])
+    model = FermionicLindbladModel(E=E, gamma_loss=np.diag([0.3, 0.05]), gamma_gain=np.diag([0.02, 0.45]), U=1.5)
+    c = contact_currents(model, fock.exact_covariance(model), [0, 1], {"bath": [2, 3], "imp": [0, 1]})
+    assert abs(c["I_bath"]) > 1e-3
+    assert abs(c["I_imp"]) < 1e-14
+    assert abs(c["F_bath"] - c["I_bath"]) < 1e-12
+    assert abs(c["I_bath"] + c["F_imp"]) < 1e-12
