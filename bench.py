@@ -27,17 +27,22 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 
 
-def _algorithmic_block_bytes(lowered, block_index: int) -> float:
-    """Compulsory bytes of one wedge-block launch: outputs, coordinates, diagonal terms, atoms read."""
+def _algorithmic_program_bytes(lowered, program: int) -> float:
+    """Compulsory bytes of one program launch: per coordinate its output, destination,
+    packed coordinate and (residual/JVP/sigma) diagonal and input value, plus every
+    distinct atom tensor the program reads once."""
 
-    program, deg, coord, dest, coeff, fstart, foff, fcode = lowered.blocks[block_index]
     n = lowered.n
-    count = dest.size
+    total = 0.0
     atoms = {}
-    for off, code in zip(foff, fcode):
-        atoms[int(off)] = n ** (int(code) & 3)
-    diag = 32 if program in (0, 2, 3) else 0  # diagonal and input value per coordinate
-    return float(count * (16 + 8 + 4 * deg + diag) + 16 * sum(atoms.values()))
+    for prog, deg, coord, dest, coeff, fstart, foff, fcode in lowered.blocks:
+        if prog != program:
+            continue
+        diag = 32 if program in (0, 2, 3) else 0
+        total += dest.size * (16 + 8 + 4 + diag)
+        for off, code in zip(foff, fcode):
+            atoms[int(off)] = n ** (int(code) & 7)
+    return total + 16.0 * sum(atoms.values())
 
 
 def _cpu_baseline(pipe, result, seconds_budget: float = 30.0) -> dict:
@@ -118,6 +123,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="siam24", choices=("siam24", "readme"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--forcing", default="legacy", choices=("legacy", "eisenstat_walker"),
+                    help="inexact-Newton forcing strategy of the CCSD solve (the reference's two options)")
+    ap.add_argument("--restart", type=int, default=40, help="GMRES restart length of the CCSD solve")
     ap.add_argument("--jvp-probe", type=int, default=0, metavar="K",
                     help="profiling workload only: one residual and K JVPs of siam24, then a 256 MiB copy")
     args = ap.parse_args()
@@ -138,6 +146,7 @@ def main() -> int:
     from c3lindblad_amd.configs import config2_grid, config2_model, readme_grid, readme_model
     from c3lindblad_amd.device import KernelTimer
     from c3lindblad_amd.pipeline import SolvePipeline
+    from c3lindblad_amd.plan_arrays import PROGRAMS
 
     if args.config == "siam24":
         model, grid, charges = config2_model(), config2_grid(), (1, -1)
@@ -145,7 +154,10 @@ def main() -> int:
     else:
         model, grid, charges = readme_model(), readme_grid(), (1,)
         workload = "readme_3site_U1.2_ccsd_eom_s13_gr_81w"
-    pipe = SolvePipeline.prepare(model, grid, 0.0, charges=charges)
+    from c3lindblad_amd.solver import NewtonKrylovPolicy
+
+    policy = NewtonKrylovPolicy(forcing_strategy=args.forcing, gmres_restart=args.restart)
+    pipe = SolvePipeline.prepare(model, grid, 0.0, charges=charges, policy=policy)
     print(f"[bench] setup {pipe.setup_seconds:.2f}s n={model.n_sites} D={pipe.static.oracle.num_amplitudes}", file=sys.stderr, flush=True)
     for k in range(args.warmup):
         r = pipe.solve()
@@ -165,14 +177,14 @@ def main() -> int:
     KernelTimer.enabled = False
     kernels = {name: d for name, d in KernelTimer.summary().items()}
     for plan in plans:
-        for i, st in enumerate(plan.block_stats()):
-            if st["launches"] == 0:
+        for name, pid in PROGRAMS.items():
+            ms, launches = plan.program_time(name)
+            if launches == 0:
                 continue
-            name = f"wedge_block[{st['program']},deg{st['deg']},{st['count']}]"
-            d = kernels.setdefault(name, {"ms": 0.0, "launches": 0, "bytes": 0.0})
-            d["ms"] += st["ms"]
-            d["launches"] += st["launches"]
-            d["bytes"] += st["launches"] * _algorithmic_block_bytes(plan.lowered, i)
+            d = kernels.setdefault(f"wedge_program[{name}]", {"ms": 0.0, "launches": 0, "bytes": 0.0})
+            d["ms"] += ms
+            d["launches"] += launches
+            d["bytes"] += launches * _algorithmic_program_bytes(plan.lowered, pid)
         plan.set_timing(False)
     dominant = max(kernels, key=lambda k: kernels[k]["ms"])
     dk = kernels[dominant]
@@ -197,6 +209,7 @@ def main() -> int:
         "config": {
             "workload": workload,
             "variant": "fp64_gpu",
+            "newton_policy": {"forcing": args.forcing, "gmres_restart": args.restart},
             "n_sites": model.n_sites,
             "ccsd_amplitudes": pipe.static.oracle.num_amplitudes,
             "eom_dimension_per_charge": {str(q): b.dimension for q, b in pipe.eom.items()},

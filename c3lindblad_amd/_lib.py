@@ -36,7 +36,8 @@ SIGNATURES = {
     "c3l_plan_apply": ([_P, _I32, _P, _P, _P, _P], ctypes.c_int),
     "c3l_plan_set_timing": ([_P, _I32], ctypes.c_int),
     "c3l_plan_block_count": ([_P], _I32),
-    "c3l_plan_block_info": ([_P, _I32, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "c3l_plan_block_info": ([_P, _I32, _P, _P, _P, _P, _P], ctypes.c_int),
+    "c3l_plan_program_time": ([_P, _I32, _P, _P], ctypes.c_int),
     "c3l_tilde_unpack": ([_P, _P, _P, _P, _P, _I64, _P], ctypes.c_int),
     "c3l_tilde_pack": ([_P, _P, _P, _P, _I64, _P], ctypes.c_int),
     "c3l_rmultidot": ([_P, _I64, _I32, _P, _I64, _P, _P, _P], ctypes.c_int),

@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -108,34 +110,103 @@ __global__ void contract_kernel(double2* __restrict__ ws, const double2* __restr
   }
 }
 
-// One thread per canonical output coordinate; the entry loop is uniform across
-// the wavefront so coefficient/factor metadata come through the scalar cache.
-__global__ void wedge_block_kernel(const double2* __restrict__ ws, double2* __restrict__ out,
-                                   const int32_t* __restrict__ coord, const int64_t* __restrict__ dest,
-                                   int64_t count, int deg, int n, int n_entries,
-                                   const double2* __restrict__ coeff, const int32_t* __restrict__ fstart,
-                                   const int64_t* __restrict__ foffset, const int32_t* __restrict__ fcode,
-                                   const double2* __restrict__ diag, const double2* __restrict__ x) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= count) return;
-  int idx[4] = {0, 0, 0, 0};
-  for (int k = 0; k < deg; ++k) idx[k] = coord[e * deg + k];
-  double2 acc = make_double2(0.0, 0.0);
-  for (int t = 0; t < n_entries; ++t) {
-    double2 prod = coeff[t];
-    const int f1 = fstart[t + 1];
-    for (int f = fstart[t]; f < f1; ++f) {
-      const int code = fcode[f];
-      const int rank = code & 7;
-      int64_t flat = 0;
-      for (int s = 0; s < rank; ++s) flat = flat * n + idx[(code >> (3 + 2 * s)) & 3];
-      prod = cmul(prod, ws[foffset[f] + flat]);
-    }
-    acc = cadd(acc, prod);
+// ---------------------------------------------------------------- program kernel
+//
+// One launch evaluates every output block of a program.  Work is cut into
+// tiles; a tile belongs to one block and covers `256 >> lanes_log2`
+// consecutive canonical coordinates of it.  With lanes_log2 = L > 0 the 2^L
+// waves of the workgroup split that block's entries between them (small
+// blocks: more workgroups in flight) and their partial sums meet in LDS.
+// Because a wave never straddles two entry groups, the entry index and all
+// entry/factor metadata are wave-uniform and come through scalar loads.
+//
+// Factor f of an entry reads ws[offset_f + sum_k idx_k * stride_f[k]]: the
+// padded-tensor flat index as a dot product of the coordinate with strides
+// prepared on the host (0 for positions the factor does not use).
+// Coordinates are packed as four 8-bit indices (n <= 255).
+
+struct WedgeFactor {
+  int64_t offset;
+  int32_t stride[4];
+};
+
+struct WedgeEntry {
+  double2 coeff;
+  int32_t first;  // first factor
+  int32_t nf;     // factor count
+  int32_t pad[2];
+};
+
+struct WedgeBlockDesc {
+  int64_t count;
+  const uint32_t* coord;  // packed indices, one word per coordinate
+  const int64_t* dest;
+  const WedgeEntry* entries;
+  const WedgeFactor* factors;
+  int32_t n_entries;
+  int32_t lanes_log2;
+};
+
+__device__ __forceinline__ int64_t factor_index(const WedgeFactor& f, int i0, int i1, int i2, int i3) {
+  return f.offset + (int64_t)(i0 * f.stride[0] + i1 * f.stride[1] + i2 * f.stride[2] + i3 * f.stride[3]);
+}
+
+template <bool kTwoFactors>
+__global__ __launch_bounds__(256) void wedge_program_kernel(const double2* __restrict__ ws, double2* __restrict__ out,
+                                                            const WedgeBlockDesc* __restrict__ descs,
+                                                            const int2* __restrict__ tiles,
+                                                            const double2* __restrict__ diag,
+                                                            const double2* __restrict__ x) {
+  __shared__ double2 part[256];
+  const int2 tile = tiles[blockIdx.x];
+  const WedgeBlockDesc d = descs[tile.x];
+  const int lg = d.lanes_log2;
+  const int per = 256 >> lg;
+  const int local = threadIdx.x & (per - 1);
+  const int group = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> (8 - lg)));
+  const int step = 1 << lg;
+  const int64_t e = (int64_t)tile.y + local;
+  const bool active = e < d.count;
+  int i0 = 0, i1 = 0, i2 = 0, i3 = 0;
+  if (active) {
+    const uint32_t c = d.coord[e];
+    i0 = c & 255u;
+    i1 = (c >> 8) & 255u;
+    i2 = (c >> 16) & 255u;
+    i3 = c >> 24;
   }
-  const int64_t d = dest[e];
-  if (diag != nullptr) acc = cadd(acc, cmul(diag[d], x[d]));
-  out[d] = acc;
+  double2 acc = make_double2(0.0, 0.0);
+  if (active) {
+    if (kTwoFactors) {
+#pragma unroll 2
+      for (int t = group; t < d.n_entries; t += step) {
+        const double2 c = d.entries[t].coeff;
+        const WedgeFactor fa = d.factors[2 * t];
+        const WedgeFactor fb = d.factors[2 * t + 1];
+        const double2 a = ws[factor_index(fa, i0, i1, i2, i3)];
+        const double2 b = ws[factor_index(fb, i0, i1, i2, i3)];
+        acc = cadd(acc, cmul(c, cmul(a, b)));
+      }
+    } else {
+      for (int t = group; t < d.n_entries; t += step) {
+        const WedgeEntry en = d.entries[t];
+        double2 prod = en.coeff;
+        for (int f = en.first; f < en.first + en.nf; ++f) prod = cmul(prod, ws[factor_index(d.factors[f], i0, i1, i2, i3)]);
+        acc = cadd(acc, prod);
+      }
+    }
+  }
+  if (lg > 0) {
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (group == 0)
+      for (int k = 1; k < step; ++k) acc = cadd(acc, part[k * per + local]);
+  }
+  if (group == 0 && active) {
+    const int64_t dd = d.dest[e];
+    if (diag != nullptr) acc = cadd(acc, cmul(diag[dd], x[dd]));
+    out[dd] = acc;
+  }
 }
 
 __global__ void fill_const_kernel(double2* __restrict__ ws, const double2* __restrict__ vectors,
@@ -376,20 +447,35 @@ struct PlanBlock {
   int32_t deg;
   int64_t count;
   int32_t n_entries;
+  int32_t n_factors;
   int64_t max_dest = -1;
-  int32_t* coord = nullptr;
-  int64_t* dest = nullptr;
-  double2* coeff = nullptr;
-  int32_t* fstart = nullptr;
-  int64_t* foffset = nullptr;
-  int32_t* fcode = nullptr;
+  bool two_factors = true;
+  // host copies, concatenated into the program's device arrays when it is built
+  std::vector<uint32_t> coord;
+  std::vector<int64_t> dest;
+  std::vector<WedgeEntry> entries;
+  std::vector<WedgeFactor> factors;
+};
+
+// Device image of one program: every block's arrays concatenated, the block
+// descriptors and the tile lists of the two kernel variants.
+struct ProgramExec {
+  bool built = false;
+  int64_t max_dest = -1;
+  int32_t tiles_two = 0;
+  int32_t tiles_general = 0;
+  WedgeBlockDesc* descs = nullptr;
+  int2* tiles = nullptr;  // two-factor tiles first, then general tiles
+  std::vector<void*> owned;
 };
 
 struct PendingTiming {
-  size_t block;
+  int32_t program;
   hipEvent_t start;
   hipEvent_t stop;
 };
+
+constexpr int kPrograms = 8;
 
 struct c3l_plan {
   int32_t n = 0;
@@ -400,12 +486,13 @@ struct c3l_plan {
   std::vector<PlanBase> bases;
   std::vector<PlanLevel> levels;
   std::vector<PlanBlock> blocks;
+  ProgramExec programs[kPrograms];
   std::vector<void*> owned;
-  // optional per-block kernel timing with HIP events
+  // optional per-program kernel timing with HIP events
   bool timing = false;
   std::vector<PendingTiming> pending;
-  std::vector<double> block_ms;
-  std::vector<int64_t> block_launches;
+  double program_ms[kPrograms] = {};
+  int64_t program_launches[kPrograms] = {};
 };
 
 namespace {
@@ -485,6 +572,8 @@ int c3l_plan_destroy(c3l_plan* plan) {
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);
   }
+  for (ProgramExec& pe : plan->programs)
+    for (void* p : pe.owned) (void)hipFree(p);
   for (void* p : plan->owned) (void)hipFree(p);
   delete plan;
   return C3L_OK;
@@ -493,40 +582,42 @@ int c3l_plan_destroy(c3l_plan* plan) {
 int c3l_plan_set_timing(c3l_plan* plan, int32_t enable) {
   if (plan == nullptr) return fail(C3L_EINVAL, "c3l_plan_set_timing: invalid argument");
   plan->timing = enable != 0;
-  plan->block_ms.assign(plan->blocks.size(), 0.0);
-  plan->block_launches.assign(plan->blocks.size(), 0);
+  for (int k = 0; k < kPrograms; ++k) {
+    plan->program_ms[k] = 0.0;
+    plan->program_launches[k] = 0;
+  }
   return C3L_OK;
 }
 
-int c3l_plan_block_info(c3l_plan* plan, int32_t index, int32_t* program, int32_t* deg, int64_t* count,
-                        int32_t* n_entries, int32_t* n_factors, double* ms_total, int64_t* launches) {
-  if (plan == nullptr || index < 0 || index >= (int32_t)plan->blocks.size())
-    return fail(C3L_EINVAL, "c3l_plan_block_info: invalid block index");
+int c3l_plan_program_time(c3l_plan* plan, int32_t program, double* ms_total, int64_t* launches) {
+  if (plan == nullptr || program < 0 || program >= kPrograms)
+    return fail(C3L_EINVAL, "c3l_plan_program_time: invalid program");
   // resolve every recorded timing (waits for the recorded work)
   for (PendingTiming& p : plan->pending) {
     C3L_HIP_CHECK(hipEventSynchronize(p.stop));
     float ms = 0.f;
     C3L_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
-    if (p.block < plan->block_ms.size()) {
-      plan->block_ms[p.block] += ms;
-      plan->block_launches[p.block] += 1;
-    }
+    plan->program_ms[p.program] += ms;
+    plan->program_launches[p.program] += 1;
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);
   }
   plan->pending.clear();
+  if (ms_total) *ms_total = plan->program_ms[program];
+  if (launches) *launches = plan->program_launches[program];
+  return C3L_OK;
+}
+
+int c3l_plan_block_info(c3l_plan* plan, int32_t index, int32_t* program, int32_t* deg, int64_t* count,
+                        int32_t* n_entries, int32_t* n_factors) {
+  if (plan == nullptr || index < 0 || index >= (int32_t)plan->blocks.size())
+    return fail(C3L_EINVAL, "c3l_plan_block_info: invalid block index");
   const PlanBlock& b = plan->blocks[(size_t)index];
   if (program) *program = b.program;
   if (deg) *deg = b.deg;
   if (count) *count = b.count;
   if (n_entries) *n_entries = b.n_entries;
-  if (n_factors) {
-    int32_t nf = 0;
-    if (b.n_entries > 0) C3L_HIP_CHECK(hipMemcpy(&nf, b.fstart + b.n_entries, sizeof(int32_t), hipMemcpyDeviceToHost));
-    *n_factors = nf;
-  }
-  if (ms_total) *ms_total = index < (int32_t)plan->block_ms.size() ? plan->block_ms[(size_t)index] : 0.0;
-  if (launches) *launches = index < (int32_t)plan->block_launches.size() ? plan->block_launches[(size_t)index] : 0;
+  if (n_factors) *n_factors = b.n_factors;
   return C3L_OK;
 }
 
@@ -577,37 +668,61 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
 int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
                        const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
                        int32_t n_factors, const int64_t* foffset, const int32_t* fcode) {
-  if (plan == nullptr || deg < 0 || deg > 4 || count < 0 || n_entries < 0 || n_factors < 0)
+  if (plan == nullptr || program < 0 || program >= kPrograms || deg < 0 || deg > 4 || count < 0 ||
+      count > INT32_MAX || n_entries < 0 || n_factors < 0)
     return fail(C3L_EINVAL, "c3l_plan_add_block: invalid argument");
-  if (n_entries > 0 && fstart[n_entries] != n_factors) return fail(C3L_EINVAL, "c3l_plan_add_block: fstart mismatch");
-  for (int32_t f = 0; f < n_factors; ++f) {
-    const int rank = fcode[f] & 7;
-    if (rank > 4 || foffset[f] < 0 || foffset[f] + ipow(plan->n, rank) > plan->workspace)
-      return fail(C3L_EINVAL, "c3l_plan_add_block: factor exceeds workspace");
-    for (int s = 0; s < rank; ++s)
-      if (((fcode[f] >> (3 + 2 * s)) & 3) >= deg) return fail(C3L_EINVAL, "c3l_plan_add_block: factor position out of range");
-  }
-  int64_t max_dest = -1;
-  for (int64_t k = 0; k < count; ++k) {
-    if (dest[k] < 0) return fail(C3L_EINVAL, "c3l_plan_add_block: negative destination");
-    if (dest[k] > max_dest) max_dest = dest[k];
-  }
-  for (int64_t k = 0; k < count * deg; ++k)
-    if (coord[k] < 0 || coord[k] >= plan->n) return fail(C3L_EINVAL, "c3l_plan_add_block: coordinate out of range");
+  if (plan->n > 255) return fail(C3L_EINVAL, "c3l_plan_add_block: more than 255 sites per group");
+  if (n_entries > 0 && (fstart[0] != 0 || fstart[n_entries] != n_factors))
+    return fail(C3L_EINVAL, "c3l_plan_add_block: fstart mismatch");
   PlanBlock b;
   b.program = program;
   b.deg = deg;
   b.count = count;
   b.n_entries = n_entries;
-  b.max_dest = max_dest;
-  int rc = upload(plan, &b.coord, coord, count * deg);
-  if (rc == C3L_OK) rc = upload(plan, &b.dest, dest, count);
-  if (rc == C3L_OK) rc = upload(plan, &b.coeff, reinterpret_cast<const double2*>(coeff), n_entries);
-  if (rc == C3L_OK) rc = upload(plan, &b.fstart, fstart, n_entries + 1);
-  if (rc == C3L_OK) rc = upload(plan, &b.foffset, foffset, n_factors);
-  if (rc == C3L_OK) rc = upload(plan, &b.fcode, fcode, n_factors);
-  if (rc != C3L_OK) return rc;
-  plan->blocks.push_back(b);
+  b.n_factors = n_factors;
+  b.factors.resize((size_t)n_factors);
+  for (int32_t f = 0; f < n_factors; ++f) {
+    const int rank = fcode[f] & 7;
+    if (rank > 4 || foffset[f] < 0 || foffset[f] + ipow(plan->n, rank) > plan->workspace)
+      return fail(C3L_EINVAL, "c3l_plan_add_block: factor exceeds workspace");
+    WedgeFactor wf;
+    wf.offset = foffset[f];
+    for (int k = 0; k < 4; ++k) wf.stride[k] = 0;
+    for (int q = 0; q < rank; ++q) {
+      const int pos = (fcode[f] >> (3 + 2 * q)) & 3;
+      if (pos >= deg) return fail(C3L_EINVAL, "c3l_plan_add_block: factor position out of range");
+      wf.stride[pos] += (int32_t)ipow(plan->n, rank - 1 - q);
+    }
+    b.factors[(size_t)f] = wf;
+  }
+  b.entries.resize((size_t)n_entries);
+  for (int32_t t = 0; t < n_entries; ++t) {
+    if (fstart[t + 1] < fstart[t]) return fail(C3L_EINVAL, "c3l_plan_add_block: fstart not monotone");
+    WedgeEntry en;
+    en.coeff = make_double2(coeff[2 * t], coeff[2 * t + 1]);
+    en.first = fstart[t];
+    en.nf = fstart[t + 1] - fstart[t];
+    en.pad[0] = en.pad[1] = 0;
+    if (en.nf != 2) b.two_factors = false;
+    b.entries[(size_t)t] = en;
+  }
+  b.coord.resize((size_t)count);
+  b.dest.assign(dest, dest + count);
+  for (int64_t k = 0; k < count; ++k) {
+    if (dest[k] < 0) return fail(C3L_EINVAL, "c3l_plan_add_block: negative destination");
+    if (dest[k] > b.max_dest) b.max_dest = dest[k];
+    uint32_t packed = 0;
+    for (int q = 0; q < deg; ++q) {
+      const int32_t c = coord[k * deg + q];
+      if (c < 0 || c >= plan->n) return fail(C3L_EINVAL, "c3l_plan_add_block: coordinate out of range");
+      packed |= (uint32_t)c << (8 * q);
+    }
+    b.coord[(size_t)k] = packed;
+  }
+  plan->blocks.push_back(std::move(b));
+  ProgramExec& pe = plan->programs[program];
+  for (void* p : pe.owned) (void)hipFree(p);
+  pe = ProgramExec();  // rebuilt at the next apply
   return C3L_OK;
 }
 
@@ -630,32 +745,118 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
   return C3L_OK;
 }
 
-int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out, void* stream) {
-  if (plan == nullptr || ((diag == nullptr) != (x == nullptr)) || (out == nullptr && diag != nullptr))
-    return fail(C3L_EINVAL, "c3l_plan_apply: invalid argument");
-  const bool internal = out == nullptr;  // creation programs write into the workspace
-  if (internal) out = reinterpret_cast<double*>(plan->ws);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  for (size_t i = 0; i < plan->blocks.size(); ++i) {
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int upload_owned(std::vector<void*>& owned, T** dst, const T* src, size_t count) {
+  *dst = nullptr;
+  if (count == 0) return C3L_OK;
+  void* ptr = nullptr;
+  if (hipMalloc(&ptr, sizeof(T) * count) != hipSuccess) return fail(C3L_ENOMEM, "hipMalloc failed in program build");
+  owned.push_back(ptr);
+  C3L_HIP_CHECK(hipMemcpy(ptr, src, sizeof(T) * count, hipMemcpyHostToDevice));
+  *dst = static_cast<T*>(ptr);
+  return C3L_OK;
+}
+
+// Lanes per coordinate: split the entries of small blocks over up to four waves
+// so that a program launch keeps at least ~2 workgroups per CU busy.
+int lanes_log2_for(const PlanBlock& b) {
+  int lg = 0;
+  while (lg < 2 && (b.count + (256 >> lg) - 1) / (256 >> lg) < 512 && b.n_entries >= (2 << lg)) ++lg;
+  return lg;
+}
+
+int build_program(c3l_plan* plan, int32_t program) {
+  ProgramExec& pe = plan->programs[program];
+  if (pe.built) return C3L_OK;
+  std::vector<size_t> members;
+  for (size_t i = 0; i < plan->blocks.size(); ++i)
+    if (plan->blocks[i].program == program && plan->blocks[i].count > 0) members.push_back(i);
+  // long-running tiles (most entries per wave) first
+  std::stable_sort(members.begin(), members.end(), [&](size_t a, size_t b) {
+    const PlanBlock& A = plan->blocks[a];
+    const PlanBlock& B = plan->blocks[b];
+    return (A.n_entries >> lanes_log2_for(A)) > (B.n_entries >> lanes_log2_for(B));
+  });
+  std::vector<WedgeBlockDesc> descs;
+  std::vector<int2> two, general;
+  for (size_t i : members) {
     const PlanBlock& b = plan->blocks[i];
-    if (b.program != program || b.count == 0) continue;
-    if (internal && b.max_dest >= plan->workspace)
-      return fail(C3L_EINVAL, "c3l_plan_apply: workspace program writes outside the workspace");
-    PendingTiming t{i, nullptr, nullptr};
-    if (plan->timing) {
-      C3L_HIP_CHECK(hipEventCreate(&t.start));
-      C3L_HIP_CHECK(hipEventCreate(&t.stop));
-      C3L_HIP_CHECK(hipEventRecord(t.start, s));
-    }
-    hipLaunchKernelGGL(wedge_block_kernel, dim3(grid_for(b.count)), dim3(kThreads), 0, s, plan->ws,
-                       reinterpret_cast<double2*>(out), b.coord, b.dest, b.count, b.deg, plan->n, b.n_entries, b.coeff,
-                       b.fstart, b.foffset, b.fcode, reinterpret_cast<const double2*>(diag),
-                       reinterpret_cast<const double2*>(x));
+    WedgeBlockDesc d;
+    d.count = b.count;
+    d.n_entries = b.n_entries;
+    d.lanes_log2 = lanes_log2_for(b);
+    uint32_t* coord = nullptr;
+    int64_t* dest = nullptr;
+    WedgeEntry* entries = nullptr;
+    WedgeFactor* factors = nullptr;
+    int rc = upload_owned(pe.owned, &coord, b.coord.data(), b.coord.size());
+    if (rc == C3L_OK) rc = upload_owned(pe.owned, &dest, b.dest.data(), b.dest.size());
+    if (rc == C3L_OK) rc = upload_owned(pe.owned, &entries, b.entries.data(), b.entries.size());
+    if (rc == C3L_OK) rc = upload_owned(pe.owned, &factors, b.factors.data(), b.factors.size());
+    if (rc != C3L_OK) return rc;
+    d.coord = coord;
+    d.dest = dest;
+    d.entries = entries;
+    d.factors = factors;
+    const int per = 256 >> d.lanes_log2;
+    std::vector<int2>& tiles = b.two_factors ? two : general;
+    for (int64_t first = 0; first < b.count; first += per) tiles.push_back(make_int2((int)descs.size(), (int)first));
+    descs.push_back(d);
+    if (b.max_dest > pe.max_dest) pe.max_dest = b.max_dest;
+  }
+  pe.tiles_two = (int32_t)two.size();
+  pe.tiles_general = (int32_t)general.size();
+  std::vector<int2> tiles(two);
+  tiles.insert(tiles.end(), general.begin(), general.end());
+  int rc = upload_owned(pe.owned, &pe.descs, descs.data(), descs.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.tiles, tiles.data(), tiles.size());
+  if (rc != C3L_OK) return rc;
+  pe.built = true;
+  return C3L_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out, void* stream) {
+  if (plan == nullptr || program < 0 || program >= kPrograms || ((diag == nullptr) != (x == nullptr)) ||
+      (out == nullptr && diag != nullptr))
+    return fail(C3L_EINVAL, "c3l_plan_apply: invalid argument");
+  int rc = build_program(plan, program);
+  if (rc != C3L_OK) return rc;
+  const ProgramExec& pe = plan->programs[program];
+  if (pe.tiles_two + pe.tiles_general == 0) return C3L_OK;
+  const bool internal = out == nullptr;  // creation programs write into the workspace
+  if (internal && pe.max_dest >= plan->workspace)
+    return fail(C3L_EINVAL, "c3l_plan_apply: workspace program writes outside the workspace");
+  double2* o = internal ? plan->ws : reinterpret_cast<double2*>(out);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PendingTiming t{program, nullptr, nullptr};
+  if (plan->timing) {
+    C3L_HIP_CHECK(hipEventCreate(&t.start));
+    C3L_HIP_CHECK(hipEventCreate(&t.stop));
+    C3L_HIP_CHECK(hipEventRecord(t.start, s));
+  }
+  const double2* dg = reinterpret_cast<const double2*>(diag);
+  const double2* xx = reinterpret_cast<const double2*>(x);
+  if (pe.tiles_two > 0) {
+    hipLaunchKernelGGL(wedge_program_kernel<true>, dim3(pe.tiles_two), dim3(kThreads), 0, s, plan->ws, o, pe.descs,
+                       pe.tiles, dg, xx);
     C3L_HIP_CHECK(hipGetLastError());
-    if (plan->timing) {
-      C3L_HIP_CHECK(hipEventRecord(t.stop, s));
-      plan->pending.push_back(t);
-    }
+  }
+  if (pe.tiles_general > 0) {
+    hipLaunchKernelGGL(wedge_program_kernel<false>, dim3(pe.tiles_general), dim3(kThreads), 0, s, plan->ws, o,
+                       pe.descs, pe.tiles + pe.tiles_two, dg, xx);
+    C3L_HIP_CHECK(hipGetLastError());
+  }
+  if (plan->timing) {
+    C3L_HIP_CHECK(hipEventRecord(t.stop, s));
+    plan->pending.push_back(t);
   }
   return C3L_OK;
 }

@@ -113,22 +113,28 @@ class DevicePlan:
         _lib.call("c3l_plan_set_timing", self.handle, 1 if enable else 0)
 
     def block_stats(self) -> list[dict]:
-        """Per output block: program, degree, coordinates, entries, factors, kernel ms, launches."""
+        """Per output block: program, degree, coordinates, entries, factors."""
 
         lib = _lib.load_library()
+        names = {v: k for k, v in PROGRAMS.items()}
         out = []
         for i in range(lib.c3l_plan_block_count(self.handle)):
             program, deg, n_entries, n_factors = (ctypes.c_int32() for _ in range(4))
-            count, launches = ctypes.c_int64(), ctypes.c_int64()
-            ms = ctypes.c_double()
+            count = ctypes.c_int64()
             _lib.call(
                 "c3l_plan_block_info", self.handle, i, ctypes.byref(program), ctypes.byref(deg), ctypes.byref(count),
-                ctypes.byref(n_entries), ctypes.byref(n_factors), ctypes.byref(ms), ctypes.byref(launches),
+                ctypes.byref(n_entries), ctypes.byref(n_factors),
             )
-            names = {v: k for k, v in PROGRAMS.items()}
             out.append(dict(program=names[program.value], deg=deg.value, count=count.value, entries=n_entries.value,
-                            factors=n_factors.value, ms=ms.value, launches=launches.value))
+                            factors=n_factors.value))
         return out
+
+    def program_time(self, program: str) -> tuple[float, int]:
+        """Accumulated kernel milliseconds and launches of one program since timing was enabled."""
+
+        ms, launches = ctypes.c_double(), ctypes.c_int64()
+        _lib.call("c3l_plan_program_time", self.handle, PROGRAMS[program], ctypes.byref(ms), ctypes.byref(launches))
+        return ms.value, launches.value
 
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:

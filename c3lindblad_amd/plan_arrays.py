@@ -153,6 +153,10 @@ def lower_plan(
                     jobs[:, 4].astype(np.int32),
                 )
             )
+    targets = {bp.target_atom for bp in plan.programs.get("creation", [])}
+    for bp in plan.programs.get("creation", []):
+        if targets.intersection(int(a) for a in bp.factor_atom):
+            raise ValueError("a creation tensor reads another creation tensor: one program launch cannot order them")
     blocks = []
     for name, programs in plan.programs.items():
         coords = out_sets.get(name)

@@ -73,11 +73,13 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
                        const int64_t* par_offset, const int32_t* par_rank, const int32_t* slot,
                        const int32_t* vec, const int32_t* sign);
 
-/* Register one output block of program `program`: `count` coordinates of degree `deg`
- * (coord[count*deg] local indices, dest[count] output positions) and `n_entries` wedge
- * entries: entry t = coeff[t] * prod_{f in [fstart[t], fstart[t+1])} ws[foffset[f] + flat_f],
- * flat_f = sum_s coord[pos_s] n^(rank-1-s) with
- * fcode[f] = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9 (rank <= 4). */
+/* Register one output block of program `program` (0 <= program < 8): `count` coordinates
+ * of degree `deg` (coord[count*deg] local indices < n <= 255, dest[count] output positions)
+ * and `n_entries` wedge entries: entry t = coeff[t] * prod_{f in [fstart[t], fstart[t+1])}
+ * ws[foffset[f] + flat_f], flat_f = sum_s coord[pos_s] n^(rank-1-s) with
+ * fcode[f] = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9 (rank <= 4).  All blocks of a
+ * program are evaluated by one launch (blocks of a program must not read each other's
+ * destinations); registering a block invalidates that program's device image. */
 int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
                        const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
                        int32_t n_factors, const int64_t* foffset, const int32_t* fcode);
@@ -94,13 +96,15 @@ int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const do
                    void* stream);
 
 /* Measurement hooks (no counterpart in the reference, which times with perf_counter,
- * cc/component_jax.py:53-65): when enabled, every block launch of c3l_plan_apply is
- * bracketed by HIP events on its stream; c3l_plan_block_info waits for them and
- * returns per-block metadata with the accumulated kernel time. */
+ * cc/component_jax.py:53-65): when enabled, every program launch of c3l_plan_apply is
+ * bracketed by HIP events on its stream; c3l_plan_program_time waits for them and
+ * returns the accumulated kernel time and launch count of one program.
+ * c3l_plan_block_info returns the metadata of one registered output block. */
 int c3l_plan_set_timing(c3l_plan* plan, int32_t enable);
+int c3l_plan_program_time(c3l_plan* plan, int32_t program, double* ms_total, int64_t* launches);
 int32_t c3l_plan_block_count(c3l_plan* plan);
 int c3l_plan_block_info(c3l_plan* plan, int32_t index, int32_t* program, int32_t* deg, int64_t* count,
-                        int32_t* n_entries, int32_t* n_factors, double* ms_total, int64_t* launches);
+                        int32_t* n_entries, int32_t* n_factors);
 
 /* ---- tilde real coordinates -------------------------------------------------------- */
 /* t[i] = ab[4i..4i+1] * x[k0[i]] + ab[4i+2..4i+3] * x[k1[i]]  (complex * real) */
