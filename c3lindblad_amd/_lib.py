@@ -40,6 +40,7 @@ SIGNATURES = {
     "c3l_plan_program_time": ([_P, _I32, _P, _P], ctypes.c_int),
     "c3l_tilde_unpack": ([_P, _P, _P, _P, _P, _I64, _P], ctypes.c_int),
     "c3l_tilde_pack": ([_P, _P, _P, _P, _I64, _P], ctypes.c_int),
+    "c3l_tilde_pack_precondition": ([_P, _P, _P, _P, _P, _I64, _P], ctypes.c_int),
     "c3l_rmultidot": ([_P, _I64, _I32, _P, _I64, _P, _P, _P], ctypes.c_int),
     "c3l_rmultiaxpy": ([_P, _I64, _I32, _P, ctypes.c_double, _P, _I64, _P], ctypes.c_int),
     "c3l_zmultidot": ([_P, _I64, _I32, _P, _I64, _I32, _P, _P, _P], ctypes.c_int),

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -145,7 +146,11 @@ struct WedgeBlockDesc {
   const WedgeFactor* factors;
   int32_t n_entries;
   int32_t lanes_log2;
+  int32_t reps;  // coordinates per thread (two-factor kernel, lanes_log2 == 0 only)
+  int32_t pad;
 };
+
+constexpr int kMaxReps = 4;
 
 __device__ __forceinline__ int64_t factor_index(const WedgeFactor& f, int i0, int i1, int i2, int i3) {
   return f.offset + (int64_t)(i0 * f.stride[0] + i1 * f.stride[1] + i2 * f.stride[2] + i3 * f.stride[3]);
@@ -165,47 +170,65 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(const double2* __res
   const int local = threadIdx.x & (per - 1);
   const int group = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> (8 - lg)));
   const int step = 1 << lg;
-  const int64_t e = (int64_t)tile.y + local;
-  const bool active = e < d.count;
-  int i0 = 0, i1 = 0, i2 = 0, i3 = 0;
-  if (active) {
-    const uint32_t c = d.coord[e];
-    i0 = c & 255u;
-    i1 = (c >> 8) & 255u;
-    i2 = (c >> 16) & 255u;
-    i3 = c >> 24;
+  constexpr int R = kTwoFactors ? kMaxReps : 1;
+  int i0[R], i1[R], i2[R], i3[R];
+  bool act[R];
+  int64_t e[R];
+  double2 acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    e[r] = (int64_t)tile.y + local + (int64_t)r * per;
+    act[r] = r < d.reps && e[r] < d.count;
+    i0[r] = i1[r] = i2[r] = i3[r] = 0;
+    if (act[r]) {
+      const uint32_t c = d.coord[e[r]];
+      i0[r] = c & 255u;
+      i1[r] = (c >> 8) & 255u;
+      i2[r] = (c >> 16) & 255u;
+      i3[r] = c >> 24;
+    }
+    acc[r] = make_double2(0.0, 0.0);
   }
-  double2 acc = make_double2(0.0, 0.0);
-  if (active) {
-    if (kTwoFactors) {
+  if (kTwoFactors) {
 #pragma unroll 2
-      for (int t = group; t < d.n_entries; t += step) {
-        const double2 c = d.entries[t].coeff;
-        const WedgeFactor fa = d.factors[2 * t];
-        const WedgeFactor fb = d.factors[2 * t + 1];
-        const double2 a = ws[factor_index(fa, i0, i1, i2, i3)];
-        const double2 b = ws[factor_index(fb, i0, i1, i2, i3)];
-        acc = cadd(acc, cmul(c, cmul(a, b)));
-      }
-    } else {
-      for (int t = group; t < d.n_entries; t += step) {
-        const WedgeEntry en = d.entries[t];
-        double2 prod = en.coeff;
-        for (int f = en.first; f < en.first + en.nf; ++f) prod = cmul(prod, ws[factor_index(d.factors[f], i0, i1, i2, i3)]);
-        acc = cadd(acc, prod);
+    for (int t = group; t < d.n_entries; t += step) {
+      const double2 c = d.entries[t].coeff;
+      const WedgeFactor fa = d.factors[2 * t];
+      const WedgeFactor fb = d.factors[2 * t + 1];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (act[r]) {
+          const double2 a = ws[factor_index(fa, i0[r], i1[r], i2[r], i3[r])];
+          const double2 b = ws[factor_index(fb, i0[r], i1[r], i2[r], i3[r])];
+          acc[r] = cadd(acc[r], cmul(c, cmul(a, b)));
+        }
       }
     }
+  } else if (act[0]) {
+    for (int t = group; t < d.n_entries; t += step) {
+      const WedgeEntry en = d.entries[t];
+      double2 prod = en.coeff;
+      for (int f = en.first; f < en.first + en.nf; ++f)
+        prod = cmul(prod, ws[factor_index(d.factors[f], i0[0], i1[0], i2[0], i3[0])]);
+      acc[0] = cadd(acc[0], prod);
+    }
   }
-  if (lg > 0) {
-    part[threadIdx.x] = acc;
+  if (lg > 0) {  // reps == 1 here
+    part[threadIdx.x] = acc[0];
     __syncthreads();
     if (group == 0)
-      for (int k = 1; k < step; ++k) acc = cadd(acc, part[k * per + local]);
+      for (int k = 1; k < step; ++k) acc[0] = cadd(acc[0], part[k * per + local]);
   }
-  if (group == 0 && active) {
-    const int64_t dd = d.dest[e];
-    if (diag != nullptr) acc = cadd(acc, cmul(diag[dd], x[dd]));
-    out[dd] = acc;
+  if (group == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (act[r]) {
+        const int64_t dd = d.dest[e[r]];
+        double2 v = acc[r];
+        if (diag != nullptr) v = cadd(v, cmul(diag[dd], x[dd]));
+        out[dd] = v;
+      }
+    }
   }
 }
 
@@ -257,6 +280,19 @@ __global__ void tilde_precondition_kernel(const double* __restrict__ x, double* 
   }
 }
 
+// x[c] = Re(w[c] * t[src[c]] * inv_diag[src[c]]): pack of the diagonally preconditioned
+// complex vector, equal to precondition(pack(t)) for tilde-symmetric t.
+__global__ void tilde_pack_precondition_kernel(const double2* __restrict__ t, double* __restrict__ x,
+                                               const int64_t* __restrict__ src, const double2* __restrict__ w,
+                                               const double2* __restrict__ inv_diag, int64_t real_dim) {
+  for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < real_dim;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = src[c];
+    const double2 v = cmul(cmul(t[i], inv_diag[i]), w[c]);
+    x[c] = v.x;
+  }
+}
+
 __global__ void rscale_kernel(const double* __restrict__ x, double* __restrict__ y, double a, int64_t dim) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x)
     y[i] = a * x[i];
@@ -269,7 +305,15 @@ __global__ void raxpby_kernel(double a, const double* __restrict__ x, double b, 
 
 // ------------------------------------------------------------------ Krylov kernels
 
-constexpr int kChunkRows = 8192;
+// Tall-skinny products V^T w / V a over Krylov bases.  A dot workgroup covers
+// kChunkRows rows (4 per thread, kept in registers) of kDotCols columns, so a
+// launch has (dim / kChunkRows) x (ncols / kDotCols) workgroups with several
+// independent loads per thread in flight; the partial sums of the row chunks
+// are added by a one-workgroup-per-column finish kernel.  An axpy workgroup
+// covers 64 rows; its four waves take every fourth column and meet in LDS.
+constexpr int kChunkRows = 1024;
+constexpr int kRowsPerThread = kChunkRows / 256;
+constexpr int kDotCols = 8;
 
 int64_t chunks_for(int64_t dim) { return (dim + kChunkRows - 1) / kChunkRows; }
 
@@ -300,18 +344,46 @@ __device__ T block_reduce(T v, T* shared) {
   return total;
 }
 
-// grid = (chunks, ncols): partial[k*chunks + c] = sum over rows of chunk c.
-__global__ void rmultidot_partial_kernel(const double* __restrict__ V, int64_t ld, const double* __restrict__ w,
-                                         int64_t dim, double* __restrict__ partial, int64_t chunks) {
-  __shared__ double shared[kThreads / 64];
-  const int k = blockIdx.y;
-  const int64_t begin = blockIdx.x * (int64_t)kChunkRows;
-  const int64_t end = begin + kChunkRows < dim ? begin + kChunkRows : dim;
-  const double* __restrict__ col = V + (int64_t)k * ld;
-  double acc = 0.0;
-  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) acc += col[i] * w[i];
-  const double total = block_reduce(acc, shared);
-  if (threadIdx.x == 0) partial[(int64_t)k * chunks + blockIdx.x] = total;
+// grid = (chunks, column groups): partial[k*chunks + c] = sum over the rows of chunk c.
+// With `extra` set, column ncols-1 is `extra` instead of V + (ncols-1)*ld.
+__global__ __launch_bounds__(256) void rmultidot_partial_kernel(const double* __restrict__ V, int64_t ld, int ncols,
+                                                                const double* __restrict__ extra,
+                                                                const double* __restrict__ w, int64_t dim,
+                                                                double* __restrict__ partial, int64_t chunks) {
+  __shared__ double red[4][kDotCols];
+  const int64_t row0 = blockIdx.x * (int64_t)kChunkRows + threadIdx.x;
+  const int k0 = blockIdx.y * kDotCols;
+  double wv[kRowsPerThread];
+#pragma unroll
+  for (int u = 0; u < kRowsPerThread; ++u) {
+    const int64_t i = row0 + u * 256;
+    wv[u] = i < dim ? w[i] : 0.0;
+  }
+  double acc[kDotCols];
+#pragma unroll
+  for (int c = 0; c < kDotCols; ++c) {
+    acc[c] = 0.0;
+    if (k0 + c < ncols) {
+      const double* __restrict__ col = (extra != nullptr && k0 + c == ncols - 1) ? extra : V + (int64_t)(k0 + c) * ld;
+#pragma unroll
+      for (int u = 0; u < kRowsPerThread; ++u) {
+        const int64_t i = row0 + u * 256;
+        if (i < dim) acc[c] += col[i] * wv[u];
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kDotCols; ++c) {
+    double v = acc[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) red[wave][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kDotCols && k0 + (int)threadIdx.x < ncols) {
+    const int c = threadIdx.x;
+    partial[(int64_t)(k0 + c) * chunks + blockIdx.x] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+  }
 }
 
 __global__ void rfinish_kernel(const double* __restrict__ partial, int64_t chunks, double* __restrict__ out) {
@@ -323,35 +395,77 @@ __global__ void rfinish_kernel(const double* __restrict__ partial, int64_t chunk
   if (threadIdx.x == 0) out[k] = total;
 }
 
-__global__ void rmultiaxpy_kernel(const double* __restrict__ V, int64_t ld, int ncols,
-                                  const double* __restrict__ alpha, double scale, double* __restrict__ w, int64_t dim) {
-  extern __shared__ double coef[];
-  for (int k = threadIdx.x; k < ncols; k += blockDim.x) coef[k] = scale * alpha[k];
-  __syncthreads();
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
-    double acc = w[i];
-    for (int k = 0; k < ncols; ++k) acc += coef[k] * V[(int64_t)k * ld + i];
-    w[i] = acc;
+// w[i] += scale * sum_k alpha[k] V[k*ld + i]; 64 rows per workgroup, columns split over the 4 waves.
+__global__ __launch_bounds__(256) void rmultiaxpy_kernel(const double* __restrict__ V, int64_t ld, int ncols,
+                                                         const double* __restrict__ alpha, double scale,
+                                                         double* __restrict__ w, int64_t dim) {
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t i = blockIdx.x * (int64_t)64 + lane;
+  double acc = 0.0;
+  if (i < dim) {
+#pragma unroll 4
+    for (int k = wave; k < ncols; k += 4) acc += alpha[k] * V[(int64_t)k * ld + i];
   }
+  part[wave][lane] = acc;
+  __syncthreads();
+  if (wave == 0 && i < dim) w[i] += scale * (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
 }
 
-__global__ void zmultidot_partial_kernel(const double2* __restrict__ V, int64_t ld, const double2* __restrict__ w,
-                                         int64_t dim, int conjugate, double2* __restrict__ partial, int64_t chunks) {
-  __shared__ cplx shared[kThreads / 64];
-  const int k = blockIdx.y;
-  const int64_t begin = blockIdx.x * (int64_t)kChunkRows;
-  const int64_t end = begin + kChunkRows < dim ? begin + kChunkRows : dim;
-  const double2* __restrict__ col = V + (int64_t)k * ld;
-  const double s = conjugate ? -1.0 : 1.0;
-  cplx acc{0.0, 0.0};
-  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
-    const double2 a = col[i];
-    const double2 b = w[i];
-    acc.x += a.x * b.x - s * a.y * b.y;
-    acc.y += a.x * b.y + s * a.y * b.x;
+// With `extra` set, column ncols-1 is `extra` (never conjugated) instead of V + (ncols-1)*ld.
+__global__ __launch_bounds__(256) void zmultidot_partial_kernel(const double2* __restrict__ V, int64_t ld, int ncols,
+                                                                const double2* __restrict__ extra,
+                                                                const double2* __restrict__ w, int64_t dim,
+                                                                int conjugate, double2* __restrict__ partial,
+                                                                int64_t chunks) {
+  __shared__ double2 red[4][kDotCols];
+  const int64_t row0 = blockIdx.x * (int64_t)kChunkRows + threadIdx.x;
+  const int k0 = blockIdx.y * kDotCols;
+  const double sg = conjugate ? -1.0 : 1.0;
+  double2 wv[kRowsPerThread];
+#pragma unroll
+  for (int u = 0; u < kRowsPerThread; ++u) {
+    const int64_t i = row0 + u * 256;
+    wv[u] = i < dim ? w[i] : make_double2(0.0, 0.0);
   }
-  const cplx total = block_reduce(acc, shared);
-  if (threadIdx.x == 0) partial[(int64_t)k * chunks + blockIdx.x] = make_double2(total.x, total.y);
+  double2 acc[kDotCols];
+#pragma unroll
+  for (int c = 0; c < kDotCols; ++c) {
+    acc[c] = make_double2(0.0, 0.0);
+    if (k0 + c < ncols) {
+      const bool ex = extra != nullptr && k0 + c == ncols - 1;
+      const double2* __restrict__ col = ex ? extra : V + (int64_t)(k0 + c) * ld;
+      const double sgc = ex ? 1.0 : sg;
+#pragma unroll
+      for (int u = 0; u < kRowsPerThread; ++u) {
+        const int64_t i = row0 + u * 256;
+        if (i < dim) {
+          const double2 a = col[i];
+          const double2 b = wv[u];
+          acc[c].x += a.x * b.x - sgc * a.y * b.y;
+          acc[c].y += a.x * b.y + sgc * a.y * b.x;
+        }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < kDotCols; ++c) {
+    double2 v = acc[c];
+    for (int off = 32; off > 0; off >>= 1) {
+      v.x += __shfl_down(v.x, off, 64);
+      v.y += __shfl_down(v.y, off, 64);
+    }
+    if (lane == 0) red[wave][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kDotCols && k0 + (int)threadIdx.x < ncols) {
+    const int c = threadIdx.x;
+    partial[(int64_t)(k0 + c) * chunks + blockIdx.x] =
+        make_double2(red[0][c].x + red[1][c].x + red[2][c].x + red[3][c].x,
+                     red[0][c].y + red[1][c].y + red[2][c].y + red[3][c].y);
+  }
 }
 
 __global__ void zfinish_kernel(const double2* __restrict__ partial, int64_t chunks, double2* __restrict__ out) {
@@ -367,17 +481,24 @@ __global__ void zfinish_kernel(const double2* __restrict__ partial, int64_t chun
   if (threadIdx.x == 0) out[k] = make_double2(total.x, total.y);
 }
 
-__global__ void zmultiaxpy_kernel(const double2* __restrict__ V, int64_t ld, int ncols,
-                                  const double2* __restrict__ alpha, double scale, double2* __restrict__ w,
-                                  int64_t dim) {
-  extern __shared__ double2 zcoef[];
-  for (int k = threadIdx.x; k < ncols; k += blockDim.x)
-    zcoef[k] = make_double2(scale * alpha[k].x, scale * alpha[k].y);
+__global__ __launch_bounds__(256) void zmultiaxpy_kernel(const double2* __restrict__ V, int64_t ld, int ncols,
+                                                         const double2* __restrict__ alpha, double scale,
+                                                         double2* __restrict__ w, int64_t dim) {
+  __shared__ double2 part[4][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t i = blockIdx.x * (int64_t)64 + lane;
+  double2 acc = make_double2(0.0, 0.0);
+  if (i < dim) {
+#pragma unroll 4
+    for (int k = wave; k < ncols; k += 4) acc = cadd(acc, cmul(alpha[k], V[(int64_t)k * ld + i]));
+  }
+  part[wave][lane] = acc;
   __syncthreads();
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
-    double2 acc = w[i];
-    for (int k = 0; k < ncols; ++k) acc = cadd(acc, cmul(zcoef[k], V[(int64_t)k * ld + i]));
-    w[i] = acc;
+  if (wave == 0 && i < dim) {
+    const double2 t = make_double2(part[0][lane].x + part[1][lane].x + part[2][lane].x + part[3][lane].x,
+                                   part[0][lane].y + part[1][lane].y + part[2][lane].y + part[3][lane].y);
+    w[i] = make_double2(w[i].x + scale * t.x, w[i].y + scale * t.y);
   }
 }
 
@@ -431,8 +552,24 @@ struct PlanBase {
   int32_t kind;
   int64_t offset;
   int64_t count;
+  std::vector<int64_t> src;  // host copies; runs of adjacent bases are uploaded together
+  std::vector<int8_t> sign;
+};
+
+// One scatter launch per run of workspace-adjacent base blocks of a kind.
+struct BaseRun {
+  int64_t offset;
+  int64_t count;
   int64_t* src = nullptr;
   int8_t* sign = nullptr;
+};
+
+constexpr int kKinds = 3;
+
+struct KindExec {
+  bool built = false;
+  std::vector<BaseRun> runs;
+  std::vector<void*> owned;
 };
 
 struct PlanLevel {
@@ -487,6 +624,7 @@ struct c3l_plan {
   std::vector<PlanLevel> levels;
   std::vector<PlanBlock> blocks;
   ProgramExec programs[kPrograms];
+  KindExec kinds[kKinds];
   std::vector<void*> owned;
   // optional per-program kernel timing with HIP events
   bool timing = false;
@@ -505,6 +643,18 @@ int upload(c3l_plan* plan, T** dst, const T* src, int64_t count) {
   if (hipMalloc(&ptr, sizeof(T) * (size_t)count) != hipSuccess) return fail(C3L_ENOMEM, "hipMalloc failed in plan upload");
   plan->owned.push_back(ptr);
   C3L_HIP_CHECK(hipMemcpy(ptr, src, sizeof(T) * (size_t)count, hipMemcpyHostToDevice));
+  *dst = static_cast<T*>(ptr);
+  return C3L_OK;
+}
+
+template <typename T>
+int upload_owned(std::vector<void*>& owned, T** dst, const T* src, size_t count) {
+  *dst = nullptr;
+  if (count == 0) return C3L_OK;
+  void* ptr = nullptr;
+  if (hipMalloc(&ptr, sizeof(T) * count) != hipSuccess) return fail(C3L_ENOMEM, "hipMalloc failed in program build");
+  owned.push_back(ptr);
+  C3L_HIP_CHECK(hipMemcpy(ptr, src, sizeof(T) * count, hipMemcpyHostToDevice));
   *dst = static_cast<T*>(ptr);
   return C3L_OK;
 }
@@ -574,6 +724,8 @@ int c3l_plan_destroy(c3l_plan* plan) {
   }
   for (ProgramExec& pe : plan->programs)
     for (void* p : pe.owned) (void)hipFree(p);
+  for (KindExec& ke : plan->kinds)
+    for (void* p : ke.owned) (void)hipFree(p);
   for (void* p : plan->owned) (void)hipFree(p);
   delete plan;
   return C3L_OK;
@@ -625,16 +777,19 @@ int32_t c3l_plan_block_count(c3l_plan* plan) { return plan == nullptr ? 0 : (int
 
 int c3l_plan_add_base(c3l_plan* plan, int32_t kind, int64_t offset, int32_t deg, const int64_t* src,
                       const int8_t* sign) {
-  if (plan == nullptr || deg < 0 || deg > 4 || offset < 0) return fail(C3L_EINVAL, "c3l_plan_add_base: invalid argument");
+  if (plan == nullptr || kind < 0 || kind >= kKinds || deg < 0 || deg > 4 || offset < 0)
+    return fail(C3L_EINVAL, "c3l_plan_add_base: invalid argument");
   PlanBase b;
   b.kind = kind;
   b.offset = offset;
   b.count = ipow(plan->n, deg);
   if (offset + b.count > plan->workspace) return fail(C3L_EINVAL, "c3l_plan_add_base: block exceeds workspace");
-  int rc = upload(plan, &b.src, src, b.count);
-  if (rc == C3L_OK) rc = upload(plan, &b.sign, sign, b.count);
-  if (rc != C3L_OK) return rc;
-  plan->bases.push_back(b);
+  b.src.assign(src, src + b.count);
+  b.sign.assign(sign, sign + b.count);
+  plan->bases.push_back(std::move(b));
+  KindExec& ke = plan->kinds[kind];
+  for (void* p : ke.owned) (void)hipFree(p);
+  ke = KindExec();  // rebuilt at the next load
   return C3L_OK;
 }
 
@@ -728,12 +883,38 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
 
 int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
   if (plan == nullptr || v == nullptr) return fail(C3L_EINVAL, "c3l_plan_load: invalid argument");
+  if (kind < 0 || kind >= kKinds) return fail(C3L_EINVAL, "c3l_plan_load: invalid kind");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double2* vv = reinterpret_cast<const double2*>(v);
-  for (const PlanBase& b : plan->bases) {
-    if (b.kind != kind) continue;
-    hipLaunchKernelGGL(scatter_padded_kernel, dim3(grid_for(b.count)), dim3(kThreads), 0, s, vv, plan->ws + b.offset,
-                       b.src, b.sign, b.count);
+  KindExec& ke = plan->kinds[kind];
+  if (!ke.built) {
+    std::vector<const PlanBase*> mine;
+    for (const PlanBase& b : plan->bases)
+      if (b.kind == kind && b.count > 0) mine.push_back(&b);
+    std::sort(mine.begin(), mine.end(), [](const PlanBase* a, const PlanBase* b) { return a->offset < b->offset; });
+    for (size_t i = 0; i < mine.size();) {
+      size_t j = i + 1;
+      while (j < mine.size() && mine[j]->offset == mine[j - 1]->offset + mine[j - 1]->count) ++j;
+      std::vector<int64_t> src;
+      std::vector<int8_t> sign;
+      for (size_t k = i; k < j; ++k) {
+        src.insert(src.end(), mine[k]->src.begin(), mine[k]->src.end());
+        sign.insert(sign.end(), mine[k]->sign.begin(), mine[k]->sign.end());
+      }
+      BaseRun run;
+      run.offset = mine[i]->offset;
+      run.count = (int64_t)src.size();
+      int rc = upload_owned(ke.owned, &run.src, src.data(), src.size());
+      if (rc == C3L_OK) rc = upload_owned(ke.owned, &run.sign, sign.data(), sign.size());
+      if (rc != C3L_OK) return rc;
+      ke.runs.push_back(run);
+      i = j;
+    }
+    ke.built = true;
+  }
+  for (const BaseRun& r : ke.runs) {
+    hipLaunchKernelGGL(scatter_padded_kernel, dim3(grid_for(r.count)), dim3(kThreads), 0, s, vv, plan->ws + r.offset,
+                       r.src, r.sign, r.count);
     C3L_HIP_CHECK(hipGetLastError());
   }
   for (const PlanLevel& l : plan->levels) {
@@ -749,17 +930,6 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
 
 namespace {
 
-template <typename T>
-int upload_owned(std::vector<void*>& owned, T** dst, const T* src, size_t count) {
-  *dst = nullptr;
-  if (count == 0) return C3L_OK;
-  void* ptr = nullptr;
-  if (hipMalloc(&ptr, sizeof(T) * count) != hipSuccess) return fail(C3L_ENOMEM, "hipMalloc failed in program build");
-  owned.push_back(ptr);
-  C3L_HIP_CHECK(hipMemcpy(ptr, src, sizeof(T) * count, hipMemcpyHostToDevice));
-  *dst = static_cast<T*>(ptr);
-  return C3L_OK;
-}
 
 // Lanes per coordinate: split the entries of small blocks over up to four waves
 // so that a program launch keeps at least ~2 workgroups per CU busy.
@@ -789,6 +959,16 @@ int build_program(c3l_plan* plan, int32_t program) {
     d.count = b.count;
     d.n_entries = b.n_entries;
     d.lanes_log2 = lanes_log2_for(b);
+    d.reps = 1;
+    if (b.two_factors && d.lanes_log2 == 0) {
+      // coordinates per thread for large two-factor blocks (more independent gathers in flight)
+      const char* env = std::getenv("C3L_WEDGE_REPS");
+      int reps = env ? std::atoi(env) : 2;  // 2 measured best on MI355X for the n = 25 JVP
+      reps = reps < 1 ? 1 : (reps > kMaxReps ? kMaxReps : reps);
+      while (reps > 1 && (b.count + 256 * reps - 1) / (256 * reps) < 512) reps >>= 1;
+      d.reps = reps;
+    }
+    d.pad = 0;
     uint32_t* coord = nullptr;
     int64_t* dest = nullptr;
     WedgeEntry* entries = nullptr;
@@ -802,7 +982,7 @@ int build_program(c3l_plan* plan, int32_t program) {
     d.dest = dest;
     d.entries = entries;
     d.factors = factors;
-    const int per = 256 >> d.lanes_log2;
+    const int per = (256 >> d.lanes_log2) * d.reps;
     std::vector<int2>& tiles = b.two_factors ? two : general;
     for (int64_t first = 0; first < b.count; first += per) tiles.push_back(make_int2((int)descs.size(), (int)first));
     descs.push_back(d);
@@ -882,6 +1062,19 @@ int c3l_tilde_pack(const double* t, double* x, const int64_t* src, const double*
   return C3L_OK;
 }
 
+int c3l_tilde_pack_precondition(const double* t, double* x, const int64_t* src, const double* w, const double* inv_diag,
+                                int64_t real_dim, void* stream) {
+  if (real_dim < 0 || (real_dim > 0 && (x == nullptr || t == nullptr || src == nullptr || w == nullptr ||
+                                        inv_diag == nullptr)))
+    return fail(C3L_EINVAL, "c3l_tilde_pack_precondition: invalid argument");
+  if (real_dim == 0) return C3L_OK;
+  hipLaunchKernelGGL(tilde_pack_precondition_kernel, dim3(grid_for(real_dim)), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(t), x, src,
+                     reinterpret_cast<const double2*>(w), reinterpret_cast<const double2*>(inv_diag), real_dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
 int64_t c3l_reduction_chunks(int64_t dim) { return chunks_for(dim); }
 
 int c3l_rmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim, double* scratch,
@@ -890,9 +1083,27 @@ int c3l_rmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, i
     return fail(C3L_EINVAL, "c3l_rmultidot: invalid argument");
   const int64_t chunks = chunks_for(dim);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(rmultidot_partial_kernel, dim3(chunks, ncols), dim3(kThreads), 0, s, V, ld, w, dim, scratch, chunks);
+  hipLaunchKernelGGL(rmultidot_partial_kernel, dim3(chunks, (ncols + kDotCols - 1) / kDotCols), dim3(kThreads), 0, s, V,
+                     ld, ncols, (const double*)nullptr, w, dim, scratch, chunks);
   C3L_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(rfinish_kernel, dim3(ncols), dim3(kThreads), 0, s, scratch, chunks, out);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+// out[k] = V_k . w for k < ncols and out[ncols] = extra . w, one pass over w (library-internal).
+int c3l_internal_rmultidot_extra(const double* V, int64_t ld, int32_t ncols, const double* extra, const double* w,
+                                 int64_t dim, double* scratch, double* out, void* stream) {
+  if (ncols < 0 || dim <= 0 || ld < dim || (ncols > 0 && V == nullptr) || extra == nullptr || w == nullptr ||
+      scratch == nullptr || out == nullptr)
+    return fail(C3L_EINVAL, "c3l_internal_rmultidot_extra: invalid argument");
+  const int64_t chunks = chunks_for(dim);
+  const int total = ncols + 1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(rmultidot_partial_kernel, dim3(chunks, (total + kDotCols - 1) / kDotCols), dim3(kThreads), 0, s, V,
+                     ld, total, extra, w, dim, scratch, chunks);
+  C3L_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(rfinish_kernel, dim3(total), dim3(kThreads), 0, s, scratch, chunks, out);
   C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;
 }
@@ -901,13 +1112,9 @@ int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alp
                    int64_t dim, void* stream) {
   if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || alpha == nullptr || w == nullptr)
     return fail(C3L_EINVAL, "c3l_rmultiaxpy: invalid argument");
-  constexpr int kCols = 2048;  // coefficients staged in LDS per launch
-  for (int32_t k0 = 0; k0 < ncols; k0 += kCols) {
-    const int32_t cols = ncols - k0 < kCols ? ncols - k0 : kCols;
-    hipLaunchKernelGGL(rmultiaxpy_kernel, dim3(grid_for(dim)), dim3(kThreads), sizeof(double) * cols,
-                       static_cast<hipStream_t>(stream), V + (int64_t)k0 * ld, ld, cols, alpha + k0, scale, w, dim);
-    C3L_HIP_CHECK(hipGetLastError());
-  }
+  hipLaunchKernelGGL(rmultiaxpy_kernel, dim3((dim + 63) / 64), dim3(kThreads), 0, static_cast<hipStream_t>(stream), V, ld,
+                     ncols, alpha, scale, w, dim);
+  C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;
 }
 
@@ -946,11 +1153,31 @@ int c3l_zmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, i
     return fail(C3L_EINVAL, "c3l_zmultidot: invalid argument");
   const int64_t chunks = chunks_for(dim);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(zmultidot_partial_kernel, dim3(chunks, ncols), dim3(kThreads), 0, s,
-                     reinterpret_cast<const double2*>(V), ld, reinterpret_cast<const double2*>(w), dim, conjugate,
-                     reinterpret_cast<double2*>(scratch), chunks);
+  hipLaunchKernelGGL(zmultidot_partial_kernel, dim3(chunks, (ncols + kDotCols - 1) / kDotCols), dim3(kThreads), 0, s,
+                     reinterpret_cast<const double2*>(V), ld, ncols, (const double2*)nullptr,
+                     reinterpret_cast<const double2*>(w), dim, conjugate, reinterpret_cast<double2*>(scratch), chunks);
   C3L_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(zfinish_kernel, dim3(ncols), dim3(kThreads), 0, s, reinterpret_cast<const double2*>(scratch), chunks,
+                     reinterpret_cast<double2*>(out));
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+// out[k] = conj?(V_k) . w for k < ncols and out[ncols] = extra . w (unconjugated), one pass over w
+// (library-internal).
+int c3l_internal_zmultidot_extra(const double* V, int64_t ld, int32_t ncols, const double* extra, const double* w,
+                                 int64_t dim, int32_t conjugate, double* scratch, double* out, void* stream) {
+  if (ncols < 0 || dim <= 0 || ld < dim || (ncols > 0 && V == nullptr) || extra == nullptr || w == nullptr ||
+      scratch == nullptr || out == nullptr)
+    return fail(C3L_EINVAL, "c3l_internal_zmultidot_extra: invalid argument");
+  const int64_t chunks = chunks_for(dim);
+  const int total = ncols + 1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(zmultidot_partial_kernel, dim3(chunks, (total + kDotCols - 1) / kDotCols), dim3(kThreads), 0, s,
+                     reinterpret_cast<const double2*>(V), ld, total, reinterpret_cast<const double2*>(extra),
+                     reinterpret_cast<const double2*>(w), dim, conjugate, reinterpret_cast<double2*>(scratch), chunks);
+  C3L_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(zfinish_kernel, dim3(total), dim3(kThreads), 0, s, reinterpret_cast<const double2*>(scratch), chunks,
                      reinterpret_cast<double2*>(out));
   C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;
@@ -960,16 +1187,10 @@ int c3l_zmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alp
                    int64_t dim, void* stream) {
   if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || alpha == nullptr || w == nullptr)
     return fail(C3L_EINVAL, "c3l_zmultiaxpy: invalid argument");
-  constexpr int kCols = 1024;  // coefficients staged in LDS per launch
-  const double2* Vz = reinterpret_cast<const double2*>(V);
-  const double2* az = reinterpret_cast<const double2*>(alpha);
-  for (int32_t k0 = 0; k0 < ncols; k0 += kCols) {
-    const int32_t cols = ncols - k0 < kCols ? ncols - k0 : kCols;
-    hipLaunchKernelGGL(zmultiaxpy_kernel, dim3(grid_for(dim)), dim3(kThreads), sizeof(double2) * cols,
-                       static_cast<hipStream_t>(stream), Vz + (int64_t)k0 * ld, ld, cols, az + k0, scale,
-                       reinterpret_cast<double2*>(w), dim);
-    C3L_HIP_CHECK(hipGetLastError());
-  }
+  hipLaunchKernelGGL(zmultiaxpy_kernel, dim3((dim + 63) / 64), dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     reinterpret_cast<const double2*>(V), ld, ncols, reinterpret_cast<const double2*>(alpha), scale,
+                     reinterpret_cast<double2*>(w), dim);
+  C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;
 }
 

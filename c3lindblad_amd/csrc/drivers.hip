@@ -25,8 +25,13 @@
 
 #include "../../include/c3l_hip.h"
 
-// Shared error slot of the library (defined in c3l_hip.hip).
+// Shared error slot and internal reductions of the library (defined in c3l_hip.hip).
 extern "C" int c3l_internal_fail(int code, const char* message);
+extern "C" int c3l_internal_rmultidot_extra(const double* V, int64_t ld, int32_t ncols, const double* extra,
+                                            const double* w, int64_t dim, double* scratch, double* out, void* stream);
+extern "C" int c3l_internal_zmultidot_extra(const double* V, int64_t ld, int32_t ncols, const double* extra,
+                                            const double* w, int64_t dim, int32_t conjugate, double* scratch,
+                                            double* out, void* stream);
 
 namespace {
 
@@ -48,18 +53,23 @@ __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// H[k, j] = c1[k] + c2[k] (k <= j); H[j+1, j] = sqrt(s); status[0] = H[j+1, j].
+// H[k, j] = c1[k] + c2[k] (k <= j); H[j+1, j] = sqrt(|w|^2); status[0] = H[j+1, j];
+// sink_v[j+1] = (sink . w) / H[j+1, j] = sink . v_{j+1} (0 after a breakdown).
+// coef holds c1 [m_max+1], c2 [m_max+1], |w|^2, sink . w (complex).
 __global__ void finish_column_kernel(const double2* __restrict__ coef, int j, int m_max, double2* __restrict__ H,
-                                     double* __restrict__ status) {
+                                     double* __restrict__ status, double2* __restrict__ sink_v, double breakdown) {
   const double2* c1 = coef;
   const double2* c2 = coef + (m_max + 1);
   double2* col = H + (int64_t)j * (m_max + 1);
   for (int k = threadIdx.x; k <= j; k += blockDim.x) col[k] = make_double2(c1[k].x + c2[k].x, c1[k].y + c2[k].y);
   if (threadIdx.x == 0) {
     const double s = coef[2 * (m_max + 1)].x;
+    const double2 sw = coef[2 * (m_max + 1) + 1];
     const double hn = sqrt(s > 0.0 ? s : 0.0);
     col[j + 1] = make_double2(hn, 0.0);
     status[0] = hn;
+    const double a = hn > breakdown ? 1.0 / hn : 0.0;
+    sink_v[j + 1] = make_double2(a * sw.x, a * sw.y);
   }
 }
 
@@ -98,10 +108,10 @@ __global__ void multishift_gmres_kernel(const double2* __restrict__ H, int j, in
                                         const double2* __restrict__ sink_v, double beta, double* __restrict__ rot_c,
                                         double2* __restrict__ rot_s, double2* __restrict__ pi,
                                         double2* __restrict__ gamma, double2* __restrict__ green,
-                                        double* __restrict__ resid) {
+                                        double* __restrict__ resid, const double* __restrict__ status) {
   const int lane = threadIdx.x & 63;
   const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (s >= n_shifts) return;  // whole wave exits together
+  if (s >= n_shifts || status[2] != 0.0) return;  // whole wave exits together; nothing after the stop
   const double2 z = shifts[s];
   const double2* col = H + (int64_t)j * (m_max + 1);
   const int64_t row = (int64_t)s * m_max;
@@ -185,10 +195,16 @@ __global__ void multishift_gmres_kernel(const double2* __restrict__ H, int j, in
 }
 
 __global__ void init_shift_state_kernel(double2* __restrict__ gamma, double2* __restrict__ green, double beta,
-                                        int n_shifts) {
+                                        int n_shifts, double* __restrict__ status) {
   for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_shifts; s += gridDim.x * blockDim.x) {
     gamma[s] = make_double2(beta, 0.0);
     green[s] = make_double2(0.0, 0.0);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    status[0] = 0.0;
+    status[1] = INFINITY;
+    status[2] = 0.0;
+    status[3] = 0.0;
   }
 }
 
@@ -196,13 +212,18 @@ __global__ void fill_kernel(double* __restrict__ a, double v, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = v;
 }
 
-// out[0] = max_i a[i] (NaN propagates as +inf)
-__global__ void max_kernel(const double* __restrict__ a, int n, double* __restrict__ out) {
+// Device-side stop test of the Arnoldi loop after step j (one workgroup).
+// status = [hn of the step, worst shift residual at the stop, stop flag, m]:
+// flag 1 = every shift converged (m >= min_dim), 2 = breakdown, 3 = nonfinite
+// recurrence.  Once the flag is set nothing here or in the multi-shift kernel
+// changes, so the host may enqueue further steps before it reads the flag.
+__global__ void arnoldi_status_kernel(const double* __restrict__ resid, int n, double* __restrict__ status, int j,
+                                      int min_dim, double tolerance, double breakdown) {
   __shared__ double part[256];
   double m = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = a[i];
-    m = (v != v) ? INFINITY : (v > m ? v : m);
+    const double v = resid[i];
+    m = (v != v) ? INFINITY : (v > m ? v : m);  // NaN counts as unconverged
   }
   part[threadIdx.x] = m;
   __syncthreads();
@@ -210,7 +231,14 @@ __global__ void max_kernel(const double* __restrict__ a, int n, double* __restri
     if ((int)threadIdx.x < w) part[threadIdx.x] = part[threadIdx.x] > part[threadIdx.x + w] ? part[threadIdx.x] : part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = part[0];
+  if (threadIdx.x == 0 && status[2] == 0.0) {
+    const double hn = status[0];
+    status[1] = part[0];
+    status[3] = (double)(j + 1);
+    if (!(hn == hn) || hn == INFINITY) status[2] = 3.0;
+    else if (hn <= breakdown) status[2] = 2.0;
+    else if (j + 1 >= min_dim && part[0] <= tolerance) status[2] = 1.0;
+  }
 }
 
 int64_t blocks_for(int64_t n, int t = 256) {
@@ -239,6 +267,15 @@ struct Gmres {
     D_CALL(c3l_plan_load(plan, 1, t_work, stream));
     D_CALL(c3l_plan_apply(plan, 2, jvp_diag, t_work, out_work, stream));
     D_CALL(c3l_tilde_pack(out_work, out, tm->src, tm->w, rd, stream));
+    if (jvp_calls) ++*jvp_calls;
+    return C3L_OK;
+  }
+  // out = M^{-1} J v with the preconditioner fused into the pack of J v
+  int apply_preconditioned(const double* v, double* out) {
+    D_CALL(c3l_tilde_unpack(v, t_work, tm->k0, tm->k1, tm->ab, tm->dim, stream));
+    D_CALL(c3l_plan_load(plan, 1, t_work, stream));
+    D_CALL(c3l_plan_apply(plan, 2, jvp_diag, t_work, out_work, stream));
+    D_CALL(c3l_tilde_pack_precondition(out_work, out, tm->src, tm->w, tm->inv_diag, rd, stream));
     if (jvp_calls) ++*jvp_calls;
     return C3L_OK;
   }
@@ -316,13 +353,12 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
     bool breakdown = false;
     double presid = 0.0;
     for (int j = 0; j < R; ++j) {
-      D_CALL(g.apply(V + (int64_t)j * rd, tmp));
-      D_CALL(g.precondition(tmp, w));
+      D_CALL(g.apply_preconditioned(V + (int64_t)j * rd, w));
       double* c1 = coef;
       double* c2 = coef + (R + 1);
       double* sq = coef + 2 * (R + 1);
-      D_CALL(c3l_rmultidot(w, rd, 1, w, rd, scratch, sq + 1, stream));  // ||w|| before orthogonalisation
-      D_CALL(c3l_rmultidot(V, rd, j + 1, w, rd, scratch, c1, stream));
+      // c1 = V^T w and, in the same pass, c1[j+1] = ||w||^2 before orthogonalisation
+      D_CALL(c3l_internal_rmultidot_extra(V, rd, j + 1, w, w, rd, scratch, c1, stream));
       D_CALL(c3l_rmultiaxpy(V, rd, j + 1, c1, -1.0, w, rd, stream));
       D_CALL(c3l_rmultidot(V, rd, j + 1, w, rd, scratch, c2, stream));
       D_CALL(c3l_rmultiaxpy(V, rd, j + 1, c2, -1.0, w, rd, stream));
@@ -331,7 +367,7 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
       D_CHECK(hipStreamSynchronize(s));
       for (int k = 0; k <= j; ++k) h(k, j) = col[k] + col[R + 1 + k];
       const double sqv = col[2 * (R + 1)];
-      const double h0v = col[2 * (R + 1) + 1];
+      const double h0v = col[j + 1];
       const double hn = std::sqrt(sqv > 0.0 ? sqv : 0.0);
       const double h0 = std::sqrt(h0v > 0.0 ? h0v : 0.0);
       if (!std::isfinite(hn)) return dfail(C3L_EINVAL, "c3l_gmres: nonfinite Krylov vector");
@@ -398,15 +434,15 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
   double* red = scratch + 2 * dim;                   // reduction partials after w
   double* c1 = coef;                                  // complex [m_max+1]
   double* c2 = coef + 2 * (m_max + 1);                // complex [m_max+1]
-  double* sq = coef + 4 * (m_max + 1);                // complex [1]
-  double* sink_v = coef + 4 * (m_max + 1) + 2;        // complex [1]
+  double* sq = coef + 4 * (m_max + 1);                // complex [2]: |w|^2, sink . w
+  double2* sink_v = reinterpret_cast<double2*>(coef + 4 * (m_max + 1) + 4);  // complex [m_max+1]: sink . v_j
   hipLaunchKernelGGL(init_shift_state_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s,
-                     reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), beta, n_shifts);
+                     reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), beta, n_shifts, status);
   D_CHECK(hipGetLastError());
-  double host[2] = {0.0, 0.0};
-  int32_t m = 0;
-  int32_t broke = 0;
-  double worst = INFINITY;
+  D_CALL(c3l_zmultidot(sink, dim, 1, V, dim, 0, red, reinterpret_cast<double*>(sink_v), stream));  // sink . v_0
+  // status = [hn, worst, stop flag, m]; the host reads it every kCheck steps
+  constexpr int kCheck = 8;
+  double host[4] = {0.0, INFINITY, 0.0, 0.0};
   const int waves_per_block = 4;
   const int64_t shift_blocks = (n_shifts + waves_per_block - 1) / waves_per_block;
   for (int32_t j = 0; j < m_max; ++j) {
@@ -414,35 +450,34 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
     double* wd = reinterpret_cast<double*>(w);
     D_CALL(c3l_plan_load(plan, 2, vj, stream));
     D_CALL(c3l_plan_apply(plan, 3, diag, vj, wd, stream));
-    D_CALL(c3l_zmultidot(sink, dim, 1, vj, dim, 0, red, sink_v, stream));
     D_CALL(c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c1, stream));
     D_CALL(c3l_zmultiaxpy(V, dim, j + 1, c1, -1.0, wd, dim, stream));
     D_CALL(c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c2, stream));
     D_CALL(c3l_zmultiaxpy(V, dim, j + 1, c2, -1.0, wd, dim, stream));
-    D_CALL(c3l_zmultidot(wd, dim, 1, wd, dim, 1, red, sq, stream));
-    hipLaunchKernelGGL(finish_column_kernel, dim3(1), dim3(256), 0, s, cz, j, m_max, Hz, status);
+    D_CALL(c3l_internal_zmultidot_extra(wd, dim, 1, sink, wd, dim, 1, red, sq, stream));
+    hipLaunchKernelGGL(finish_column_kernel, dim3(1), dim3(256), 0, s, cz, j, m_max, Hz, status, sink_v, breakdown_tol);
     D_CHECK(hipGetLastError());
     hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(dim)), dim3(256), 0, s, w, Vz + (int64_t)(j + 1) * dim, status,
                        breakdown_tol, dim);
     D_CHECK(hipGetLastError());
     hipLaunchKernelGGL(multishift_gmres_kernel, dim3(shift_blocks), dim3(64 * waves_per_block), 0, s, Hz, j, m_max,
-                       reinterpret_cast<const double2*>(shifts), n_shifts, reinterpret_cast<const double2*>(sink_v), beta,
+                       reinterpret_cast<const double2*>(shifts), n_shifts, sink_v + j, beta,
                        rot_c, reinterpret_cast<double2*>(rot_s), reinterpret_cast<double2*>(pi),
-                       reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), resid);
+                       reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), resid, status);
     D_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(max_kernel, dim3(1), dim3(256), 0, s, resid, n_shifts, status + 1);
+    hipLaunchKernelGGL(arnoldi_status_kernel, dim3(1), dim3(256), 0, s, resid, n_shifts, status, j, min_dim, tolerance,
+                       breakdown_tol);
     D_CHECK(hipGetLastError());
-    D_CHECK(hipMemcpyAsync(host, status, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-    D_CHECK(hipStreamSynchronize(s));
-    m = j + 1;
-    worst = host[1];
-    if (!std::isfinite(host[0])) return dfail(C3L_EINVAL, "c3l_arnoldi: nonfinite Arnoldi recurrence");
-    if (host[0] <= breakdown_tol) {
-      broke = 1;
-      break;
+    if ((j + 1) % kCheck == 0 || j + 1 == m_max) {
+      D_CHECK(hipMemcpyAsync(host, status, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+      D_CHECK(hipStreamSynchronize(s));
+      if (host[2] != 0.0) break;
     }
-    if (m >= min_dim && worst <= tolerance) break;
   }
+  if (host[2] == 3.0) return dfail(C3L_EINVAL, "c3l_arnoldi: nonfinite Arnoldi recurrence");
+  const int32_t m = (int32_t)host[3];
+  const int32_t broke = host[2] == 2.0 ? 1 : 0;
+  const double worst = host[1];
   *m_out = m;
   *breakdown_out = broke;
   *worst_out = worst;

@@ -114,6 +114,12 @@ int c3l_tilde_unpack(const double* x, double* t, const int64_t* k0, const int64_
 int c3l_tilde_pack(const double* t, double* x, const int64_t* src, const double* w, int64_t real_dim,
                    void* stream);
 
+/* x[c] = Re(w[c] * t[src[c]] * inv_diag[src[c]]): the preconditioned pack of a tilde-symmetric
+ * complex vector in one pass (TildeMap.pack after _preconditioner, symmetry.py:94-107,
+ * newton_krylov.py:158-192). */
+int c3l_tilde_pack_precondition(const double* t, double* x, const int64_t* src, const double* w, const double* inv_diag,
+                                int64_t real_dim, void* stream);
+
 /* ---- Krylov building blocks --------------------------------------------------------- */
 /* Real: out[k] = sum_i V[k*ld + i] * w[i], k < ncols (partial sums in scratch of
  * at least ncols*c3l_reduction_chunks(dim) doubles). */
@@ -183,8 +189,10 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
  * V[0] = b/beta on entry; H complex, column j at H + j*(m_max+1); rot_c [n_shifts*m_max]
  * doubles, rot_s and pi [n_shifts*m_max] complex, gamma and green [n_shifts] complex, resid
  * [n_shifts] doubles (relative GMRES residual); scratch >= 2*dim + 2*chunks*(m_max+1) doubles;
- * coef >= 4*(m_max+1)+4 doubles; status >= 2 doubles.  Stops at the first m >= min_dim with
- * every shift's residual <= tolerance, at a breakdown, or at m_max. */
+ * coef >= 6*(m_max+1)+4 doubles; status >= 4 doubles.  Stops at the first m >= min_dim with
+ * every shift's residual <= tolerance, at a breakdown, or at m_max.  The stop test runs on
+ * the device after every step and freezes the shift state; the host reads it every 8 steps,
+ * so up to 7 further basis vectors may be computed (and ignored) after the stop. */
 int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
                 const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
                 double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
