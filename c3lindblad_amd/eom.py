@@ -78,6 +78,53 @@ class ComponentEOMProblem:
     def dimension(self) -> int:
         return self.coordinates.dimension
 
+    def source_plan(self, site: int) -> LoweredPlan:
+        """Plan whose ``source_addition``/``source_removal`` programs evaluate the two sources of ``site``.
+
+        A source is ``pack(F_creator + ι_F T)`` (``component_tensor.py:2788-2797``): the
+        degree-1 and degree-3 part of ``exp(-T) F exp(T)|0>`` for the channel operator ``F``
+        of the charge (``component_tensor.py:2816-2858``), so the term compiler expands it
+        like one factor of a residual product.
+        """
+
+        plans = self.__dict__.setdefault("_source_plans", {})
+        if site not in plans:
+            cc = self.oracle.cc
+            ops = channel_operators(cc.reference, site, self.oracle.response_spin, self.charge)
+            compiler = TermCompiler(cc.model.n_sites, ())
+            actions = {}
+            for name in ("addition", "removal"):
+                op, coefficient = ops[name]
+                terms = compiler.apply_factor({(): complex(coefficient)}, compiler._lin(op), 3)
+                actions["source_" + name] = ({k: v for k, v in terms.items() if term_key(k) in self.keys}, self.keys)
+            plan = emit_plan(actions, cc.model.n_sites, compiler.table.array())
+            plans[site] = lower_plan(
+                plan,
+                {"T": (cc.coordinates, STATIC_KEYS)},
+                {"source_addition": self.coordinates, "source_removal": self.coordinates},
+            )
+        return plans[site]
+
+    def source_device(self, site: int):
+        """Uploaded source plan of ``site`` (shared by every context of this problem)."""
+
+        plans = self.__dict__.setdefault("_source_devices", {})
+        if site not in plans:
+            from .device import DevicePlan
+
+            plans[site] = DevicePlan(self.source_plan(site))
+        return plans[site]
+
+    def sink(self, site: int) -> np.ndarray:
+        """Rank-one annihilator coefficients of the channel's sink operator (cached per site)."""
+
+        sinks = self.__dict__.setdefault("_sinks", {})
+        if site not in sinks:
+            cc = self.oracle.cc
+            op = channel_operators(cc.reference, site, self.oracle.response_spin, self.charge)["sink"][0]
+            sinks[site] = _sink(self, op.components)
+        return sinks[site]
+
     def device_plan(self, slot: int = 0):
         """Uploaded sigma plan number ``slot``; independent slots can run concurrently."""
 
@@ -107,11 +154,17 @@ class ComponentEOMContext:
         return self._devices[slot]
 
 
-def component_eom_context(problem: ComponentEOMProblem, amplitudes) -> ComponentEOMContext:
+def component_eom_context(problem: ComponentEOMProblem, amplitudes, *, device_amplitudes=None) -> ComponentEOMContext:
+    """``device_amplitudes`` (a GPU copy of the same amplitudes, e.g. from the solver) skips the upload."""
+
     t = np.asarray(amplitudes, dtype=np.complex128)
     if t.shape != (problem.static.oracle.num_amplitudes,) or not np.all(np.isfinite(t)):
         raise ValueError("amplitudes must be finite and match the CCSD basis")
     context = ComponentEOMContext(problem, t)
+    if device_amplitudes is not None:
+        if tuple(device_amplitudes.shape) != t.shape or not device_amplitudes.is_cuda:
+            raise ValueError("device_amplitudes must be a GPU vector matching the CCSD basis")
+        context.__dict__["_device_t"] = device_amplitudes
     context.device  # make the amplitude atoms resident on plan slot 0
     return context
 
@@ -123,9 +176,17 @@ class EOMDevice:
         torch = _lib.require_device()
         self.torch = torch
         self.plan = context.problem.device_plan(slot)
-        self.t = torch.from_numpy(np.ascontiguousarray(context.amplitudes)).to(_lib.DEVICE)
+        shared = context.__dict__.get("_device_t")
+        if shared is None:
+            shared = torch.from_numpy(np.ascontiguousarray(context.amplitudes)).to(_lib.DEVICE)
+            context.__dict__["_device_t"] = shared
+        self.t = shared
         self.token = self.plan.load("T", self.t)
-        self.diag = torch.from_numpy(np.ascontiguousarray(context.problem.diagonal)).to(_lib.DEVICE)
+        diag = context.problem.__dict__.get("_device_diagonal")
+        if diag is None:
+            diag = torch.from_numpy(np.ascontiguousarray(context.problem.diagonal)).to(_lib.DEVICE)
+            context.problem.__dict__["_device_diagonal"] = diag
+        self.diag = diag
         self.dim = context.problem.dimension
 
     def sigma(self, x, out) -> None:
@@ -200,44 +261,24 @@ class ComponentEOMSources:
     site: int
 
 
-def _block_values(coords: CoordinateSet, key, values: np.ndarray, out: np.ndarray) -> None:
-    block = coords.block(key)
-    if block is not None and block.positions.size:
-        out[block.positions] += values[tuple(block.indices.T)]
+def channel_operators(reference, site: int, spin: int, charge: int) -> dict:
+    """Source operators (with their scalar factors) and sink of one charge channel.
 
+    Charge +1: addition ``c†``, removal ``-i c̃``, sink ``c``; charge -1: addition
+    ``-i c̃†``, removal ``c``, sink ``c†`` (``component_tensor.py:2829-2851``).
+    """
 
-def _source(problem: ComponentEOMProblem, amplitudes: np.ndarray, components) -> np.ndarray:
-    """``pack(F_creator + ι_F T)`` (``component_tensor.py:2788-2797``), host FP64."""
-
-    n = problem.static.oracle.model.n_sites
-    coords = problem.coordinates
-    out = np.zeros(coords.dimension, dtype=np.complex128)
-    create = np.zeros((4, n), dtype=np.complex128)
-    annihilate = np.zeros((4, n), dtype=np.complex128)
-    for kind, mode, c in components:
-        g, q = divmod(mode, n)
-        (create if kind == "create" else annihilate)[g, q] += c
-    for g in range(4):
-        key = tuple(1 if k == g else 0 for k in range(4))
-        _block_values(coords, key, create[g], out)
-    # interior of the amplitudes with the annihilator part, group by group
-    static_coords = problem.static.oracle.coordinates
-    from .plan_arrays import padded_scatter
-
-    for key in STATIC_KEYS:
-        src, sign = padded_scatter(static_coords, key)
-        deg = sum(key)
-        padded = (np.where(src >= 0, amplitudes[np.maximum(src, 0)], 0.0) * sign).reshape((n,) * deg)
-        slots = [g for g, c in enumerate(key) for _ in range(c)]
-        for g in range(4):
-            if key[g] == 0 or not np.any(annihilate[g]):
-                continue
-            s = slots.index(g)
-            value = np.tensordot(padded, annihilate[g], axes=([s], [0])) * (-1.0 if s % 2 else 1.0)
-            rest = list(key)
-            rest[g] -= 1
-            _block_values(coords, tuple(rest), value, out)
-    return out
+    if charge == 1:
+        return {
+            "addition": (physical_mode_operator(reference, site, spin, "c_dagger"), 1.0),
+            "removal": (physical_mode_operator(reference, site, spin, "tilde_c"), -1j),
+            "sink": (physical_mode_operator(reference, site, spin, "c"), 1.0),
+        }
+    return {
+        "addition": (physical_mode_operator(reference, site, spin, "tilde_c_dagger"), -1j),
+        "removal": (physical_mode_operator(reference, site, spin, "c"), 1.0),
+        "sink": (physical_mode_operator(reference, site, spin, "c_dagger"), 1.0),
+    }
 
 
 def _sink(problem: ComponentEOMProblem, components) -> np.ndarray:
@@ -256,22 +297,20 @@ def _sink(problem: ComponentEOMProblem, components) -> np.ndarray:
 
 
 def component_eom_sources(context: ComponentEOMContext, *, site: int | None = None) -> ComponentEOMSources:
-    """Addition/removal sources and the trace sink (``component_tensor.py:2816-2858``)."""
+    """Addition/removal sources and the trace sink (``component_tensor.py:2816-2858``), sources on the GPU."""
 
     cc = context.problem.static.oracle
     target = cc.model.impurity_site if site is None else site
     if isinstance(target, (bool, np.bool_)) or not isinstance(target, (int, np.integer)) or not 0 <= int(target) < cc.model.n_sites:
         raise ValueError("site is outside the auxiliary model")
     target = int(target)
-    spin = context.problem.oracle.response_spin
-    ref = cc.reference
-    t = context.amplitudes
-    if context.problem.charge == 1:
-        addition = _source(context.problem, t, physical_mode_operator(ref, target, spin, "c_dagger").components)
-        removal = -1j * _source(context.problem, t, physical_mode_operator(ref, target, spin, "tilde_c").components)
-        sink_op = physical_mode_operator(ref, target, spin, "c")
-    else:
-        addition = -1j * _source(context.problem, t, physical_mode_operator(ref, target, spin, "tilde_c_dagger").components)
-        removal = _source(context.problem, t, physical_mode_operator(ref, target, spin, "c").components)
-        sink_op = physical_mode_operator(ref, target, spin, "c_dagger")
-    return ComponentEOMSources(addition, removal, _sink(context.problem, sink_op.components), context.problem.charge, target)
+    problem = context.problem
+    plan = problem.source_device(target)
+    torch = _lib.require_device()
+    dev = context.device
+    plan.load("T", dev.t)
+    add = torch.zeros(problem.dimension, dtype=torch.complex128, device=_lib.DEVICE)
+    rem = torch.zeros_like(add)
+    plan.apply("source_addition", add)
+    plan.apply("source_removal", rem)
+    return ComponentEOMSources(add.cpu().numpy(), rem.cpu().numpy(), problem.sink(target), problem.charge, target)

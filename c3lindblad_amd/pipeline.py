@@ -57,6 +57,8 @@ class SolvePipeline:
         for b in blocks.values():
             b.device_plan(0)
             b.device_plan(1)
+            b.source_device(model.impurity_site)
+            b.sink(model.impurity_site)
         return cls(static, blocks, np.asarray(omega, dtype=np.float64), float(eta), perf_counter() - started,
                    policy or NewtonKrylovPolicy(), rtol)
 
@@ -72,8 +74,9 @@ class SolvePipeline:
         charges = list(self.eom)
         pairs = []
         for q in charges:
-            ctx = component_eom_context(self.eom[q], sol.amplitudes)
+            ctx = component_eom_context(self.eom[q], sol.amplitudes, device_amplitudes=sol.device_amplitudes)
             pairs.append((ComponentSigmaOperator(ctx), component_eom_sources(ctx)))
+        t_sources = perf_counter()
         results = retarded_green_blocks(pairs, self.omega, self.eta, rtol=self.rtol, concurrent=self.concurrent)
         for q, result in zip(charges, results):
             green[q] = result.retarded
@@ -88,6 +91,6 @@ class SolvePipeline:
             residual_evaluations=sol.residual_evaluations,
             krylov_dimensions=dims,
             max_shift_residual=worst,
-            seconds={"ccsd": t1 - t0, "eom_green": t2 - t1},
+            seconds={"ccsd": t1 - t0, "eom_green": t2 - t1, "eom_contexts_sources": t_sources - t1},
             amplitudes=sol.amplitudes,
         )

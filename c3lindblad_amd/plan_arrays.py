@@ -26,7 +26,16 @@ from .algebra import Plan, key_slots
 from .lowering import BlockKey, CoordinateSet
 
 KINDS = {"T": 0, "D": 1, "X": 2}
-PROGRAMS = {"residual": 0, "reference": 1, "jvp": 2, "sigma": 3, "creation": 4, "observable": 5}
+PROGRAMS = {
+    "residual": 0,
+    "reference": 1,
+    "jvp": 2,
+    "sigma": 3,
+    "creation": 4,
+    "observable": 5,
+    "source_addition": 6,
+    "source_removal": 7,
+}
 
 
 def encode_factors(ranks: np.ndarray, positions: np.ndarray) -> np.ndarray:

@@ -136,14 +136,14 @@ def arnoldi_resolvent(
     dev = _lib.DEVICE
     c128, f64 = torch.complex128, torch.float64
     ns = z.size
-    V = torch.zeros(m_max + 1, n, dtype=c128, device=dev)
+    V = torch.empty(m_max + 1, n, dtype=c128, device=dev)  # columns are written before they are read
     V[0].copy_(torch.from_numpy(b / beta))
     H = torch.zeros(m_max, m_max + 1, dtype=c128, device=dev)  # row j = column j of H
     zs = torch.from_numpy(z).to(dev)
     sink_d = torch.from_numpy(np.ascontiguousarray(sink_h)).to(dev)
-    rot_c = torch.zeros(ns * m_max, dtype=f64, device=dev)
-    rot_s = torch.zeros(ns * m_max, dtype=c128, device=dev)
-    pi = torch.zeros(ns * m_max, dtype=c128, device=dev)
+    rot_c = torch.empty(ns * m_max, dtype=f64, device=dev)  # per-shift state, written before read
+    rot_s = torch.empty(ns * m_max, dtype=c128, device=dev)
+    pi = torch.empty(ns * m_max, dtype=c128, device=dev)
     gamma = torch.zeros(ns, dtype=c128, device=dev)
     green = torch.zeros(ns, dtype=c128, device=dev)
     resid = torch.zeros(ns, dtype=f64, device=dev)

@@ -25,7 +25,7 @@ import numpy as np
 
 from . import _lib
 from .device import Reducer, raxpby, rscale
-from .static import ComponentStaticAction, ComponentStaticProblem, component_static_residual
+from .static import ComponentStaticAction, ComponentStaticProblem
 
 
 @dataclass(frozen=True)
@@ -106,6 +106,7 @@ class ComponentCCSolution:
     wall_seconds: float
     peak_rss_kib: int
     device_seconds: dict = field(default_factory=dict)
+    device_amplitudes: object = field(default=None, repr=False)  # torch complex128 copy on the GPU
 
 
 class _DeviceNewton:
@@ -127,12 +128,16 @@ class _DeviceNewton:
         self.coef = torch.zeros(2 * (restart + 2) + 2, dtype=f64, device=_lib.DEVICE)
         self.host = _lib.pinned(torch.zeros(2 * (restart + 2) + 2, dtype=f64))
         self.reducer = Reducer(D, restart + 2, complex_=False)
-        diag = -1j * problem.oracle.rapidity_sums()
-        mags = np.abs(diag)
-        threshold = 1.0e-12 * max(float(np.max(mags, initial=0.0)), 1.0)
-        small = mags < threshold
-        diag[small] = threshold * np.exp(1j * np.angle(diag[small] + 1e-300))
-        self.inv_diag = torch.from_numpy(1.0 / diag).to(_lib.DEVICE)
+        cached = problem.__dict__.get("_inverse_preconditioner")
+        if cached is None:
+            diag = -1j * problem.oracle.rapidity_sums()
+            mags = np.abs(diag)
+            threshold = 1.0e-12 * max(float(np.max(mags, initial=0.0)), 1.0)
+            small = mags < threshold
+            diag[small] = threshold * np.exp(1j * np.angle(diag[small] + 1e-300))
+            cached = torch.from_numpy(1.0 / diag).to(_lib.DEVICE)
+            problem.__dict__["_inverse_preconditioner"] = cached
+        self.inv_diag = cached
         self.jvp_calls = 0
         self.residual_calls = 0
 
@@ -218,8 +223,14 @@ def solve_component_ccsd(
     converged = False
     previous_norm = None
     previous_tol = None
+    # `current`: state.r and the device residual/amplitude buffers belong to x (an accepted
+    # line-search trial already evaluated the residual at the new x, so it is not recomputed)
+    current = False
+    norm = np.inf
     for iteration in range(controls.maximum_newton_iterations + 1):
-        norm = state.residual(x, state.r)
+        if not current:
+            norm = state.residual(x, state.r)
+            current = True
         if not np.isfinite(norm):
             break
         if norm <= controls.tolerance:
@@ -235,31 +246,38 @@ def solve_component_ccsd(
         )
         if controls.forcing_strategy == "eisenstat_walker" and not state.acceptable_step(norm, true, tol, atol):
             history.append(NewtonKrylovStep(iteration, norm, tol, info, its, true, state.norm(state.step), 0.0, norm))
+            current = False  # GMRES reused the residual buffers
             break
         damping = 1.0
         accepted = False
         accepted_norm = np.inf
+        step_norm = state.norm(state.step)
         for _ in range(controls.maximum_line_search_steps):
             state.xtrial.copy_(x)
             raxpby(damping, state.step, 1.0, state.xtrial, torch)
             accepted_norm = state.residual(state.xtrial, state.trial)
             if accepted_norm <= (1.0 - controls.armijo * damping) * norm:
                 x.copy_(state.xtrial)
+                state.r, state.trial = state.trial, state.r
                 accepted = True
                 if accepted_step_callback is not None:
                     accepted_step_callback(iteration, oracle.tilde.unpack(x.cpu().numpy()), accepted_norm)
                 break
             damping *= 0.5
-        history.append(
-            NewtonKrylovStep(iteration, norm, tol, info, its, true, state.norm(state.step), damping, accepted_norm)
-        )
+        history.append(NewtonKrylovStep(iteration, norm, tol, info, its, true, step_norm, damping, accepted_norm))
         if not accepted:
+            current = False
             break
         previous_norm, previous_tol = norm, tol
+        norm = accepted_norm
+    if not current:
+        state.residual(x, state.r)
+    dev = state.dev
     xh = x.cpu().numpy()
-    amplitudes = oracle.tilde.unpack(xh)
-    final = component_static_residual(problem, amplitudes)
-    packed = float(np.linalg.norm(oracle.tilde.pack(final.retained, check=False)))
+    amplitudes = dev.t.cpu().numpy()
+    retained = dev.out.cpu().numpy()
+    final = ComponentStaticAction(retained, complex(dev.ref.cpu().numpy()[0]), 0.0)
+    packed = float(np.linalg.norm(state.r.cpu().numpy()))
     return ComponentCCSolution(
         problem=problem,
         variables=xh,
@@ -270,9 +288,10 @@ def solve_component_ccsd(
         reference_residual=float(abs(final.reference)),
         converged=bool(converged and packed <= controls.tolerance),
         iterations=len(history),
-        residual_evaluations=state.residual_calls + 1,
+        residual_evaluations=state.residual_calls,
         jvp_evaluations=state.jvp_calls,
         history=tuple(history),
         wall_seconds=float(perf_counter() - started),
         peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
+        device_amplitudes=dev.t.clone(),
     )

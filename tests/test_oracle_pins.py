@@ -92,7 +92,7 @@ def test_compiled_plan_equals_polynomial_oracle(n, U, seed):
 
 @pytest.mark.parametrize("n,charge", [(2, 1), (3, -1), (4, 1)])
 def test_compiled_sigma_and_sources_equal_polynomial_oracle(n, charge):
-    from c3lindblad_amd.eom import ComponentEOMProblem, _sink, _source, eom_ccsd_problem
+    from c3lindblad_amd.eom import ComponentEOMProblem, _sink, channel_operators, eom_ccsd_problem
     from c3lindblad_amd.lowering import physical_mode_operator
     from oracle import polynomial as poly
     from oracle.port import PortPlan
@@ -110,9 +110,17 @@ def test_compiled_sigma_and_sources_equal_polynomial_oracle(n, charge):
     assert _rel(got, poly.sigma(p, t, masks, x)) < 1e-13
     for channel in ("c", "c_dagger", "tilde_c", "tilde_c_dagger"):
         comps = physical_mode_operator(p.reference, 0, 0, channel).components
-        want = poly.source(p, t, masks, comps)
-        assert np.max(np.abs(_source(ep, t, comps) - want)) < 1e-14 * max(1.0, np.max(np.abs(want)))
         assert np.array_equal(_sink(ep, comps), poly.sink(masks, comps))
+    for site in range(n):
+        ops = channel_operators(p.reference, site, 0, charge)
+        port = PortPlan(ep.source_plan(site))
+        port.load(0, t)
+        for name in ("addition", "removal"):
+            op, coefficient = ops[name]
+            want = coefficient * poly.source(p, t, masks, op.components)
+            got = port.apply(6 if name == "addition" else 7, np.zeros(ep.dimension, complex))
+            assert np.max(np.abs(got - want)) < 1e-14 * max(1.0, np.max(np.abs(want)))
+        assert np.array_equal(ep.sink(site), poly.sink(masks, ops["sink"][0].components))
 
 
 def test_empty_and_degenerate_inputs_are_rejected():

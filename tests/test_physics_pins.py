@@ -37,7 +37,7 @@ def test_lowered_generator_spectrum_equals_lindbladian(n, U, seed):
 
 
 def _oracle_green(model, omega, eta, charge=1):
-    from c3lindblad_amd.eom import ComponentEOMProblem, _sink, _source, eom_ccsd_problem
+    from c3lindblad_amd.eom import ComponentEOMProblem, eom_ccsd_problem
     from c3lindblad_amd.lowering import factorized_cc_problem, physical_mode_operator
     from c3lindblad_amd.static import ComponentStaticProblem
     from oracle import polynomial as poly
@@ -52,9 +52,9 @@ def _oracle_green(model, omega, eta, charge=1):
     rmasks = poly.coordinate_masks(ep.coordinates)
     A = solvers.dense_matrix(lambda v: poly.sigma(p, t, rmasks, v, masks), ep.dimension)
     ref = p.reference
-    add = _source(ep, t, physical_mode_operator(ref, 0, 0, "c_dagger").components)
-    rem = -1j * _source(ep, t, physical_mode_operator(ref, 0, 0, "tilde_c").components)
-    sink = _sink(ep, physical_mode_operator(ref, 0, 0, "c").components)
+    add = poly.source(p, t, rmasks, physical_mode_operator(ref, 0, 0, "c_dagger").components, masks)
+    rem = -1j * poly.source(p, t, rmasks, physical_mode_operator(ref, 0, 0, "tilde_c").components, masks)
+    sink = poly.sink(rmasks, physical_mode_operator(ref, 0, 0, "c").components)
     a, r = solvers.retarded_green_dense(A, add, rem, sink, omega, eta)
     return packed, a + r, p
 
