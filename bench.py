@@ -126,6 +126,8 @@ def main() -> int:
     ap.add_argument("--forcing", default="legacy", choices=("legacy", "eisenstat_walker"),
                     help="inexact-Newton forcing strategy of the CCSD solve (the reference's two options)")
     ap.add_argument("--restart", type=int, default=40, help="GMRES restart length of the CCSD solve")
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"),
+                    help="precision of the Jacobian-vector products inside GMRES (fp32 = mixed precision)")
     ap.add_argument("--jvp-probe", type=int, default=0, metavar="K",
                     help="profiling workload only: one residual and K JVPs of siam24, then a 256 MiB copy")
     args = ap.parse_args()
@@ -156,7 +158,10 @@ def main() -> int:
         workload = "readme_3site_U1.2_ccsd_eom_s13_gr_81w"
     from c3lindblad_amd.solver import NewtonKrylovPolicy
 
-    policy = NewtonKrylovPolicy(forcing_strategy=args.forcing, gmres_restart=args.restart)
+    if args.precision == "fp32":
+        policy = NewtonKrylovPolicy.mixed_precision(forcing_strategy=args.forcing, gmres_restart=args.restart)
+    else:
+        policy = NewtonKrylovPolicy(forcing_strategy=args.forcing, gmres_restart=args.restart)
     pipe = SolvePipeline.prepare(model, grid, 0.0, charges=charges, policy=policy)
     print(f"[bench] setup {pipe.setup_seconds:.2f}s n={model.n_sites} D={pipe.static.oracle.num_amplitudes}", file=sys.stderr, flush=True)
     for k in range(args.warmup):
@@ -204,12 +209,14 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64 (JVP f32)",
         "data": "synthetic",
         "config": {
             "workload": workload,
-            "variant": "fp64_gpu",
-            "newton_policy": {"forcing": args.forcing, "gmres_restart": args.restart},
+            "variant": "fp64_gpu" if args.precision == "fp64" else "mixed_fp32_jvp",
+            "newton_policy": {"forcing": args.forcing, "gmres_restart": args.restart,
+                              "jacobian_precision": args.precision,
+                              "minimum_gmres_relative_tolerance": policy.minimum_gmres_relative_tolerance},
             "n_sites": model.n_sites,
             "ccsd_amplitudes": pipe.static.oracle.num_amplitudes,
             "eom_dimension_per_charge": {str(q): b.dimension for q, b in pipe.eom.items()},

@@ -45,6 +45,34 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+
+// Storage/arithmetic type conversions of the plan kernels (FP64 = double2, FP32 = float2).
+template <typename C>
+__device__ __forceinline__ C from_d2(double2 v);
+template <>
+__device__ __forceinline__ double2 from_d2<double2>(double2 v) {
+  return v;
+}
+template <>
+__device__ __forceinline__ float2 from_d2<float2>(double2 v) {
+  return make_float2((float)v.x, (float)v.y);
+}
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+template <typename C>
+__device__ __forceinline__ C czero();
+template <>
+__device__ __forceinline__ double2 czero<double2>() {
+  return make_double2(0.0, 0.0);
+}
+template <>
+__device__ __forceinline__ float2 czero<float2>() {
+  return make_float2(0.f, 0.f);
+}
 
 constexpr int kThreads = 256;
 
@@ -57,7 +85,8 @@ int64_t grid_for(int64_t count, int threads = kThreads) {
 
 // ------------------------------------------------------------------ plan kernels
 
-__global__ void scatter_padded_kernel(const double2* __restrict__ v, double2* __restrict__ pad,
+template <typename C>
+__global__ void scatter_padded_kernel(const double2* __restrict__ v, C* __restrict__ pad,
                                       const int64_t* __restrict__ src, const int8_t* __restrict__ sign,
                                       int64_t count) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
@@ -71,8 +100,21 @@ __global__ void scatter_padded_kernel(const double2* __restrict__ v, double2* __
         x.y = -x.y;
       }
     }
-    pad[e] = x;
+    pad[e] = from_d2<C>(x);
   }
+}
+
+__global__ void narrow_kernel(const double2* __restrict__ a, float2* __restrict__ b, int64_t count) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = from_d2<float2>(a[i]);
+}
+
+// out[i] = in[i] (FP64 -> FP32) over the listed workspace ranges; grid.y = range.
+__global__ void narrow_ranges_kernel(const double2* __restrict__ ws, float2* __restrict__ ws32,
+                                     const int64_t* __restrict__ ranges) {
+  const int64_t begin = ranges[2 * blockIdx.y], count = ranges[2 * blockIdx.y + 1];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+    ws32[begin + i] = from_d2<float2>(ws[begin + i]);
 }
 
 struct ContractJob {
@@ -85,7 +127,8 @@ struct ContractJob {
 };
 
 // grid.y = job; threads cover the n^(R-1) outputs of that job.
-__global__ void contract_kernel(double2* __restrict__ ws, const double2* __restrict__ vectors,
+template <typename C>
+__global__ void contract_kernel(C* __restrict__ ws, const C* __restrict__ vectors,
                                 const ContractJob* __restrict__ jobs, int n) {
   const ContractJob job = jobs[blockIdx.y];
   const int r = job.par_rank - 1;
@@ -93,15 +136,15 @@ __global__ void contract_kernel(double2* __restrict__ ws, const double2* __restr
   for (int k = 0; k < r; ++k) out_size *= n;
   int64_t inner = 1;
   for (int k = 0; k < job.par_rank - 1 - job.slot; ++k) inner *= n;
-  const double2* __restrict__ par = ws + job.par_offset;
-  const double2* __restrict__ vec = vectors + (int64_t)job.vec * n;
-  double2* __restrict__ out = ws + job.out_offset;
+  const C* __restrict__ par = ws + job.par_offset;
+  const C* __restrict__ vec = vectors + (int64_t)job.vec * n;
+  C* __restrict__ out = ws + job.out_offset;
   for (int64_t J = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; J < out_size;
        J += (int64_t)gridDim.x * blockDim.x) {
     const int64_t hi = J / inner;
     const int64_t lo = J - hi * inner;
     const int64_t base = hi * n * inner + lo;
-    double2 acc = make_double2(0.0, 0.0);
+    C acc = czero<C>();
     for (int q = 0; q < n; ++q) acc = cadd(acc, cmul(vec[q], par[base + (int64_t)q * inner]));
     if (job.sign < 0) {
       acc.x = -acc.x;
@@ -156,8 +199,8 @@ __device__ __forceinline__ int64_t factor_index(const WedgeFactor& f, int i0, in
   return f.offset + (int64_t)(i0 * f.stride[0] + i1 * f.stride[1] + i2 * f.stride[2] + i3 * f.stride[3]);
 }
 
-template <bool kTwoFactors>
-__global__ __launch_bounds__(256) void wedge_program_kernel(const double2* __restrict__ ws, double2* __restrict__ out,
+template <bool kTwoFactors, typename C>
+__global__ __launch_bounds__(256) void wedge_program_kernel(const C* __restrict__ ws, double2* __restrict__ out,
                                                             const WedgeBlockDesc* __restrict__ descs,
                                                             const int2* __restrict__ tiles,
                                                             const double2* __restrict__ diag,
@@ -174,7 +217,7 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(const double2* __res
   int i0[R], i1[R], i2[R], i3[R];
   bool act[R];
   int64_t e[R];
-  double2 acc[R];
+  C acc[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     e[r] = (int64_t)tile.y + local + (int64_t)r * per;
@@ -187,19 +230,19 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(const double2* __res
       i2[r] = (c >> 16) & 255u;
       i3[r] = c >> 24;
     }
-    acc[r] = make_double2(0.0, 0.0);
+    acc[r] = czero<C>();
   }
   if (kTwoFactors) {
 #pragma unroll 2
     for (int t = group; t < d.n_entries; t += step) {
-      const double2 c = d.entries[t].coeff;
+      const C c = from_d2<C>(d.entries[t].coeff);
       const WedgeFactor fa = d.factors[2 * t];
       const WedgeFactor fb = d.factors[2 * t + 1];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         if (act[r]) {
-          const double2 a = ws[factor_index(fa, i0[r], i1[r], i2[r], i3[r])];
-          const double2 b = ws[factor_index(fb, i0[r], i1[r], i2[r], i3[r])];
+          const C a = ws[factor_index(fa, i0[r], i1[r], i2[r], i3[r])];
+          const C b = ws[factor_index(fb, i0[r], i1[r], i2[r], i3[r])];
           acc[r] = cadd(acc[r], cmul(c, cmul(a, b)));
         }
       }
@@ -207,24 +250,24 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(const double2* __res
   } else if (act[0]) {
     for (int t = group; t < d.n_entries; t += step) {
       const WedgeEntry en = d.entries[t];
-      double2 prod = en.coeff;
+      C prod = from_d2<C>(en.coeff);
       for (int f = en.first; f < en.first + en.nf; ++f)
         prod = cmul(prod, ws[factor_index(d.factors[f], i0[0], i1[0], i2[0], i3[0])]);
       acc[0] = cadd(acc[0], prod);
     }
   }
   if (lg > 0) {  // reps == 1 here
-    part[threadIdx.x] = acc[0];
+    part[threadIdx.x] = to_d2(acc[0]);
     __syncthreads();
     if (group == 0)
-      for (int k = 1; k < step; ++k) acc[0] = cadd(acc[0], part[k * per + local]);
+      for (int k = 1; k < step; ++k) acc[0] = cadd(acc[0], from_d2<C>(part[k * per + local]));
   }
   if (group == 0) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (act[r]) {
         const int64_t dd = d.dest[e[r]];
-        double2 v = acc[r];
+        double2 v = to_d2(acc[r]);
         if (diag != nullptr) v = cadd(v, cmul(diag[dd], x[dd]));
         out[dd] = v;
       }
@@ -577,6 +620,7 @@ struct PlanLevel {
   int32_t n_jobs;
   int64_t max_out;
   ContractJob* jobs = nullptr;
+  std::vector<ContractJob> host_jobs;
 };
 
 struct PlanBlock {
@@ -626,6 +670,19 @@ struct c3l_plan {
   ProgramExec programs[kPrograms];
   KindExec kinds[kKinds];
   std::vector<void*> owned;
+  // mixed precision: kinds loaded and programs evaluated in FP32 (complex64) use ws32,
+  // a float2 mirror of the workspace; FP64-produced atoms they read are narrowed into it
+  // whenever the FP64 side changed (fixed_version) since the last narrowing.
+  int32_t kind_bits[kKinds] = {64, 64, 64};
+  int32_t program_bits[kPrograms] = {64, 64, 64, 64, 64, 64, 64, 64};
+  float2* ws32 = nullptr;
+  float2* vectors32 = nullptr;
+  int64_t fixed_version = 0;
+  int64_t narrowed_version = -1;
+  bool ranges_built = false;
+  int32_t n_ranges = 0;
+  int64_t max_range = 0;
+  int64_t* ranges = nullptr;  // device [2 * n_ranges]: (begin, count)
   // optional per-program kernel timing with HIP events
   bool timing = false;
   std::vector<PendingTiming> pending;
@@ -787,6 +844,7 @@ int c3l_plan_add_base(c3l_plan* plan, int32_t kind, int64_t offset, int32_t deg,
   b.src.assign(src, src + b.count);
   b.sign.assign(sign, sign + b.count);
   plan->bases.push_back(std::move(b));
+  plan->ranges_built = false;
   KindExec& ke = plan->kinds[kind];
   for (void* p : ke.owned) (void)hipFree(p);
   ke = KindExec();  // rebuilt at the next load
@@ -816,7 +874,9 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
   level.max_out = max_out;
   int rc = upload(plan, &level.jobs, jobs.data(), n_jobs);
   if (rc != C3L_OK) return rc;
+  level.host_jobs = jobs;
   plan->levels.push_back(level);
+  plan->ranges_built = false;
   return C3L_OK;
 }
 
@@ -875,6 +935,7 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
     b.coord[(size_t)k] = packed;
   }
   plan->blocks.push_back(std::move(b));
+  plan->ranges_built = false;
   ProgramExec& pe = plan->programs[program];
   for (void* p : pe.owned) (void)hipFree(p);
   pe = ProgramExec();  // rebuilt at the next apply
@@ -912,15 +973,25 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
     }
     ke.built = true;
   }
+  const bool narrow = plan->kind_bits[kind] == 32;
   for (const BaseRun& r : ke.runs) {
-    hipLaunchKernelGGL(scatter_padded_kernel, dim3(grid_for(r.count)), dim3(kThreads), 0, s, vv, plan->ws + r.offset,
-                       r.src, r.sign, r.count);
+    if (narrow)
+      hipLaunchKernelGGL(scatter_padded_kernel<float2>, dim3(grid_for(r.count)), dim3(kThreads), 0, s, vv,
+                         plan->ws32 + r.offset, r.src, r.sign, r.count);
+    else
+      hipLaunchKernelGGL(scatter_padded_kernel<double2>, dim3(grid_for(r.count)), dim3(kThreads), 0, s, vv,
+                         plan->ws + r.offset, r.src, r.sign, r.count);
     C3L_HIP_CHECK(hipGetLastError());
   }
+  if (!narrow) ++plan->fixed_version;
   for (const PlanLevel& l : plan->levels) {
     if (l.kind != kind || l.n_jobs == 0) continue;
-    hipLaunchKernelGGL(contract_kernel, dim3(grid_for(l.max_out), l.n_jobs), dim3(kThreads), 0, s, plan->ws,
-                       plan->vectors, l.jobs, plan->n);
+    if (narrow)
+      hipLaunchKernelGGL(contract_kernel<float2>, dim3(grid_for(l.max_out), l.n_jobs), dim3(kThreads), 0, s, plan->ws32,
+                         plan->vectors32, l.jobs, plan->n);
+    else
+      hipLaunchKernelGGL(contract_kernel<double2>, dim3(grid_for(l.max_out), l.n_jobs), dim3(kThreads), 0, s, plan->ws,
+                         plan->vectors, l.jobs, plan->n);
     C3L_HIP_CHECK(hipGetLastError());
   }
   return C3L_OK;
@@ -999,9 +1070,89 @@ int build_program(c3l_plan* plan, int32_t program) {
   return C3L_OK;
 }
 
+// Workspace ranges read by FP32 programs that FP64 loads/programs produce (constant
+// atoms, amplitude atoms, creation tensors): narrowed into ws32 before an FP32 program
+// runs whenever the FP64 side changed.
+int narrow_fixed_atoms(c3l_plan* plan, hipStream_t s) {
+  if (!plan->ranges_built) {
+    std::vector<std::pair<int64_t, int64_t>> produced;  // FP32-produced [begin, end)
+    for (const PlanBase& b : plan->bases)
+      if (plan->kind_bits[b.kind] == 32) produced.push_back({b.offset, b.offset + b.count});
+    for (const PlanLevel& l : plan->levels) {
+      if (plan->kind_bits[l.kind] != 32) continue;
+      for (const ContractJob& j : l.host_jobs) produced.push_back({j.out_offset, j.out_offset + ipow(plan->n, j.par_rank - 1)});
+    }
+    std::vector<std::pair<int64_t, int64_t>> need;
+    for (const PlanBlock& b : plan->blocks) {
+      if (plan->program_bits[b.program] != 32) continue;
+      for (const WedgeFactor& f : b.factors) {
+        int rank = 0;
+        int64_t size = 1;
+        for (int k = 0; k < 4; ++k)
+          if (f.stride[k] != 0) ++rank;
+        size = ipow(plan->n, rank);
+        bool mine = false;
+        for (const auto& p : produced)
+          if (f.offset < p.second && f.offset + size > p.first) mine = true;
+        if (!mine) need.push_back({f.offset, f.offset + size});
+      }
+    }
+    std::sort(need.begin(), need.end());
+    std::vector<int64_t> flat;
+    int64_t max_count = 0;
+    for (size_t i = 0; i < need.size();) {
+      int64_t b = need[i].first, e = need[i].second;
+      size_t j = i + 1;
+      while (j < need.size() && need[j].first <= e) e = std::max(e, need[j++].second);
+      flat.push_back(b);
+      flat.push_back(e - b);
+      max_count = std::max(max_count, e - b);
+      i = j;
+    }
+    plan->n_ranges = (int32_t)(flat.size() / 2);
+    plan->max_range = max_count;
+    int rc = upload(plan, &plan->ranges, flat.data(), (int64_t)flat.size());
+    if (rc != C3L_OK) return rc;
+    plan->ranges_built = true;
+  }
+  if (plan->narrowed_version == plan->fixed_version || plan->n_ranges == 0) return C3L_OK;
+  hipLaunchKernelGGL(narrow_ranges_kernel, dim3(grid_for(plan->max_range), plan->n_ranges), dim3(kThreads), 0, s,
+                     plan->ws, plan->ws32, plan->ranges);
+  C3L_HIP_CHECK(hipGetLastError());
+  plan->narrowed_version = plan->fixed_version;
+  return C3L_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int c3l_plan_set_precision(c3l_plan* plan, int32_t kind, int32_t program, int32_t bits) {
+  if (plan == nullptr || kind < 0 || kind >= kKinds || program < 0 || program >= kPrograms || (bits != 32 && bits != 64))
+    return fail(C3L_EINVAL, "c3l_plan_set_precision: invalid argument");
+  if (bits == 32 && plan->ws32 == nullptr) {
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(float2) * (size_t)(plan->workspace > 0 ? plan->workspace : 1)) != hipSuccess)
+      return fail(C3L_ENOMEM, "c3l_plan_set_precision: FP32 workspace allocation failed");
+    plan->owned.push_back(p);
+    plan->ws32 = static_cast<float2*>(p);
+    const int64_t nv = (int64_t)plan->n_vectors * plan->n;
+    if (nv > 0) {
+      if (hipMalloc(&p, sizeof(float2) * (size_t)nv) != hipSuccess)
+        return fail(C3L_ENOMEM, "c3l_plan_set_precision: FP32 vector table allocation failed");
+      plan->owned.push_back(p);
+      plan->vectors32 = static_cast<float2*>(p);
+      hipLaunchKernelGGL(narrow_kernel, dim3(grid_for(nv)), dim3(kThreads), 0, 0, plan->vectors, plan->vectors32, nv);
+      C3L_HIP_CHECK(hipGetLastError());
+      C3L_HIP_CHECK(hipDeviceSynchronize());
+    }
+  }
+  plan->kind_bits[kind] = bits;
+  plan->program_bits[program] = bits;
+  plan->ranges_built = false;
+  plan->narrowed_version = -1;
+  return C3L_OK;
+}
 
 int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out, void* stream) {
   if (plan == nullptr || program < 0 || program >= kPrograms || ((diag == nullptr) != (x == nullptr)) ||
@@ -1024,15 +1175,32 @@ int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const do
   }
   const double2* dg = reinterpret_cast<const double2*>(diag);
   const double2* xx = reinterpret_cast<const double2*>(x);
-  if (pe.tiles_two > 0) {
-    hipLaunchKernelGGL(wedge_program_kernel<true>, dim3(pe.tiles_two), dim3(kThreads), 0, s, plan->ws, o, pe.descs,
-                       pe.tiles, dg, xx);
-    C3L_HIP_CHECK(hipGetLastError());
-  }
-  if (pe.tiles_general > 0) {
-    hipLaunchKernelGGL(wedge_program_kernel<false>, dim3(pe.tiles_general), dim3(kThreads), 0, s, plan->ws, o,
-                       pe.descs, pe.tiles + pe.tiles_two, dg, xx);
-    C3L_HIP_CHECK(hipGetLastError());
+  if (plan->program_bits[program] == 32) {
+    if (internal) return fail(C3L_EINVAL, "c3l_plan_apply: workspace programs run in FP64 only");
+    rc = narrow_fixed_atoms(plan, s);
+    if (rc != C3L_OK) return rc;
+    if (pe.tiles_two > 0) {
+      hipLaunchKernelGGL((wedge_program_kernel<true, float2>), dim3(pe.tiles_two), dim3(kThreads), 0, s, plan->ws32, o,
+                         pe.descs, pe.tiles, dg, xx);
+      C3L_HIP_CHECK(hipGetLastError());
+    }
+    if (pe.tiles_general > 0) {
+      hipLaunchKernelGGL((wedge_program_kernel<false, float2>), dim3(pe.tiles_general), dim3(kThreads), 0, s,
+                         plan->ws32, o, pe.descs, pe.tiles + pe.tiles_two, dg, xx);
+      C3L_HIP_CHECK(hipGetLastError());
+    }
+  } else {
+    if (pe.tiles_two > 0) {
+      hipLaunchKernelGGL((wedge_program_kernel<true, double2>), dim3(pe.tiles_two), dim3(kThreads), 0, s, plan->ws, o,
+                         pe.descs, pe.tiles, dg, xx);
+      C3L_HIP_CHECK(hipGetLastError());
+    }
+    if (pe.tiles_general > 0) {
+      hipLaunchKernelGGL((wedge_program_kernel<false, double2>), dim3(pe.tiles_general), dim3(kThreads), 0, s,
+                         plan->ws, o, pe.descs, pe.tiles + pe.tiles_two, dg, xx);
+      C3L_HIP_CHECK(hipGetLastError());
+    }
+    if (internal) ++plan->fixed_version;
   }
   if (plan->timing) {
     C3L_HIP_CHECK(hipEventRecord(t.stop, s));

@@ -109,6 +109,11 @@ class DevicePlan:
     def has_creation(self) -> bool:
         return any(b[0] == PROGRAMS["creation"] for b in self.lowered.blocks)
 
+    def set_precision(self, kind: str, program: str, bits: int) -> None:
+        """Build ``kind`` atoms and evaluate ``program`` in FP32 (``bits=32``) or FP64."""
+
+        _lib.call("c3l_plan_set_precision", self.handle, KINDS[kind], PROGRAMS[program], int(bits))
+
     def set_timing(self, enable: bool) -> None:
         _lib.call("c3l_plan_set_timing", self.handle, 1 if enable else 0)
 

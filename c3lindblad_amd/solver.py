@@ -43,6 +43,18 @@ class NewtonKrylovPolicy:
     eisenstat_walker_gamma: float = 0.9
     eisenstat_walker_exponent: float = 1.5
     gmres_absolute_tolerance_factor: float = 0.1
+    # Precision of the Jacobian-vector products inside GMRES ("fp64" or "fp32").  The
+    # residual, the Newton update and the GMRES orthogonalisation stay FP64, so Newton
+    # acts as iterative refinement and converges to the FP64 root at `tolerance`.
+    jacobian_precision: str = "fp64"
+
+    @classmethod
+    def mixed_precision(cls, **overrides) -> "NewtonKrylovPolicy":
+        """FP32 JVPs with a GMRES forcing floor (1e-6) that an FP32 operator can reach."""
+
+        settings = dict(jacobian_precision="fp32", minimum_gmres_relative_tolerance=1.0e-6)
+        settings.update(overrides)
+        return cls(**settings)
 
     def __post_init__(self) -> None:
         positive = (
@@ -58,6 +70,8 @@ class NewtonKrylovPolicy:
             raise ValueError("minimum GMRES tolerance exceeds the maximum")
         if self.forcing_strategy not in ("legacy", "eisenstat_walker"):
             raise ValueError("unknown inexact-Newton forcing strategy")
+        if self.jacobian_precision not in ("fp64", "fp32"):
+            raise ValueError("jacobian_precision must be 'fp64' or 'fp32'")
         for v in (self.maximum_newton_iterations, self.gmres_restart, self.gmres_maximum_restart_cycles, self.maximum_line_search_steps):
             if isinstance(v, bool) or not isinstance(v, int) or v <= 0:
                 raise ValueError("Newton--Krylov iteration limits must be positive integers")
@@ -211,6 +225,8 @@ def solve_component_ccsd(
     started = perf_counter()
     state = _DeviceNewton(problem, controls.gmres_restart)
     torch = state.torch
+    bits = 32 if controls.jacobian_precision == "fp32" else 64
+    state.dev.plan.set_precision("D", "jvp", bits)
     if initial_amplitudes is None:
         x0 = np.zeros(oracle.num_variables)
     else:
@@ -272,6 +288,8 @@ def solve_component_ccsd(
         norm = accepted_norm
     if not current:
         state.residual(x, state.r)
+    if bits != 64:
+        state.dev.plan.set_precision("D", "jvp", 64)  # the JVP entry points stay FP64
     dev = state.dev
     xh = x.cpu().numpy()
     amplitudes = dev.t.cpu().numpy()

@@ -95,6 +95,14 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream);
 int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out,
                    void* stream);
 
+/* Mixed precision (no counterpart in the reference, which is FP64 throughout): with
+ * bits = 32, loads of atom kind `kind` build its atoms in a complex64 mirror of the
+ * workspace and program `program` evaluates there with FP32 accumulation (its output,
+ * including the diag*x term, stays complex128); FP64-produced atoms that the program
+ * reads are narrowed into the mirror whenever they changed.  bits = 64 restores FP64.
+ * Workspace (creation) programs always run in FP64. */
+int c3l_plan_set_precision(c3l_plan* plan, int32_t kind, int32_t program, int32_t bits);
+
 /* Measurement hooks (no counterpart in the reference, which times with perf_counter,
  * cc/component_jax.py:53-65): when enabled, every program launch of c3l_plan_apply is
  * bracketed by HIP events on its stream; c3l_plan_program_time waits for them and

@@ -345,3 +345,46 @@ def test_single_site_device_occupations_equal_lindblad_ed(torch_gpu):
     sol = solve_component_ccsd(sp)
     got = factorized_ccsd_covariance(sp, sol.amplitudes)
     assert np.max(np.abs(got - fock.exact_covariance(model))) < 1e-10
+
+
+def test_fp32_jvp_matches_fp64_jvp(torch_gpu):
+    """Mixed-precision JVP: complex64 atoms with FP32 accumulation against the FP64
+    JVP of the same plan, relative (max-norm) error <= 2e-6 at n = 7."""
+
+    print("case fp32 jvp n=7", flush=True)
+    p, sp = _static(7, 1.3, 7)
+    dev = sp.device
+    torch = torch_gpu
+    rng = np.random.default_rng(3)
+    x = torch.from_numpy(0.05 * rng.normal(size=p.num_variables)).cuda()
+    r = torch.empty_like(x)
+    dev.residual_real(x, r)
+    dx = torch.from_numpy(rng.normal(size=p.num_variables)).cuda()
+    want = torch.empty_like(x)
+    dev.jvp_real(dx, want)
+    got = torch.empty_like(x)
+    dev.plan.set_precision("D", "jvp", 32)
+    try:
+        dev.jvp_real(dx, got)
+        dev.jvp_real(dx, got)  # second call: narrowed atoms are reused
+    finally:
+        dev.plan.set_precision("D", "jvp", 64)
+    g, w = got.cpu().numpy(), want.cpu().numpy()
+    err = _rel(g, w)
+    assert 1e-12 < err < 2e-6  # FP32 rounding is visible, and bounded
+    again = torch.empty_like(x)
+    dev.jvp_real(dx, again)
+    assert _rel(again.cpu().numpy(), w) < RTOL  # back to FP64
+
+
+def test_mixed_precision_newton_reaches_fp64_root(torch_gpu):
+    from c3lindblad_amd.solver import NewtonKrylovPolicy, solve_component_ccsd
+
+    print("case mixed precision newton", flush=True)
+    p, sp = _static(4, 1.6, 11)
+    ref = solve_component_ccsd(sp)
+    for policy in (NewtonKrylovPolicy.mixed_precision(),
+                   NewtonKrylovPolicy.mixed_precision(forcing_strategy="eisenstat_walker")):
+        sol = solve_component_ccsd(sp, policy=policy)
+        assert sol.converged and sol.packed_residual_norm <= 1e-10
+        assert _rel(sol.amplitudes, ref.amplitudes) < 1e-8
