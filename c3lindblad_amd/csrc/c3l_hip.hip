@@ -193,7 +193,8 @@ struct WedgeBlockDesc {
   int32_t pad;
 };
 
-constexpr int kMaxReps = 4;
+constexpr int kMaxReps = 2;
+constexpr int kStagedEntries = 64;
 
 __device__ __forceinline__ int64_t factor_index(const WedgeFactor& f, int i0, int i1, int i2, int i3) {
   return f.offset + (int64_t)(i0 * f.stride[0] + i1 * f.stride[1] + i2 * f.stride[2] + i3 * f.stride[3]);
@@ -206,8 +207,17 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(const C* __restrict_
                                                             const double2* __restrict__ diag,
                                                             const double2* __restrict__ x) {
   __shared__ double2 part[256];
+  __shared__ double2 s_coeff[kStagedEntries];
+  __shared__ WedgeFactor s_fac[2 * kStagedEntries];
   const int2 tile = tiles[blockIdx.x];
   const WedgeBlockDesc d = descs[tile.x];
+  // two-factor entry metadata is staged in LDS once per workgroup (uniform reads later)
+  const bool staged = kTwoFactors && d.n_entries <= kStagedEntries;
+  if (staged) {
+    for (int q = threadIdx.x; q < d.n_entries; q += blockDim.x) s_coeff[q] = d.entries[q].coeff;
+    for (int q = threadIdx.x; q < 2 * d.n_entries; q += blockDim.x) s_fac[q] = d.factors[q];
+  }
+  __syncthreads();
   const int lg = d.lanes_log2;
   const int per = 256 >> lg;
   const int local = threadIdx.x & (per - 1);
@@ -233,14 +243,29 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(const C* __restrict_
     acc[r] = czero<C>();
   }
   if (kTwoFactors) {
+    // Branch-free body: inactive coordinates read index 0 (a valid address) and are
+    // never stored, so the gathers of every coordinate and unrolled entry overlap.
+    if (staged) {
 #pragma unroll 2
-    for (int t = group; t < d.n_entries; t += step) {
-      const C c = from_d2<C>(d.entries[t].coeff);
-      const WedgeFactor fa = d.factors[2 * t];
-      const WedgeFactor fb = d.factors[2 * t + 1];
+      for (int t = group; t < d.n_entries; t += step) {
+        const C c = from_d2<C>(s_coeff[t]);
+        const WedgeFactor fa = s_fac[2 * t];
+        const WedgeFactor fb = s_fac[2 * t + 1];
 #pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if (act[r]) {
+        for (int r = 0; r < R; ++r) {
+          const C a = ws[factor_index(fa, i0[r], i1[r], i2[r], i3[r])];
+          const C b = ws[factor_index(fb, i0[r], i1[r], i2[r], i3[r])];
+          acc[r] = cadd(acc[r], cmul(c, cmul(a, b)));
+        }
+      }
+    } else {
+#pragma unroll 2
+      for (int t = group; t < d.n_entries; t += step) {
+        const C c = from_d2<C>(d.entries[t].coeff);
+        const WedgeFactor fa = d.factors[2 * t];
+        const WedgeFactor fb = d.factors[2 * t + 1];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
           const C a = ws[factor_index(fa, i0[r], i1[r], i2[r], i3[r])];
           const C b = ws[factor_index(fb, i0[r], i1[r], i2[r], i3[r])];
           acc[r] = cadd(acc[r], cmul(c, cmul(a, b)));
@@ -1059,6 +1084,37 @@ int build_program(c3l_plan* plan, int32_t program) {
     descs.push_back(d);
     if (b.max_dest > pe.max_dest) pe.max_dest = b.max_dest;
   }
+  // Workgroup w runs on XCD w % 8 (round-robin dispatch).  Each block's tiles are cut
+  // into 8 contiguous coordinate ranges, range x is queued for XCD x and the queues are
+  // interleaved, so every XCD's L2 serves one slice of the block's larger factor tensors.
+  auto xcd_order = [](std::vector<int2>& list) {
+    constexpr int kXcds = 8;
+    std::vector<std::vector<int2>> queue(kXcds);
+    for (size_t i = 0; i < list.size();) {
+      size_t j = i;
+      while (j < list.size() && list[j].x == list[i].x) ++j;
+      const size_t count = j - i;
+      for (size_t k = 0; k < count; ++k) queue[(k * kXcds) / count].push_back(list[i + k]);
+      i = j;
+    }
+    std::vector<size_t> head(kXcds, 0);
+    std::vector<int2> out;
+    out.reserve(list.size());
+    while (out.size() < list.size()) {
+      for (int x = 0; x < kXcds && out.size() < list.size(); ++x) {
+        int q = x;
+        if (head[q] == queue[q].size()) {  // this queue is drained: take from the fullest one
+          size_t best = 0;
+          for (int y = 0; y < kXcds; ++y)
+            if (queue[y].size() - head[y] > best) best = queue[y].size() - head[y], q = y;
+        }
+        out.push_back(queue[q][head[q]++]);
+      }
+    }
+    list.swap(out);
+  };
+  xcd_order(two);
+  xcd_order(general);
   pe.tiles_two = (int32_t)two.size();
   pe.tiles_general = (int32_t)general.size();
   std::vector<int2> tiles(two);
