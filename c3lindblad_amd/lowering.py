@@ -401,18 +401,27 @@ def amplitude_tilde_map(coords: CoordinateSet, mode_tilde: ModeTilde) -> TildeMa
     k1 = np.zeros(dim, dtype=np.int64)
     pack_src = np.empty(real_size, dtype=np.int64)
     pack_w = np.empty(real_size, dtype=np.complex128)
-    for i, c, s in zip(independent, cursor, is_self):
-        j = partner[i]
-        if s:
-            sc = _self_scale(phase[i])
-            alpha[i], k0[i], k1[i] = sc, c, c
-            pack_src[c], pack_w[c] = i, 1.0 / sc
-        else:
-            alpha[i], beta[i], k0[i], k1[i] = 1.0, 1.0j, c, c + 1
-            alpha[j], beta[j], k0[j], k1[j] = phase[i], -1.0j * phase[i], c, c + 1
-            pack_src[c], pack_w[c] = i, 1.0
-            pack_src[c + 1], pack_w[c + 1] = i, -1.0j
+    # self orbits: t_i = s_i x_c with the branch of _self_scale
+    si, sc_ = independent[is_self], cursor[is_self]
+    scale = _self_scales(phase[si])
+    alpha[si], k0[si], k1[si] = scale, sc_, sc_
+    pack_src[sc_], pack_w[sc_] = si, 1.0 / scale
+    # pairs: t_i = x_c + i x_{c+1}, t_j = eta_i conj(t_i)
+    pi_, pc = independent[~is_self], cursor[~is_self]
+    pj = partner[pi_]
+    alpha[pi_], beta[pi_], k0[pi_], k1[pi_] = 1.0, 1.0j, pc, pc + 1
+    alpha[pj], beta[pj], k0[pj], k1[pj] = phase[pi_], -1.0j * phase[pi_], pc, pc + 1
+    pack_src[pc], pack_w[pc] = pi_, 1.0
+    pack_src[pc + 1], pack_w[pc + 1] = pi_, -1.0j
     return TildeMap(partner, phase, alpha, beta, k0, k1, pack_src, pack_w)
+
+
+def _self_scales(eta: np.ndarray) -> np.ndarray:
+    """Vectorised ``_self_scale``."""
+
+    scale = np.sqrt(eta / np.abs(eta))
+    flip = np.abs(scale - eta * scale.conj()) > np.abs(-scale - eta * (-scale).conj())
+    return np.where(flip, -scale, scale)
 
 
 # ----------------------------------------------------------------------------- problem
