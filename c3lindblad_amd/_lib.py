@@ -56,6 +56,15 @@ SIGNATURES = {
          _P],
         ctypes.c_int,
     ),
+    "c3l_bf16_gemm": ([_P, _I64, _I64, _P, _I64, _I64, _P, _I64, _I32, _I32, _I32, ctypes.c_float, ctypes.c_float, _P],
+                      ctypes.c_int),
+    "c3l_mlp_bias_act": ([_P, _P, _P, _I32, _I32, _I32, _P], ctypes.c_int),
+    "c3l_mlp_gelu_backward": ([_P, _P, _P, _P, _I64, _P], ctypes.c_int),
+    "c3l_mlp_column_sum": ([_P, _P, _I32, _I32, _P], ctypes.c_int),
+    "c3l_mlp_mse": ([_P, _P, _P, _P, _P, _I64, ctypes.c_float, _I32, _P], ctypes.c_int),
+    "c3l_mlp_adam": ([_P, _P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I32, _P],
+                     ctypes.c_int),
+    "c3l_to_bf16": ([_P, _P, _I64, _P], ctypes.c_int),
     "c3l_arnoldi": (
         [_P, _P, _P, _P, _I64, _I32, _P, _I32, _P, ctypes.c_double, _P, _P, _P, _P, _P, _P, _P, _P, _P,
          ctypes.c_double, _I32, ctypes.c_double, _P, _P, _P, _P],

@@ -121,6 +121,7 @@ class ComponentCCSolution:
     peak_rss_kib: int
     device_seconds: dict = field(default_factory=dict)
     device_amplitudes: object = field(default=None, repr=False)  # torch complex128 copy on the GPU
+    warm_start_fallback: bool = False
 
 
 class _DeviceNewton:
@@ -217,9 +218,30 @@ def solve_component_ccsd(
     *,
     policy: NewtonKrylovPolicy | None = None,
     accepted_step_callback=None,
+    initial_variables=None,
 ) -> ComponentCCSolution:
-    """Solve the tilde-real CCSD equations to ``||pack R|| <= policy.tolerance`` on the GPU."""
+    """Solve the tilde-real CCSD equations to ``||pack R|| <= policy.tolerance`` on the GPU.
 
+    ``initial_variables`` (real tilde variables, NumPy or a GPU tensor, e.g. from a learned
+    warm start) replaces the initial amplitudes; when Newton does not converge from it the
+    solve restarts from the zero guess and ``warm_start_fallback`` is set.
+    """
+
+    if initial_variables is not None:
+        if initial_amplitudes is not None:
+            raise ValueError("give initial_amplitudes or initial_variables, not both")
+        first = _solve(problem, None, policy, accepted_step_callback, initial_variables)
+        if first.converged:
+            return first
+        second = _solve(problem, None, policy, accepted_step_callback, None)
+        second.warm_start_fallback = True
+        second.jvp_evaluations += first.jvp_evaluations
+        second.residual_evaluations += first.residual_evaluations
+        return second
+    return _solve(problem, initial_amplitudes, policy, accepted_step_callback, None)
+
+
+def _solve(problem, initial_amplitudes, policy, accepted_step_callback, initial_variables) -> ComponentCCSolution:
     controls = NewtonKrylovPolicy() if policy is None else policy
     oracle = problem.oracle
     started = perf_counter()
@@ -227,14 +249,24 @@ def solve_component_ccsd(
     torch = state.torch
     bits = 32 if controls.jacobian_precision == "fp32" else 64
     state.dev.plan.set_precision("D", "jvp", bits)
-    if initial_amplitudes is None:
+    if initial_variables is not None:
+        x0 = None
+    elif initial_amplitudes is None:
         x0 = np.zeros(oracle.num_variables)
     else:
         initial = np.asarray(initial_amplitudes, dtype=np.complex128)
         if initial.shape != (oracle.num_amplitudes,) or not np.all(np.isfinite(initial)):
             raise ValueError("initial amplitudes do not match the component CCSD problem")
         x0 = oracle.tilde.pack(initial, check=False)
-    x = torch.from_numpy(np.ascontiguousarray(x0)).to(_lib.DEVICE)
+    if x0 is None:
+        if hasattr(initial_variables, "is_cuda"):
+            x = initial_variables.to(device=_lib.DEVICE, dtype=torch.float64).contiguous().clone()
+        else:
+            x = torch.from_numpy(np.ascontiguousarray(initial_variables, dtype=np.float64)).to(_lib.DEVICE)
+        if tuple(x.shape) != (oracle.num_variables,) or not bool(torch.isfinite(x).all()):
+            raise ValueError("initial_variables must be finite real tilde variables of the problem")
+    else:
+        x = torch.from_numpy(np.ascontiguousarray(x0)).to(_lib.DEVICE)
     history = []
     converged = False
     previous_norm = None

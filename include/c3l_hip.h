@@ -207,6 +207,31 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
                 double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
                 int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
 
+/* ---- learned warm start (MLP) ---------------------------------------------------------
+ * No counterpart in the reference: a small MLP (bf16 weights, FP32 accumulation on the
+ * matrix cores) maps model parameters to compressed initial amplitudes.  Matrices are
+ * strided views: A(m, k) = A[m*a_rs + k*a_cs], B(k, n) = B[k*b_rs + n*b_cs]; C is
+ * row-major with leading dimension ldc.  bf16 values are passed as uint16_t bit patterns. */
+/* C = alpha * A B + beta * C (C is FP32) */
+int c3l_bf16_gemm(const uint16_t* A, int64_t a_rs, int64_t a_cs, const uint16_t* B, int64_t b_rs, int64_t b_cs,
+                  float* C, int64_t ldc, int32_t M, int32_t N, int32_t K, float alpha, float beta, void* stream);
+/* z[m, n] += bias[n]; h = bf16(act ? GELU(z) : z) when h != NULL (GELU in its tanh form) */
+int c3l_mlp_bias_act(float* z, const float* bias, uint16_t* h, int32_t M, int32_t N, int32_t act, void* stream);
+/* dz = dh * GELU'(z), also as bf16 */
+int c3l_mlp_gelu_backward(const float* z, const float* dh, float* dz, uint16_t* dz16, int64_t total, void* stream);
+/* db[n] = sum_m dz[m, n] */
+int c3l_mlp_column_sum(const float* dz, float* db, int32_t M, int32_t N, void* stream);
+/* dy = scale * (y - target) (FP32 and bf16); loss_parts[b] = partial sums of (y - target)^2
+ * over n_parts workgroups */
+int c3l_mlp_mse(const float* y, const float* target, float* dy, uint16_t* dy16, float* loss_parts, int64_t total,
+                float scale, int32_t n_parts, void* stream);
+/* Adam step `step` (>= 1) on FP32 master weights w with moments m, v; refreshes the bf16
+ * copy w16 when non-NULL */
+int c3l_mlp_adam(float* w, float* m, float* v, const float* g, uint16_t* w16, int64_t n, float lr, float b1, float b2,
+                 float eps, int32_t step, void* stream);
+/* b = bf16(a), round to nearest even */
+int c3l_to_bf16(const float* a, uint16_t* b, int64_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
