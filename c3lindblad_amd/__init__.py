@@ -33,7 +33,7 @@ from .eom import (
     component_eom_sources,
     eom_ccsd_problem,
 )
-from .response import arnoldi_resolvent, component_retarded_green
+from .response import arnoldi_resolvent, component_retarded_green, solve_component_shifted
 from .observables import (
     ComponentObservableProblem,
     contact_currents,
@@ -66,5 +66,6 @@ __all__ = [
     "eom_ccsd_problem",
     "factorized_cc_problem",
     "solve_component_ccsd",
+    "solve_component_shifted",
     "two_body_expectations",
 ]

@@ -388,3 +388,39 @@ def test_mixed_precision_newton_reaches_fp64_root(torch_gpu):
         sol = solve_component_ccsd(sp, policy=policy)
         assert sol.converged and sol.packed_residual_norm <= 1e-10
         assert _rel(sol.amplitudes, ref.amplitudes) < 1e-8
+
+
+def test_solve_component_shifted_matches_dense_solve(torch_gpu):
+    from c3lindblad_amd import (
+        ComponentEOMProblem,
+        ComponentSigmaOperator,
+        ComponentStaticProblem,
+        component_eom_context,
+        component_eom_sources,
+        eom_ccsd_problem,
+        factorized_cc_problem,
+        solve_component_ccsd,
+        solve_component_shifted,
+    )
+    from c3lindblad_amd.configs import readme_model
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    print("case shifted solve readme", flush=True)
+    p = factorized_cc_problem(readme_model())
+    sp = ComponentStaticProblem.compile(p)
+    sol = solve_component_ccsd(sp)
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    ctx = component_eom_context(ep, sol.amplitudes)
+    op = ComponentSigmaOperator(ctx)
+    b = component_eom_sources(ctx).addition
+    masks = poly.coordinate_masks(ep.coordinates)
+    A = solvers.dense_matrix(lambda v: poly.sigma(p, sol.amplitudes, masks, v, poly.coordinate_masks(p.coordinates)),
+                             ep.dimension)
+    for z in (0.3 + 0.0j, -1.7 + 0.05j):
+        out = solve_component_shifted(op, z, b)
+        want = np.linalg.solve(z * np.eye(ep.dimension) - A, b)
+        assert out.converged and out.true_relative_residual <= 1e-9
+        assert _rel(out.solution, want) < 1e-8
+        warm = solve_component_shifted(op, z, b, initial_solution=want)
+        assert warm.converged and _rel(warm.solution, want) < 1e-9
