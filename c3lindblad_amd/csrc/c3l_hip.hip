@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/c3l_hip.h"
@@ -151,6 +152,43 @@ __global__ void contract_kernel(C* __restrict__ ws, const C* __restrict__ vector
       acc.y = -acc.y;
     }
     out[J] = acc;
+  }
+}
+
+// All contraction levels of one atom kind in a single workgroup (small kinds such
+// as the EOM input): levels run in order with a workgroup barrier between them, so
+// one launch replaces one launch per level.  Jobs of level l are
+// jobs[level_start[l] .. level_start[l+1]).
+template <typename C>
+__global__ __launch_bounds__(1024) void contract_levels_kernel(C* __restrict__ ws, const C* __restrict__ vectors,
+                                                               const ContractJob* __restrict__ jobs,
+                                                               const int32_t* __restrict__ level_start, int n_levels,
+                                                               int n) {
+  for (int lv = 0; lv < n_levels; ++lv) {
+    for (int jb = level_start[lv]; jb < level_start[lv + 1]; ++jb) {
+      const ContractJob job = jobs[jb];
+      const int r = job.par_rank - 1;
+      int64_t out_size = 1;
+      for (int k = 0; k < r; ++k) out_size *= n;
+      int64_t inner = 1;
+      for (int k = 0; k < job.par_rank - 1 - job.slot; ++k) inner *= n;
+      const C* __restrict__ par = ws + job.par_offset;
+      const C* __restrict__ vec = vectors + (int64_t)job.vec * n;
+      C* __restrict__ out = ws + job.out_offset;
+      for (int64_t J = threadIdx.x; J < out_size; J += blockDim.x) {
+        const int64_t hi = J / inner;
+        const int64_t lo = J - hi * inner;
+        const int64_t base = hi * n * inner + lo;
+        C acc = czero<C>();
+        for (int q = 0; q < n; ++q) acc = cadd(acc, cmul(vec[q], par[base + (int64_t)q * inner]));
+        if (job.sign < 0) {
+          acc.x = -acc.x;
+          acc.y = -acc.y;
+        }
+        out[J] = acc;
+      }
+    }
+    __syncthreads();  // this level's atoms are the next level's parents
   }
 }
 
@@ -373,17 +411,34 @@ __global__ void raxpby_kernel(double a, const double* __restrict__ x, double b, 
 
 // ------------------------------------------------------------------ Krylov kernels
 
-// Tall-skinny products V^T w / V a over Krylov bases.  A dot workgroup covers
-// kChunkRows rows (4 per thread, kept in registers) of kDotCols columns, so a
-// launch has (dim / kChunkRows) x (ncols / kDotCols) workgroups with several
-// independent loads per thread in flight; the partial sums of the row chunks
-// are added by a one-workgroup-per-column finish kernel.  An axpy workgroup
-// covers 64 rows; its four waves take every fourth column and meet in LDS.
-constexpr int kChunkRows = 1024;
-constexpr int kRowsPerThread = kChunkRows / 256;
-constexpr int kDotCols = 8;
+// Tall-skinny products V^T w / V a over Krylov bases.  A dot launch gives every
+// column one wave per chunk of rows (512, 1024 or 2048 rows: shorter chunks for
+// few columns keep the GPU full, longer ones amortise the reduction; measured on
+// MI355X at 23,150 rows); a one-workgroup-per-column finish kernel adds the chunk
+// partials.  An axpy workgroup covers 64 rows; its four waves take every fourth
+// column and meet in LDS.
+constexpr int kMinChunkRows = 512;
 
-int64_t chunks_for(int64_t dim) { return (dim + kChunkRows - 1) / kChunkRows; }
+int dot_rows_for(int ncols) { return ncols <= 64 ? 512 : (ncols <= 256 ? 1024 : 2048); }
+
+// partial-sum slots per column: the largest count any chunk length produces
+int64_t chunks_for(int64_t dim) { return (dim + kMinChunkRows - 1) / kMinChunkRows; }
+
+template <typename Launch>
+int dispatch_rows(int rows, Launch launch) {
+  switch (rows) {
+    case 512:
+      launch(std::integral_constant<int, 512>());
+      break;
+    case 1024:
+      launch(std::integral_constant<int, 1024>());
+      break;
+    default:
+      launch(std::integral_constant<int, 2048>());
+      break;
+  }
+  return 0;
+}
 
 struct cplx {
   double x, y;
@@ -412,46 +467,27 @@ __device__ T block_reduce(T v, T* shared) {
   return total;
 }
 
-// grid = (chunks, column groups): partial[k*chunks + c] = sum over the rows of chunk c.
-// With `extra` set, column ncols-1 is `extra` instead of V + (ncols-1)*ld.
+// One wave per column over a chunk of ROWS rows (4 columns per workgroup):
+// coalesced 64-lane loads, a wave-level reduction and one partial per (column,
+// chunk).  With `extra` set, column ncols-1 is `extra` instead of V + (ncols-1)*ld.
+template <int ROWS>
 __global__ __launch_bounds__(256) void rmultidot_partial_kernel(const double* __restrict__ V, int64_t ld, int ncols,
                                                                 const double* __restrict__ extra,
                                                                 const double* __restrict__ w, int64_t dim,
                                                                 double* __restrict__ partial, int64_t chunks) {
-  __shared__ double red[4][kDotCols];
-  const int64_t row0 = blockIdx.x * (int64_t)kChunkRows + threadIdx.x;
-  const int k0 = blockIdx.y * kDotCols;
-  double wv[kRowsPerThread];
-#pragma unroll
-  for (int u = 0; u < kRowsPerThread; ++u) {
-    const int64_t i = row0 + u * 256;
-    wv[u] = i < dim ? w[i] : 0.0;
-  }
-  double acc[kDotCols];
-#pragma unroll
-  for (int c = 0; c < kDotCols; ++c) {
-    acc[c] = 0.0;
-    if (k0 + c < ncols) {
-      const double* __restrict__ col = (extra != nullptr && k0 + c == ncols - 1) ? extra : V + (int64_t)(k0 + c) * ld;
-#pragma unroll
-      for (int u = 0; u < kRowsPerThread; ++u) {
-        const int64_t i = row0 + u * 256;
-        if (i < dim) acc[c] += col[i] * wv[u];
-      }
-    }
-  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int c = 0; c < kDotCols; ++c) {
-    double v = acc[c];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (lane == 0) red[wave][c] = v;
+  const int k = blockIdx.y * 4 + wave;
+  if (k >= ncols) return;  // whole wave
+  const int64_t r0 = blockIdx.x * (int64_t)ROWS;
+  const double* __restrict__ col = (extra != nullptr && k == ncols - 1) ? extra : V + (int64_t)k * ld;
+  double acc = 0.0;
+#pragma unroll 8
+  for (int t = 0; t < ROWS / 64; ++t) {
+    const int64_t i = r0 + t * 64 + lane;
+    if (i < dim) acc += col[i] * w[i];
   }
-  __syncthreads();
-  if (threadIdx.x < kDotCols && k0 + (int)threadIdx.x < ncols) {
-    const int c = threadIdx.x;
-    partial[(int64_t)(k0 + c) * chunks + blockIdx.x] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
-  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) partial[(int64_t)k * chunks + blockIdx.x] = acc;
 }
 
 __global__ void rfinish_kernel(const double* __restrict__ partial, int64_t chunks, double* __restrict__ out) {
@@ -481,59 +517,37 @@ __global__ __launch_bounds__(256) void rmultiaxpy_kernel(const double* __restric
   if (wave == 0 && i < dim) w[i] += scale * (part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane]);
 }
 
-// With `extra` set, column ncols-1 is `extra` (never conjugated) instead of V + (ncols-1)*ld.
+// Complex twin of rmultidot_partial_kernel; conj(V) when `conjugate`, and the
+// `extra` column (when set) is never conjugated.
+template <int ROWS>
 __global__ __launch_bounds__(256) void zmultidot_partial_kernel(const double2* __restrict__ V, int64_t ld, int ncols,
                                                                 const double2* __restrict__ extra,
                                                                 const double2* __restrict__ w, int64_t dim,
                                                                 int conjugate, double2* __restrict__ partial,
                                                                 int64_t chunks) {
-  __shared__ double2 red[4][kDotCols];
-  const int64_t row0 = blockIdx.x * (int64_t)kChunkRows + threadIdx.x;
-  const int k0 = blockIdx.y * kDotCols;
-  const double sg = conjugate ? -1.0 : 1.0;
-  double2 wv[kRowsPerThread];
-#pragma unroll
-  for (int u = 0; u < kRowsPerThread; ++u) {
-    const int64_t i = row0 + u * 256;
-    wv[u] = i < dim ? w[i] : make_double2(0.0, 0.0);
-  }
-  double2 acc[kDotCols];
-#pragma unroll
-  for (int c = 0; c < kDotCols; ++c) {
-    acc[c] = make_double2(0.0, 0.0);
-    if (k0 + c < ncols) {
-      const bool ex = extra != nullptr && k0 + c == ncols - 1;
-      const double2* __restrict__ col = ex ? extra : V + (int64_t)(k0 + c) * ld;
-      const double sgc = ex ? 1.0 : sg;
-#pragma unroll
-      for (int u = 0; u < kRowsPerThread; ++u) {
-        const int64_t i = row0 + u * 256;
-        if (i < dim) {
-          const double2 a = col[i];
-          const double2 b = wv[u];
-          acc[c].x += a.x * b.x - sgc * a.y * b.y;
-          acc[c].y += a.x * b.y + sgc * a.y * b.x;
-        }
-      }
-    }
-  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int c = 0; c < kDotCols; ++c) {
-    double2 v = acc[c];
-    for (int off = 32; off > 0; off >>= 1) {
-      v.x += __shfl_down(v.x, off, 64);
-      v.y += __shfl_down(v.y, off, 64);
+  const int k = blockIdx.y * 4 + wave;
+  if (k >= ncols) return;  // whole wave
+  const int64_t r0 = blockIdx.x * (int64_t)ROWS;
+  const bool ex = extra != nullptr && k == ncols - 1;
+  const double2* __restrict__ col = ex ? extra : V + (int64_t)k * ld;
+  const double sg = (conjugate && !ex) ? -1.0 : 1.0;
+  double ax = 0.0, ay = 0.0;
+#pragma unroll 8
+  for (int t = 0; t < ROWS / 64; ++t) {
+    const int64_t i = r0 + t * 64 + lane;
+    if (i < dim) {
+      const double2 a = col[i];
+      const double2 b = w[i];
+      ax += a.x * b.x - sg * a.y * b.y;
+      ay += a.x * b.y + sg * a.y * b.x;
     }
-    if (lane == 0) red[wave][c] = v;
   }
-  __syncthreads();
-  if (threadIdx.x < kDotCols && k0 + (int)threadIdx.x < ncols) {
-    const int c = threadIdx.x;
-    partial[(int64_t)(k0 + c) * chunks + blockIdx.x] =
-        make_double2(red[0][c].x + red[1][c].x + red[2][c].x + red[3][c].x,
-                     red[0][c].y + red[1][c].y + red[2][c].y + red[3][c].y);
+  for (int off = 32; off > 0; off >>= 1) {
+    ax += __shfl_down(ax, off, 64);
+    ay += __shfl_down(ay, off, 64);
   }
+  if (lane == 0) partial[(int64_t)k * chunks + blockIdx.x] = make_double2(ax, ay);
 }
 
 __global__ void zfinish_kernel(const double2* __restrict__ partial, int64_t chunks, double2* __restrict__ out) {
@@ -638,7 +652,17 @@ struct KindExec {
   bool built = false;
   std::vector<BaseRun> runs;
   std::vector<void*> owned;
+  // small kinds: every level in one single-workgroup launch
+  bool single = false;
+  int32_t n_levels = 0;
+  ContractJob* jobs = nullptr;
+  int32_t* level_start = nullptr;
 };
+
+// Complex MACs below which one workgroup runs all levels of a kind (measured: one
+// workgroup takes ~150 us for the 2e5 MACs of the n = 25 EOM input, three separate
+// level launches ~33 us, so only tiny kinds qualify).
+constexpr int64_t kSingleLevelMacs = 1 << 15;
 
 struct PlanLevel {
   int32_t kind;
@@ -902,6 +926,11 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
   level.host_jobs = jobs;
   plan->levels.push_back(level);
   plan->ranges_built = false;
+  if (kind >= 0 && kind < kKinds) {
+    KindExec& ke = plan->kinds[kind];
+    for (void* p : ke.owned) (void)hipFree(p);
+    ke = KindExec();  // rebuilt at the next load
+  }
   return C3L_OK;
 }
 
@@ -996,6 +1025,25 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
       ke.runs.push_back(run);
       i = j;
     }
+    // all levels of this kind in one launch when the total work is small
+    std::vector<ContractJob> all;
+    std::vector<int32_t> starts{0};
+    int64_t macs = 0;
+    for (const PlanLevel& l : plan->levels) {
+      if (l.kind != kind || l.n_jobs == 0) continue;
+      for (const ContractJob& j : l.host_jobs) {
+        all.push_back(j);
+        macs += ipow(plan->n, j.par_rank);
+      }
+      starts.push_back((int32_t)all.size());
+    }
+    if (!all.empty() && macs <= kSingleLevelMacs) {
+      int rc = upload_owned(ke.owned, &ke.jobs, all.data(), all.size());
+      if (rc == C3L_OK) rc = upload_owned(ke.owned, &ke.level_start, starts.data(), starts.size());
+      if (rc != C3L_OK) return rc;
+      ke.single = true;
+      ke.n_levels = (int32_t)starts.size() - 1;
+    }
     ke.built = true;
   }
   const bool narrow = plan->kind_bits[kind] == 32;
@@ -1009,6 +1057,16 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
     C3L_HIP_CHECK(hipGetLastError());
   }
   if (!narrow) ++plan->fixed_version;
+  if (ke.single) {
+    if (narrow)
+      hipLaunchKernelGGL(contract_levels_kernel<float2>, dim3(1), dim3(1024), 0, s, plan->ws32, plan->vectors32, ke.jobs,
+                         ke.level_start, ke.n_levels, plan->n);
+    else
+      hipLaunchKernelGGL(contract_levels_kernel<double2>, dim3(1), dim3(1024), 0, s, plan->ws, plan->vectors, ke.jobs,
+                         ke.level_start, ke.n_levels, plan->n);
+    C3L_HIP_CHECK(hipGetLastError());
+    return C3L_OK;
+  }
   for (const PlanLevel& l : plan->levels) {
     if (l.kind != kind || l.n_jobs == 0) continue;
     if (narrow)
@@ -1305,10 +1363,13 @@ int c3l_rmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, i
                   double* out, void* stream) {
   if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || w == nullptr || scratch == nullptr || out == nullptr)
     return fail(C3L_EINVAL, "c3l_rmultidot: invalid argument");
-  const int64_t chunks = chunks_for(dim);
+  const int rows = dot_rows_for(ncols);
+  const int64_t chunks = (dim + rows - 1) / rows;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(rmultidot_partial_kernel, dim3(chunks, (ncols + kDotCols - 1) / kDotCols), dim3(kThreads), 0, s, V,
-                     ld, ncols, (const double*)nullptr, w, dim, scratch, chunks);
+  dispatch_rows(rows, [&](auto R) {
+    hipLaunchKernelGGL((rmultidot_partial_kernel<decltype(R)::value>), dim3(chunks, (ncols + 3) / 4), dim3(kThreads), 0,
+                       s, V, ld, ncols, (const double*)nullptr, w, dim, scratch, chunks);
+  });
   C3L_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(rfinish_kernel, dim3(ncols), dim3(kThreads), 0, s, scratch, chunks, out);
   C3L_HIP_CHECK(hipGetLastError());
@@ -1321,11 +1382,14 @@ int c3l_internal_rmultidot_extra(const double* V, int64_t ld, int32_t ncols, con
   if (ncols < 0 || dim <= 0 || ld < dim || (ncols > 0 && V == nullptr) || extra == nullptr || w == nullptr ||
       scratch == nullptr || out == nullptr)
     return fail(C3L_EINVAL, "c3l_internal_rmultidot_extra: invalid argument");
-  const int64_t chunks = chunks_for(dim);
   const int total = ncols + 1;
+  const int rows = dot_rows_for(total);
+  const int64_t chunks = (dim + rows - 1) / rows;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(rmultidot_partial_kernel, dim3(chunks, (total + kDotCols - 1) / kDotCols), dim3(kThreads), 0, s, V,
-                     ld, total, extra, w, dim, scratch, chunks);
+  dispatch_rows(rows, [&](auto R) {
+    hipLaunchKernelGGL((rmultidot_partial_kernel<decltype(R)::value>), dim3(chunks, (total + 3) / 4), dim3(kThreads), 0,
+                       s, V, ld, total, extra, w, dim, scratch, chunks);
+  });
   C3L_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(rfinish_kernel, dim3(total), dim3(kThreads), 0, s, scratch, chunks, out);
   C3L_HIP_CHECK(hipGetLastError());
@@ -1375,11 +1439,14 @@ int c3l_zmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, i
                   double* scratch, double* out, void* stream) {
   if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || w == nullptr || scratch == nullptr || out == nullptr)
     return fail(C3L_EINVAL, "c3l_zmultidot: invalid argument");
-  const int64_t chunks = chunks_for(dim);
+  const int rows = dot_rows_for(ncols);
+  const int64_t chunks = (dim + rows - 1) / rows;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(zmultidot_partial_kernel, dim3(chunks, (ncols + kDotCols - 1) / kDotCols), dim3(kThreads), 0, s,
-                     reinterpret_cast<const double2*>(V), ld, ncols, (const double2*)nullptr,
-                     reinterpret_cast<const double2*>(w), dim, conjugate, reinterpret_cast<double2*>(scratch), chunks);
+  dispatch_rows(rows, [&](auto R) {
+    hipLaunchKernelGGL((zmultidot_partial_kernel<decltype(R)::value>), dim3(chunks, (ncols + 3) / 4), dim3(kThreads), 0,
+                       s, reinterpret_cast<const double2*>(V), ld, ncols, (const double2*)nullptr,
+                       reinterpret_cast<const double2*>(w), dim, conjugate, reinterpret_cast<double2*>(scratch), chunks);
+  });
   C3L_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(zfinish_kernel, dim3(ncols), dim3(kThreads), 0, s, reinterpret_cast<const double2*>(scratch), chunks,
                      reinterpret_cast<double2*>(out));
@@ -1394,12 +1461,15 @@ int c3l_internal_zmultidot_extra(const double* V, int64_t ld, int32_t ncols, con
   if (ncols < 0 || dim <= 0 || ld < dim || (ncols > 0 && V == nullptr) || extra == nullptr || w == nullptr ||
       scratch == nullptr || out == nullptr)
     return fail(C3L_EINVAL, "c3l_internal_zmultidot_extra: invalid argument");
-  const int64_t chunks = chunks_for(dim);
   const int total = ncols + 1;
+  const int rows = dot_rows_for(total);
+  const int64_t chunks = (dim + rows - 1) / rows;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(zmultidot_partial_kernel, dim3(chunks, (total + kDotCols - 1) / kDotCols), dim3(kThreads), 0, s,
-                     reinterpret_cast<const double2*>(V), ld, total, reinterpret_cast<const double2*>(extra),
-                     reinterpret_cast<const double2*>(w), dim, conjugate, reinterpret_cast<double2*>(scratch), chunks);
+  dispatch_rows(rows, [&](auto R) {
+    hipLaunchKernelGGL((zmultidot_partial_kernel<decltype(R)::value>), dim3(chunks, (total + 3) / 4), dim3(kThreads), 0,
+                       s, reinterpret_cast<const double2*>(V), ld, total, reinterpret_cast<const double2*>(extra),
+                       reinterpret_cast<const double2*>(w), dim, conjugate, reinterpret_cast<double2*>(scratch), chunks);
+  });
   C3L_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(zfinish_kernel, dim3(total), dim3(kThreads), 0, s, reinterpret_cast<const double2*>(scratch), chunks,
                      reinterpret_cast<double2*>(out));
