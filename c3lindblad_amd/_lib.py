@@ -65,6 +65,8 @@ SIGNATURES = {
     "c3l_mlp_adam": ([_P, _P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I32, _P],
                      ctypes.c_int),
     "c3l_to_bf16": ([_P, _P, _I64, _P], ctypes.c_int),
+    "c3l_krylov_set_timing": ([_I32], ctypes.c_int),
+    "c3l_krylov_timing": ([_I32, _P, _P, _P], ctypes.c_int),
     "c3l_arnoldi": (
         [_P, _P, _P, _P, _I64, _I32, _P, _I32, _P, ctypes.c_double, _P, _P, _P, _P, _P, _P, _P, _P, _P,
          ctypes.c_double, _I32, ctypes.c_double, _P, _P, _P, _P],

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,54 @@ extern "C" int c3l_internal_zmultidot_extra(const double* V, int64_t ld, int32_t
 namespace {
 
 int dfail(int code, const std::string& msg) { return c3l_internal_fail(code, msg.c_str()); }
+
+// Optional HIP-event timing of the orthogonalisation kernels inside the drivers
+// (families: 0 complex dots, 1 complex axpys, 2 real dots, 3 real axpys), with the
+// algorithmic bytes of every timed call.  Shared by all host threads.
+struct KrylovTimer {
+  std::mutex mu;
+  bool enabled = false;
+  struct Pending {
+    int which;
+    double bytes;
+    hipEvent_t start, stop;
+  };
+  std::vector<Pending> pending;
+  double ms[4] = {0, 0, 0, 0};
+  double bytes[4] = {0, 0, 0, 0};
+  int64_t launches[4] = {0, 0, 0, 0};
+};
+
+KrylovTimer& ktimer() {
+  static KrylovTimer t;
+  return t;
+}
+
+struct KrylovScope {
+  bool on = false;
+  int which;
+  double bytes;
+  hipStream_t stream;
+  hipEvent_t start = nullptr, stop = nullptr;
+  KrylovScope(int w, double b, void* s) : which(w), bytes(b), stream(static_cast<hipStream_t>(s)) {
+    on = ktimer().enabled && hipEventCreate(&start) == hipSuccess && hipEventCreate(&stop) == hipSuccess &&
+         hipEventRecord(start, stream) == hipSuccess;
+  }
+  void done() {
+    if (!on) return;
+    on = false;
+    if (hipEventRecord(stop, stream) != hipSuccess) return;
+    std::lock_guard<std::mutex> guard(ktimer().mu);
+    ktimer().pending.push_back({which, bytes, start, stop});
+  }
+};
+
+#define K_TIMED(which, bytes, expr)            \
+  do {                                         \
+    KrylovScope scope__((which), (bytes), stream); \
+    D_CALL(expr);                              \
+    scope__.done();                            \
+  } while (0)
 
 #define D_CHECK(expr)                                                                       \
   do {                                                                                      \
@@ -358,11 +407,12 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
       double* c2 = coef + (R + 1);
       double* sq = coef + 2 * (R + 1);
       // c1 = V^T w and, in the same pass, c1[j+1] = ||w||^2 before orthogonalisation
-      D_CALL(c3l_internal_rmultidot_extra(V, rd, j + 1, w, w, rd, scratch, c1, stream));
-      D_CALL(c3l_rmultiaxpy(V, rd, j + 1, c1, -1.0, w, rd, stream));
-      D_CALL(c3l_rmultidot(V, rd, j + 1, w, rd, scratch, c2, stream));
-      D_CALL(c3l_rmultiaxpy(V, rd, j + 1, c2, -1.0, w, rd, stream));
-      D_CALL(c3l_rmultidot(w, rd, 1, w, rd, scratch, sq, stream));
+      const double col_bytes = 8.0 * (double)rd;
+      K_TIMED(2, (j + 2) * col_bytes, c3l_internal_rmultidot_extra(V, rd, j + 1, w, w, rd, scratch, c1, stream));
+      K_TIMED(3, (j + 3) * col_bytes, c3l_rmultiaxpy(V, rd, j + 1, c1, -1.0, w, rd, stream));
+      K_TIMED(2, (j + 2) * col_bytes, c3l_rmultidot(V, rd, j + 1, w, rd, scratch, c2, stream));
+      K_TIMED(3, (j + 3) * col_bytes, c3l_rmultiaxpy(V, rd, j + 1, c2, -1.0, w, rd, stream));
+      K_TIMED(2, col_bytes, c3l_rmultidot(w, rd, 1, w, rd, scratch, sq, stream));
       D_CHECK(hipMemcpyAsync(col.data(), coef, sizeof(double) * (2 * (R + 1) + 2), hipMemcpyDeviceToHost, s));
       D_CHECK(hipStreamSynchronize(s));
       for (int k = 0; k <= j; ++k) h(k, j) = col[k] + col[R + 1 + k];
@@ -416,6 +466,35 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   return C3L_OK;
 }
 
+int c3l_krylov_set_timing(int32_t enable) {
+  KrylovTimer& t = ktimer();
+  std::lock_guard<std::mutex> guard(t.mu);
+  t.enabled = enable != 0;
+  for (int k = 0; k < 4; ++k) t.ms[k] = t.bytes[k] = 0.0, t.launches[k] = 0;
+  return C3L_OK;
+}
+
+int c3l_krylov_timing(int32_t family, double* ms_total, int64_t* launches, double* bytes) {
+  if (family < 0 || family > 3) return dfail(C3L_EINVAL, "c3l_krylov_timing: invalid family");
+  KrylovTimer& t = ktimer();
+  std::lock_guard<std::mutex> guard(t.mu);
+  for (KrylovTimer::Pending& p : t.pending) {
+    D_CHECK(hipEventSynchronize(p.stop));
+    float ms = 0.f;
+    D_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+    t.ms[p.which] += ms;
+    t.bytes[p.which] += p.bytes;
+    t.launches[p.which] += 1;
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+  }
+  t.pending.clear();
+  if (ms_total) *ms_total = t.ms[family];
+  if (launches) *launches = t.launches[family];
+  if (bytes) *bytes = t.bytes[family];
+  return C3L_OK;
+}
+
 int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
                 const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
                 double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
@@ -450,11 +529,12 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
     double* wd = reinterpret_cast<double*>(w);
     D_CALL(c3l_plan_load(plan, 2, vj, stream));
     D_CALL(c3l_plan_apply(plan, 3, diag, vj, wd, stream));
-    D_CALL(c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c1, stream));
-    D_CALL(c3l_zmultiaxpy(V, dim, j + 1, c1, -1.0, wd, dim, stream));
-    D_CALL(c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c2, stream));
-    D_CALL(c3l_zmultiaxpy(V, dim, j + 1, c2, -1.0, wd, dim, stream));
-    D_CALL(c3l_internal_zmultidot_extra(wd, dim, 1, sink, wd, dim, 1, red, sq, stream));
+    const double col_bytes = 16.0 * (double)dim;
+    K_TIMED(0, (j + 2) * col_bytes, c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c1, stream));
+    K_TIMED(1, (j + 3) * col_bytes, c3l_zmultiaxpy(V, dim, j + 1, c1, -1.0, wd, dim, stream));
+    K_TIMED(0, (j + 2) * col_bytes, c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c2, stream));
+    K_TIMED(1, (j + 3) * col_bytes, c3l_zmultiaxpy(V, dim, j + 1, c2, -1.0, wd, dim, stream));
+    K_TIMED(0, 2 * col_bytes, c3l_internal_zmultidot_extra(wd, dim, 1, sink, wd, dim, 1, red, sq, stream));
     hipLaunchKernelGGL(finish_column_kernel, dim3(1), dim3(256), 0, s, cz, j, m_max, Hz, status, sink_v, breakdown_tol);
     D_CHECK(hipGetLastError());
     hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(dim)), dim3(256), 0, s, w, Vz + (int64_t)(j + 1) * dim, status,

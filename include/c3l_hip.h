@@ -207,6 +207,14 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
                 double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
                 int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
 
+/* Measurement hooks of the drivers (no counterpart in the reference): when enabled, every
+ * orthogonalisation call inside c3l_gmres / c3l_arnoldi is bracketed by HIP events on its
+ * stream.  c3l_krylov_timing waits for them and returns, for one family (0 complex dots,
+ * 1 complex axpys, 2 real dots, 3 real axpys), the accumulated milliseconds, the number of
+ * calls and their algorithmic bytes (columns read, vector read and written). */
+int c3l_krylov_set_timing(int32_t enable);
+int c3l_krylov_timing(int32_t family, double* ms_total, int64_t* launches, double* bytes);
+
 /* ---- learned warm start (MLP) ---------------------------------------------------------
  * No counterpart in the reference: a small MLP (bf16 weights, FP32 accumulation on the
  * matrix cores) maps model parameters to compressed initial amplitudes.  Matrices are
