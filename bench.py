@@ -246,8 +246,8 @@ def main() -> int:
         plan.set_timing(False)
     import ctypes
 
-    for family, name in enumerate(("zmultidot[arnoldi cgs2]", "zmultiaxpy[arnoldi cgs2]", "rmultidot[gmres cgs2]",
-                                   "rmultiaxpy[gmres cgs2]")):
+    for family, name in enumerate(("reduce_kernel<double2>[arnoldi]", "update_kernel<double2>[arnoldi]",
+                                   "reduce_kernel<double>[gmres]", "update_kernel<double>[gmres]")):
         ms, launches, nbytes = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
         lib.c3l_krylov_timing(family, ctypes.byref(ms), ctypes.byref(launches), ctypes.byref(nbytes))
         if launches.value:

@@ -127,6 +127,23 @@ struct ContractJob {
   int32_t sign;
 };
 
+// sum_q vec[q] par[q*inner] in order q = 0..n-1; the loads are issued eight at a
+// time so that one thread has eight in flight instead of one.
+template <typename C>
+__device__ __forceinline__ C contract_one(const C* __restrict__ par, const C* __restrict__ vec, int64_t inner, int n) {
+  C acc = czero<C>();
+  int q = 0;
+  for (; q + 8 <= n; q += 8) {
+    C p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) p[u] = par[(int64_t)(q + u) * inner];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = cadd(acc, cmul(vec[q + u], p[u]));
+  }
+  for (; q < n; ++q) acc = cadd(acc, cmul(vec[q], par[(int64_t)q * inner]));
+  return acc;
+}
+
 // grid.y = job; threads cover the n^(R-1) outputs of that job.
 template <typename C>
 __global__ void contract_kernel(C* __restrict__ ws, const C* __restrict__ vectors,
@@ -145,8 +162,7 @@ __global__ void contract_kernel(C* __restrict__ ws, const C* __restrict__ vector
     const int64_t hi = J / inner;
     const int64_t lo = J - hi * inner;
     const int64_t base = hi * n * inner + lo;
-    C acc = czero<C>();
-    for (int q = 0; q < n; ++q) acc = cadd(acc, cmul(vec[q], par[base + (int64_t)q * inner]));
+    C acc = contract_one(par + base, vec, inner, n);
     if (job.sign < 0) {
       acc.x = -acc.x;
       acc.y = -acc.y;
@@ -179,8 +195,7 @@ __global__ __launch_bounds__(1024) void contract_levels_kernel(C* __restrict__ w
         const int64_t hi = J / inner;
         const int64_t lo = J - hi * inner;
         const int64_t base = hi * n * inner + lo;
-        C acc = czero<C>();
-        for (int q = 0; q < n; ++q) acc = cadd(acc, cmul(vec[q], par[base + (int64_t)q * inner]));
+        C acc = contract_one(par + base, vec, inner, n);
         if (job.sign < 0) {
           acc.x = -acc.x;
           acc.y = -acc.y;

@@ -2,19 +2,23 @@
 // shift-invariant Arnoldi resolvent, both running their whole loop in C++ so
 // that the host only sees one call per linear solve / per Krylov basis.
 //
-// GMRES: left-preconditioned restarted GMRES with two-pass classical
-// Gram-Schmidt on the device; the (j+2)-number Hessenberg column is copied to
-// the host once per iteration for the Givens update (SciPy's tolerance control:
-// target max(atol, rtol ||b||) on the true residual, checked after every restart,
-// with an adaptively rescaled inner tolerance on the preconditioned residual).
+// Both orthogonalise with the one-reduction delayed-reorthogonalisation step of
+// ortho.hip: the operator is applied to the once-projected vector, and one
+// reduction plus one update pass per step complete the previous Hessenberg
+// column and project the new vector.  Column j-1 is therefore final after step j.
 //
-// Arnoldi: two-pass CGS on the device; the Hessenberg matrix stays on the
-// device; every shift z runs GMRES on the shared basis: its Givens QR of
-// (z I - H) advances by one column per step (one wave per shift, O(m) work)
-// and the sink projection of its solution is updated incrementally, so the
-// Green function and the exact residual of every shift are available at every
-// step without any dense factorisation.  One 16-byte transfer per step carries
-// the new sub-diagonal entry and the worst shift residual to the host.
+// GMRES: left-preconditioned restarted GMRES; the final Hessenberg column is
+// copied to the host once per iteration for the Givens update (SciPy's tolerance
+// control: target max(atol, rtol ||b||) on the true residual, checked after every
+// restart, with an adaptively rescaled inner tolerance on the preconditioned
+// residual).  The last column of a cycle is completed by a step without operator.
+//
+// Arnoldi: the Hessenberg matrix stays on the device; every shift z runs GMRES on
+// the shared basis: its Givens QR of (z I - H) advances by one column per step
+// (one wave per shift, O(m) work) and the sink projection of its solution is
+// updated incrementally, so the Green function and the exact residual of every
+// shift are available at every step without any dense factorisation.  A device
+// stop flag is read by the host every few steps.
 
 #include <hip/hip_runtime.h>
 
@@ -25,22 +29,19 @@
 #include <vector>
 
 #include "../../include/c3l_hip.h"
+#include "ortho.h"
 
-// Shared error slot and internal reductions of the library (defined in c3l_hip.hip).
+// Shared error slot of the library (defined in c3l_hip.hip).
 extern "C" int c3l_internal_fail(int code, const char* message);
-extern "C" int c3l_internal_rmultidot_extra(const double* V, int64_t ld, int32_t ncols, const double* extra,
-                                            const double* w, int64_t dim, double* scratch, double* out, void* stream);
-extern "C" int c3l_internal_zmultidot_extra(const double* V, int64_t ld, int32_t ncols, const double* extra,
-                                            const double* w, int64_t dim, int32_t conjugate, double* scratch,
-                                            double* out, void* stream);
 
 namespace {
 
 int dfail(int code, const std::string& msg) { return c3l_internal_fail(code, msg.c_str()); }
 
-// Optional HIP-event timing of the orthogonalisation kernels inside the drivers
-// (families: 0 complex dots, 1 complex axpys, 2 real dots, 3 real axpys), with the
-// algorithmic bytes of every timed call.  Shared by all host threads.
+// Optional HIP-event timing of the two bandwidth passes of every orthogonalisation
+// step inside the drivers (families: 0 Arnoldi reduction, 1 Arnoldi update,
+// 2 GMRES reduction, 3 GMRES update), with the algorithmic bytes of every timed
+// launch.  Shared by all host threads.
 struct KrylovTimer {
   std::mutex mu;
   bool enabled = false;
@@ -60,31 +61,38 @@ KrylovTimer& ktimer() {
   return t;
 }
 
-struct KrylovScope {
-  bool on = false;
-  int which;
-  double bytes;
-  hipStream_t stream;
-  hipEvent_t start = nullptr, stop = nullptr;
-  KrylovScope(int w, double b, void* s) : which(w), bytes(b), stream(static_cast<hipStream_t>(s)) {
-    on = ktimer().enabled && hipEventCreate(&start) == hipSuccess && hipEventCreate(&stop) == hipSuccess &&
-         hipEventRecord(start, stream) == hipSuccess;
+// Brackets one pass of an orthogonalisation step with events on its stream:
+// family `base` + pass (0 reduction, 1 update).
+struct DriverClock : c3l_ortho::Clock {
+  int base;
+  hipEvent_t start = nullptr;
+  explicit DriverClock(int b) : base(b) {}
+  void begin(int, hipStream_t stream) override {
+    start = nullptr;
+    if (!ktimer().enabled) return;
+    if (hipEventCreate(&start) != hipSuccess) {
+      start = nullptr;
+      return;
+    }
+    if (hipEventRecord(start, stream) != hipSuccess) {
+      (void)hipEventDestroy(start);
+      start = nullptr;
+    }
   }
-  void done() {
-    if (!on) return;
-    on = false;
-    if (hipEventRecord(stop, stream) != hipSuccess) return;
+  void end(int pass, double bytes, hipStream_t stream) override {
+    if (start == nullptr) return;
+    hipEvent_t stop = nullptr;
+    if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, stream) != hipSuccess) {
+      if (stop != nullptr) (void)hipEventDestroy(stop);
+      (void)hipEventDestroy(start);
+      start = nullptr;
+      return;
+    }
     std::lock_guard<std::mutex> guard(ktimer().mu);
-    ktimer().pending.push_back({which, bytes, start, stop});
+    ktimer().pending.push_back({base + pass, bytes, start, stop});
+    start = nullptr;
   }
 };
-
-#define K_TIMED(which, bytes, expr)            \
-  do {                                         \
-    KrylovScope scope__((which), (bytes), stream); \
-    D_CALL(expr);                              \
-    scope__.done();                            \
-  } while (0)
 
 #define D_CHECK(expr)                                                                       \
   do {                                                                                      \
@@ -100,35 +108,6 @@ struct KrylovScope {
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-
-// H[k, j] = c1[k] + c2[k] (k <= j); H[j+1, j] = sqrt(|w|^2); status[0] = H[j+1, j];
-// sink_v[j+1] = (sink . w) / H[j+1, j] = sink . v_{j+1} (0 after a breakdown).
-// coef holds c1 [m_max+1], c2 [m_max+1], |w|^2, sink . w (complex).
-__global__ void finish_column_kernel(const double2* __restrict__ coef, int j, int m_max, double2* __restrict__ H,
-                                     double* __restrict__ status, double2* __restrict__ sink_v, double breakdown) {
-  const double2* c1 = coef;
-  const double2* c2 = coef + (m_max + 1);
-  double2* col = H + (int64_t)j * (m_max + 1);
-  for (int k = threadIdx.x; k <= j; k += blockDim.x) col[k] = make_double2(c1[k].x + c2[k].x, c1[k].y + c2[k].y);
-  if (threadIdx.x == 0) {
-    const double s = coef[2 * (m_max + 1)].x;
-    const double2 sw = coef[2 * (m_max + 1) + 1];
-    const double hn = sqrt(s > 0.0 ? s : 0.0);
-    col[j + 1] = make_double2(hn, 0.0);
-    status[0] = hn;
-    const double a = hn > breakdown ? 1.0 / hn : 0.0;
-    sink_v[j + 1] = make_double2(a * sw.x, a * sw.y);
-  }
-}
-
-// V_next = w / hn (or 0 after a breakdown), hn read from device memory.
-__global__ void normalize_kernel(const double2* __restrict__ w, double2* __restrict__ v, const double* __restrict__ hn,
-                                 double breakdown, int64_t dim) {
-  const double h = hn[0];
-  const double a = h > breakdown ? 1.0 / h : 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x)
-    v[i] = make_double2(a * w[i].x, a * w[i].y);
 }
 
 // ---------------------------------------------------------------------------
@@ -295,6 +274,47 @@ int64_t blocks_for(int64_t n, int t = 256) {
   return b < 1 ? 1 : (b > 65536 ? 65536 : b);
 }
 
+// Workspace layouts in doubles.
+// GMRES (restart R, real basis of rd rows):
+//   coef:    red [2(R+2)] | ab [2(R+1)] | sc [8] | H [R(R+1)] | y [R]
+//   scratch: partial | hs   (the partial region also serves the norms)
+// Arnoldi (m_max columns, complex basis of dim rows), complex entries doubled:
+//   coef:    red [2(m+2)] | ab [2(m+1)] | sc [8] | sink_v [m+1]
+//   scratch: w [dim] | partial | hs
+struct Layout {
+  int64_t red, ab, sc, H, y, sink_v, coef_total;
+  int64_t w, partial, hs, scratch_total;
+};
+
+Layout gmres_layout(int64_t rd, int32_t R) {
+  Layout L{};
+  L.red = 0;
+  L.ab = L.red + 2 * ((int64_t)R + 2);
+  L.sc = L.ab + 2 * ((int64_t)R + 1);
+  L.H = L.sc + 8;
+  L.y = L.H + (int64_t)R * (R + 1);
+  L.coef_total = L.y + R;
+  L.w = 0;
+  L.partial = 0;
+  L.hs = L.partial + c3l_ortho::partial_entries(false, rd, R);
+  L.scratch_total = L.hs + c3l_ortho::hs_entries(R);
+  return L;
+}
+
+Layout arnoldi_layout(int64_t dim, int32_t m) {
+  Layout L{};
+  L.red = 0;
+  L.ab = L.red + 2 * 2 * ((int64_t)m + 2);
+  L.sc = L.ab + 2 * 2 * ((int64_t)m + 1);
+  L.sink_v = L.sc + 2 * 8;
+  L.coef_total = L.sink_v + 2 * ((int64_t)m + 1);
+  L.w = 0;
+  L.partial = 2 * dim;
+  L.hs = L.partial + 2 * c3l_ortho::partial_entries(true, dim, m);
+  L.scratch_total = L.hs + 2 * c3l_ortho::hs_entries(m);
+  return L;
+}
+
 }  // namespace
 
 namespace {
@@ -385,7 +405,12 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
     return C3L_OK;
   }
   const int R = restart;
-  std::vector<double> H((size_t)(R + 1) * R), cs(R), sn(R), S(R + 1), y(R), col(2 * (R + 1) + 2);
+  const Layout L = gmres_layout(rd, R);
+  c3l_ortho::Basis<double> basis{V, rd, rd, coef + L.H, R + 1, nullptr, nullptr, scratch + L.partial,
+                                 scratch + L.hs, coef + L.red, coef + L.ab, coef + L.sc, nullptr};
+  DriverClock clock(2);
+  std::vector<double> H((size_t)(R + 1) * R), cs(R), sn(R), S(R + 1), y(R), col(R + 2);
+  double tail[2] = {0.0, 0.0};
   auto h = [&](int i, int j) -> double& { return H[(size_t)i * R + j]; };
   D_CHECK(hipMemcpyAsync(w, b, sizeof(double) * rd, hipMemcpyDeviceToDevice, s));  // r = b - A 0
   double rn = bnrm;
@@ -401,49 +426,44 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
     int used = 0;
     bool breakdown = false;
     double presid = 0.0;
-    for (int j = 0; j < R; ++j) {
-      D_CALL(g.apply_preconditioned(V + (int64_t)j * rd, w));
-      double* c1 = coef;
-      double* c2 = coef + (R + 1);
-      double* sq = coef + 2 * (R + 1);
-      // c1 = V^T w and, in the same pass, c1[j+1] = ||w||^2 before orthogonalisation
-      const double col_bytes = 8.0 * (double)rd;
-      K_TIMED(2, (j + 2) * col_bytes, c3l_internal_rmultidot_extra(V, rd, j + 1, w, w, rd, scratch, c1, stream));
-      K_TIMED(3, (j + 3) * col_bytes, c3l_rmultiaxpy(V, rd, j + 1, c1, -1.0, w, rd, stream));
-      K_TIMED(2, (j + 2) * col_bytes, c3l_rmultidot(V, rd, j + 1, w, rd, scratch, c2, stream));
-      K_TIMED(3, (j + 3) * col_bytes, c3l_rmultiaxpy(V, rd, j + 1, c2, -1.0, w, rd, stream));
-      K_TIMED(2, col_bytes, c3l_rmultidot(w, rd, 1, w, rd, scratch, sq, stream));
-      D_CHECK(hipMemcpyAsync(col.data(), coef, sizeof(double) * (2 * (R + 1) + 2), hipMemcpyDeviceToHost, s));
+    // step j applies the operator to the once-projected v_j (none at j = R) and
+    // completes column j-1, which the host then rotates
+    for (int j = 0; j <= R; ++j) {
+      const bool has_w = j < R;
+      if (has_w) D_CALL(g.apply_preconditioned(V + (int64_t)j * rd, w));
+      D_CALL(c3l_ortho::step(basis, j, j == 0, has_w ? w : nullptr, 0.0, s, &clock));
+      if (j == 0) continue;
+      const int c = j - 1;
+      D_CHECK(hipMemcpyAsync(col.data(), coef + L.H + (int64_t)c * (R + 1), sizeof(double) * (j + 1),
+                             hipMemcpyDeviceToHost, s));
+      D_CHECK(hipMemcpyAsync(tail, coef + L.sc + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
       D_CHECK(hipStreamSynchronize(s));
-      for (int k = 0; k <= j; ++k) h(k, j) = col[k] + col[R + 1 + k];
-      const double sqv = col[2 * (R + 1)];
-      const double h0v = col[j + 1];
-      const double hn = std::sqrt(sqv > 0.0 ? sqv : 0.0);
-      const double h0 = std::sqrt(h0v > 0.0 ? h0v : 0.0);
+      for (int k = 0; k <= c; ++k) h(k, c) = col[k];
+      const double hn = col[j];
+      const double h0 = tail[1];  // |A v_c| before projection
       if (!std::isfinite(hn)) return dfail(C3L_EINVAL, "c3l_gmres: nonfinite Krylov vector");
       if (hn <= eps * h0) {  // exact solution indicator
-        h(j + 1, j) = 0.0;
+        h(c + 1, c) = 0.0;
         breakdown = true;
       } else {
-        h(j + 1, j) = hn;
-        D_CALL(c3l_rscale(w, V + (int64_t)(j + 1) * rd, 1.0 / hn, rd, stream));
+        h(c + 1, c) = hn;
       }
-      for (int k = 0; k < j; ++k) {
-        const double a = h(k, j), bb = h(k + 1, j);
-        h(k, j) = cs[k] * a + sn[k] * bb;
-        h(k + 1, j) = -sn[k] * a + cs[k] * bb;
+      for (int k = 0; k < c; ++k) {
+        const double a = h(k, c), bb = h(k + 1, c);
+        h(k, c) = cs[k] * a + sn[k] * bb;
+        h(k + 1, c) = -sn[k] * a + cs[k] * bb;
       }
-      const double a = h(j, j), bb = h(j + 1, j);
+      const double a = h(c, c), bb = h(c + 1, c);
       const double r = std::hypot(a, bb);
-      cs[j] = r == 0.0 ? 1.0 : a / r;
-      sn[j] = r == 0.0 ? 0.0 : bb / r;
-      h(j, j) = r;
-      h(j + 1, j) = 0.0;
-      S[j + 1] = -sn[j] * S[j];
-      S[j] = cs[j] * S[j];
-      presid = std::fabs(S[j + 1]);
+      cs[c] = r == 0.0 ? 1.0 : a / r;
+      sn[c] = r == 0.0 ? 0.0 : bb / r;
+      h(c, c) = r;
+      h(c + 1, c) = 0.0;
+      S[c + 1] = -sn[c] * S[c];
+      S[c] = cs[c] * S[c];
+      presid = std::fabs(S[c + 1]);
       ++*iterations;
-      used = j + 1;
+      used = c + 1;
       if (presid <= ptol || breakdown) break;
     }
     if (h(used - 1, used - 1) == 0.0) S[used - 1] = 0.0;
@@ -452,8 +472,8 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
       for (int l = k + 1; l < used; ++l) acc -= h(k, l) * y[l];
       y[k] = h(k, k) == 0.0 ? 0.0 : acc / h(k, k);
     }
-    D_CHECK(hipMemcpyAsync(coef, y.data(), sizeof(double) * used, hipMemcpyHostToDevice, s));
-    D_CALL(c3l_rmultiaxpy(V, rd, used, coef, 1.0, x, rd, stream));
+    D_CHECK(hipMemcpyAsync(coef + L.y, y.data(), sizeof(double) * used, hipMemcpyHostToDevice, s));
+    D_CALL(c3l_rmultiaxpy(V, rd, used, coef + L.y, 1.0, x, rd, stream));
     D_CALL(g.apply(x, w));
     D_CALL(c3l_raxpby(1.0, b, -1.0, w, rd, stream));
     D_CALL(g.norm(w, &rn));  // synchronises: y may be reused by the next cycle
@@ -463,6 +483,16 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   }
   *true_rel = rn / bnrm;
   *info = rn <= atol ? 0 : cycles;
+  return C3L_OK;
+}
+
+int c3l_krylov_workspace(int32_t complex_basis, int64_t dim, int32_t max_columns, int64_t* scratch_doubles,
+                         int64_t* coef_doubles) {
+  if (dim <= 0 || max_columns <= 0 || scratch_doubles == nullptr || coef_doubles == nullptr)
+    return dfail(C3L_EINVAL, "c3l_krylov_workspace: invalid argument");
+  const Layout L = complex_basis ? arnoldi_layout(dim, max_columns) : gmres_layout(dim, max_columns);
+  *scratch_doubles = L.scratch_total;
+  *coef_doubles = L.coef_total;
   return C3L_OK;
 }
 
@@ -506,49 +536,46 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
       m_out == nullptr || breakdown_out == nullptr || worst_out == nullptr)
     return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const Layout L = arnoldi_layout(dim, m_max);
   double2* Vz = reinterpret_cast<double2*>(V);
   double2* Hz = reinterpret_cast<double2*>(H);
-  double2* cz = reinterpret_cast<double2*>(coef);
-  double2* w = reinterpret_cast<double2*>(scratch);  // first dim complex entries of scratch hold w
-  double* red = scratch + 2 * dim;                   // reduction partials after w
-  double* c1 = coef;                                  // complex [m_max+1]
-  double* c2 = coef + 2 * (m_max + 1);                // complex [m_max+1]
-  double* sq = coef + 4 * (m_max + 1);                // complex [2]: |w|^2, sink . w
-  double2* sink_v = reinterpret_cast<double2*>(coef + 4 * (m_max + 1) + 4);  // complex [m_max+1]: sink . v_j
+  double2* w = reinterpret_cast<double2*>(scratch + L.w);
+  double2* sink_v = reinterpret_cast<double2*>(coef + L.sink_v);
+  c3l_ortho::Basis<double2> basis{Vz, dim, dim, Hz, (int64_t)m_max + 1, sink_v, reinterpret_cast<const double2*>(sink),
+                                  reinterpret_cast<double2*>(scratch + L.partial),
+                                  reinterpret_cast<double2*>(scratch + L.hs), reinterpret_cast<double2*>(coef + L.red),
+                                  reinterpret_cast<double2*>(coef + L.ab), reinterpret_cast<double2*>(coef + L.sc),
+                                  status};
+  DriverClock clock(0);
   hipLaunchKernelGGL(init_shift_state_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s,
                      reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), beta, n_shifts, status);
   D_CHECK(hipGetLastError());
-  D_CALL(c3l_zmultidot(sink, dim, 1, V, dim, 0, red, reinterpret_cast<double*>(sink_v), stream));  // sink . v_0
   // status = [hn, worst, stop flag, m]; the host reads it every kCheck steps
   constexpr int kCheck = 8;
   double host[4] = {0.0, INFINITY, 0.0, 0.0};
   const int waves_per_block = 4;
   const int64_t shift_blocks = (n_shifts + waves_per_block - 1) / waves_per_block;
-  for (int32_t j = 0; j < m_max; ++j) {
-    double* vj = V + (int64_t)2 * j * dim;
-    double* wd = reinterpret_cast<double*>(w);
-    D_CALL(c3l_plan_load(plan, 2, vj, stream));
-    D_CALL(c3l_plan_apply(plan, 3, diag, vj, wd, stream));
-    const double col_bytes = 16.0 * (double)dim;
-    K_TIMED(0, (j + 2) * col_bytes, c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c1, stream));
-    K_TIMED(1, (j + 3) * col_bytes, c3l_zmultiaxpy(V, dim, j + 1, c1, -1.0, wd, dim, stream));
-    K_TIMED(0, (j + 2) * col_bytes, c3l_zmultidot(V, dim, j + 1, wd, dim, 1, red, c2, stream));
-    K_TIMED(1, (j + 3) * col_bytes, c3l_zmultiaxpy(V, dim, j + 1, c2, -1.0, wd, dim, stream));
-    K_TIMED(0, 2 * col_bytes, c3l_internal_zmultidot_extra(wd, dim, 1, sink, wd, dim, 1, red, sq, stream));
-    hipLaunchKernelGGL(finish_column_kernel, dim3(1), dim3(256), 0, s, cz, j, m_max, Hz, status, sink_v, breakdown_tol);
-    D_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(dim)), dim3(256), 0, s, w, Vz + (int64_t)(j + 1) * dim, status,
-                       breakdown_tol, dim);
-    D_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(multishift_gmres_kernel, dim3(shift_blocks), dim3(64 * waves_per_block), 0, s, Hz, j, m_max,
-                       reinterpret_cast<const double2*>(shifts), n_shifts, sink_v + j, beta,
+  // step j applies sigma to the once-projected v_j (none at j = m_max) and
+  // completes column j-1, which every shift then absorbs
+  for (int32_t j = 0; j <= m_max; ++j) {
+    const bool has_w = j < m_max;
+    if (has_w) {
+      double* vj = V + (int64_t)2 * j * dim;
+      D_CALL(c3l_plan_load(plan, 2, vj, stream));
+      D_CALL(c3l_plan_apply(plan, 3, diag, vj, reinterpret_cast<double*>(w), stream));
+    }
+    D_CALL(c3l_ortho::step(basis, j, j == 0, has_w ? w : nullptr, breakdown_tol, s, &clock));
+    if (j == 0) continue;
+    const int c = j - 1;
+    hipLaunchKernelGGL(multishift_gmres_kernel, dim3(shift_blocks), dim3(64 * waves_per_block), 0, s, Hz, c, m_max,
+                       reinterpret_cast<const double2*>(shifts), n_shifts, sink_v + c, beta,
                        rot_c, reinterpret_cast<double2*>(rot_s), reinterpret_cast<double2*>(pi),
                        reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), resid, status);
     D_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(arnoldi_status_kernel, dim3(1), dim3(256), 0, s, resid, n_shifts, status, j, min_dim, tolerance,
+    hipLaunchKernelGGL(arnoldi_status_kernel, dim3(1), dim3(256), 0, s, resid, n_shifts, status, c, min_dim, tolerance,
                        breakdown_tol);
     D_CHECK(hipGetLastError());
-    if ((j + 1) % kCheck == 0 || j + 1 == m_max) {
+    if (j % kCheck == 0 || j == m_max) {
       D_CHECK(hipMemcpyAsync(host, status, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
       D_CHECK(hipStreamSynchronize(s));
       if (host[2] != 0.0) break;

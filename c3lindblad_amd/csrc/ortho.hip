@@ -1,0 +1,410 @@
+// One-reduction Gram-Schmidt step with delayed reorthogonalisation (see ortho.h).
+//
+// Per step five launches on the caller's stream:
+//   reduce_kernel  one pass over basis columns 0..j (column j = q) against q and w,
+//                  one wave per column per chunk of rows, q and w held in registers:
+//                  partials of s = V^H q (s_j = |q|^2), z = V^H w (z_j = q^H w),
+//                  |w|^2 and sink . q;
+//   finish_kernel  one workgroup per reduced number;
+//   hs_kernel      partials of (H s)_k over 32-column slices of the final H;
+//   column_kernel  one workgroup: r, final column j-1, sink . v_j, tentative
+//                  column j, update coefficients;
+//   update_kernel  one pass over columns 0..j-1: v_j = q/r - V a and
+//                  q_{j+1} = w/g - (c/(r g)) q - V b.
+// Algebra (columns 0..j-1 orthonormal, q the once-projected v_j, nu its tentative
+// sub-diagonal, A v_{j-1} = V h + nu q):
+//   r = sqrt(|q|^2 - |s|^2), v_j = (q - V s)/r, H[:j, j-1] = h + nu s, H[j, j-1] = nu r;
+//   c = (q^H w - s^H z)/r = v_j^H w;
+//   A v_j = (w - A V s)/r = V_{j+1} ([z; c] - H s)/r + u/r, u = w - V z - c v_j,
+//   |u|^2 = |w|^2 - |z|^2 - |c|^2, q_{j+1} = u/|u|, tentative nu_j = |u|/r.
+
+#include "ortho.h"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "../../include/c3l_hip.h"
+
+extern "C" int c3l_internal_fail(int code, const char* message);
+
+namespace c3l_ortho {
+namespace {
+
+constexpr int kHsChunk = 32;       // Hessenberg columns per hs_kernel workgroup
+constexpr int kColumnThreads = 1024;
+
+template <typename T>
+struct Num;
+
+template <>
+struct Num<double> {
+  static __device__ __forceinline__ double zero() { return 0.0; }
+  static __device__ __forceinline__ double make(double re) { return re; }
+  static __device__ __forceinline__ double re(double a) { return a; }
+  static __device__ __forceinline__ double add(double a, double b) { return a + b; }
+  static __device__ __forceinline__ double sub(double a, double b) { return a - b; }
+  static __device__ __forceinline__ double mul(double a, double b) { return a * b; }
+  static __device__ __forceinline__ double scale(double a, double s) { return a * s; }
+  static __device__ __forceinline__ double abs2(double a) { return a * a; }
+  static __device__ __forceinline__ double fma(double a, double b, double acc) { return acc + a * b; }   // acc + a b
+  static __device__ __forceinline__ double cfma(double a, double b, double acc) { return acc + a * b; }  // acc + conj(a) b
+  static __device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+  }
+};
+
+template <>
+struct Num<double2> {
+  static __device__ __forceinline__ double2 zero() { return make_double2(0.0, 0.0); }
+  static __device__ __forceinline__ double2 make(double re) { return make_double2(re, 0.0); }
+  static __device__ __forceinline__ double re(double2 a) { return a.x; }
+  static __device__ __forceinline__ double2 add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+  static __device__ __forceinline__ double2 sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+  static __device__ __forceinline__ double2 mul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  }
+  static __device__ __forceinline__ double2 scale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+  static __device__ __forceinline__ double abs2(double2 a) { return a.x * a.x + a.y * a.y; }
+  static __device__ __forceinline__ double2 fma(double2 a, double2 b, double2 acc) {
+    return make_double2(acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x);
+  }
+  static __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
+    return make_double2(acc.x + a.x * b.x + a.y * b.y, acc.y + a.x * b.y - a.y * b.x);
+  }
+  static __device__ __forceinline__ double2 wave_sum(double2 v) {
+    for (int off = 32; off > 0; off >>= 1) {
+      v.x += __shfl_down(v.x, off, 64);
+      v.y += __shfl_down(v.y, off, 64);
+    }
+    return v;
+  }
+};
+
+// Rows per reduction chunk (8 per lane: q and w stay in registers at full
+// occupancy) and waves per update workgroup.  Complex bases are short (tens of
+// thousands of rows): 8-wave update groups keep the GPU full.  Real bases (the
+// CCSD Newton correction) are long.
+template <typename T>
+struct Shape;
+template <>
+struct Shape<double> {
+  static constexpr int kRows = 512;
+  static constexpr int kWaves = 4;
+};
+template <>
+struct Shape<double2> {
+  static constexpr int kRows = 512;
+  static constexpr int kWaves = 8;
+};
+
+// Grid (chunks, column groups), 4 waves per group; wave `wave` of group g takes
+// columns g*4*cpw + wave + 4c (c < cpw).  Column nv is the extra one: its q-dot
+// is sink . q (unconjugated, 0 without a sink) and its w-dot |w|^2.
+// Partials: s-side output k at partial[k*chunks + chunk], w-side at
+// partial[(nv+1+k)*chunks + chunk].
+template <typename T, int ROWS>
+__global__ __launch_bounds__(256) void reduce_kernel(const T* __restrict__ V, int64_t ld, int nv, int cpw,
+                                                     const T* __restrict__ w, const T* __restrict__ sink,
+                                                     int64_t dim, T* __restrict__ partial, int64_t chunks) {
+  using N = Num<T>;
+  constexpr int kPer = ROWS / 64;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kbase = blockIdx.y * 4 * cpw + wave;
+  if (kbase > nv) return;  // whole wave
+  const int64_t r0 = blockIdx.x * (int64_t)ROWS + lane;
+  const T* __restrict__ q = V + (int64_t)(nv - 1) * ld;
+  T qv[kPer], wv[kPer];
+#pragma unroll
+  for (int t = 0; t < kPer; ++t) {
+    const int64_t i = r0 + t * 64;
+    qv[t] = i < dim ? q[i] : N::zero();
+    wv[t] = (w != nullptr && i < dim) ? w[i] : N::zero();
+  }
+  for (int c = 0; c < cpw; ++c) {
+    const int k = kbase + 4 * c;
+    if (k > nv) break;
+    T as = N::zero(), az = N::zero();
+    if (k < nv) {
+      const T* __restrict__ col = V + (int64_t)k * ld;
+#pragma unroll
+      for (int t = 0; t < kPer; ++t) {
+        const int64_t i = r0 + t * 64;
+        if (i < dim) {
+          const T v = col[i];
+          as = N::cfma(v, qv[t], as);
+          az = N::cfma(v, wv[t], az);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < kPer; ++t) {
+        const int64_t i = r0 + t * 64;
+        if (sink != nullptr && i < dim) as = N::fma(sink[i], qv[t], as);
+        az = N::cfma(wv[t], wv[t], az);
+      }
+    }
+    as = N::wave_sum(as);
+    az = N::wave_sum(az);
+    if (lane == 0) {
+      partial[(int64_t)k * chunks + blockIdx.x] = as;
+      if (w != nullptr) partial[(int64_t)(nv + 1 + k) * chunks + blockIdx.x] = az;
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void finish_kernel(const T* __restrict__ partial, int64_t chunks, T* __restrict__ out) {
+  using N = Num<T>;
+  __shared__ T part[4];
+  const int o = blockIdx.x;
+  T acc = N::zero();
+  for (int64_t c = threadIdx.x; c < chunks; c += 256) acc = N::add(acc, partial[(int64_t)o * chunks + c]);
+  acc = N::wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) out[o] = N::add(N::add(part[0], part[1]), N::add(part[2], part[3]));
+}
+
+// P[g*ldh + k] = sum over l in slice g of Hf[k, l] s_l for rows k < j, where Hf is
+// H with column j-1 completed (H[k, j-1] + nu s_k); Hessenberg: l >= k-1.  Every
+// load of a slice is issued up front (clamped addresses inside the allocated H,
+// whose entries below the sub-diagonal stay zero; masked terms are selected away).
+template <typename T>
+__global__ __launch_bounds__(64) void hs_kernel(const T* __restrict__ H, int64_t ldh, int j, const T* __restrict__ s,
+                                                T* __restrict__ P) {
+  using N = Num<T>;
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  if (k >= j) return;
+  const int l0 = blockIdx.y * kHsChunk;
+  const T nu = H[(int64_t)(j - 1) * ldh + j];
+  const T sk = s[k];
+  T h[kHsChunk];
+#pragma unroll
+  for (int t = 0; t < kHsChunk; ++t) h[t] = H[(int64_t)min(l0 + t, j - 1) * ldh + k];
+  T acc = N::zero();
+#pragma unroll
+  for (int t = 0; t < kHsChunk; ++t) {
+    const int l = l0 + t;
+    const T hl = (l == j - 1) ? N::fma(nu, sk, h[t]) : h[t];
+    const T term = N::mul(hl, s[min(l, j - 1)]);
+    acc = (l < j && l >= k - 1) ? N::add(acc, term) : acc;
+  }
+  P[(int64_t)blockIdx.y * ldh + k] = acc;
+}
+
+// Sums of six doubles over the workgroup with one barrier (every thread gets them).
+__device__ void block_sum6(double (&v)[6], double (*shared)[6]) {
+  for (int q = 0; q < 6; ++q)
+    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int q = 0; q < 6; ++q) shared[wave][q] = v[q];
+  __syncthreads();
+  for (int q = 0; q < 6; ++q) {
+    double t = 0.0;
+    for (int u = 0; u < (int)(blockDim.x >> 6); ++u) t += shared[u][q];
+    v[q] = t;
+  }
+}
+
+__device__ __forceinline__ void put(double (&v)[6], int at, double x) { v[at] += x; }
+__device__ __forceinline__ void put(double (&v)[6], int at, double2 x) {
+  v[at] += x.x;
+  v[at + 1] += x.y;
+}
+__device__ __forceinline__ double take(const double (&v)[6], int at, double) { return v[at]; }
+__device__ __forceinline__ double2 take(const double (&v)[6], int at, double2) { return make_double2(v[at], v[at + 1]); }
+
+// sum_g P[g*ldh + k], eight independent loads at a time
+template <typename T>
+__device__ __forceinline__ T hs_row(const T* __restrict__ P, int64_t ldh, int G, int k) {
+  using N = Num<T>;
+  T acc = N::zero();
+  for (int g0 = 0; g0 < G; g0 += 8) {
+    T v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = P[(int64_t)min(g0 + u, G - 1) * ldh + k];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc = (g0 + u < G) ? N::add(acc, v[u]) : acc;
+  }
+  return acc;
+}
+
+// One workgroup.  red = [s_0..s_{j-1}, |q|^2, sink.q, z_0..z_{j-1}, q^H w, |w|^2].
+// sc = [1/r, 1/g, c/(r g), final H[j, j-1], its h0, h0 of the tentative column j]
+// where h0 is |w|/r, the norm of A v before projection.
+template <typename T>
+__global__ __launch_bounds__(kColumnThreads) void column_kernel(T* __restrict__ H, int64_t ldh, int j, int fresh,
+                                                                int has_w, const T* __restrict__ red,
+                                                                const T* __restrict__ P, int G,
+                                                                T* __restrict__ sink_v, T* __restrict__ ab,
+                                                                T* __restrict__ sc, double* __restrict__ status,
+                                                                double breakdown) {
+  using N = Num<T>;
+  __shared__ double shared[kColumnThreads / 64][6];
+  const int nv = j + 1;
+  const T* S = red;
+  const T* Z = red + nv + 1;
+  const int tid = threadIdx.x;
+  const double nu = fresh ? 0.0 : N::re(H[(int64_t)(j - 1) * ldh + j]);
+  const double h0_prev = N::re(sc[5]);
+  // [|s|^2, |z|^2, s^H z, s . sink_v] (complex sums in two slots; a real basis leaves slots 3, 5 at 0)
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int k = tid; k < j; k += kColumnThreads) {
+    const T s = fresh ? N::zero() : S[k];
+    const T z = has_w ? Z[k] : N::zero();
+    acc[0] += N::abs2(s);
+    acc[1] += N::abs2(z);
+    put(acc, 2, N::cfma(s, z, N::zero()));
+    if (sink_v != nullptr) put(acc, 4, N::fma(s, sink_v[k], N::zero()));
+  }
+  // one barrier: every thread has read nu and sc[5] before they change below
+  block_sum6(acc, shared);
+  const double ss = acc[0], zz = acc[1];
+  const T sz = take(acc, 2, T{}), sv = take(acc, 4, T{});
+  const double r = fresh ? 1.0 : sqrt(fmax(N::re(S[j]) - ss, 0.0));
+  const double hn = nu * r;
+  double ir = r > 0.0 ? 1.0 / r : 0.0;
+  if (!fresh && !(hn > breakdown)) ir = 0.0;  // breakdown: v_j = 0
+  T c = N::zero();
+  double omega = 0.0, g = 0.0, ig = 0.0;
+  if (has_w) {
+    c = N::scale(N::sub(Z[j], sz), ir);
+    omega = N::re(Z[nv]);
+    g = sqrt(fmax(omega - zz - N::abs2(c), 0.0));
+    ig = g > 0.0 ? 1.0 / g : 0.0;
+  }
+  const T cr = N::scale(c, ir);
+  T* prev = H + (int64_t)(j - 1) * ldh;  // column j-1 (touched only when !fresh)
+  T* col = H + (int64_t)j * ldh;         // column j (written only when has_w)
+  for (int k = tid; k < j; k += kColumnThreads) {
+    const T s = fresh ? N::zero() : S[k];
+    if (!fresh) prev[k] = N::fma(N::make(nu), s, prev[k]);
+    ab[k] = N::scale(s, ir);
+    if (has_w) {
+      const T z = Z[k];
+      const T hs = fresh ? N::zero() : hs_row(P, ldh, G, k);
+      col[k] = N::scale(N::sub(z, hs), ir);
+      ab[ldh + k] = N::scale(N::sub(z, N::mul(cr, s)), ig);
+    }
+  }
+  if (tid == 0) {
+    if (!fresh) prev[j] = N::make(hn);
+    if (has_w) {
+      const T hs = fresh ? N::zero() : N::scale(S[j - 1], hn);
+      col[j] = N::scale(N::sub(c, hs), ir);
+      col[j + 1] = N::make(g * ir);
+    }
+    if (sink_v != nullptr) sink_v[j] = N::scale(N::sub(S[nv], sv), ir);
+    sc[0] = N::make(ir);
+    sc[1] = N::make(ig);
+    sc[2] = N::scale(c, ir * ig);
+    sc[3] = N::make(hn);
+    sc[4] = N::make(h0_prev);
+    sc[5] = N::make(has_w ? sqrt(fmax(omega, 0.0)) * ir : 0.0);
+    if (status != nullptr && !fresh) status[0] = hn;
+  }
+}
+
+// 64 rows per workgroup, WAVES waves splitting the columns, partial sums met in LDS.
+template <typename T, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void update_kernel(T* __restrict__ V, int64_t ld, int j, int fresh,
+                                                            const T* __restrict__ w, const T* __restrict__ ab,
+                                                            int64_t ldh, const T* __restrict__ sc, int64_t dim) {
+  using N = Num<T>;
+  __shared__ T pa[WAVES][64];
+  __shared__ T pb[WAVES][64];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t i = blockIdx.x * (int64_t)64 + lane;
+  T A = N::zero(), B = N::zero();
+  if (i < dim) {
+    const T* a = ab;
+    const T* b = ab + ldh;
+#pragma unroll 4
+    for (int k = wave; k < j; k += WAVES) {
+      const T v = V[(int64_t)k * ld + i];
+      A = N::fma(a[k], v, A);
+      B = N::fma(b[k], v, B);
+    }
+  }
+  pa[wave][lane] = A;
+  pb[wave][lane] = B;
+  __syncthreads();
+  if (wave == 0 && i < dim) {
+    for (int u = 1; u < WAVES; ++u) {
+      A = N::add(A, pa[u][lane]);
+      B = N::add(B, pb[u][lane]);
+    }
+    T* q = V + (int64_t)j * ld;
+    const T qi = q[i];
+    if (!fresh) q[i] = N::sub(N::scale(qi, N::re(sc[0])), A);
+    if (w != nullptr) V[(int64_t)(j + 1) * ld + i] = N::sub(N::sub(N::scale(w[i], N::re(sc[1])), N::mul(sc[2], qi)), B);
+  }
+}
+
+int fail(const std::string& msg) { return c3l_internal_fail(C3L_EINVAL, msg.c_str()); }
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return C3L_OK;
+  return c3l_internal_fail(C3L_EHIP, (std::string(what) + ": " + hipGetErrorString(e)).c_str());
+}
+
+}  // namespace
+
+int64_t partial_entries(bool complex_basis, int64_t dim, int32_t max_columns) {
+  const int rows = complex_basis ? Shape<double2>::kRows : Shape<double>::kRows;
+  return 2 * ((int64_t)max_columns + 2) * ((dim + rows - 1) / rows);
+}
+
+int64_t hs_entries(int32_t max_columns) {
+  return (((int64_t)max_columns + kHsChunk - 1) / kHsChunk) * ((int64_t)max_columns + 1);
+}
+
+template <typename T>
+int step(const Basis<T>& b, int j, bool fresh, const T* w, double breakdown, hipStream_t s, Clock* clock) {
+  if (b.V == nullptr || b.H == nullptr || b.partial == nullptr || b.hs == nullptr || b.red == nullptr ||
+      b.ab == nullptr || b.sc == nullptr || b.dim <= 0 || b.ld < b.dim || j < 0 ||
+      (w != nullptr ? j + 2 > b.ldh : j + 1 > b.ldh) ||
+      (j == 0 && !fresh))
+    return fail("Krylov orthogonalisation step: invalid argument");
+  constexpr int ROWS = Shape<T>::kRows;
+  constexpr int WAVES = Shape<T>::kWaves;
+  const int nv = j + 1;
+  const int64_t chunks = (b.dim + ROWS - 1) / ROWS;
+  const int cols = nv + 1;
+  const int cpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, (int64_t)cols * chunks / 4096));
+  const int groups = (cols + 4 * cpw - 1) / (4 * cpw);
+  const double vec = (double)sizeof(T) * (double)b.dim;
+  if (clock) clock->begin(0, s);
+  hipLaunchKernelGGL((reduce_kernel<T, ROWS>), dim3(chunks, groups), dim3(256), 0, s, b.V, b.ld, nv, cpw, w, b.sink,
+                     b.dim, b.partial, chunks);
+  if (int rc = check_launch("reduce_kernel")) return rc;
+  if (clock) clock->end(0, vec * (nv + (w ? 1 : 0) + (b.sink ? 1 : 0)), s);
+  const int outs = w ? 2 * (nv + 1) : nv + 1;
+  hipLaunchKernelGGL((finish_kernel<T>), dim3(outs), dim3(256), 0, s, b.partial, chunks, b.red);
+  if (int rc = check_launch("finish_kernel")) return rc;
+  const int G = (j + kHsChunk - 1) / kHsChunk;
+  if (!fresh && w != nullptr) {
+    hipLaunchKernelGGL((hs_kernel<T>), dim3((j + 63) / 64, G), dim3(64), 0, s, b.H, b.ldh, j, b.red, b.hs);
+    if (int rc = check_launch("hs_kernel")) return rc;
+  }
+  hipLaunchKernelGGL((column_kernel<T>), dim3(1), dim3(kColumnThreads), 0, s, b.H, b.ldh, j, fresh ? 1 : 0,
+                     w ? 1 : 0, b.red, b.hs, G, b.sink_v, b.ab, b.sc, b.status, breakdown);
+  if (int rc = check_launch("column_kernel")) return rc;
+  if (clock) clock->begin(1, s);
+  hipLaunchKernelGGL((update_kernel<T, WAVES>), dim3((b.dim + 63) / 64), dim3(64 * WAVES), 0, s, b.V, b.ld, j,
+                     fresh ? 1 : 0, w, b.ab, b.ldh, b.sc, b.dim);
+  if (int rc = check_launch("update_kernel")) return rc;
+  if (clock) clock->end(1, vec * (j + 1 + (w ? 2 : 0) + (fresh ? 0 : 1)), s);
+  return C3L_OK;
+}
+
+template int step<double>(const Basis<double>&, int, bool, const double*, double, hipStream_t, Clock*);
+template int step<double2>(const Basis<double2>&, int, bool, const double2*, double, hipStream_t, Clock*);
+
+}  // namespace c3l_ortho

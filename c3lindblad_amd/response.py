@@ -147,9 +147,9 @@ def arnoldi_resolvent(
     gamma = torch.zeros(ns, dtype=c128, device=dev)
     green = torch.zeros(ns, dtype=c128, device=dev)
     resid = torch.zeros(ns, dtype=f64, device=dev)
-    chunks = int(_lib.load_library().c3l_reduction_chunks(n))
-    scratch = torch.zeros(2 * n + 2 * chunks * (m_max + 1), dtype=f64, device=dev)
-    coef = torch.zeros(3 * (m_max + 1) + 2, dtype=c128, device=dev)
+    scratch_size, coef_size = _lib.krylov_workspace(True, n, m_max)
+    scratch = torch.empty(scratch_size, dtype=f64, device=dev)
+    coef = torch.zeros(coef_size, dtype=f64, device=dev)
     status = torch.zeros(4, dtype=f64, device=dev)
     m_out, broke = ctypes.c_int32(), ctypes.c_int32()
     worst = ctypes.c_double()

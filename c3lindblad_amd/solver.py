@@ -143,6 +143,9 @@ class _DeviceNewton:
         self.coef = torch.zeros(2 * (restart + 2) + 2, dtype=f64, device=_lib.DEVICE)
         self.host = _lib.pinned(torch.zeros(2 * (restart + 2) + 2, dtype=f64))
         self.reducer = Reducer(D, restart + 2, complex_=False)
+        scratch, coef = _lib.krylov_workspace(False, D, restart)
+        self.gmres_scratch = torch.empty(scratch, dtype=f64, device=_lib.DEVICE)
+        self.gmres_coef = torch.zeros(coef, dtype=f64, device=_lib.DEVICE)
         cached = problem.__dict__.get("_inverse_preconditioner")
         if cached is None:
             diag = -1j * problem.oracle.rapidity_sums()
@@ -204,7 +207,7 @@ class _DeviceNewton:
         _lib.call(
             "c3l_gmres", self.dev.plan.handle, ctypes.byref(maps), _lib.dptr(self.dev.diag), _lib.dptr(self.dev.dt),
             _lib.dptr(self.dev.out), _lib.dptr(b), _lib.dptr(x), _lib.dptr(self.V), _lib.dptr(self.w),
-            _lib.dptr(self.tmp), _lib.dptr(self.reducer.scratch), _lib.dptr(self.coef), float(rtol), float(atol), int(restart),
+            _lib.dptr(self.tmp), _lib.dptr(self.gmres_scratch), _lib.dptr(self.gmres_coef), float(rtol), float(atol), int(restart),
             int(cycles), ctypes.byref(info), ctypes.byref(iterations), ctypes.byref(true), ctypes.byref(calls),
             _lib.stream_ptr(self.torch),
         )

@@ -182,8 +182,12 @@ typedef struct c3l_tilde_maps {
  * (initial ||M b|| min(1, target/||b||), rescaled after each restart).  info = 0 when the
  * target is met, else `cycles`.  On return w holds the true residual b - J x.
  * The plan's T atoms must be loaded (linearisation point).  Workspace: t_work, out_work
- * complex [dim]; V [(restart+1) x real_dim]; w, tmp [real_dim]; scratch >= chunks*(restart+2)
- * doubles; coef >= 2*(restart+1)+2 doubles. x is overwritten with the step. */
+ * complex [dim]; V [(restart+1) x real_dim]; w, tmp [real_dim]; scratch and coef at least
+ * the sizes c3l_krylov_workspace(0, real_dim, restart, ...) returns.  Orthogonalisation:
+ * classical Gram-Schmidt with delayed reorthogonalisation (one reduction and one update pass
+ * per iteration); each iteration applies J to the once-projected basis vector, so a cycle
+ * that stops on the inner test has applied J once more than it has iterations.
+ * x is overwritten with the step. */
 int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
               const double* b, double* x, double* V, double* w, double* tmp, double* scratch, double* coef,
               double rtol, double atol, int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations,
@@ -191,13 +195,15 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
 
 /* Shift-invariant Krylov resolvent on the sigma program (replaces the per-frequency GMRES
  * loop of component_retarded_green, src/c3l/cc/component_response.py:180-245, and the Arnoldi
- * resolvent arnoldi_resolvent, src/c3l/green/krylov.py:284-417).  One Arnoldi basis (two-pass
- * CGS) serves every shift; each shift runs GMRES on it with its sink projection
+ * resolvent arnoldi_resolvent, src/c3l/green/krylov.py:284-417).  One Arnoldi basis (classical
+ * Gram-Schmidt with delayed reorthogonalisation) serves every shift; each shift runs GMRES on
+ * it with its sink projection
  * green[s] = sink . x_m(z_s) updated incrementally.  V [(m_max+1) x dim] complex with
  * V[0] = b/beta on entry; H complex, column j at H + j*(m_max+1); rot_c [n_shifts*m_max]
  * doubles, rot_s and pi [n_shifts*m_max] complex, gamma and green [n_shifts] complex, resid
- * [n_shifts] doubles (relative GMRES residual); scratch >= 2*dim + 2*chunks*(m_max+1) doubles;
- * coef >= 6*(m_max+1)+4 doubles; status >= 4 doubles.  Stops at the first m >= min_dim with
+ * [n_shifts] doubles (relative GMRES residual); scratch and coef at least the sizes
+ * c3l_krylov_workspace(1, dim, m_max, ...) returns; status >= 4 doubles.  Stops at the first
+ * m >= min_dim with
  * every shift's residual <= tolerance, at a breakdown, or at m_max.  The stop test runs on
  * the device after every step and freezes the shift state; the host reads it every 8 steps,
  * so up to 7 further basis vectors may be computed (and ignored) after the stop. */
@@ -207,11 +213,19 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
                 double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
                 int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
 
-/* Measurement hooks of the drivers (no counterpart in the reference): when enabled, every
- * orthogonalisation call inside c3l_gmres / c3l_arnoldi is bracketed by HIP events on its
- * stream.  c3l_krylov_timing waits for them and returns, for one family (0 complex dots,
- * 1 complex axpys, 2 real dots, 3 real axpys), the accumulated milliseconds, the number of
- * calls and their algorithmic bytes (columns read, vector read and written). */
+/* Workspace sizes (in doubles) of c3l_arnoldi (complex_basis = 1, dim complex rows, m_max =
+ * max_columns) and c3l_gmres (complex_basis = 0, real_dim rows, restart = max_columns).  No
+ * counterpart in the reference. */
+int c3l_krylov_workspace(int32_t complex_basis, int64_t dim, int32_t max_columns, int64_t* scratch_doubles,
+                         int64_t* coef_doubles);
+
+/* Measurement hooks of the drivers (no counterpart in the reference): when enabled, the two
+ * bandwidth passes of every orthogonalisation step inside c3l_arnoldi / c3l_gmres (the
+ * reduction kernel and the update kernel) are bracketed by HIP events on their stream.
+ * c3l_krylov_timing waits for them and returns, for one family (0 Arnoldi reduction,
+ * 1 Arnoldi update, 2 GMRES reduction, 3 GMRES update), the accumulated milliseconds, the
+ * number of launches and their algorithmic bytes (basis columns and vectors read and
+ * written). */
 int c3l_krylov_set_timing(int32_t enable);
 int c3l_krylov_timing(int32_t family, double* ms_total, int64_t* launches, double* bytes);
 
