@@ -424,3 +424,39 @@ def test_solve_component_shifted_matches_dense_solve(torch_gpu):
         assert _rel(out.solution, want) < 1e-8
         warm = solve_component_shifted(op, z, b, initial_solution=want)
         assert warm.converged and _rel(warm.solution, want) < 1e-9
+
+
+def test_arnoldi_basis_is_orthonormal_and_satisfies_the_arnoldi_relation(torch_gpu):
+    """Size-independent certificate of the delayed-reorthogonalisation Arnoldi driver:
+    V^H V = I and sigma V_m = V_{m+1} H_m on a 300-column basis over a ragged
+    dimension (P = 744, not a multiple of the reduction chunk), including the
+    last column, which a step without operator completes."""
+
+    import torch
+
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, eom_ccsd_problem
+    from c3lindblad_amd.response import ComponentSigmaOperator, arnoldi_resolvent
+
+    print("case arnoldi relation n=8 m=300", flush=True)
+    p, sp = _static(8, 3.0, 9)
+    rng = np.random.default_rng(21)
+    t = p.tilde.unpack(0.02 * rng.normal(size=p.num_variables))
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    op = ComponentSigmaOperator(component_eom_context(ep, t))
+    b = rng.normal(size=ep.dimension) + 1j * rng.normal(size=ep.dimension)
+    sink = rng.normal(size=ep.dimension) + 1j * rng.normal(size=ep.dimension)
+    proj = arnoldi_resolvent(op, b, np.array([0.5 + 0.1j]), maximum_dimension=300, residual_tolerance=1e-300,
+                             minimum_dimension=300, sink=sink)
+    m = proj.dimension
+    assert m == 300 and not proj.breakdown
+    V = proj.basis[: m + 1]
+    gram = (V.conj() @ V.T).cpu().numpy()
+    assert np.max(np.abs(gram - np.eye(m + 1))) < 1e-12
+    AV = torch.empty(m, ep.dimension, dtype=torch.complex128, device="cuda")
+    for k in range(m):
+        op.matvec_device(V[k].contiguous(), AV[k])
+    H = torch.from_numpy(np.ascontiguousarray(proj.hessenberg[: m + 1, :m])).cuda()
+    defect = (AV.T - V.T @ H).cpu().numpy()
+    assert np.max(np.abs(defect)) / np.max(np.abs(proj.hessenberg)) < 1e-12
+    # first basis vector is the normalised source, untouched by the driver
+    assert np.max(np.abs(V[0].cpu().numpy() - b / np.linalg.norm(b))) < 1e-15
