@@ -145,6 +145,8 @@ def main() -> int:
                     help="skip the extra solve with the reference's default FP64 policy")
     ap.add_argument("--precision", default="fp32", choices=("fp64", "fp32"),
                     help="precision of the Jacobian-vector products inside GMRES (fp32 = mixed precision)")
+    ap.add_argument("--serial-probe", action="store_true",
+                    help="also time one solve with the EOM channels on one stream (no instrumentation)")
     ap.add_argument("--jvp-probe", type=int, default=0, metavar="K",
                     help="profiling workload only: one residual and K JVPs of siam24, then a 256 MiB copy")
     args = ap.parse_args()
@@ -220,6 +222,12 @@ def main() -> int:
     # Kernel timing: one extra solve with the EOM channels run one after the other, every
     # plan program and every driver orthogonalisation call bracketed by HIP events on its
     # stream (events of concurrent streams would also count the other streams' work).
+    serial_phases = None
+    if args.serial_probe:
+        pipe.concurrent = False
+        serial_phases = pipe.solve().seconds
+        pipe.concurrent = True
+        print(f"[bench] serial EOM solve {serial_phases}", file=sys.stderr, flush=True)
     lib = _lib_handle()
     for plan in plans:
         plan.set_timing(True)
@@ -294,6 +302,7 @@ def main() -> int:
             "max_shift_relative_residual": last.max_shift_residual,
             "gr_max_rel_spread_between_steps": spread,
             "seconds_per_phase": last.seconds,
+            **({"serial_eom_seconds_per_phase": serial_phases} if serial_phases else {}),
             "batch_instances": len(pipes),
             "batch_packed_residual_max": max(r.packed_residual_norm for rs in (batch_results or [results]) for r in rs),
             "batch_max_shift_residual": max(r.max_shift_residual for rs in (batch_results or [results]) for r in rs),

@@ -276,13 +276,13 @@ int64_t blocks_for(int64_t n, int t = 256) {
 
 // Workspace layouts in doubles.
 // GMRES (restart R, real basis of rd rows):
-//   coef:    red [2(R+2)] | ab [2(R+1)] | sc [8] | H [R(R+1)] | y [R]
+//   coef:    red [2(R+2)] | ab [2(R+1)] | sc [8] | H [R(R+1)] | y [R] | stage [R+2]
 //   scratch: partial | hs   (the partial region also serves the norms)
 // Arnoldi (m_max columns, complex basis of dim rows), complex entries doubled:
 //   coef:    red [2(m+2)] | ab [2(m+1)] | sc [8] | sink_v [m+1]
 //   scratch: w [dim] | partial | hs
 struct Layout {
-  int64_t red, ab, sc, H, y, sink_v, coef_total;
+  int64_t red, ab, sc, H, y, stage, sink_v, coef_total;
   int64_t w, partial, hs, scratch_total;
 };
 
@@ -293,7 +293,8 @@ Layout gmres_layout(int64_t rd, int32_t R) {
   L.sc = L.ab + 2 * ((int64_t)R + 1);
   L.H = L.sc + 8;
   L.y = L.H + (int64_t)R * (R + 1);
-  L.coef_total = L.y + R;
+  L.stage = L.y + R;
+  L.coef_total = L.stage + R + 2;
   L.w = 0;
   L.partial = 0;
   L.hs = L.partial + c3l_ortho::partial_entries(false, rd, R);
@@ -361,6 +362,22 @@ struct Gmres {
   }
 };
 
+// Two pinned host slots and their events: the host reads Hessenberg column c from slot
+// c % 2 while the GPU already works on the next step.
+struct HostColumns {
+  double* host = nullptr;
+  hipEvent_t done[2] = {nullptr, nullptr};
+  int64_t stride = 0;
+  ~HostColumns() {
+    for (hipEvent_t e : done)
+      if (e != nullptr) {
+        (void)hipEventSynchronize(e);  // no copy may still target the slots when they are freed
+        (void)hipEventDestroy(e);
+      }
+    if (host != nullptr) (void)hipHostFree(host);
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -407,10 +424,14 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   const int R = restart;
   const Layout L = gmres_layout(rd, R);
   c3l_ortho::Basis<double> basis{V, rd, rd, coef + L.H, R + 1, nullptr, nullptr, scratch + L.partial,
-                                 scratch + L.hs, coef + L.red, coef + L.ab, coef + L.sc, nullptr};
+                                 scratch + L.hs, coef + L.red, coef + L.ab, coef + L.sc, nullptr,
+                                 coef + L.stage};
   DriverClock clock(2);
-  std::vector<double> H((size_t)(R + 1) * R), cs(R), sn(R), S(R + 1), y(R), col(R + 2);
-  double tail[2] = {0.0, 0.0};
+  HostColumns pinned;
+  pinned.stride = R + 2;
+  D_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned.host), sizeof(double) * 2 * pinned.stride, 0));
+  for (hipEvent_t& e : pinned.done) D_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  std::vector<double> H((size_t)(R + 1) * R), cs(R), sn(R), S(R + 1), y(R);
   auto h = [&](int i, int j) -> double& { return H[(size_t)i * R + j]; };
   D_CHECK(hipMemcpyAsync(w, b, sizeof(double) * rd, hipMemcpyDeviceToDevice, s));  // r = b - A 0
   double rn = bnrm;
@@ -424,23 +445,15 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
     std::fill(S.begin(), S.end(), 0.0);
     S[0] = beta;
     int used = 0;
-    bool breakdown = false;
+    bool breakdown = false, stop = false;
     double presid = 0.0;
-    // step j applies the operator to the once-projected v_j (none at j = R) and
-    // completes column j-1, which the host then rotates
-    for (int j = 0; j <= R; ++j) {
-      const bool has_w = j < R;
-      if (has_w) D_CALL(g.apply_preconditioned(V + (int64_t)j * rd, w));
-      D_CALL(c3l_ortho::step(basis, j, j == 0, has_w ? w : nullptr, 0.0, s, &clock));
-      if (j == 0) continue;
-      const int c = j - 1;
-      D_CHECK(hipMemcpyAsync(col.data(), coef + L.H + (int64_t)c * (R + 1), sizeof(double) * (j + 1),
-                             hipMemcpyDeviceToHost, s));
-      D_CHECK(hipMemcpyAsync(tail, coef + L.sc + 3, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-      D_CHECK(hipStreamSynchronize(s));
+    // Givens update of completed column c from its pinned copy; true when the cycle stops.
+    auto absorb = [&](int c) -> int {
+      const double* col = pinned.host + (int64_t)(c % 2) * pinned.stride;
+      D_CHECK(hipEventSynchronize(pinned.done[c % 2]));
       for (int k = 0; k <= c; ++k) h(k, c) = col[k];
-      const double hn = col[j];
-      const double h0 = tail[1];  // |A v_c| before projection
+      const double hn = col[c + 1];
+      const double h0 = col[c + 2];  // |A v_c| before projection
       if (!std::isfinite(hn)) return dfail(C3L_EINVAL, "c3l_gmres: nonfinite Krylov vector");
       if (hn <= eps * h0) {  // exact solution indicator
         h(c + 1, c) = 0.0;
@@ -464,7 +477,30 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
       presid = std::fabs(S[c + 1]);
       ++*iterations;
       used = c + 1;
-      if (presid <= ptol || breakdown) break;
+      stop = presid <= ptol || breakdown;
+      return C3L_OK;
+    };
+    // Step j applies the operator to the once-projected v_j (none at j = R) and completes
+    // column j-1, which is copied to pinned slot (j-1) % 2; the host absorbs column j-2
+    // meanwhile, so it never waits for the step it has just queued.
+    int absorbed = 0;
+    for (int j = 0; j <= R && !stop; ++j) {
+      const bool has_w = j < R;
+      if (has_w) D_CALL(g.apply_preconditioned(V + (int64_t)j * rd, w));
+      D_CALL(c3l_ortho::step(basis, j, j == 0, has_w ? w : nullptr, 0.0, s, &clock));
+      if (j == 0) continue;
+      const int c = j - 1;
+      D_CHECK(hipMemcpyAsync(pinned.host + (int64_t)(c % 2) * pinned.stride, coef + L.stage,
+                             sizeof(double) * (c + 3), hipMemcpyDeviceToHost, s));
+      D_CHECK(hipEventRecord(pinned.done[c % 2], s));
+      if (c >= 1) {
+        D_CALL(absorb(c - 1));
+        absorbed = c;
+      }
+    }
+    while (!stop && absorbed < R) {  // columns still in flight at the end of the cycle
+      D_CALL(absorb(absorbed));
+      ++absorbed;
     }
     if (h(used - 1, used - 1) == 0.0) S[used - 1] = 0.0;
     for (int k = used - 1; k >= 0; --k) {

@@ -34,6 +34,7 @@ struct Basis {
   T* ab;           // update coefficients: a at ab, b at ab + ldh
   T* sc;           // scalars: [1/r, 1/g, c/(r g), hn(j-1), h0(j-1), h0(j)] (8 entries)
   double* status;  // status[0] = final H[j, j-1] when set (nullable)
+  T* stage;        // when set: final column j-1 rows 0..j, then its h0 (j+2 entries; nullable)
 };
 
 // Scratch sizes in units of T (entries) for a basis of `dim` rows and at most

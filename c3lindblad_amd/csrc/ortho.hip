@@ -235,14 +235,15 @@ __device__ __forceinline__ T hs_row(const T* __restrict__ P, int64_t ldh, int G,
 
 // One workgroup.  red = [s_0..s_{j-1}, |q|^2, sink.q, z_0..z_{j-1}, q^H w, |w|^2].
 // sc = [1/r, 1/g, c/(r g), final H[j, j-1], its h0, h0 of the tentative column j]
-// where h0 is |w|/r, the norm of A v before projection.
+// where h0 is |w|/r, the norm of A v before projection.  `stage` (optional) gets the
+// final column j-1 followed by its h0, contiguous for one copy to the host.
 template <typename T>
 __global__ __launch_bounds__(kColumnThreads) void column_kernel(T* __restrict__ H, int64_t ldh, int j, int fresh,
                                                                 int has_w, const T* __restrict__ red,
                                                                 const T* __restrict__ P, int G,
                                                                 T* __restrict__ sink_v, T* __restrict__ ab,
                                                                 T* __restrict__ sc, double* __restrict__ status,
-                                                                double breakdown) {
+                                                                T* __restrict__ stage, double breakdown) {
   using N = Num<T>;
   __shared__ double shared[kColumnThreads / 64][6];
   const int nv = j + 1;
@@ -282,7 +283,11 @@ __global__ __launch_bounds__(kColumnThreads) void column_kernel(T* __restrict__ 
   T* col = H + (int64_t)j * ldh;         // column j (written only when has_w)
   for (int k = tid; k < j; k += kColumnThreads) {
     const T s = fresh ? N::zero() : S[k];
-    if (!fresh) prev[k] = N::fma(N::make(nu), s, prev[k]);
+    if (!fresh) {
+      const T h = N::fma(N::make(nu), s, prev[k]);
+      prev[k] = h;
+      if (stage != nullptr) stage[k] = h;
+    }
     ab[k] = N::scale(s, ir);
     if (has_w) {
       const T z = Z[k];
@@ -293,6 +298,10 @@ __global__ __launch_bounds__(kColumnThreads) void column_kernel(T* __restrict__ 
   }
   if (tid == 0) {
     if (!fresh) prev[j] = N::make(hn);
+    if (!fresh && stage != nullptr) {
+      stage[j] = N::make(hn);
+      stage[j + 1] = N::make(h0_prev);
+    }
     if (has_w) {
       const T hs = fresh ? N::zero() : N::scale(S[j - 1], hn);
       col[j] = N::scale(N::sub(c, hs), ir);
@@ -394,7 +403,7 @@ int step(const Basis<T>& b, int j, bool fresh, const T* w, double breakdown, hip
     if (int rc = check_launch("hs_kernel")) return rc;
   }
   hipLaunchKernelGGL((column_kernel<T>), dim3(1), dim3(kColumnThreads), 0, s, b.H, b.ldh, j, fresh ? 1 : 0,
-                     w ? 1 : 0, b.red, b.hs, G, b.sink_v, b.ab, b.sc, b.status, breakdown);
+                     w ? 1 : 0, b.red, b.hs, G, b.sink_v, b.ab, b.sc, b.status, b.stage, breakdown);
   if (int rc = check_launch("column_kernel")) return rc;
   if (clock) clock->begin(1, s);
   hipLaunchKernelGGL((update_kernel<T, WAVES>), dim3((b.dim + 63) / 64), dim3(64 * WAVES), 0, s, b.V, b.ld, j,
