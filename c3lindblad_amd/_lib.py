@@ -65,7 +65,7 @@ SIGNATURES = {
     "c3l_mlp_adam": ([_P, _P, _P, _P, _P, _I64, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _I32, _P],
                      ctypes.c_int),
     "c3l_to_bf16": ([_P, _P, _I64, _P], ctypes.c_int),
-    "c3l_krylov_workspace": ([_I32, _I64, _I32, _P, _P], ctypes.c_int),
+    "c3l_krylov_workspace": ([_I32, _I64, _I32, _I32, _P, _P], ctypes.c_int),
     "c3l_krylov_set_timing": ([_I32], ctypes.c_int),
     "c3l_krylov_timing": ([_I32, _P, _P, _P], ctypes.c_int),
     "c3l_arnoldi": (
@@ -141,12 +141,12 @@ def host_ptr(array: np.ndarray) -> int:
     return array.ctypes.data
 
 
-def krylov_workspace(complex_basis: bool, dim: int, max_columns: int) -> tuple[int, int]:
+def krylov_workspace(complex_basis: bool, dim: int, max_columns: int, n_shifts: int = 0) -> tuple[int, int]:
     """``(scratch, coef)`` sizes in doubles of ``c3l_arnoldi`` / ``c3l_gmres`` workspaces."""
 
     scratch, coef = ctypes.c_int64(), ctypes.c_int64()
-    call("c3l_krylov_workspace", 1 if complex_basis else 0, int(dim), int(max_columns), ctypes.byref(scratch),
-         ctypes.byref(coef))
+    call("c3l_krylov_workspace", 1 if complex_basis else 0, int(dim), int(max_columns), int(n_shifts),
+         ctypes.byref(scratch), ctypes.byref(coef))
     return scratch.value, coef.value
 
 

@@ -131,96 +131,150 @@ __device__ __forceinline__ double2 shfl2(double2 v, int lane) {
 }
 __device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
-__global__ void multishift_gmres_kernel(const double2* __restrict__ H, int j, int m_max,
+// Multi-shift GMRES over a batch of nb <= kShiftBatch completed columns c0..c0+nb-1, one
+// 64-lane wave per shift.  The rotations 0..c0-1 computed by earlier launches act on
+// all nb columns in one pass over the shift's state: the affine maps of a rotation
+// share their multiplier -conj(s_k) across columns, so one scan of the multipliers
+// serves nb scans of the offsets.  Lane 0 then applies the rotations of the batch
+// itself, column after column, and forms each new rotation, pi and Green increment.
+// The residual and Green value after every column go to resid_hist / green_hist
+// ([b][shift]) so that the stop test can take the state of the exact stop column.
+constexpr int kShiftBatch = 4;
+
+__global__ void multishift_batch_kernel(const double2* __restrict__ H, int c0, int nb, int m_max,
                                         const double2* __restrict__ shifts, int n_shifts,
                                         const double2* __restrict__ sink_v, double beta, double* __restrict__ rot_c,
                                         double2* __restrict__ rot_s, double2* __restrict__ pi,
                                         double2* __restrict__ gamma, double2* __restrict__ green,
-                                        double* __restrict__ resid, const double* __restrict__ status) {
+                                        double* __restrict__ resid, double* __restrict__ resid_hist,
+                                        double2* __restrict__ green_hist, const double* __restrict__ status) {
   const int lane = threadIdx.x & 63;
   const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (s >= n_shifts || status[2] != 0.0) return;  // whole wave exits together; nothing after the stop
   const double2 z = shifts[s];
-  const double2* col = H + (int64_t)j * (m_max + 1);
   const int64_t row = (int64_t)s * m_max;
-  // running x at the start of the current block, and sum_k R[k, j] pi_k
-  double2 x = make_double2(-col[0].x + (j == 0 ? z.x : 0.0), -col[0].y + (j == 0 ? z.y : 0.0));
-  double2 acc = make_double2(0.0, 0.0);
-  for (int base = 0; base < j; base += 64) {
+  const int64_t ldh = m_max + 1;
+  double2 x[kShiftBatch], acc[kShiftBatch];
+#pragma unroll
+  for (int b = 0; b < kShiftBatch; ++b) {
+    const int cb = c0 + b;
+    const double2 h = b < nb ? H[(int64_t)cb * ldh] : make_double2(0.0, 0.0);
+    x[b] = make_double2(-h.x + (cb == 0 ? z.x : 0.0), -h.y + (cb == 0 ? z.y : 0.0));
+    acc[b] = make_double2(0.0, 0.0);
+  }
+  for (int base = 0; base < c0; base += 64) {
     const int k = base + lane;
-    double2 A = make_double2(1.0, 0.0), B = make_double2(0.0, 0.0);  // identity map
+    const bool on = k < c0;
+    double2 A = make_double2(1.0, 0.0);
     double c = 1.0;
-    double2 sn = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0), pk = make_double2(0.0, 0.0);
-    if (k < j) {
+    double2 sn = make_double2(0.0, 0.0), pk = make_double2(0.0, 0.0);
+    double2 a1[kShiftBatch], B[kShiftBatch];
+    if (on) {
       c = rot_c[row + k];
       sn = rot_s[row + k];
       pk = pi[row + k];
-      a1 = make_double2(-col[k + 1].x + (k + 1 == j ? z.x : 0.0), -col[k + 1].y + (k + 1 == j ? z.y : 0.0));
       A = make_double2(-sn.x, sn.y);  // -conj(s)
-      B = make_double2(c * a1.x, c * a1.y);
     }
-    // inclusive scan: lane l holds map_l o ... o map_base
+#pragma unroll
+    for (int b = 0; b < kShiftBatch; ++b) {
+      const int cb = c0 + b;
+      a1[b] = make_double2(0.0, 0.0);
+      if (on && b < nb) {
+        const double2 h = H[(int64_t)cb * ldh + k + 1];
+        a1[b] = make_double2(-h.x + (k + 1 == cb ? z.x : 0.0), -h.y + (k + 1 == cb ? z.y : 0.0));
+      }
+      B[b] = make_double2(c * a1[b].x, c * a1[b].y);
+    }
+    // inclusive scan: lane l holds map_l o ... o map_base for every column
     for (int off = 1; off < 64; off <<= 1) {
       const double2 Ap = shfl_up2(A, off);
-      const double2 Bp = shfl_up2(B, off);
-      if (lane >= off) {
-        B = zadd(zmul(A, Bp), B);
-        A = zmul(A, Ap);
+#pragma unroll
+      for (int b = 0; b < kShiftBatch; ++b) {
+        const double2 Bp = shfl_up2(B[b], off);
+        if (lane >= off) B[b] = zadd(zmul(A, Bp), B[b]);
       }
+      if (lane >= off) A = zmul(A, Ap);
     }
-    // exclusive prefix for this lane: previous lane's inclusive map (identity for lane 0)
-    double2 Ae = shfl_up2(A, 1), Be = shfl_up2(B, 1);
-    if (lane == 0) {
-      Ae = make_double2(1.0, 0.0);
-      Be = make_double2(0.0, 0.0);
+    double2 Ae = shfl_up2(A, 1);
+    if (lane == 0) Ae = make_double2(1.0, 0.0);
+    const int last = (c0 - base) < 64 ? (c0 - base - 1) : 63;
+    const double2 Al = shfl2(A, last);
+#pragma unroll
+    for (int b = 0; b < kShiftBatch; ++b) {
+      double2 Be = shfl_up2(B[b], 1);
+      if (lane == 0) Be = make_double2(0.0, 0.0);
+      const double2 xk = zadd(zmul(Ae, x[b]), Be);
+      if (on) {
+        const double2 Rkj = zadd(make_double2(c * xk.x, c * xk.y), zmul(sn, a1[b]));
+        acc[b] = zadd(acc[b], zmul(Rkj, pk));
+      }
+      const double2 Bl = shfl2(B[b], last);
+      x[b] = zadd(zmul(Al, x[b]), Bl);
     }
-    const double2 xk = zadd(zmul(Ae, x), Be);
-    if (k < j) {
-      const double2 Rkj = zadd(make_double2(c * xk.x, c * xk.y), zmul(sn, a1));
-      acc = zadd(acc, zmul(Rkj, pk));
-    }
-    // advance x past the whole block (last active lane's inclusive map)
-    const int last = (j - base) < 64 ? (j - base - 1) : 63;
-    const double2 Al = shfl2(A, last), Bl = shfl2(B, last);
-    x = zadd(zmul(Al, x), Bl);
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    acc.x += __shfl_down(acc.x, off, 64);
-    acc.y += __shfl_down(acc.y, off, 64);
-  }
-  if (lane == 0) {
-    const double2 b = make_double2(-col[j + 1].x, -col[j + 1].y);
-    const double ax = hypot(x.x, x.y);
-    const double bx = hypot(b.x, b.y);
-    double c;
-    double2 sn, r;
+#pragma unroll
+  for (int b = 0; b < kShiftBatch; ++b)
+    for (int off = 32; off > 0; off >>= 1) {
+      acc[b].x += __shfl_down(acc[b].x, off, 64);
+      acc[b].y += __shfl_down(acc[b].y, off, 64);
+    }
+  if (lane != 0) return;
+  double cr[kShiftBatch];
+  double2 sr[kShiftBatch], pr[kShiftBatch];
+  double2 g = gamma[s], G = green[s];
+  double res = 0.0;
+#pragma unroll
+  for (int b = 0; b < kShiftBatch; ++b) {
+    if (b >= nb) break;
+    const int cb = c0 + b;
+    const double2* col = H + (int64_t)cb * ldh;
+    double2 xb = x[b], ab = acc[b];
+    for (int bb = 0; bb < b; ++bb) {  // rotations of this batch
+      const int k = c0 + bb;
+      const double2 a1 = make_double2(-col[k + 1].x + (k + 1 == cb ? z.x : 0.0), -col[k + 1].y + (k + 1 == cb ? z.y : 0.0));
+      const double2 R = zadd(make_double2(cr[bb] * xb.x, cr[bb] * xb.y), zmul(sr[bb], a1));
+      ab = zadd(ab, zmul(R, pr[bb]));
+      xb = zadd(zmul(make_double2(-sr[bb].x, sr[bb].y), xb), make_double2(cr[bb] * a1.x, cr[bb] * a1.y));
+    }
+    const double2 bsub = make_double2(-col[cb + 1].x, -col[cb + 1].y);
+    const double ax = hypot(xb.x, xb.y);
+    const double bx = hypot(bsub.x, bsub.y);
+    double cn;
+    double2 snn, r;
     if (ax == 0.0) {
-      c = 0.0;
-      sn = make_double2(1.0, 0.0);
-      r = b;
+      cn = 0.0;
+      snn = make_double2(1.0, 0.0);
+      r = bsub;
     } else {
       const double nrm = hypot(ax, bx);
-      const double2 u = make_double2(x.x / ax, x.y / ax);
-      c = ax / nrm;
-      sn = zmul(u, make_double2(b.x / nrm, -b.y / nrm));
+      const double2 u = make_double2(xb.x / ax, xb.y / ax);
+      cn = ax / nrm;
+      snn = zmul(u, make_double2(bsub.x / nrm, -bsub.y / nrm));
       r = make_double2(u.x * nrm, u.y * nrm);
     }
-    rot_c[row + j] = c;
-    rot_s[row + j] = sn;
-    // pi_j = (sink.v_j - acc) / R[j, j]
-    const double2 num = make_double2(sink_v[0].x - acc.x, sink_v[0].y - acc.y);
+    rot_c[row + cb] = cn;
+    rot_s[row + cb] = snn;
+    cr[b] = cn;
+    sr[b] = snn;
+    // pi_cb = (sink.v_cb - sum_k R[k, cb] pi_k) / R[cb, cb]
+    const double2 num = make_double2(sink_v[cb].x - ab.x, sink_v[cb].y - ab.y);
     const double den = r.x * r.x + r.y * r.y;
     const double2 p = den > 0.0 ? make_double2((num.x * r.x + num.y * r.y) / den, (num.y * r.x - num.x * r.y) / den)
                                 : make_double2(0.0, 0.0);
-    pi[row + j] = p;
-    const double2 g = gamma[s];  // residual entry gamma~ before this rotation
-    const double2 gj = make_double2(c * g.x, c * g.y);
-    const double2 gn = zmul(make_double2(-sn.x, sn.y), g);
-    gamma[s] = gn;
-    green[s] = zadd(green[s], zmul(gj, p));
-    resid[s] = hypot(gn.x, gn.y) / beta;
+    pi[row + cb] = p;
+    pr[b] = p;
+    const double2 gj = make_double2(cn * g.x, cn * g.y);
+    g = zmul(make_double2(-snn.x, snn.y), g);  // residual entry after this rotation
+    G = zadd(G, zmul(gj, p));
+    res = hypot(g.x, g.y) / beta;
+    resid_hist[(int64_t)b * n_shifts + s] = res;
+    green_hist[(int64_t)b * n_shifts + s] = G;
   }
+  gamma[s] = g;
+  green[s] = G;
+  resid[s] = res;
 }
+
 
 __global__ void init_shift_state_kernel(double2* __restrict__ gamma, double2* __restrict__ green, double beta,
                                         int n_shifts, double* __restrict__ status) {
@@ -240,32 +294,72 @@ __global__ void fill_kernel(double* __restrict__ a, double v, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = v;
 }
 
-// Device-side stop test of the Arnoldi loop after step j (one workgroup).
-// status = [hn of the step, worst shift residual at the stop, stop flag, m]:
-// flag 1 = every shift converged (m >= min_dim), 2 = breakdown, 3 = nonfinite
-// recurrence.  Once the flag is set nothing here or in the multi-shift kernel
-// changes, so the host may enqueue further steps before it reads the flag.
-__global__ void arnoldi_status_kernel(const double* __restrict__ resid, int n, double* __restrict__ status, int j,
-                                      int min_dim, double tolerance, double breakdown) {
-  __shared__ double part[256];
-  double m = 0.0;
+// Stop test after a batch (one workgroup).  For the first column c of the batch with a
+// nonfinite or broken-down sub-diagonal H[c+1, c], or (c+1 >= min_dim) with every shift's
+// residual <= tolerance: status = [hn, worst, flag (3 nonfinite, 2 breakdown, 1 converged),
+// c+1], and green/resid take the values after that column.  Without a stop, status[1] is
+// the worst residual after the last column and status[3] its count.
+__global__ __launch_bounds__(1024) void arnoldi_batch_status_kernel(const double* __restrict__ resid_hist,
+                                                                    const double2* __restrict__ green_hist,
+                                                                    double2* __restrict__ green,
+                                                                    double* __restrict__ resid, int n,
+                                                                    double* __restrict__ status,
+                                                                    const double2* __restrict__ H, int m_max, int c0,
+                                                                    int nb, int min_dim, double tolerance,
+                                                                    double breakdown) {
+  __shared__ double part[kShiftBatch][16];
+  __shared__ int stop_b;
+  if (status[2] != 0.0) return;  // uniform: frozen after an earlier stop
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double m[kShiftBatch];
+#pragma unroll
+  for (int b = 0; b < kShiftBatch; ++b) m[b] = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = resid[i];
-    m = (v != v) ? INFINITY : (v > m ? v : m);  // NaN counts as unconverged
+#pragma unroll
+    for (int b = 0; b < kShiftBatch; ++b) {
+      if (b >= nb) break;
+      const double v = resid_hist[(int64_t)b * n + i];
+      m[b] = (v != v) ? INFINITY : (v > m[b] ? v : m[b]);  // NaN counts as unconverged
+    }
   }
-  part[threadIdx.x] = m;
+#pragma unroll
+  for (int b = 0; b < kShiftBatch; ++b) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(m[b], off, 64);
+      m[b] = o > m[b] ? o : m[b];
+    }
+    if (lane == 0) part[b][wave] = m[b];
+  }
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] = part[threadIdx.x] > part[threadIdx.x + w] ? part[threadIdx.x] : part[threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    stop_b = -1;
+    const int waves = (int)(blockDim.x >> 6);
+    const int64_t ldh = m_max + 1;
+    for (int b = 0; b < nb; ++b) {
+      double worst = 0.0;
+      for (int u = 0; u < waves; ++u) worst = part[b][u] > worst ? part[b][u] : worst;
+      const int c = c0 + b;
+      const double hn = H[(int64_t)c * ldh + c + 1].x;
+      status[0] = hn;
+      status[1] = worst;
+      status[3] = (double)(c + 1);
+      double flag = 0.0;
+      if (!(hn == hn) || hn == INFINITY) flag = 3.0;
+      else if (hn <= breakdown) flag = 2.0;
+      else if (c + 1 >= min_dim && worst <= tolerance) flag = 1.0;
+      if (flag != 0.0) {
+        status[2] = flag;
+        stop_b = b;
+        break;
+      }
+    }
   }
-  if (threadIdx.x == 0 && status[2] == 0.0) {
-    const double hn = status[0];
-    status[1] = part[0];
-    status[3] = (double)(j + 1);
-    if (!(hn == hn) || hn == INFINITY) status[2] = 3.0;
-    else if (hn <= breakdown) status[2] = 2.0;
-    else if (j + 1 >= min_dim && part[0] <= tolerance) status[2] = 1.0;
+  __syncthreads();
+  const int sb = stop_b;
+  if (sb < 0 || sb == nb - 1) return;  // the shift state already belongs to the last column
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    green[i] = green_hist[(int64_t)sb * n + i];
+    resid[i] = resid_hist[(int64_t)sb * n + i];
   }
 }
 
@@ -278,12 +372,12 @@ int64_t blocks_for(int64_t n, int t = 256) {
 // GMRES (restart R, real basis of rd rows):
 //   coef:    red [2(R+2)] | ab [2(R+1)] | sc [8] | H [R(R+1)] | y [R] | stage [R+2]
 //   scratch: partial | hs   (the partial region also serves the norms)
-// Arnoldi (m_max columns, complex basis of dim rows), complex entries doubled:
+// Arnoldi (m_max columns, complex basis of dim rows, n_shifts shifts), complex entries doubled:
 //   coef:    red [2(m+2)] | ab [2(m+1)] | sc [8] | sink_v [m+1]
-//   scratch: w [dim] | partial | hs
+//   scratch: w [dim] | partial | hs | green_hist [kShiftBatch n_shifts] | resid_hist (doubles)
 struct Layout {
   int64_t red, ab, sc, H, y, stage, sink_v, coef_total;
-  int64_t w, partial, hs, scratch_total;
+  int64_t w, partial, hs, green_hist, resid_hist, scratch_total;
 };
 
 Layout gmres_layout(int64_t rd, int32_t R) {
@@ -302,7 +396,7 @@ Layout gmres_layout(int64_t rd, int32_t R) {
   return L;
 }
 
-Layout arnoldi_layout(int64_t dim, int32_t m) {
+Layout arnoldi_layout(int64_t dim, int32_t m, int32_t n_shifts) {
   Layout L{};
   L.red = 0;
   L.ab = L.red + 2 * 2 * ((int64_t)m + 2);
@@ -312,7 +406,9 @@ Layout arnoldi_layout(int64_t dim, int32_t m) {
   L.w = 0;
   L.partial = 2 * dim;
   L.hs = L.partial + 2 * c3l_ortho::partial_entries(true, dim, m);
-  L.scratch_total = L.hs + 2 * c3l_ortho::hs_entries(m);
+  L.green_hist = L.hs + 2 * c3l_ortho::hs_entries(m);
+  L.resid_hist = L.green_hist + 2 * (int64_t)kShiftBatch * n_shifts;
+  L.scratch_total = L.resid_hist + (int64_t)kShiftBatch * n_shifts;
   return L;
 }
 
@@ -522,11 +618,11 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   return C3L_OK;
 }
 
-int c3l_krylov_workspace(int32_t complex_basis, int64_t dim, int32_t max_columns, int64_t* scratch_doubles,
-                         int64_t* coef_doubles) {
-  if (dim <= 0 || max_columns <= 0 || scratch_doubles == nullptr || coef_doubles == nullptr)
+int c3l_krylov_workspace(int32_t complex_basis, int64_t dim, int32_t max_columns, int32_t n_shifts,
+                         int64_t* scratch_doubles, int64_t* coef_doubles) {
+  if (dim <= 0 || max_columns <= 0 || n_shifts < 0 || scratch_doubles == nullptr || coef_doubles == nullptr)
     return dfail(C3L_EINVAL, "c3l_krylov_workspace: invalid argument");
-  const Layout L = complex_basis ? arnoldi_layout(dim, max_columns) : gmres_layout(dim, max_columns);
+  const Layout L = complex_basis ? arnoldi_layout(dim, max_columns, n_shifts) : gmres_layout(dim, max_columns);
   *scratch_doubles = L.scratch_total;
   *coef_doubles = L.coef_total;
   return C3L_OK;
@@ -572,7 +668,7 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
       m_out == nullptr || breakdown_out == nullptr || worst_out == nullptr)
     return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const Layout L = arnoldi_layout(dim, m_max);
+  const Layout L = arnoldi_layout(dim, m_max, n_shifts);
   double2* Vz = reinterpret_cast<double2*>(V);
   double2* Hz = reinterpret_cast<double2*>(H);
   double2* w = reinterpret_cast<double2*>(scratch + L.w);
@@ -587,12 +683,15 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
                      reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), beta, n_shifts, status);
   D_CHECK(hipGetLastError());
   // status = [hn, worst, stop flag, m]; the host reads it every kCheck steps
-  constexpr int kCheck = 8;
+  constexpr int kCheck = 2 * kShiftBatch;
   double host[4] = {0.0, INFINITY, 0.0, 0.0};
   const int waves_per_block = 4;
   const int64_t shift_blocks = (n_shifts + waves_per_block - 1) / waves_per_block;
-  // step j applies sigma to the once-projected v_j (none at j = m_max) and
-  // completes column j-1, which every shift then absorbs
+  double* resid_hist = scratch + L.resid_hist;
+  double2* green_hist = reinterpret_cast<double2*>(scratch + L.green_hist);
+  int c0 = 0;  // first column the shifts have not absorbed yet
+  // step j applies sigma to the once-projected v_j (none at j = m_max) and completes
+  // column j-1; every kShiftBatch columns (and at the end) the shifts absorb them
   for (int32_t j = 0; j <= m_max; ++j) {
     const bool has_w = j < m_max;
     if (has_w) {
@@ -602,15 +701,20 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
     }
     D_CALL(c3l_ortho::step(basis, j, j == 0, has_w ? w : nullptr, breakdown_tol, s, &clock));
     if (j == 0) continue;
-    const int c = j - 1;
-    hipLaunchKernelGGL(multishift_gmres_kernel, dim3(shift_blocks), dim3(64 * waves_per_block), 0, s, Hz, c, m_max,
-                       reinterpret_cast<const double2*>(shifts), n_shifts, sink_v + c, beta,
-                       rot_c, reinterpret_cast<double2*>(rot_s), reinterpret_cast<double2*>(pi),
-                       reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), resid, status);
-    D_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(arnoldi_status_kernel, dim3(1), dim3(256), 0, s, resid, n_shifts, status, c, min_dim, tolerance,
-                       breakdown_tol);
-    D_CHECK(hipGetLastError());
+    if (j - c0 == kShiftBatch || j == m_max) {
+      const int nb = j - c0;
+      hipLaunchKernelGGL(multishift_batch_kernel, dim3(shift_blocks), dim3(64 * waves_per_block), 0, s, Hz, c0, nb,
+                         m_max, reinterpret_cast<const double2*>(shifts), n_shifts, sink_v, beta, rot_c,
+                         reinterpret_cast<double2*>(rot_s), reinterpret_cast<double2*>(pi),
+                         reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), resid, resid_hist,
+                         green_hist, status);
+      D_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(arnoldi_batch_status_kernel, dim3(1), dim3(1024), 0, s, resid_hist, green_hist,
+                         reinterpret_cast<double2*>(green), resid, n_shifts, status, Hz, m_max, c0, nb, min_dim,
+                         tolerance, breakdown_tol);
+      D_CHECK(hipGetLastError());
+      c0 = j;
+    }
     if (j % kCheck == 0 || j == m_max) {
       D_CHECK(hipMemcpyAsync(host, status, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
       D_CHECK(hipStreamSynchronize(s));

@@ -147,7 +147,7 @@ def arnoldi_resolvent(
     gamma = torch.zeros(ns, dtype=c128, device=dev)
     green = torch.zeros(ns, dtype=c128, device=dev)
     resid = torch.zeros(ns, dtype=f64, device=dev)
-    scratch_size, coef_size = _lib.krylov_workspace(True, n, m_max)
+    scratch_size, coef_size = _lib.krylov_workspace(True, n, m_max, ns)
     scratch = torch.empty(scratch_size, dtype=f64, device=dev)
     coef = torch.zeros(coef_size, dtype=f64, device=dev)
     status = torch.zeros(4, dtype=f64, device=dev)

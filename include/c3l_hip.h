@@ -183,7 +183,7 @@ typedef struct c3l_tilde_maps {
  * target is met, else `cycles`.  On return w holds the true residual b - J x.
  * The plan's T atoms must be loaded (linearisation point).  Workspace: t_work, out_work
  * complex [dim]; V [(restart+1) x real_dim]; w, tmp [real_dim]; scratch and coef at least
- * the sizes c3l_krylov_workspace(0, real_dim, restart, ...) returns.  Orthogonalisation:
+ * the sizes c3l_krylov_workspace(0, real_dim, restart, 0, ...) returns.  Orthogonalisation:
  * classical Gram-Schmidt with delayed reorthogonalisation (one reduction and one update pass
  * per iteration); each iteration applies J to the once-projected basis vector, so a cycle
  * that stops on the inner test has applied J once more than it has iterations.
@@ -202,11 +202,12 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
  * V[0] = b/beta on entry; H complex, column j at H + j*(m_max+1); rot_c [n_shifts*m_max]
  * doubles, rot_s and pi [n_shifts*m_max] complex, gamma and green [n_shifts] complex, resid
  * [n_shifts] doubles (relative GMRES residual); scratch and coef at least the sizes
- * c3l_krylov_workspace(1, dim, m_max, ...) returns; status >= 4 doubles.  Stops at the first
- * m >= min_dim with
- * every shift's residual <= tolerance, at a breakdown, or at m_max.  The stop test runs on
- * the device after every step and freezes the shift state; the host reads it every 8 steps,
- * so up to 7 further basis vectors may be computed (and ignored) after the stop. */
+ * c3l_krylov_workspace(1, dim, m_max, n_shifts, ...) returns; status >= 4 doubles.  Stops at
+ * the first m >= min_dim with every shift's residual <= tolerance, at a breakdown, or at m_max.
+ * The shifts absorb completed columns four at a time; the stop test after each batch finds
+ * the exact first column that meets the criterion and freezes the shift state there.  The
+ * host reads it every 8 steps, so up to 12 further basis vectors may be computed (and
+ * ignored) after the stop. */
 int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
                 const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
                 double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
@@ -214,10 +215,10 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
                 int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
 
 /* Workspace sizes (in doubles) of c3l_arnoldi (complex_basis = 1, dim complex rows, m_max =
- * max_columns) and c3l_gmres (complex_basis = 0, real_dim rows, restart = max_columns).  No
- * counterpart in the reference. */
-int c3l_krylov_workspace(int32_t complex_basis, int64_t dim, int32_t max_columns, int64_t* scratch_doubles,
-                         int64_t* coef_doubles);
+ * max_columns, n_shifts shifts) and c3l_gmres (complex_basis = 0, real_dim rows, restart =
+ * max_columns; n_shifts ignored).  No counterpart in the reference. */
+int c3l_krylov_workspace(int32_t complex_basis, int64_t dim, int32_t max_columns, int32_t n_shifts,
+                         int64_t* scratch_doubles, int64_t* coef_doubles);
 
 /* Measurement hooks of the drivers (no counterpart in the reference): when enabled, the two
  * bandwidth passes of every orthogonalisation step inside c3l_arnoldi / c3l_gmres (the
