@@ -370,14 +370,14 @@ int64_t blocks_for(int64_t n, int t = 256) {
 
 // Workspace layouts in doubles.
 // GMRES (restart R, real basis of rd rows):
-//   coef:    red [2(R+2)] | ab [2(R+1)] | sc [8] | H [R(R+1)] | y [R] | stage [R+2]
-//   scratch: partial | hs   (the partial region also serves the norms)
+//   coef:    red [2(R+2)] | ab [2(R+1)] | sc [16] | H [R(R+1)] | y [R] | stage [R+2]
+//   scratch: partial   (also the norms' partials)
 // Arnoldi (m_max columns, complex basis of dim rows, n_shifts shifts), complex entries doubled:
-//   coef:    red [2(m+2)] | ab [2(m+1)] | sc [8] | sink_v [m+1]
-//   scratch: w [dim] | partial | hs | green_hist [kShiftBatch n_shifts] | resid_hist (doubles)
+//   coef:    red [2(m+2)] | ab [2(m+1)] | sc [16] | sink_v [m+1]
+//   scratch: w [dim] | partial | green_hist [kShiftBatch n_shifts] | resid_hist (doubles)
 struct Layout {
   int64_t red, ab, sc, H, y, stage, sink_v, coef_total;
-  int64_t w, partial, hs, green_hist, resid_hist, scratch_total;
+  int64_t w, partial, green_hist, resid_hist, scratch_total;
 };
 
 Layout gmres_layout(int64_t rd, int32_t R) {
@@ -385,14 +385,13 @@ Layout gmres_layout(int64_t rd, int32_t R) {
   L.red = 0;
   L.ab = L.red + 2 * ((int64_t)R + 2);
   L.sc = L.ab + 2 * ((int64_t)R + 1);
-  L.H = L.sc + 8;
+  L.H = L.sc + 16;
   L.y = L.H + (int64_t)R * (R + 1);
   L.stage = L.y + R;
   L.coef_total = L.stage + R + 2;
   L.w = 0;
   L.partial = 0;
-  L.hs = L.partial + c3l_ortho::partial_entries(false, rd, R);
-  L.scratch_total = L.hs + c3l_ortho::hs_entries(R);
+  L.scratch_total = L.partial + c3l_ortho::partial_entries(false, rd, R);
   return L;
 }
 
@@ -401,12 +400,11 @@ Layout arnoldi_layout(int64_t dim, int32_t m, int32_t n_shifts) {
   L.red = 0;
   L.ab = L.red + 2 * 2 * ((int64_t)m + 2);
   L.sc = L.ab + 2 * 2 * ((int64_t)m + 1);
-  L.sink_v = L.sc + 2 * 8;
+  L.sink_v = L.sc + 2 * 16;
   L.coef_total = L.sink_v + 2 * ((int64_t)m + 1);
   L.w = 0;
   L.partial = 2 * dim;
-  L.hs = L.partial + 2 * c3l_ortho::partial_entries(true, dim, m);
-  L.green_hist = L.hs + 2 * c3l_ortho::hs_entries(m);
+  L.green_hist = L.partial + 2 * c3l_ortho::partial_entries(true, dim, m);
   L.resid_hist = L.green_hist + 2 * (int64_t)kShiftBatch * n_shifts;
   L.scratch_total = L.resid_hist + (int64_t)kShiftBatch * n_shifts;
   return L;
@@ -520,8 +518,7 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   const int R = restart;
   const Layout L = gmres_layout(rd, R);
   c3l_ortho::Basis<double> basis{V, rd, rd, coef + L.H, R + 1, nullptr, nullptr, scratch + L.partial,
-                                 scratch + L.hs, coef + L.red, coef + L.ab, coef + L.sc, nullptr,
-                                 coef + L.stage};
+                                 coef + L.red, coef + L.ab, coef + L.sc, nullptr, coef + L.stage};
   DriverClock clock(2);
   HostColumns pinned;
   pinned.stride = R + 2;
@@ -674,8 +671,7 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
   double2* w = reinterpret_cast<double2*>(scratch + L.w);
   double2* sink_v = reinterpret_cast<double2*>(coef + L.sink_v);
   c3l_ortho::Basis<double2> basis{Vz, dim, dim, Hz, (int64_t)m_max + 1, sink_v, reinterpret_cast<const double2*>(sink),
-                                  reinterpret_cast<double2*>(scratch + L.partial),
-                                  reinterpret_cast<double2*>(scratch + L.hs), reinterpret_cast<double2*>(coef + L.red),
+                                  reinterpret_cast<double2*>(scratch + L.partial), reinterpret_cast<double2*>(coef + L.red),
                                   reinterpret_cast<double2*>(coef + L.ab), reinterpret_cast<double2*>(coef + L.sc),
                                   status};
   DriverClock clock(0);

@@ -29,10 +29,9 @@ struct Basis {
   T* sink_v;       // sink . v_j per column (nullable)
   const T* sink;   // dotted unconjugated with every basis vector (nullable)
   T* partial;      // reduction partials: 2*(ldh+1)*chunks entries
-  T* hs;           // H s partials: ceil(ldh/32)*ldh entries
   T* red;          // finished reductions: 2*(ldh+1) entries
   T* ab;           // update coefficients: a at ab, b at ab + ldh
-  T* sc;           // scalars: [1/r, 1/g, c/(r g), hn(j-1), h0(j-1), h0(j)] (8 entries)
+  T* sc;           // step scalars, two alternating blocks of 8 (16 entries)
   double* status;  // status[0] = final H[j, j-1] when set (nullable)
   T* stage;        // when set: final column j-1 rows 0..j, then its h0 (j+2 entries; nullable)
 };
@@ -40,7 +39,6 @@ struct Basis {
 // Scratch sizes in units of T (entries) for a basis of `dim` rows and at most
 // `max_columns` columns (Hessenberg leading dimension max_columns + 1).
 int64_t partial_entries(bool complex_basis, int64_t dim, int32_t max_columns);
-int64_t hs_entries(int32_t max_columns);
 
 // Optional HIP-event brackets around the two bandwidth passes of a step
 // (pass 0: the reduction kernel, pass 1: the update kernel), with the

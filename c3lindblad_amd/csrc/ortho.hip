@@ -1,14 +1,13 @@
 // One-reduction Gram-Schmidt step with delayed reorthogonalisation (see ortho.h).
 //
-// Per step five launches on the caller's stream:
+// Per step four launches on the caller's stream:
 //   reduce_kernel  one pass over basis columns 0..j (column j = q) against q and w,
 //                  one wave per column per chunk of rows, q and w held in registers:
 //                  partials of s = V^H q (s_j = |q|^2), z = V^H w (z_j = q^H w),
 //                  |w|^2 and sink . q;
 //   finish_kernel  one workgroup per reduced number;
-//   hs_kernel      partials of (H s)_k over 32-column slices of the final H;
-//   column_kernel  one workgroup: r, final column j-1, sink . v_j, tentative
-//                  column j, update coefficients;
+//   column_kernel  r, final column j-1, sink . v_j, H s, tentative column j and
+//                  the update coefficients, one workgroup per 16 Hessenberg rows;
 //   update_kernel  one pass over columns 0..j-1: v_j = q/r - V a and
 //                  q_{j+1} = w/g - (c/(r g)) q - V b.
 // Algebra (columns 0..j-1 orthonormal, q the once-projected v_j, nu its tentative
@@ -31,8 +30,7 @@ extern "C" int c3l_internal_fail(int code, const char* message);
 namespace c3l_ortho {
 namespace {
 
-constexpr int kHsChunk = 32;       // Hessenberg columns per hs_kernel workgroup
-constexpr int kColumnThreads = 1024;
+constexpr int kGroupRows = 16;  // Hessenberg rows per column_kernel workgroup
 
 template <typename T>
 struct Num;
@@ -168,33 +166,6 @@ __global__ __launch_bounds__(256) void finish_kernel(const T* __restrict__ parti
   if (threadIdx.x == 0) out[o] = N::add(N::add(part[0], part[1]), N::add(part[2], part[3]));
 }
 
-// P[g*ldh + k] = sum over l in slice g of Hf[k, l] s_l for rows k < j, where Hf is
-// H with column j-1 completed (H[k, j-1] + nu s_k); Hessenberg: l >= k-1.  Every
-// load of a slice is issued up front (clamped addresses inside the allocated H,
-// whose entries below the sub-diagonal stay zero; masked terms are selected away).
-template <typename T>
-__global__ __launch_bounds__(64) void hs_kernel(const T* __restrict__ H, int64_t ldh, int j, const T* __restrict__ s,
-                                                T* __restrict__ P) {
-  using N = Num<T>;
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  if (k >= j) return;
-  const int l0 = blockIdx.y * kHsChunk;
-  const T nu = H[(int64_t)(j - 1) * ldh + j];
-  const T sk = s[k];
-  T h[kHsChunk];
-#pragma unroll
-  for (int t = 0; t < kHsChunk; ++t) h[t] = H[(int64_t)min(l0 + t, j - 1) * ldh + k];
-  T acc = N::zero();
-#pragma unroll
-  for (int t = 0; t < kHsChunk; ++t) {
-    const int l = l0 + t;
-    const T hl = (l == j - 1) ? N::fma(nu, sk, h[t]) : h[t];
-    const T term = N::mul(hl, s[min(l, j - 1)]);
-    acc = (l < j && l >= k - 1) ? N::add(acc, term) : acc;
-  }
-  P[(int64_t)blockIdx.y * ldh + k] = acc;
-}
-
 // Sums of six doubles over the workgroup with one barrier (every thread gets them).
 __device__ void block_sum6(double (&v)[6], double (*shared)[6]) {
   for (int q = 0; q < 6; ++q)
@@ -218,43 +189,36 @@ __device__ __forceinline__ void put(double (&v)[6], int at, double2 x) {
 __device__ __forceinline__ double take(const double (&v)[6], int at, double) { return v[at]; }
 __device__ __forceinline__ double2 take(const double (&v)[6], int at, double2) { return make_double2(v[at], v[at + 1]); }
 
-// sum_g P[g*ldh + k], eight independent loads at a time
+// Step scalars and Hessenberg columns: one workgroup of 256 threads per kGroupRows rows
+// (at least one workgroup).  red = [s_0..s_{j-1}, |q|^2, sink.q, z_0..z_{j-1}, q^H w, |w|^2].
+// Every workgroup recomputes the step scalars (four sums over j numbers).  The scalar
+// block alternates with the step parity, so no workgroup overwrites what another still
+// reads: sc_prev (step j-1) is only read, sc_cur (step j) is written by workgroup 0:
+//   [1/r, 1/g, c/(r g), final H[j, j-1], its h0, h0 of the tentative column j,
+//    tentative sub-diagonal nu of column j]
+// (h0 = |w|/r, the norm of A v before projection).  Row k < j: final
+// H[k, j-1] = H[k, j-1] + nu s_k and tentative H[k, j] = (z_k - (H s)_k)/r with
+// (H s)_k = sum_{l >= k-1} H~[k, l] s_l (H~: column j-1 completed), the row's 16 lanes
+// splitting l; threads of one l read consecutive rows.  `stage` (optional) gets the final
+// column j-1 followed by its h0, contiguous for one copy to the host.
 template <typename T>
-__device__ __forceinline__ T hs_row(const T* __restrict__ P, int64_t ldh, int G, int k) {
+__global__ __launch_bounds__(256) void column_kernel(T* __restrict__ H, int64_t ldh, int j, int fresh, int has_w,
+                                                     const T* __restrict__ red, T* __restrict__ sink_v,
+                                                     T* __restrict__ ab, const T* __restrict__ sc_prev,
+                                                     T* __restrict__ sc_cur, double* __restrict__ status,
+                                                     T* __restrict__ stage, double breakdown) {
   using N = Num<T>;
-  T acc = N::zero();
-  for (int g0 = 0; g0 < G; g0 += 8) {
-    T v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = P[(int64_t)min(g0 + u, G - 1) * ldh + k];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) acc = (g0 + u < G) ? N::add(acc, v[u]) : acc;
-  }
-  return acc;
-}
-
-// One workgroup.  red = [s_0..s_{j-1}, |q|^2, sink.q, z_0..z_{j-1}, q^H w, |w|^2].
-// sc = [1/r, 1/g, c/(r g), final H[j, j-1], its h0, h0 of the tentative column j]
-// where h0 is |w|/r, the norm of A v before projection.  `stage` (optional) gets the
-// final column j-1 followed by its h0, contiguous for one copy to the host.
-template <typename T>
-__global__ __launch_bounds__(kColumnThreads) void column_kernel(T* __restrict__ H, int64_t ldh, int j, int fresh,
-                                                                int has_w, const T* __restrict__ red,
-                                                                const T* __restrict__ P, int G,
-                                                                T* __restrict__ sink_v, T* __restrict__ ab,
-                                                                T* __restrict__ sc, double* __restrict__ status,
-                                                                T* __restrict__ stage, double breakdown) {
-  using N = Num<T>;
-  __shared__ double shared[kColumnThreads / 64][6];
+  __shared__ double shared[4][6];
+  __shared__ T part[256 / kGroupRows][kGroupRows];
   const int nv = j + 1;
   const T* S = red;
   const T* Z = red + nv + 1;
   const int tid = threadIdx.x;
-  const double nu = fresh ? 0.0 : N::re(H[(int64_t)(j - 1) * ldh + j]);
-  const double h0_prev = N::re(sc[5]);
+  const double nu = fresh ? 0.0 : N::re(sc_prev[6]);
+  const double h0_prev = N::re(sc_prev[5]);
   // [|s|^2, |z|^2, s^H z, s . sink_v] (complex sums in two slots; a real basis leaves slots 3, 5 at 0)
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int k = tid; k < j; k += kColumnThreads) {
+  for (int k = tid; k < j; k += 256) {
     const T s = fresh ? N::zero() : S[k];
     const T z = has_w ? Z[k] : N::zero();
     acc[0] += N::abs2(s);
@@ -262,7 +226,6 @@ __global__ __launch_bounds__(kColumnThreads) void column_kernel(T* __restrict__ 
     put(acc, 2, N::cfma(s, z, N::zero()));
     if (sink_v != nullptr) put(acc, 4, N::fma(s, sink_v[k], N::zero()));
   }
-  // one barrier: every thread has read nu and sc[5] before they change below
   block_sum6(acc, shared);
   const double ss = acc[0], zz = acc[1];
   const T sz = take(acc, 2, T{}), sv = take(acc, 4, T{});
@@ -278,10 +241,24 @@ __global__ __launch_bounds__(kColumnThreads) void column_kernel(T* __restrict__ 
     g = sqrt(fmax(omega - zz - N::abs2(c), 0.0));
     ig = g > 0.0 ? 1.0 / g : 0.0;
   }
-  const T cr = N::scale(c, ir);
   T* prev = H + (int64_t)(j - 1) * ldh;  // column j-1 (touched only when !fresh)
   T* col = H + (int64_t)j * ldh;         // column j (written only when has_w)
-  for (int k = tid; k < j; k += kColumnThreads) {
+  const int row = tid % kGroupRows, lr = tid / kGroupRows;
+  const int k = blockIdx.x * kGroupRows + row;
+  T hs = N::zero();
+  if (!fresh && has_w && k < j) {
+    const T sk = S[k];
+#pragma unroll 4
+    for (int l = lr; l < j; l += 256 / kGroupRows) {
+      T h = H[(int64_t)l * ldh + k];
+      if (l == j - 1) h = N::fma(N::make(nu), sk, h);
+      hs = l >= k - 1 ? N::fma(h, S[l], hs) : hs;
+    }
+  }
+  part[lr][row] = hs;
+  __syncthreads();  // every read of this workgroup's rows of column j-1 precedes the writes below
+  if (lr == 0 && k < j) {
+    for (int u = 1; u < 256 / kGroupRows; ++u) hs = N::add(hs, part[u][row]);
     const T s = fresh ? N::zero() : S[k];
     if (!fresh) {
       const T h = N::fma(N::make(nu), s, prev[k]);
@@ -291,29 +268,31 @@ __global__ __launch_bounds__(kColumnThreads) void column_kernel(T* __restrict__ 
     ab[k] = N::scale(s, ir);
     if (has_w) {
       const T z = Z[k];
-      const T hs = fresh ? N::zero() : hs_row(P, ldh, G, k);
       col[k] = N::scale(N::sub(z, hs), ir);
-      ab[ldh + k] = N::scale(N::sub(z, N::mul(cr, s)), ig);
+      ab[ldh + k] = N::scale(N::sub(z, N::mul(N::scale(c, ir), s)), ig);
     }
   }
-  if (tid == 0) {
-    if (!fresh) prev[j] = N::make(hn);
-    if (!fresh && stage != nullptr) {
-      stage[j] = N::make(hn);
-      stage[j + 1] = N::make(h0_prev);
+  if (blockIdx.x == 0 && tid == 0) {
+    if (!fresh) {
+      prev[j] = N::make(hn);
+      if (stage != nullptr) {
+        stage[j] = N::make(hn);
+        stage[j + 1] = N::make(h0_prev);
+      }
     }
     if (has_w) {
-      const T hs = fresh ? N::zero() : N::scale(S[j - 1], hn);
-      col[j] = N::scale(N::sub(c, hs), ir);
+      const T hsj = fresh ? N::zero() : N::scale(S[j - 1], hn);  // (H s)_j = H[j, j-1] s_{j-1}
+      col[j] = N::scale(N::sub(c, hsj), ir);
       col[j + 1] = N::make(g * ir);
     }
     if (sink_v != nullptr) sink_v[j] = N::scale(N::sub(S[nv], sv), ir);
-    sc[0] = N::make(ir);
-    sc[1] = N::make(ig);
-    sc[2] = N::scale(c, ir * ig);
-    sc[3] = N::make(hn);
-    sc[4] = N::make(h0_prev);
-    sc[5] = N::make(has_w ? sqrt(fmax(omega, 0.0)) * ir : 0.0);
+    sc_cur[0] = N::make(ir);
+    sc_cur[1] = N::make(ig);
+    sc_cur[2] = N::scale(c, ir * ig);
+    sc_cur[3] = N::make(hn);
+    sc_cur[4] = N::make(h0_prev);
+    sc_cur[5] = N::make(has_w ? sqrt(fmax(omega, 0.0)) * ir : 0.0);
+    sc_cur[6] = N::make(has_w ? g * ir : 0.0);
     if (status != nullptr && !fresh) status[0] = hn;
   }
 }
@@ -370,13 +349,10 @@ int64_t partial_entries(bool complex_basis, int64_t dim, int32_t max_columns) {
   return 2 * ((int64_t)max_columns + 2) * ((dim + rows - 1) / rows);
 }
 
-int64_t hs_entries(int32_t max_columns) {
-  return (((int64_t)max_columns + kHsChunk - 1) / kHsChunk) * ((int64_t)max_columns + 1);
-}
 
 template <typename T>
 int step(const Basis<T>& b, int j, bool fresh, const T* w, double breakdown, hipStream_t s, Clock* clock) {
-  if (b.V == nullptr || b.H == nullptr || b.partial == nullptr || b.hs == nullptr || b.red == nullptr ||
+  if (b.V == nullptr || b.H == nullptr || b.partial == nullptr || b.red == nullptr ||
       b.ab == nullptr || b.sc == nullptr || b.dim <= 0 || b.ld < b.dim || j < 0 ||
       (w != nullptr ? j + 2 > b.ldh : j + 1 > b.ldh) ||
       (j == 0 && !fresh))
@@ -397,17 +373,15 @@ int step(const Basis<T>& b, int j, bool fresh, const T* w, double breakdown, hip
   const int outs = w ? 2 * (nv + 1) : nv + 1;
   hipLaunchKernelGGL((finish_kernel<T>), dim3(outs), dim3(256), 0, s, b.partial, chunks, b.red);
   if (int rc = check_launch("finish_kernel")) return rc;
-  const int G = (j + kHsChunk - 1) / kHsChunk;
-  if (!fresh && w != nullptr) {
-    hipLaunchKernelGGL((hs_kernel<T>), dim3((j + 63) / 64, G), dim3(64), 0, s, b.H, b.ldh, j, b.red, b.hs);
-    if (int rc = check_launch("hs_kernel")) return rc;
-  }
-  hipLaunchKernelGGL((column_kernel<T>), dim3(1), dim3(kColumnThreads), 0, s, b.H, b.ldh, j, fresh ? 1 : 0,
-                     w ? 1 : 0, b.red, b.hs, G, b.sink_v, b.ab, b.sc, b.status, b.stage, breakdown);
+  T* sc_cur = b.sc + 8 * (j & 1);
+  const T* sc_prev = b.sc + 8 * ((j + 1) & 1);
+  const int groups_h = std::max(1, (j + kGroupRows - 1) / kGroupRows);
+  hipLaunchKernelGGL((column_kernel<T>), dim3(groups_h), dim3(256), 0, s, b.H, b.ldh, j, fresh ? 1 : 0, w ? 1 : 0,
+                     b.red, b.sink_v, b.ab, sc_prev, sc_cur, b.status, b.stage, breakdown);
   if (int rc = check_launch("column_kernel")) return rc;
   if (clock) clock->begin(1, s);
   hipLaunchKernelGGL((update_kernel<T, WAVES>), dim3((b.dim + 63) / 64), dim3(64 * WAVES), 0, s, b.V, b.ld, j,
-                     fresh ? 1 : 0, w, b.ab, b.ldh, b.sc, b.dim);
+                     fresh ? 1 : 0, w, b.ab, b.ldh, sc_cur, b.dim);
   if (int rc = check_launch("update_kernel")) return rc;
   if (clock) clock->end(1, vec * (j + 1 + (w ? 2 : 0) + (fresh ? 0 : 1)), s);
   return C3L_OK;
