@@ -114,6 +114,7 @@ __global__ __launch_bounds__(256) void reduce_kernel(const T* __restrict__ V, in
   if (kbase > nv) return;  // whole wave
   const int64_t r0 = blockIdx.x * (int64_t)ROWS + lane;
   const T* __restrict__ q = V + (int64_t)(nv - 1) * ld;
+  // rows past the end read the last row (always valid) against zero q and w entries
   T qv[kPer], wv[kPer];
 #pragma unroll
   for (int t = 0; t < kPer; ++t) {
@@ -121,34 +122,56 @@ __global__ __launch_bounds__(256) void reduce_kernel(const T* __restrict__ V, in
     qv[t] = i < dim ? q[i] : N::zero();
     wv[t] = (w != nullptr && i < dim) ? w[i] : N::zero();
   }
-  for (int c = 0; c < cpw; ++c) {
-    const int k = kbase + 4 * c;
-    if (k > nv) break;
+  // basis columns, two per iteration so that 2 kPer loads are in flight
+  for (int c = 0; c < cpw; c += 2) {
+    const int k0 = kbase + 4 * c, k1 = k0 + 4;
+    if (k0 >= nv) break;
+    const bool two = c + 1 < cpw && k1 < nv;
+    const T* __restrict__ col0 = V + (int64_t)k0 * ld;
+    const T* __restrict__ col1 = V + (int64_t)(two ? k1 : k0) * ld;
+    T v0[kPer], v1[kPer];
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+      const int64_t i = min(r0 + t * 64, dim - 1);
+      v0[t] = col0[i];
+      v1[t] = col1[i];
+    }
+    T as0 = N::zero(), az0 = N::zero(), as1 = N::zero(), az1 = N::zero();
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) {
+      as0 = N::cfma(v0[t], qv[t], as0);
+      az0 = N::cfma(v0[t], wv[t], az0);
+      as1 = N::cfma(v1[t], qv[t], as1);
+      az1 = N::cfma(v1[t], wv[t], az1);
+    }
+    as0 = N::wave_sum(as0);
+    az0 = N::wave_sum(az0);
+    as1 = N::wave_sum(as1);
+    az1 = N::wave_sum(az1);
+    if (lane == 0) {
+      partial[(int64_t)k0 * chunks + blockIdx.x] = as0;
+      if (w != nullptr) partial[(int64_t)(nv + 1 + k0) * chunks + blockIdx.x] = az0;
+      if (two) {
+        partial[(int64_t)k1 * chunks + blockIdx.x] = as1;
+        if (w != nullptr) partial[(int64_t)(nv + 1 + k1) * chunks + blockIdx.x] = az1;
+      }
+    }
+  }
+  // the extra column nv belongs to the wave whose column list contains it
+  const int off = nv - kbase;
+  if (off % 4 == 0 && off / 4 < cpw) {
     T as = N::zero(), az = N::zero();
-    if (k < nv) {
-      const T* __restrict__ col = V + (int64_t)k * ld;
 #pragma unroll
-      for (int t = 0; t < kPer; ++t) {
-        const int64_t i = r0 + t * 64;
-        if (i < dim) {
-          const T v = col[i];
-          as = N::cfma(v, qv[t], as);
-          az = N::cfma(v, wv[t], az);
-        }
-      }
-    } else {
-#pragma unroll
-      for (int t = 0; t < kPer; ++t) {
-        const int64_t i = r0 + t * 64;
-        if (sink != nullptr && i < dim) as = N::fma(sink[i], qv[t], as);
-        az = N::cfma(wv[t], wv[t], az);
-      }
+    for (int t = 0; t < kPer; ++t) {
+      const int64_t i = r0 + t * 64;
+      if (sink != nullptr && i < dim) as = N::fma(sink[i], qv[t], as);
+      az = N::cfma(wv[t], wv[t], az);
     }
     as = N::wave_sum(as);
     az = N::wave_sum(az);
     if (lane == 0) {
-      partial[(int64_t)k * chunks + blockIdx.x] = as;
-      if (w != nullptr) partial[(int64_t)(nv + 1 + k) * chunks + blockIdx.x] = az;
+      partial[(int64_t)nv * chunks + blockIdx.x] = as;
+      if (w != nullptr) partial[(int64_t)(2 * nv + 1) * chunks + blockIdx.x] = az;
     }
   }
 }
