@@ -145,6 +145,8 @@ def main() -> int:
                     help="skip the extra solve with the reference's default FP64 policy")
     ap.add_argument("--precision", default="fp32", choices=("fp64", "fp32"),
                     help="precision of the Jacobian-vector products inside GMRES (fp32 = mixed precision)")
+    ap.add_argument("--eom-streams", type=int, default=None,
+                    help="streams for the EOM resolvents (default: one per resolvent)")
     ap.add_argument("--serial-probe", action="store_true",
                     help="also time one solve with the EOM channels on one stream (no instrumentation)")
     ap.add_argument("--jvp-probe", type=int, default=0, metavar="K",
@@ -182,6 +184,7 @@ def main() -> int:
     else:
         policy = NewtonKrylovPolicy(forcing_strategy=args.forcing, gmres_restart=args.restart)
     pipe = SolvePipeline.prepare(model, grid, 0.0, charges=charges, policy=policy)
+    pipe.eom_streams = args.eom_streams
     print(f"[bench] setup {pipe.setup_seconds:.2f}s n={model.n_sites} D={pipe.static.oracle.num_amplitudes}", file=sys.stderr, flush=True)
     extra = []
     if args.batch > 1:

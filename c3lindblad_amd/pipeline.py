@@ -45,6 +45,7 @@ class SolvePipeline:
     policy: NewtonKrylovPolicy
     rtol: float = 1e-10
     concurrent: bool = True
+    eom_streams: int | None = None  # streams for the EOM resolvents (None: one per resolvent)
 
     @classmethod
     def prepare(cls, model, omega, eta=0.0, charges=(1, -1), policy=None, rtol=1e-10):
@@ -77,7 +78,8 @@ class SolvePipeline:
             ctx = component_eom_context(self.eom[q], sol.amplitudes, device_amplitudes=sol.device_amplitudes)
             pairs.append((ComponentSigmaOperator(ctx), component_eom_sources(ctx)))
         t_sources = perf_counter()
-        results = retarded_green_blocks(pairs, self.omega, self.eta, rtol=self.rtol, concurrent=self.concurrent)
+        results = retarded_green_blocks(pairs, self.omega, self.eta, rtol=self.rtol, concurrent=self.concurrent,
+                                        streams=self.eom_streams)
         for q, result in zip(charges, results):
             green[q] = result.retarded
             dims[q] = tuple(p.dimension if p is not None else 0 for p in (result.addition_projection, result.removal_projection))

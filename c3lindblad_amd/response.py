@@ -278,31 +278,36 @@ class ComponentRetardedGreen:
     peak_rss_kib: int
 
 
-def run_concurrently(tasks):
-    """Run independent device tasks on their own default-priority streams, one host thread each.
+def run_concurrently(tasks, streams: int | None = None):
+    """Run independent device tasks on default-priority streams, one host thread per stream.
 
-    The native drivers release the GIL, so the tasks overlap on the GPU.  Every
-    stream first waits for the caller's stream; the caller's stream waits for
+    With ``streams`` = k < len(tasks), stream i runs tasks i, i+k, ... one after the
+    other.  The native drivers release the GIL, so the streams overlap on the GPU.
+    Every stream first waits for the caller's stream; the caller's stream waits for
     all of them before returning.
     """
 
     torch = _lib.require_device()
-    if len(tasks) <= 1 or _lib.DEVICE != "cuda":
+    k = len(tasks) if streams is None else max(1, min(int(streams), len(tasks)))
+    if k <= 1 or _lib.DEVICE != "cuda":
         return [task() for task in tasks]
     from concurrent.futures import ThreadPoolExecutor
 
     main = torch.cuda.current_stream()
-    streams = [torch.cuda.Stream() for _ in tasks]
-    for stream in streams:
+    pool_streams = [torch.cuda.Stream() for _ in range(k)]
+    for stream in pool_streams:
         stream.wait_stream(main)
 
     def run(i):
-        with torch.cuda.stream(streams[i]):
-            return tasks[i]()
+        with torch.cuda.stream(pool_streams[i]):
+            return [(t, tasks[t]()) for t in range(i, len(tasks), k)]
 
-    with ThreadPoolExecutor(max_workers=len(tasks)) as pool:
-        results = list(pool.map(run, range(len(tasks))))
-    for stream in streams:
+    results = [None] * len(tasks)
+    with ThreadPoolExecutor(max_workers=k) as pool:
+        for group in pool.map(run, range(k)):
+            for t, value in group:
+                results[t] = value
+    for stream in pool_streams:
         main.wait_stream(stream)
     return results
 
@@ -320,11 +325,12 @@ def _channel(operator, source, sink, z, cap, rtol, limit, name):
 
 
 def retarded_green_blocks(pairs, omega, eta: float, *, rtol: float = 1.0e-10, maximum_dimension: int | None = None,
-                          concurrent: bool = True) -> list["ComponentRetardedGreen"]:
+                          concurrent: bool = True, streams: int | None = None) -> list["ComponentRetardedGreen"]:
     """``component_retarded_green`` for several ``(operator, sources)`` charge blocks at once.
 
     The addition and removal resolvents of every block run concurrently (plan
-    slots 0 and 1 of each block, one stream each) when ``concurrent``.
+    slots 0 and 1 of each block) when ``concurrent``, on ``streams`` streams (one
+    per resolvent by default).
     """
 
     started = perf_counter()
@@ -338,7 +344,7 @@ def retarded_green_blocks(pairs, omega, eta: float, *, rtol: float = 1.0e-10, ma
             op = operator.on_slot(slot) if concurrent else operator
             tasks.append(lambda op=op, src=src, snk=sources.sink, cap=cap, name=name:
                          _channel(op, src, snk, z, cap, rtol, limit, name))
-    results = run_concurrently(tasks) if concurrent else [t() for t in tasks]
+    results = run_concurrently(tasks, streams) if concurrent else [t() for t in tasks]
     out = []
     for b, (operator, _) in enumerate(pairs):
         (ga, pa, ra), (gr, pr, rr) = results[2 * b], results[2 * b + 1]
