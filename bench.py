@@ -40,6 +40,25 @@ def _lib_handle():
     return _lib.load_library()
 
 
+def _measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed counter summaries in ``profiles/``
+    (lines ``traffic_bytes_per_launch[<kernel>] = <bytes>``, FETCH_SIZE + WRITE_SIZE scaled by a
+    copy of known size), or ``(None, None)``."""
+
+    import glob
+    import re
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    pattern = re.compile(r"^traffic_bytes_per_launch\[(.+)\] = ([0-9.eE+]+)\s*$")
+    for path in sorted(glob.glob(os.path.join(here, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                m = pattern.match(line.strip())
+                if m and m.group(1) == kernel:
+                    return float(m.group(2)), os.path.relpath(path, here)
+    return None, None
+
+
 def _algorithmic_program_bytes(lowered, program: int) -> float:
     """Compulsory bytes of one program launch: per coordinate its output, destination,
     packed coordinate and (residual/JVP/sigma) diagonal and input value, plus every
@@ -149,6 +168,8 @@ def main() -> int:
                     help="streams for the EOM resolvents (default: one per resolvent)")
     ap.add_argument("--serial-probe", action="store_true",
                     help="also time one solve with the EOM channels on one stream (no instrumentation)")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="end with one 256 MiB device copy (known bytes for hardware-counter calibration)")
     ap.add_argument("--jvp-probe", type=int, default=0, metavar="K",
                     help="profiling workload only: one residual and K JVPs of siam24, then a 256 MiB copy")
     args = ap.parse_args()
@@ -265,6 +286,7 @@ def main() -> int:
             kernels[name] = {"ms": ms.value, "launches": launches.value, "bytes": nbytes.value}
     lib.c3l_krylov_set_timing(0)
     dominant = max(kernels, key=lambda k: kernels[k]["ms"])
+    traffic, traffic_source = _measured_traffic(dominant)
     dk = kernels[dominant]
     achieved = dk["bytes"] / (dk["ms"] * 1e-3) / 1e9
     last = results[-1]
@@ -317,7 +339,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_source,
             "avg_launch_us": 1e3 * dk["ms"] / dk["launches"],
             "launches": dk["launches"],
             "share_of_step": dk["ms"] * 1e-3 / instrumented,
@@ -350,6 +373,10 @@ def main() -> int:
         }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = _cpu_baseline(pipe, last)
+    if args.calibration_copy:
+        a = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
     print(json.dumps(line), flush=True)
     return 0
 
