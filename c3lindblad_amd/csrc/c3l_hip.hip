@@ -153,7 +153,7 @@ __global__ void contract_kernel(C* __restrict__ ws, const C* __restrict__ vector
   int64_t out_size = 1;
   for (int k = 0; k < r; ++k) out_size *= n;
   int64_t inner = 1;
-  for (int k = 0; k < job.par_rank - 1 - job.slot; ++k) inner *= n;
+  for (int k = 0; k < job.slot; ++k) inner *= n;  // stride of the contracted slot
   const C* __restrict__ par = ws + job.par_offset;
   const C* __restrict__ vec = vectors + (int64_t)job.vec * n;
   C* __restrict__ out = ws + job.out_offset;
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(1024) void contract_levels_kernel(C* __restrict__ w
       int64_t out_size = 1;
       for (int k = 0; k < r; ++k) out_size *= n;
       int64_t inner = 1;
-      for (int k = 0; k < job.par_rank - 1 - job.slot; ++k) inner *= n;
+      for (int k = 0; k < job.slot; ++k) inner *= n;  // stride of the contracted slot
       const C* __restrict__ par = ws + job.par_offset;
       const C* __restrict__ vec = vectors + (int64_t)job.vec * n;
       C* __restrict__ out = ws + job.out_offset;
@@ -975,7 +975,7 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
     for (int q = 0; q < rank; ++q) {
       const int pos = (fcode[f] >> (3 + 2 * q)) & 3;
       if (pos >= deg) return fail(C3L_EINVAL, "c3l_plan_add_block: factor position out of range");
-      wf.stride[pos] += (int32_t)ipow(plan->n, rank - 1 - q);
+      wf.stride[pos] += (int32_t)ipow(plan->n, q);  // atoms store their first index fastest
     }
     b.factors[(size_t)f] = wf;
   }

@@ -13,6 +13,13 @@ workspace and converts the symbolic plan of ``algebra.emit_plan`` into:
   the wedge entries (``fcode = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9``);
   creation programs write every padded index tuple of a ``C`` atom into the
   workspace (destinations are workspace offsets).
+
+Storage order: a padded rank-r tensor stores index tuple ``(i_0, .., i_{r-1})`` at
+``sum_s i_s n^s`` (first index fastest), and the coordinates of an output block
+are listed in increasing destination.  The canonical string order (PH rank,
+then integer mask) runs over the first index fastest, so consecutive
+coordinates of a block write consecutive outputs and read consecutive elements
+of the rank-r atoms indexed by all of their positions.
 """
 
 from __future__ import annotations
@@ -80,7 +87,7 @@ def padded_scatter(coords: CoordinateSet, key: BlockKey) -> tuple[np.ndarray, np
     groups = sorted(set(slots))
     group_slots = [[s for s, g in enumerate(slots) if g == grp] for grp in groups]
     idx = block.indices
-    strides = n ** np.arange(deg - 1, -1, -1, dtype=np.int64)
+    strides = n ** np.arange(deg, dtype=np.int64)  # first index fastest
     per_group_perms = []
     for gs in group_slots:
         perms = []
@@ -173,8 +180,10 @@ def lower_plan(
             deg = sum(bp.key)
             if bp.target_atom >= 0:
                 # creation tensor: every index tuple of its padded storage, written in place
-                coord = np.indices((n,) * deg).reshape(deg, -1).T.astype(np.int32) if deg else np.zeros((1, 0), np.int32)
-                dest = plan.atom_offsets[bp.target_atom] + np.arange(n**deg, dtype=np.int64)
+                flat = np.arange(n**deg, dtype=np.int64)
+                coord = (np.stack([(flat // n**d) % n for d in range(deg)], axis=1).astype(np.int32) if deg
+                         else np.zeros((1, 0), np.int32))
+                dest = plan.atom_offsets[bp.target_atom] + flat
             elif deg == 0:
                 coord = np.zeros((1, 0), dtype=np.int32)
                 dest = np.full(1, max(bp.scalar_index, 0), dtype=np.int64)
@@ -182,8 +191,9 @@ def lower_plan(
                 block = coords.block(bp.key)
                 if block is None or block.positions.size == 0:
                     continue
-                coord = block.indices.astype(np.int32)
-                dest = block.positions.astype(np.int64)
+                order = np.argsort(block.positions, kind="stable")  # walk the output vector in order
+                coord = block.indices[order].astype(np.int32)
+                dest = block.positions[order].astype(np.int64)
             fcode = encode_factors(np.array([plan.atoms[a].rank for a in bp.factor_atom], dtype=np.int32),
                                    bp.factor_positions)
             foffset = plan.atom_offsets[bp.factor_atom] if bp.factor_atom.size else np.zeros(0, dtype=np.int64)

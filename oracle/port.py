@@ -37,9 +37,9 @@ class PortPlan:
             if k != kind:
                 continue
             for o, p, r, s, vi, sg in zip(out_off, par_off, par_rank, slot, vec, sign):
-                parent = self.ws[p : p + n**r].reshape((n,) * r)
+                parent = self.ws[p : p + n**r].reshape((n,) * r, order="F")  # first index fastest
                 out = np.tensordot(parent, self.lp.vectors[vi], axes=([s], [0]))
-                self.ws[o : o + n ** (r - 1)] = sg * np.ravel(out)
+                self.ws[o : o + n ** (r - 1)] = sg * np.ravel(out, order="F")
         if kind == 0:  # amplitudes changed: rebuild the creation tensors that depend on them
             self.apply(4, None)
 
@@ -58,7 +58,7 @@ class PortPlan:
                     code = int(fcode[f])
                     rank = code & 7
                     flat = np.zeros(dest.size, dtype=np.int64)
-                    for s in range(rank):
+                    for s in range(rank - 1, -1, -1):  # slot s has stride n**s
                         flat = flat * n + coord[:, (code >> (3 + 2 * s)) & 3]
                     prod *= self.ws[foff[f] + flat]
                 acc += prod
