@@ -52,8 +52,8 @@ SIGNATURES = {
     "c3l_reduction_chunks": ([_I64], _I64),
     "c3l_shifted_triangular_green": ([_P, _I32, _P, _P, _P, _P, _I32, _P, _P, _P], ctypes.c_int),
     "c3l_gmres": (
-        [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _I32, _I32, _P, _P, _P, _P,
-         _P],
+        [_P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _I32, _I32, _P, _P, _P,
+         _P, _P],
         ctypes.c_int,
     ),
     "c3l_bf16_gemm": ([_P, _I64, _I64, _P, _I64, _I64, _P, _I64, _I32, _I32, _I32, ctypes.c_float, ctypes.c_float, _P],

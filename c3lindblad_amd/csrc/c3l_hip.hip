@@ -361,15 +361,22 @@ __global__ void fill_const_kernel(double2* __restrict__ ws, const double2* __res
 
 // ------------------------------------------------------------------ tilde maps
 
-__global__ void tilde_unpack_kernel(const double* __restrict__ x, double2* __restrict__ t,
+template <typename X>
+__global__ void tilde_unpack_kernel(const X* __restrict__ x, double2* __restrict__ t,
                                     const int64_t* __restrict__ k0, const int64_t* __restrict__ k1,
                                     const double4* __restrict__ ab, int64_t dim) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x) {
     const double4 c = ab[i];
-    const double a = x[k0[i]];
-    const double b = x[k1[i]];
+    const double a = (double)x[k0[i]];
+    const double b = (double)x[k1[i]];
     t[i] = make_double2(c.x * a + c.z * b, c.y * a + c.w * b);
   }
+}
+
+// y = (float)(a x), the FP32 copy of a scaled FP64 vector.
+__global__ void narrow_scale_kernel(const double* __restrict__ x, float* __restrict__ y, double a, int64_t dim) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)(a * x[i]);
 }
 
 __global__ void tilde_pack_kernel(const double2* __restrict__ t, double* __restrict__ x,
@@ -515,7 +522,8 @@ __global__ void rfinish_kernel(const double* __restrict__ partial, int64_t chunk
 }
 
 // w[i] += scale * sum_k alpha[k] V[k*ld + i]; 64 rows per workgroup, columns split over the 4 waves.
-__global__ __launch_bounds__(256) void rmultiaxpy_kernel(const double* __restrict__ V, int64_t ld, int ncols,
+template <typename S>
+__global__ __launch_bounds__(256) void rmultiaxpy_kernel(const S* __restrict__ V, int64_t ld, int ncols,
                                                          const double* __restrict__ alpha, double scale,
                                                          double* __restrict__ w, int64_t dim) {
   __shared__ double part[4][64];
@@ -525,7 +533,7 @@ __global__ __launch_bounds__(256) void rmultiaxpy_kernel(const double* __restric
   double acc = 0.0;
   if (i < dim) {
 #pragma unroll 4
-    for (int k = wave; k < ncols; k += 4) acc += alpha[k] * V[(int64_t)k * ld + i];
+    for (int k = wave; k < ncols; k += 4) acc += alpha[k] * (double)V[(int64_t)k * ld + i];
   }
   part[wave][lane] = acc;
   __syncthreads();
@@ -1343,7 +1351,7 @@ int c3l_tilde_unpack(const double* x, double* t, const int64_t* k0, const int64_
   if (dim < 0 || (dim > 0 && (x == nullptr || t == nullptr || k0 == nullptr || k1 == nullptr || ab == nullptr)))
     return fail(C3L_EINVAL, "c3l_tilde_unpack: invalid argument");
   if (dim == 0) return C3L_OK;
-  hipLaunchKernelGGL(tilde_unpack_kernel, dim3(grid_for(dim)), dim3(kThreads), 0, static_cast<hipStream_t>(stream), x,
+  hipLaunchKernelGGL(tilde_unpack_kernel<double>, dim3(grid_for(dim)), dim3(kThreads), 0, static_cast<hipStream_t>(stream), x,
                      reinterpret_cast<double2*>(t), k0, k1, reinterpret_cast<const double4*>(ab), dim);
   C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;
@@ -1368,6 +1376,36 @@ int c3l_tilde_pack_precondition(const double* t, double* x, const int64_t* src, 
   hipLaunchKernelGGL(tilde_pack_precondition_kernel, dim3(grid_for(real_dim)), dim3(kThreads), 0,
                      static_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(t), x, src,
                      reinterpret_cast<const double2*>(w), reinterpret_cast<const double2*>(inv_diag), real_dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+// FP32 Krylov-basis helpers of the GMRES driver (library-internal).
+int c3l_internal_tilde_unpack_f32(const float* x, double* t, const int64_t* k0, const int64_t* k1, const double* ab,
+                                  int64_t dim, void* stream) {
+  if (dim <= 0 || x == nullptr || t == nullptr || k0 == nullptr || k1 == nullptr || ab == nullptr)
+    return fail(C3L_EINVAL, "c3l_internal_tilde_unpack_f32: invalid argument");
+  hipLaunchKernelGGL(tilde_unpack_kernel<float>, dim3(grid_for(dim)), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), x, reinterpret_cast<double2*>(t), k0, k1,
+                     reinterpret_cast<const double4*>(ab), dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_internal_narrow_scale(const double* x, float* y, double a, int64_t dim, void* stream) {
+  if (dim <= 0 || x == nullptr || y == nullptr) return fail(C3L_EINVAL, "c3l_internal_narrow_scale: invalid argument");
+  hipLaunchKernelGGL(narrow_scale_kernel, dim3(grid_for(dim)), dim3(kThreads), 0, static_cast<hipStream_t>(stream), x, y,
+                     a, dim);
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+int c3l_internal_rmultiaxpy_f32(const float* V, int64_t ld, int32_t ncols, const double* alpha, double scale,
+                                double* w, int64_t dim, void* stream) {
+  if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || alpha == nullptr || w == nullptr)
+    return fail(C3L_EINVAL, "c3l_internal_rmultiaxpy_f32: invalid argument");
+  hipLaunchKernelGGL(rmultiaxpy_kernel<float>, dim3((dim + 63) / 64), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), V, ld, ncols, alpha, scale, w, dim);
   C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;
 }
@@ -1415,7 +1453,7 @@ int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alp
                    int64_t dim, void* stream) {
   if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || alpha == nullptr || w == nullptr)
     return fail(C3L_EINVAL, "c3l_rmultiaxpy: invalid argument");
-  hipLaunchKernelGGL(rmultiaxpy_kernel, dim3((dim + 63) / 64), dim3(kThreads), 0, static_cast<hipStream_t>(stream), V, ld,
+  hipLaunchKernelGGL(rmultiaxpy_kernel<double>, dim3((dim + 63) / 64), dim3(kThreads), 0, static_cast<hipStream_t>(stream), V, ld,
                      ncols, alpha, scale, w, dim);
   C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;

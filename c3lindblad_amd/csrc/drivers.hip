@@ -31,8 +31,13 @@
 #include "../../include/c3l_hip.h"
 #include "ortho.h"
 
-// Shared error slot of the library (defined in c3l_hip.hip).
+// Shared error slot and FP32-basis helpers of the library (defined in c3l_hip.hip).
 extern "C" int c3l_internal_fail(int code, const char* message);
+extern "C" int c3l_internal_tilde_unpack_f32(const float* x, double* t, const int64_t* k0, const int64_t* k1,
+                                             const double* ab, int64_t dim, void* stream);
+extern "C" int c3l_internal_narrow_scale(const double* x, float* y, double a, int64_t dim, void* stream);
+extern "C" int c3l_internal_rmultiaxpy_f32(const float* V, int64_t ld, int32_t ncols, const double* alpha,
+                                           double scale, double* w, int64_t dim, void* stream);
 
 namespace {
 
@@ -434,9 +439,22 @@ struct Gmres {
     if (jvp_calls) ++*jvp_calls;
     return C3L_OK;
   }
+  int unpack(const double* v) { return c3l_tilde_unpack(v, t_work, tm->k0, tm->k1, tm->ab, tm->dim, stream); }
+  int unpack(const float* v) { return c3l_internal_tilde_unpack_f32(v, t_work, tm->k0, tm->k1, tm->ab, tm->dim, stream); }
+  // basis column = a x
+  int store_scaled(const double* x, double* col, double a) { return c3l_rscale(x, col, a, rd, stream); }
+  int store_scaled(const double* x, float* col, double a) { return c3l_internal_narrow_scale(x, col, a, rd, stream); }
+  // x += V y
+  int add_combination(const double* V, int used, const double* y, double* x) {
+    return c3l_rmultiaxpy(V, rd, used, y, 1.0, x, rd, stream);
+  }
+  int add_combination(const float* V, int used, const double* y, double* x) {
+    return c3l_internal_rmultiaxpy_f32(V, rd, used, y, 1.0, x, rd, stream);
+  }
   // out = M^{-1} J v with the preconditioner fused into the pack of J v
-  int apply_preconditioned(const double* v, double* out) {
-    D_CALL(c3l_tilde_unpack(v, t_work, tm->k0, tm->k1, tm->ab, tm->dim, stream));
+  template <typename S>
+  int apply_preconditioned(const S* v, double* out) {
+    D_CALL(unpack(v));
     D_CALL(c3l_plan_load(plan, 1, t_work, stream));
     D_CALL(c3l_plan_apply(plan, 2, jvp_diag, t_work, out_work, stream));
     D_CALL(c3l_tilde_pack_precondition(out_work, out, tm->src, tm->w, tm->inv_diag, rd, stream));
@@ -472,12 +490,9 @@ struct HostColumns {
   }
 };
 
-}  // namespace
-
-extern "C" {
-
-int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
-              const double* b, double* x, double* V, double* w, double* tmp, double* scratch, double* coef,
+template <typename Store>
+int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
+              const double* b, double* x, Store* V, double* w, double* tmp, double* scratch, double* coef,
               double rtol, double atol_in, int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations,
               double* true_rel, int32_t* jvp_calls, void* stream) {
   if (plan == nullptr || tm == nullptr || b == nullptr || x == nullptr || V == nullptr || w == nullptr ||
@@ -517,7 +532,7 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   }
   const int R = restart;
   const Layout L = gmres_layout(rd, R);
-  c3l_ortho::Basis<double> basis{V, rd, rd, coef + L.H, R + 1, nullptr, nullptr, scratch + L.partial,
+  c3l_ortho::Basis<double, Store> basis{V, rd, rd, coef + L.H, R + 1, nullptr, nullptr, scratch + L.partial,
                                  coef + L.red, coef + L.ab, coef + L.sc, nullptr, coef + L.stage};
   DriverClock clock(2);
   HostColumns pinned;
@@ -529,11 +544,11 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   D_CHECK(hipMemcpyAsync(w, b, sizeof(double) * rd, hipMemcpyDeviceToDevice, s));  // r = b - A 0
   double rn = bnrm;
   for (int32_t cycle = 0; cycle < cycles; ++cycle) {
-    D_CALL(g.precondition(w, V));
+    D_CALL(g.precondition(w, tmp));
     double beta = 0.0;
-    D_CALL(g.norm(V, &beta));
+    D_CALL(g.norm(tmp, &beta));
     if (!(beta > 0.0) || !std::isfinite(beta)) return dfail(C3L_EINVAL, "c3l_gmres: singular preconditioned residual");
-    D_CALL(c3l_rscale(V, V, 1.0 / beta, rd, stream));
+    D_CALL(g.store_scaled(tmp, V, 1.0 / beta));
     std::fill(H.begin(), H.end(), 0.0);
     std::fill(S.begin(), S.end(), 0.0);
     S[0] = beta;
@@ -602,7 +617,7 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
       y[k] = h(k, k) == 0.0 ? 0.0 : acc / h(k, k);
     }
     D_CHECK(hipMemcpyAsync(coef + L.y, y.data(), sizeof(double) * used, hipMemcpyHostToDevice, s));
-    D_CALL(c3l_rmultiaxpy(V, rd, used, coef + L.y, 1.0, x, rd, stream));
+    D_CALL(g.add_combination(V, used, coef + L.y, x));
     D_CALL(g.apply(x, w));
     D_CALL(c3l_raxpby(1.0, b, -1.0, w, rd, stream));
     D_CALL(g.norm(w, &rn));  // synchronises: y may be reused by the next cycle
@@ -613,6 +628,23 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   *true_rel = rn / bnrm;
   *info = rn <= atol ? 0 : cycles;
   return C3L_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
+              const double* b, double* x, void* V, int32_t basis_bits, double* w, double* tmp, double* scratch,
+              double* coef, double rtol, double atol, int32_t restart, int32_t cycles, int32_t* info,
+              int32_t* iterations, double* true_rel, int32_t* jvp_calls, void* stream) {
+  if (basis_bits == 64)
+    return gmres_run(plan, tm, jvp_diag, t_work, out_work, b, x, static_cast<double*>(V), w, tmp, scratch, coef, rtol,
+                     atol, restart, cycles, info, iterations, true_rel, jvp_calls, stream);
+  if (basis_bits == 32)
+    return gmres_run(plan, tm, jvp_diag, t_work, out_work, b, x, static_cast<float*>(V), w, tmp, scratch, coef, rtol,
+                     atol, restart, cycles, info, iterations, true_rel, jvp_calls, stream);
+  return dfail(C3L_EINVAL, "c3l_gmres: basis_bits must be 32 or 64");
 }
 
 int c3l_krylov_workspace(int32_t complex_basis, int64_t dim, int32_t max_columns, int32_t n_shifts,

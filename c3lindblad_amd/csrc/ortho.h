@@ -18,10 +18,12 @@
 
 namespace c3l_ortho {
 
-// Device buffers of one Krylov basis, all owned by the caller.
-template <typename T>
+// Device buffers of one Krylov basis, all owned by the caller.  T is the arithmetic
+// type (double or double2), S the storage type of the basis columns (T, or float for
+// a real basis kept in FP32).
+template <typename T, typename S = T>
 struct Basis {
-  T* V;            // columns 0..m (column j at V + j*ld)
+  S* V;            // columns 0..m (column j at V + j*ld)
   int64_t ld;      // >= dim
   int64_t dim;     // rows
   T* H;            // Hessenberg columns: column j at H + j*ldh, rows 0..j+1
@@ -54,7 +56,7 @@ struct Clock {
 // w == nullptr only finalise column j-1 and v_j.  `fresh`: q_j is already final
 // (first step), so no reorthogonalisation and no column j-1.  `breakdown`: a
 // final H[j, j-1] at or below it zeroes v_j.
-template <typename T>
-int step(const Basis<T>& b, int j, bool fresh, const T* w, double breakdown, hipStream_t stream, Clock* clock);
+template <typename T, typename S>
+int step(const Basis<T, S>& b, int j, bool fresh, const T* w, double breakdown, hipStream_t stream, Clock* clock);
 
 }  // namespace c3l_ortho

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <type_traits>
 
 #include "../../include/c3l_hip.h"
 
@@ -80,6 +81,16 @@ struct Num<double2> {
   }
 };
 
+// Basis storage: FP64 (real or complex) or FP32 (real) columns; arithmetic in T.
+__device__ __forceinline__ double widen(float v) { return (double)v; }
+__device__ __forceinline__ double widen(double v) { return v; }
+__device__ __forceinline__ double2 widen(double2 v) { return v; }
+template <typename S, typename T>
+__device__ __forceinline__ S narrow_to(T v) {
+  if constexpr (std::is_same<S, T>::value) return v;
+  else return static_cast<S>(v);
+}
+
 // Rows per reduction chunk (8 per lane: q and w stay in registers at full
 // occupancy) and waves per update workgroup.  Complex bases are short (tens of
 // thousands of rows): 8-wave update groups keep the GPU full.  Real bases (the
@@ -102,8 +113,8 @@ struct Shape<double2> {
 // is sink . q (unconjugated, 0 without a sink) and its w-dot |w|^2.
 // Partials: s-side output k at partial[k*chunks + chunk], w-side at
 // partial[(nv+1+k)*chunks + chunk].
-template <typename T, int ROWS>
-__global__ __launch_bounds__(256) void reduce_kernel(const T* __restrict__ V, int64_t ld, int nv, int cpw,
+template <typename T, typename S, int ROWS>
+__global__ __launch_bounds__(256) void reduce_kernel(const S* __restrict__ V, int64_t ld, int nv, int cpw,
                                                      const T* __restrict__ w, const T* __restrict__ sink,
                                                      int64_t dim, T* __restrict__ partial, int64_t chunks) {
   using N = Num<T>;
@@ -113,13 +124,13 @@ __global__ __launch_bounds__(256) void reduce_kernel(const T* __restrict__ V, in
   const int kbase = blockIdx.y * 4 * cpw + wave;
   if (kbase > nv) return;  // whole wave
   const int64_t r0 = blockIdx.x * (int64_t)ROWS + lane;
-  const T* __restrict__ q = V + (int64_t)(nv - 1) * ld;
+  const S* __restrict__ q = V + (int64_t)(nv - 1) * ld;
   // rows past the end read the last row (always valid) against zero q and w entries
   T qv[kPer], wv[kPer];
 #pragma unroll
   for (int t = 0; t < kPer; ++t) {
     const int64_t i = r0 + t * 64;
-    qv[t] = i < dim ? q[i] : N::zero();
+    qv[t] = i < dim ? T(widen(q[i])) : N::zero();
     wv[t] = (w != nullptr && i < dim) ? w[i] : N::zero();
   }
   // basis columns, two per iteration so that 2 kPer loads are in flight
@@ -127,14 +138,14 @@ __global__ __launch_bounds__(256) void reduce_kernel(const T* __restrict__ V, in
     const int k0 = kbase + 4 * c, k1 = k0 + 4;
     if (k0 >= nv) break;
     const bool two = c + 1 < cpw && k1 < nv;
-    const T* __restrict__ col0 = V + (int64_t)k0 * ld;
-    const T* __restrict__ col1 = V + (int64_t)(two ? k1 : k0) * ld;
+    const S* __restrict__ col0 = V + (int64_t)k0 * ld;
+    const S* __restrict__ col1 = V + (int64_t)(two ? k1 : k0) * ld;
     T v0[kPer], v1[kPer];
 #pragma unroll
     for (int t = 0; t < kPer; ++t) {
       const int64_t i = min(r0 + t * 64, dim - 1);
-      v0[t] = col0[i];
-      v1[t] = col1[i];
+      v0[t] = widen(col0[i]);
+      v1[t] = widen(col1[i]);
     }
     T as0 = N::zero(), az0 = N::zero(), as1 = N::zero(), az1 = N::zero();
 #pragma unroll
@@ -321,8 +332,8 @@ __global__ __launch_bounds__(256) void column_kernel(T* __restrict__ H, int64_t 
 }
 
 // 64 rows per workgroup, WAVES waves splitting the columns, partial sums met in LDS.
-template <typename T, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void update_kernel(T* __restrict__ V, int64_t ld, int j, int fresh,
+template <typename T, typename S, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void update_kernel(S* __restrict__ V, int64_t ld, int j, int fresh,
                                                             const T* __restrict__ w, const T* __restrict__ ab,
                                                             int64_t ldh, const T* __restrict__ sc, int64_t dim) {
   using N = Num<T>;
@@ -337,7 +348,7 @@ __global__ __launch_bounds__(64 * WAVES) void update_kernel(T* __restrict__ V, i
     const T* b = ab + ldh;
 #pragma unroll 4
     for (int k = wave; k < j; k += WAVES) {
-      const T v = V[(int64_t)k * ld + i];
+      const T v = widen(V[(int64_t)k * ld + i]);
       A = N::fma(a[k], v, A);
       B = N::fma(b[k], v, B);
     }
@@ -350,10 +361,11 @@ __global__ __launch_bounds__(64 * WAVES) void update_kernel(T* __restrict__ V, i
       A = N::add(A, pa[u][lane]);
       B = N::add(B, pb[u][lane]);
     }
-    T* q = V + (int64_t)j * ld;
-    const T qi = q[i];
-    if (!fresh) q[i] = N::sub(N::scale(qi, N::re(sc[0])), A);
-    if (w != nullptr) V[(int64_t)(j + 1) * ld + i] = N::sub(N::sub(N::scale(w[i], N::re(sc[1])), N::mul(sc[2], qi)), B);
+    S* q = V + (int64_t)j * ld;
+    const T qi = widen(q[i]);
+    if (!fresh) q[i] = narrow_to<S>(N::sub(N::scale(qi, N::re(sc[0])), A));
+    if (w != nullptr)
+      V[(int64_t)(j + 1) * ld + i] = narrow_to<S>(N::sub(N::sub(N::scale(w[i], N::re(sc[1])), N::mul(sc[2], qi)), B));
   }
 }
 
@@ -373,8 +385,8 @@ int64_t partial_entries(bool complex_basis, int64_t dim, int32_t max_columns) {
 }
 
 
-template <typename T>
-int step(const Basis<T>& b, int j, bool fresh, const T* w, double breakdown, hipStream_t s, Clock* clock) {
+template <typename T, typename S>
+int step(const Basis<T, S>& b, int j, bool fresh, const T* w, double breakdown, hipStream_t s, Clock* clock) {
   if (b.V == nullptr || b.H == nullptr || b.partial == nullptr || b.red == nullptr ||
       b.ab == nullptr || b.sc == nullptr || b.dim <= 0 || b.ld < b.dim || j < 0 ||
       (w != nullptr ? j + 2 > b.ldh : j + 1 > b.ldh) ||
@@ -387,12 +399,13 @@ int step(const Basis<T>& b, int j, bool fresh, const T* w, double breakdown, hip
   const int cols = nv + 1;
   const int cpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, (int64_t)cols * chunks / 4096));
   const int groups = (cols + 4 * cpw - 1) / (4 * cpw);
-  const double vec = (double)sizeof(T) * (double)b.dim;
+  const double vec = (double)sizeof(T) * (double)b.dim;   // w, sink
+  const double col = (double)sizeof(S) * (double)b.dim;   // basis columns
   if (clock) clock->begin(0, s);
-  hipLaunchKernelGGL((reduce_kernel<T, ROWS>), dim3(chunks, groups), dim3(256), 0, s, b.V, b.ld, nv, cpw, w, b.sink,
+  hipLaunchKernelGGL((reduce_kernel<T, S, ROWS>), dim3(chunks, groups), dim3(256), 0, s, b.V, b.ld, nv, cpw, w, b.sink,
                      b.dim, b.partial, chunks);
   if (int rc = check_launch("reduce_kernel")) return rc;
-  if (clock) clock->end(0, vec * (nv + (w ? 1 : 0) + (b.sink ? 1 : 0)), s);
+  if (clock) clock->end(0, col * nv + vec * ((w ? 1 : 0) + (b.sink ? 1 : 0)), s);
   const int outs = w ? 2 * (nv + 1) : nv + 1;
   hipLaunchKernelGGL((finish_kernel<T>), dim3(outs), dim3(256), 0, s, b.partial, chunks, b.red);
   if (int rc = check_launch("finish_kernel")) return rc;
@@ -403,14 +416,16 @@ int step(const Basis<T>& b, int j, bool fresh, const T* w, double breakdown, hip
                      b.red, b.sink_v, b.ab, sc_prev, sc_cur, b.status, b.stage, breakdown);
   if (int rc = check_launch("column_kernel")) return rc;
   if (clock) clock->begin(1, s);
-  hipLaunchKernelGGL((update_kernel<T, WAVES>), dim3((b.dim + 63) / 64), dim3(64 * WAVES), 0, s, b.V, b.ld, j,
+  hipLaunchKernelGGL((update_kernel<T, S, WAVES>), dim3((b.dim + 63) / 64), dim3(64 * WAVES), 0, s, b.V, b.ld, j,
                      fresh ? 1 : 0, w, b.ab, b.ldh, sc_cur, b.dim);
   if (int rc = check_launch("update_kernel")) return rc;
-  if (clock) clock->end(1, vec * (j + 1 + (w ? 2 : 0) + (fresh ? 0 : 1)), s);
+  if (clock) clock->end(1, col * (j + 1 + (w ? 1 : 0) + (fresh ? 0 : 1)) + vec * (w ? 1 : 0), s);
   return C3L_OK;
 }
 
-template int step<double>(const Basis<double>&, int, bool, const double*, double, hipStream_t, Clock*);
-template int step<double2>(const Basis<double2>&, int, bool, const double2*, double, hipStream_t, Clock*);
+template int step<double, double>(const Basis<double, double>&, int, bool, const double*, double, hipStream_t, Clock*);
+template int step<double, float>(const Basis<double, float>&, int, bool, const double*, double, hipStream_t, Clock*);
+template int step<double2, double2>(const Basis<double2, double2>&, int, bool, const double2*, double, hipStream_t,
+                                    Clock*);
 
 }  // namespace c3l_ortho

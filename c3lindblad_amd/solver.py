@@ -127,13 +127,16 @@ class ComponentCCSolution:
 class _DeviceNewton:
     """Device buffers of one Newton--GMRES run."""
 
-    def __init__(self, problem: ComponentStaticProblem, restart: int):
+    def __init__(self, problem: ComponentStaticProblem, restart: int, basis_bits: int = 64):
         self.torch = torch = _lib.require_device()
         self.dev = problem.device
         D = problem.oracle.num_variables
         self.D = D
         f64 = torch.float64
-        self.V = torch.zeros(restart + 1, D, dtype=f64, device=_lib.DEVICE)
+        # FP32 Krylov basis when the Jacobian products are FP32 (dots and updates stay FP64)
+        self.basis_bits = int(basis_bits)
+        basis_dtype = torch.float32 if self.basis_bits == 32 else f64
+        self.V = torch.zeros(restart + 1, D, dtype=basis_dtype, device=_lib.DEVICE)
         self.w = torch.empty(D, dtype=f64, device=_lib.DEVICE)
         self.tmp = torch.empty(D, dtype=f64, device=_lib.DEVICE)
         self.r = torch.empty(D, dtype=f64, device=_lib.DEVICE)
@@ -206,7 +209,7 @@ class _DeviceNewton:
         true = ctypes.c_double()
         _lib.call(
             "c3l_gmres", self.dev.plan.handle, ctypes.byref(maps), _lib.dptr(self.dev.diag), _lib.dptr(self.dev.dt),
-            _lib.dptr(self.dev.out), _lib.dptr(b), _lib.dptr(x), _lib.dptr(self.V), _lib.dptr(self.w),
+            _lib.dptr(self.dev.out), _lib.dptr(b), _lib.dptr(x), _lib.dptr(self.V), self.basis_bits, _lib.dptr(self.w),
             _lib.dptr(self.tmp), _lib.dptr(self.gmres_scratch), _lib.dptr(self.gmres_coef), float(rtol), float(atol), int(restart),
             int(cycles), ctypes.byref(info), ctypes.byref(iterations), ctypes.byref(true), ctypes.byref(calls),
             _lib.stream_ptr(self.torch),
@@ -248,7 +251,7 @@ def _solve(problem, initial_amplitudes, policy, accepted_step_callback, initial_
     controls = NewtonKrylovPolicy() if policy is None else policy
     oracle = problem.oracle
     started = perf_counter()
-    state = _DeviceNewton(problem, controls.gmres_restart)
+    state = _DeviceNewton(problem, controls.gmres_restart, 32 if controls.jacobian_precision == "fp32" else 64)
     torch = state.torch
     bits = 32 if controls.jacobian_precision == "fp32" else 64
     state.dev.plan.set_precision("D", "jvp", bits)

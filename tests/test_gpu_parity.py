@@ -292,6 +292,44 @@ def test_native_gmres_matches_scipy(torch_gpu, restart, cycles, rtol, atol):
     assert (info == 0) == (true * norm <= max(rtol * norm, atol))
 
 
+def test_native_gmres_with_fp32_basis_meets_the_tolerance(torch_gpu):
+    """FP32-stored Krylov basis (FP64 dots, coefficients and updates): the true residual
+    meets rtol 1e-6 and the step agrees with SciPy's to the tolerance's scale."""
+
+    from scipy.sparse.linalg import LinearOperator, gmres
+
+    from c3lindblad_amd.solver import _DeviceNewton
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    print("case gmres fp32 basis", flush=True)
+    p, sp = _static(3, 1.2, 3)
+    torch = torch_gpu
+    rng = np.random.default_rng(7)
+    D = p.tilde.real_size
+    x = 0.05 * rng.normal(size=D)
+    state = _DeviceNewton(sp, 40, basis_bits=32)
+    assert state.V.dtype == torch.float32
+    norm = state.residual(torch.from_numpy(x).cuda(), state.r)
+    b = (-state.r).contiguous()
+    step = torch.empty_like(b)
+    info, its, true = state.gmres(b, step, 1e-6, 0.0, 40, 12)
+    assert info == 0 and true <= 1e-6
+    masks = poly.coordinate_masks(p.coordinates)
+    t = p.tilde.unpack(x)
+    r = p.tilde.pack(poly.residual(p, t, masks)[0], check=False)
+    diag = solvers.preconditioner_diagonal(p.rapidity_sums())
+    A = LinearOperator((D, D), matvec=lambda v: p.tilde.pack(poly.jvp(p, t, p.tilde.unpack(v), masks), check=False),
+                       dtype=np.float64)
+    M = LinearOperator((D, D), matvec=lambda v: p.tilde.pack(p.tilde.unpack(v) / diag, check=False), dtype=np.float64)
+    want, want_info = gmres(A, -r, M=M, rtol=1e-10, atol=0.0, restart=200, maxiter=5)
+    assert want_info == 0
+    assert _rel(step.cpu().numpy(), want) < 1e-4
+    # the returned residual is the exact FP64 one of the returned step
+    w = state.w.cpu().numpy()
+    assert float(np.max(np.abs(w - (-r - A.matvec(step.cpu().numpy())))) / np.max(np.abs(r))) < 1e-12
+
+
 def test_eisenstat_walker_newton_reaches_legacy_root(torch_gpu):
     from c3lindblad_amd.solver import NewtonKrylovPolicy, solve_component_ccsd
 
