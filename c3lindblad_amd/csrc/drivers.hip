@@ -144,9 +144,9 @@ __device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_doub
 // itself, column after column, and forms each new rotation, pi and Green increment.
 // The residual and Green value after every column go to resid_hist / green_hist
 // ([b][shift]) so that the stop test can take the state of the exact stop column.
-constexpr int kShiftBatch = 4;
+constexpr int kShiftBatch = 8;
 
-__global__ void multishift_batch_kernel(const double2* __restrict__ H, int c0, int nb, int m_max,
+__global__ __launch_bounds__(256) void multishift_batch_kernel(const double2* __restrict__ H, int c0, int nb, int m_max,
                                         const double2* __restrict__ shifts, int n_shifts,
                                         const double2* __restrict__ sink_v, double beta, double* __restrict__ rot_c,
                                         double2* __restrict__ rot_s, double2* __restrict__ pi,
@@ -711,7 +711,7 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
                      reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), beta, n_shifts, status);
   D_CHECK(hipGetLastError());
   // status = [hn, worst, stop flag, m]; the host reads it every kCheck steps
-  constexpr int kCheck = 2 * kShiftBatch;
+  constexpr int kCheck = kShiftBatch;
   double host[4] = {0.0, INFINITY, 0.0, 0.0};
   const int waves_per_block = 4;
   const int64_t shift_blocks = (n_shifts + waves_per_block - 1) / waves_per_block;

@@ -206,9 +206,9 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
  * [n_shifts] doubles (relative GMRES residual); scratch and coef at least the sizes
  * c3l_krylov_workspace(1, dim, m_max, n_shifts, ...) returns; status >= 4 doubles.  Stops at
  * the first m >= min_dim with every shift's residual <= tolerance, at a breakdown, or at m_max.
- * The shifts absorb completed columns four at a time; the stop test after each batch finds
+ * The shifts absorb completed columns eight at a time; the stop test after each batch finds
  * the exact first column that meets the criterion and freezes the shift state there.  The
- * host reads it every 8 steps, so up to 12 further basis vectors may be computed (and
+ * host reads it after every batch, so up to 16 further basis vectors may be computed (and
  * ignored) after the stop. */
 int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
                 const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
