@@ -73,6 +73,18 @@ SIGNATURES = {
          ctypes.c_double, _I32, ctypes.c_double, _P, _P, _P, _P],
         ctypes.c_int,
     ),
+    "c3l_arnoldi_batch": (
+        [_P, _I32, _P, _P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+         ctypes.c_double, _I32, ctypes.c_double, _P, _P, _P, _P],
+        ctypes.c_int,
+    ),
+    "c3l_arnoldi_begin": (
+        [_P, _I32, _P, _P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+         ctypes.c_double, _I32, ctypes.c_double, _P, _P],
+        ctypes.c_int,
+    ),
+    "c3l_arnoldi_drive": ([_P, _I32], ctypes.c_int),
+    "c3l_arnoldi_end": ([_P, _P, _P, _P], ctypes.c_int),
 }
 
 

@@ -85,11 +85,27 @@ int64_t grid_for(int64_t count, int threads = kThreads) {
 }
 
 // ------------------------------------------------------------------ plan kernels
+//
+// Every plan kernel runs over blockIdx.z = instance: twin plans (compiled from one
+// lowered plan, so identical layouts, descriptors and index arrays) evaluate their
+// own workspaces and vectors in one launch.  A single plan is the one-instance case.
+constexpr int kMaxTwins = 4;
+template <typename P>
+struct Twins {
+  P p[kMaxTwins];
+};
+template <typename P>
+Twins<P> twin(P a) {
+  Twins<P> t{};
+  t.p[0] = a;
+  return t;
+}
 
 template <typename C>
-__global__ void scatter_padded_kernel(const double2* __restrict__ v, C* __restrict__ pad,
-                                      const int64_t* __restrict__ src, const int8_t* __restrict__ sign,
-                                      int64_t count) {
+__global__ void scatter_padded_kernel(Twins<const double2*> vs, Twins<C*> pads, const int64_t* __restrict__ src,
+                                      const int8_t* __restrict__ sign, int64_t count) {
+  const double2* __restrict__ v = vs.p[blockIdx.z];
+  C* __restrict__ pad = pads.p[blockIdx.z];
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t s = src[e];
@@ -146,8 +162,9 @@ __device__ __forceinline__ C contract_one(const C* __restrict__ par, const C* __
 
 // grid.y = job; threads cover the n^(R-1) outputs of that job.
 template <typename C>
-__global__ void contract_kernel(C* __restrict__ ws, const C* __restrict__ vectors,
-                                const ContractJob* __restrict__ jobs, int n) {
+__global__ void contract_kernel(Twins<C*> wss, const C* __restrict__ vectors, const ContractJob* __restrict__ jobs,
+                                int n) {
+  C* __restrict__ ws = wss.p[blockIdx.z];
   const ContractJob job = jobs[blockIdx.y];
   const int r = job.par_rank - 1;
   int64_t out_size = 1;
@@ -176,10 +193,11 @@ __global__ void contract_kernel(C* __restrict__ ws, const C* __restrict__ vector
 // one launch replaces one launch per level.  Jobs of level l are
 // jobs[level_start[l] .. level_start[l+1]).
 template <typename C>
-__global__ __launch_bounds__(1024) void contract_levels_kernel(C* __restrict__ ws, const C* __restrict__ vectors,
+__global__ __launch_bounds__(1024) void contract_levels_kernel(Twins<C*> wss, const C* __restrict__ vectors,
                                                                const ContractJob* __restrict__ jobs,
                                                                const int32_t* __restrict__ level_start, int n_levels,
                                                                int n) {
+  C* __restrict__ ws = wss.p[blockIdx.z];
   for (int lv = 0; lv < n_levels; ++lv) {
     for (int jb = level_start[lv]; jb < level_start[lv + 1]; ++jb) {
       const ContractJob job = jobs[jb];
@@ -254,11 +272,14 @@ __device__ __forceinline__ int64_t factor_index(const WedgeFactor& f, int i0, in
 }
 
 template <bool kTwoFactors, typename C>
-__global__ __launch_bounds__(256) void wedge_program_kernel(const C* __restrict__ ws, double2* __restrict__ out,
+__global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss, Twins<double2*> outs,
                                                             const WedgeBlockDesc* __restrict__ descs,
                                                             const int2* __restrict__ tiles,
                                                             const double2* __restrict__ diag,
-                                                            const double2* __restrict__ x) {
+                                                            Twins<const double2*> xs) {
+  const C* __restrict__ ws = wss.p[blockIdx.z];
+  double2* __restrict__ out = outs.p[blockIdx.z];
+  const double2* __restrict__ x = xs.p[blockIdx.z];
   __shared__ double2 part[256];
   __shared__ double2 s_coeff[kStagedEntries];
   __shared__ WedgeFactor s_fac[2 * kStagedEntries];
@@ -760,6 +781,10 @@ struct c3l_plan {
   std::vector<PendingTiming> pending;
   double program_ms[kPrograms] = {};
   int64_t program_launches[kPrograms] = {};
+  // FNV-1a digest of everything the plan was built from (sizes, vectors, constant
+  // atoms, bases, levels, blocks): plans with equal digests are twins and can be
+  // evaluated together in one launch per kernel, each on its own workspace.
+  uint64_t digest = 1469598103934665603ull;
 };
 
 namespace {
@@ -794,6 +819,18 @@ int64_t ipow(int64_t n, int r) {
   return v;
 }
 
+void mix(c3l_plan* plan, const void* data, size_t bytes) {
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  uint64_t h = plan->digest;
+  for (size_t i = 0; i < bytes; ++i) h = (h ^ p[i]) * 1099511628211ull;
+  plan->digest = h;
+}
+
+template <typename T>
+void mix_value(c3l_plan* plan, T v) {
+  mix(plan, &v, sizeof(T));
+}
+
 }  // namespace
 
 extern "C" {
@@ -825,6 +862,12 @@ int c3l_plan_create(int32_t n, int32_t n_vectors, const double* vectors, int64_t
   }
   plan->owned.push_back(ws);
   plan->ws = static_cast<double2*>(ws);
+  mix_value(plan, n);
+  mix_value(plan, n_vectors);
+  mix_value(plan, workspace);
+  mix(plan, vectors, sizeof(double2) * (size_t)n_vectors * (size_t)n);
+  mix(plan, const_offset, sizeof(int64_t) * (size_t)n_const);
+  mix(plan, const_vec, sizeof(int32_t) * (size_t)n_const);
   int rc = upload(plan, &plan->vectors, reinterpret_cast<const double2*>(vectors), (int64_t)n_vectors * n);
   int64_t* d_off = nullptr;
   int32_t* d_vec = nullptr;
@@ -915,6 +958,12 @@ int c3l_plan_add_base(c3l_plan* plan, int32_t kind, int64_t offset, int32_t deg,
   if (offset + b.count > plan->workspace) return fail(C3L_EINVAL, "c3l_plan_add_base: block exceeds workspace");
   b.src.assign(src, src + b.count);
   b.sign.assign(sign, sign + b.count);
+  mix_value(plan, (int32_t)1);
+  mix_value(plan, kind);
+  mix_value(plan, offset);
+  mix_value(plan, deg);
+  mix(plan, src, sizeof(int64_t) * (size_t)b.count);
+  mix(plan, sign, (size_t)b.count);
   plan->bases.push_back(std::move(b));
   plan->ranges_built = false;
   KindExec& ke = plan->kinds[kind];
@@ -940,6 +989,9 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
     jobs[(size_t)j] = ContractJob{out_offset[j], par_offset[j], par_rank[j], slot[j], vec[j], sign[j]};
     if (outsz > max_out) max_out = outsz;
   }
+  mix_value(plan, (int32_t)2);
+  mix_value(plan, kind);
+  mix(plan, jobs.data(), sizeof(ContractJob) * jobs.size());
   PlanLevel level;
   level.kind = kind;
   level.n_jobs = n_jobs;
@@ -1011,6 +1063,15 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
     }
     b.coord[(size_t)k] = packed;
   }
+  mix_value(plan, (int32_t)3);
+  mix_value(plan, program);
+  mix_value(plan, deg);
+  mix(plan, b.coord.data(), sizeof(uint32_t) * b.coord.size());
+  mix(plan, dest, sizeof(int64_t) * (size_t)count);
+  mix(plan, coeff, sizeof(double) * 2 * (size_t)n_entries);
+  if (n_entries > 0) mix(plan, fstart, sizeof(int32_t) * ((size_t)n_entries + 1));
+  mix(plan, foffset, sizeof(int64_t) * (size_t)n_factors);
+  mix(plan, fcode, sizeof(int32_t) * (size_t)n_factors);
   plan->blocks.push_back(std::move(b));
   plan->ranges_built = false;
   ProgramExec& pe = plan->programs[program];
@@ -1019,11 +1080,29 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
   return C3L_OK;
 }
 
+int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t kind, const double* const* v,
+                                void* stream);
+int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t program, const double* diag,
+                                 const double* const* x, double* const* out, void* stream);
+
 int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
-  if (plan == nullptr || v == nullptr) return fail(C3L_EINVAL, "c3l_plan_load: invalid argument");
+  return c3l_internal_plan_load_many(&plan, 1, kind, &v, stream);
+}
+
+// Loads vector v[i] into twin plan plans[i] for i < count with one launch per kernel
+// (blockIdx.z = instance).  Twins are checked by digest; several plans load in FP64 only.
+int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t kind, const double* const* v,
+                                void* stream) {
+  if (plans == nullptr || v == nullptr || count < 1 || count > kMaxTwins)
+    return fail(C3L_EINVAL, "c3l_plan_load: invalid argument");
   if (kind < 0 || kind >= kKinds) return fail(C3L_EINVAL, "c3l_plan_load: invalid kind");
+  for (int32_t i = 0; i < count; ++i) {
+    if (plans[i] == nullptr || v[i] == nullptr) return fail(C3L_EINVAL, "c3l_plan_load: invalid argument");
+    if (i > 0 && (plans[i]->digest != plans[0]->digest || plans[i]->kind_bits[kind] != 64))
+      return fail(C3L_EINVAL, "c3l_plan_load: plans loaded together must be FP64 twins");
+  }
+  c3l_plan* plan = plans[0];
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const double2* vv = reinterpret_cast<const double2*>(v);
   KindExec& ke = plan->kinds[kind];
   if (!ke.built) {
     std::vector<const PlanBase*> mine;
@@ -1070,34 +1149,49 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
     ke.built = true;
   }
   const bool narrow = plan->kind_bits[kind] == 32;
+  Twins<const double2*> vv{};
+  Twins<double2*> ws{};
+  Twins<float2*> ws32{};
+  for (int32_t i = 0; i < count; ++i) {
+    vv.p[i] = reinterpret_cast<const double2*>(v[i]);
+    ws.p[i] = plans[i]->ws;
+    ws32.p[i] = plans[i]->ws32;
+  }
+  const dim3 inst(1, 1, (unsigned)count);
   for (const BaseRun& r : ke.runs) {
+    Twins<double2*> pad = ws;
+    Twins<float2*> pad32 = ws32;
+    for (int32_t i = 0; i < count; ++i) {
+      pad.p[i] += r.offset;
+      if (narrow) pad32.p[i] += r.offset;
+    }
     if (narrow)
-      hipLaunchKernelGGL(scatter_padded_kernel<float2>, dim3(grid_for(r.count)), dim3(kThreads), 0, s, vv,
-                         plan->ws32 + r.offset, r.src, r.sign, r.count);
+      hipLaunchKernelGGL(scatter_padded_kernel<float2>, dim3(grid_for(r.count), 1, count), dim3(kThreads), 0, s, vv,
+                         pad32, r.src, r.sign, r.count);
     else
-      hipLaunchKernelGGL(scatter_padded_kernel<double2>, dim3(grid_for(r.count)), dim3(kThreads), 0, s, vv,
-                         plan->ws + r.offset, r.src, r.sign, r.count);
+      hipLaunchKernelGGL(scatter_padded_kernel<double2>, dim3(grid_for(r.count), 1, count), dim3(kThreads), 0, s, vv,
+                         pad, r.src, r.sign, r.count);
     C3L_HIP_CHECK(hipGetLastError());
   }
-  if (!narrow) ++plan->fixed_version;
+  if (!narrow)
+    for (int32_t i = 0; i < count; ++i) ++plans[i]->fixed_version;
   if (ke.single) {
     if (narrow)
-      hipLaunchKernelGGL(contract_levels_kernel<float2>, dim3(1), dim3(1024), 0, s, plan->ws32, plan->vectors32, ke.jobs,
+      hipLaunchKernelGGL(contract_levels_kernel<float2>, inst, dim3(1024), 0, s, ws32, plan->vectors32, ke.jobs,
                          ke.level_start, ke.n_levels, plan->n);
     else
-      hipLaunchKernelGGL(contract_levels_kernel<double2>, dim3(1), dim3(1024), 0, s, plan->ws, plan->vectors, ke.jobs,
+      hipLaunchKernelGGL(contract_levels_kernel<double2>, inst, dim3(1024), 0, s, ws, plan->vectors, ke.jobs,
                          ke.level_start, ke.n_levels, plan->n);
     C3L_HIP_CHECK(hipGetLastError());
     return C3L_OK;
   }
   for (const PlanLevel& l : plan->levels) {
     if (l.kind != kind || l.n_jobs == 0) continue;
+    const dim3 grid(grid_for(l.max_out), l.n_jobs, count);
     if (narrow)
-      hipLaunchKernelGGL(contract_kernel<float2>, dim3(grid_for(l.max_out), l.n_jobs), dim3(kThreads), 0, s, plan->ws32,
-                         plan->vectors32, l.jobs, plan->n);
+      hipLaunchKernelGGL(contract_kernel<float2>, grid, dim3(kThreads), 0, s, ws32, plan->vectors32, l.jobs, plan->n);
     else
-      hipLaunchKernelGGL(contract_kernel<double2>, dim3(grid_for(l.max_out), l.n_jobs), dim3(kThreads), 0, s, plan->ws,
-                         plan->vectors, l.jobs, plan->n);
+      hipLaunchKernelGGL(contract_kernel<double2>, grid, dim3(kThreads), 0, s, ws, plan->vectors, l.jobs, plan->n);
     C3L_HIP_CHECK(hipGetLastError());
   }
   return C3L_OK;
@@ -1292,17 +1386,32 @@ int c3l_plan_set_precision(c3l_plan* plan, int32_t kind, int32_t program, int32_
 }
 
 int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out, void* stream) {
-  if (plan == nullptr || program < 0 || program >= kPrograms || ((diag == nullptr) != (x == nullptr)) ||
-      (out == nullptr && diag != nullptr))
+  return c3l_internal_plan_apply_many(&plan, 1, program, diag, &x, &out, stream);
+}
+
+// Program `program` of twin plans plans[i] on x[i] into out[i], i < count, one launch
+// per kernel variant (blockIdx.z = instance).  Several plans: FP64 programs with
+// external outputs only, one shared diagonal.
+int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t program, const double* diag,
+                                 const double* const* x, double* const* out, void* stream) {
+  if (plans == nullptr || x == nullptr || out == nullptr || count < 1 || count > kMaxTwins || program < 0 ||
+      program >= kPrograms)
     return fail(C3L_EINVAL, "c3l_plan_apply: invalid argument");
+  for (int32_t i = 0; i < count; ++i) {
+    if (plans[i] == nullptr || ((diag == nullptr) != (x[i] == nullptr)) || (out[i] == nullptr && diag != nullptr))
+      return fail(C3L_EINVAL, "c3l_plan_apply: invalid argument");
+    if (count > 1 && (plans[i]->digest != plans[0]->digest || plans[i]->program_bits[program] != 64 ||
+                      out[i] == nullptr))
+      return fail(C3L_EINVAL, "c3l_plan_apply: plans applied together must be FP64 twins with external outputs");
+  }
+  c3l_plan* plan = plans[0];
   int rc = build_program(plan, program);
   if (rc != C3L_OK) return rc;
   const ProgramExec& pe = plan->programs[program];
   if (pe.tiles_two + pe.tiles_general == 0) return C3L_OK;
-  const bool internal = out == nullptr;  // creation programs write into the workspace
+  const bool internal = out[0] == nullptr;  // creation programs write into the workspace
   if (internal && pe.max_dest >= plan->workspace)
     return fail(C3L_EINVAL, "c3l_plan_apply: workspace program writes outside the workspace");
-  double2* o = internal ? plan->ws : reinterpret_cast<double2*>(out);
   hipStream_t s = static_cast<hipStream_t>(stream);
   PendingTiming t{program, nullptr, nullptr};
   if (plan->timing) {
@@ -1311,30 +1420,38 @@ int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const do
     C3L_HIP_CHECK(hipEventRecord(t.start, s));
   }
   const double2* dg = reinterpret_cast<const double2*>(diag);
-  const double2* xx = reinterpret_cast<const double2*>(x);
+  Twins<double2*> o{};
+  Twins<const double2*> xx{}, ws{};
+  for (int32_t i = 0; i < count; ++i) {
+    o.p[i] = internal ? plan->ws : reinterpret_cast<double2*>(out[i]);
+    xx.p[i] = reinterpret_cast<const double2*>(x[i]);
+    ws.p[i] = plans[i]->ws;
+  }
+  const dim3 two(pe.tiles_two, 1, count), general(pe.tiles_general, 1, count);
   if (plan->program_bits[program] == 32) {
     if (internal) return fail(C3L_EINVAL, "c3l_plan_apply: workspace programs run in FP64 only");
     rc = narrow_fixed_atoms(plan, s);
     if (rc != C3L_OK) return rc;
+    const Twins<const float2*> ws32 = twin<const float2*>(plan->ws32);
     if (pe.tiles_two > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<true, float2>), dim3(pe.tiles_two), dim3(kThreads), 0, s, plan->ws32, o,
-                         pe.descs, pe.tiles, dg, xx);
+      hipLaunchKernelGGL((wedge_program_kernel<true, float2>), two, dim3(kThreads), 0, s, ws32, o, pe.descs, pe.tiles,
+                         dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
     if (pe.tiles_general > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<false, float2>), dim3(pe.tiles_general), dim3(kThreads), 0, s,
-                         plan->ws32, o, pe.descs, pe.tiles + pe.tiles_two, dg, xx);
+      hipLaunchKernelGGL((wedge_program_kernel<false, float2>), general, dim3(kThreads), 0, s, ws32, o, pe.descs,
+                         pe.tiles + pe.tiles_two, dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
   } else {
     if (pe.tiles_two > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<true, double2>), dim3(pe.tiles_two), dim3(kThreads), 0, s, plan->ws, o,
-                         pe.descs, pe.tiles, dg, xx);
+      hipLaunchKernelGGL((wedge_program_kernel<true, double2>), two, dim3(kThreads), 0, s, ws, o, pe.descs, pe.tiles,
+                         dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
     if (pe.tiles_general > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<false, double2>), dim3(pe.tiles_general), dim3(kThreads), 0, s,
-                         plan->ws, o, pe.descs, pe.tiles + pe.tiles_two, dg, xx);
+      hipLaunchKernelGGL((wedge_program_kernel<false, double2>), general, dim3(kThreads), 0, s, ws, o, pe.descs,
+                         pe.tiles + pe.tiles_two, dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
     if (internal) ++plan->fixed_version;

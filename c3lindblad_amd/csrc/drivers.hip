@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -38,6 +39,12 @@ extern "C" int c3l_internal_tilde_unpack_f32(const float* x, double* t, const in
 extern "C" int c3l_internal_narrow_scale(const double* x, float* y, double a, int64_t dim, void* stream);
 extern "C" int c3l_internal_rmultiaxpy_f32(const float* V, int64_t ld, int32_t ncols, const double* alpha,
                                            double scale, double* w, int64_t dim, void* stream);
+// Twin plans (equal digests) loaded / applied together, one launch per kernel.
+extern "C" int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t kind,
+                                           const double* const* v, void* stream);
+extern "C" int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t program,
+                                            const double* diag, const double* const* x, double* const* out,
+                                            void* stream);
 
 namespace {
 
@@ -146,13 +153,15 @@ __device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_doub
 // ([b][shift]) so that the stop test can take the state of the exact stop column.
 constexpr int kShiftBatch = 8;
 
-__global__ __launch_bounds__(256) void multishift_batch_kernel(const double2* __restrict__ H, int c0, int nb, int m_max,
-                                        const double2* __restrict__ shifts, int n_shifts,
-                                        const double2* __restrict__ sink_v, double beta, double* __restrict__ rot_c,
-                                        double2* __restrict__ rot_s, double2* __restrict__ pi,
-                                        double2* __restrict__ gamma, double2* __restrict__ green,
-                                        double* __restrict__ resid, double* __restrict__ resid_hist,
-                                        double2* __restrict__ green_hist, const double* __restrict__ status) {
+__device__ __forceinline__ void multishift_batch_body(const double2* __restrict__ H, int c0, int nb, int m_max,
+                                                      const double2* __restrict__ shifts, int n_shifts,
+                                                      const double2* __restrict__ sink_v, double beta,
+                                                      double* __restrict__ rot_c, double2* __restrict__ rot_s,
+                                                      double2* __restrict__ pi, double2* __restrict__ gamma,
+                                                      double2* __restrict__ green, double* __restrict__ resid,
+                                                      double* __restrict__ resid_hist,
+                                                      double2* __restrict__ green_hist,
+                                                      const double* __restrict__ status) {
   const int lane = threadIdx.x & 63;
   const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (s >= n_shifts || status[2] != 0.0) return;  // whole wave exits together; nothing after the stop
@@ -304,14 +313,12 @@ __global__ void fill_kernel(double* __restrict__ a, double v, int64_t n) {
 // residual <= tolerance: status = [hn, worst, flag (3 nonfinite, 2 breakdown, 1 converged),
 // c+1], and green/resid take the values after that column.  Without a stop, status[1] is
 // the worst residual after the last column and status[3] its count.
-__global__ __launch_bounds__(1024) void arnoldi_batch_status_kernel(const double* __restrict__ resid_hist,
-                                                                    const double2* __restrict__ green_hist,
-                                                                    double2* __restrict__ green,
-                                                                    double* __restrict__ resid, int n,
-                                                                    double* __restrict__ status,
-                                                                    const double2* __restrict__ H, int m_max, int c0,
-                                                                    int nb, int min_dim, double tolerance,
-                                                                    double breakdown) {
+__device__ __forceinline__ void arnoldi_batch_status_body(const double* __restrict__ resid_hist,
+                                                          const double2* __restrict__ green_hist,
+                                                          double2* __restrict__ green, double* __restrict__ resid,
+                                                          int n, double* __restrict__ status,
+                                                          const double2* __restrict__ H, int m_max, int c0, int nb,
+                                                          int min_dim, double tolerance, double breakdown) {
   __shared__ double part[kShiftBatch][16];
   __shared__ int stop_b;
   if (status[2] != 0.0) return;  // uniform: frozen after an earlier stop
@@ -366,6 +373,39 @@ __global__ __launch_bounds__(1024) void arnoldi_batch_status_kernel(const double
     green[i] = green_hist[(int64_t)sb * n + i];
     resid[i] = resid_hist[(int64_t)sb * n + i];
   }
+}
+
+// Device state of one Arnoldi basis's shifts; the batch kernels run over several
+// bases of equal shape at once (blockIdx.z = basis).
+struct ShiftState {
+  const double2* H;
+  const double2* sink_v;
+  double beta;
+  double* rot_c;
+  double2* rot_s;
+  double2* pi;
+  double2* gamma;
+  double2* green;
+  double* resid;
+  double* resid_hist;
+  double2* green_hist;
+  double* status;
+};
+using ShiftStates = c3l_ortho::Many<ShiftState>;
+
+__global__ __launch_bounds__(256) void multishift_batch_kernel(ShiftStates states, int c0, int nb, int m_max,
+                                                               const double2* __restrict__ shifts, int n_shifts) {
+  const ShiftState& t = states.p[blockIdx.z];
+  multishift_batch_body(t.H, c0, nb, m_max, shifts, n_shifts, t.sink_v, t.beta, t.rot_c, t.rot_s, t.pi, t.gamma,
+                        t.green, t.resid, t.resid_hist, t.green_hist, t.status);
+}
+
+__global__ __launch_bounds__(1024) void arnoldi_batch_status_kernel(ShiftStates states, int n, int m_max, int c0,
+                                                                    int nb, int min_dim, double tolerance,
+                                                                    double breakdown) {
+  const ShiftState& t = states.p[blockIdx.z];
+  arnoldi_batch_status_body(t.resid_hist, t.green_hist, t.green, t.resid, n, t.status, t.H, m_max, c0, nb, min_dim,
+                            tolerance, breakdown);
 }
 
 int64_t blocks_for(int64_t n, int t = 256) {
@@ -632,6 +672,174 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
 
 }  // namespace
 
+// One batched Arnoldi resolvent in flight (c3l_arnoldi_begin / _drive / _end): the
+// per-basis device pointers, the lockstep position and the bases still running.
+// Stop flags travel to pinned host memory with an event per shift batch.  The host
+// reads a batch's flags as soon as its event has completed (polled once per step) and
+// waits for it only when the next batch is enqueued, so it never drains its stream
+// while other resolvents have work queued.
+struct c3l_arnoldi_run {
+  std::vector<c3l_plan*> plans;
+  std::vector<c3l_ortho::Basis<double2>> basis;
+  std::vector<ShiftState> state;
+  std::vector<double*> w;  // sigma output (scratch) per basis
+  std::vector<double*> V;  // basis columns per basis
+  std::vector<int32_t> live;
+  const double* diag = nullptr;
+  const double2* shifts = nullptr;
+  double* status = nullptr;
+  int64_t dim = 0;
+  int32_t count = 0, m_max = 0, n_shifts = 0, min_dim = 0;
+  double tolerance = 0.0, breakdown = 0.0;
+  hipStream_t stream = nullptr;
+  int32_t j = 0, c0 = 0;
+  double* pinned = nullptr;  // two slots of 4 * count doubles
+  hipEvent_t copied[2] = {nullptr, nullptr};
+  int32_t batches = 0;     // status copies enqueued so far
+  int32_t flags_read = 0;  // status copies read so far
+  DriverClock clock{0};
+
+  bool done() const { return live.empty() || j > m_max; }
+
+  ~c3l_arnoldi_run() {
+    for (hipEvent_t& e : copied)
+      if (e != nullptr) (void)hipEventDestroy(e);
+    if (pinned != nullptr) (void)hipHostFree(pinned);
+  }
+
+  // Reads every status copy whose event has completed, and waits for the oldest one
+  // when both slots are in use; drops the bases whose stop flag is set.
+  int poll() {
+    while (flags_read < batches) {
+      const int slot = flags_read & 1;
+      if (batches - flags_read < 2) {
+        const hipError_t e = hipEventQuery(copied[slot]);
+        if (e == hipErrorNotReady) break;
+        D_CHECK(e);
+      }
+      D_CHECK(hipEventSynchronize(copied[slot]));
+      const double* h = pinned + (size_t)slot * 4 * count;
+      std::vector<int32_t> next;
+      for (int32_t i : live)
+        if (h[(size_t)i * 4 + 2] == 0.0) next.push_back(i);
+      live.swap(next);
+      ++flags_read;
+    }
+    return C3L_OK;
+  }
+
+  // Step j: sigma on the once-projected v_j of every live basis (none at j = m_max),
+  // one orthogonalisation step, and every kShiftBatch columns the shift update.
+  int advance() {
+    const int k = (int)live.size();
+    const bool has_w = j < m_max;
+    c3l_plan* lp[c3l_ortho::kMaxInstances];
+    const double* vj[c3l_ortho::kMaxInstances];
+    double* wj[c3l_ortho::kMaxInstances];
+    const double2* wz[c3l_ortho::kMaxInstances];
+    c3l_ortho::Basis<double2> lb[c3l_ortho::kMaxInstances];
+    ShiftStates ls{};
+    for (int u = 0; u < k; ++u) {
+      const int32_t i = live[(size_t)u];
+      lp[u] = plans[(size_t)i];
+      vj[u] = V[(size_t)i] + (int64_t)2 * j * dim;
+      wj[u] = w[(size_t)i];
+      wz[u] = has_w ? reinterpret_cast<const double2*>(wj[u]) : nullptr;
+      lb[u] = basis[(size_t)i];
+      ls.p[u] = state[(size_t)i];
+    }
+    if (has_w) {
+      D_CALL(c3l_internal_plan_load_many(lp, k, 2, vj, stream));
+      D_CALL(c3l_internal_plan_apply_many(lp, k, 3, diag, vj, wj, stream));
+    }
+    D_CALL(c3l_ortho::step_many(lb, k, j, j == 0, wz, breakdown, stream, &clock));
+    if (j > 0 && (j - c0 == kShiftBatch || j == m_max)) {
+      const int nb = j - c0;
+      const int waves_per_block = 4;
+      const int64_t shift_blocks = (n_shifts + waves_per_block - 1) / waves_per_block;
+      hipLaunchKernelGGL(multishift_batch_kernel, dim3(shift_blocks, 1, k), dim3(64 * waves_per_block), 0, stream, ls,
+                         c0, nb, m_max, shifts, n_shifts);
+      D_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(arnoldi_batch_status_kernel, dim3(1, 1, k), dim3(1024), 0, stream, ls, n_shifts, m_max, c0,
+                         nb, min_dim, tolerance, breakdown);
+      D_CHECK(hipGetLastError());
+      c0 = j;
+      const int slot = batches & 1;
+      D_CHECK(hipMemcpyAsync(pinned + (size_t)slot * 4 * count, status, 4 * sizeof(double) * (size_t)count,
+                             hipMemcpyDeviceToHost, stream));
+      D_CHECK(hipEventRecord(copied[slot], stream));
+      ++batches;
+    }
+    ++j;
+    return poll();
+  }
+};
+
+namespace {
+
+// Sets up `count` bases of c3l_arnoldi_batch on `stream` (validation, per-basis pointers,
+// shift-state initialisation).
+int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, const double* diag, double* V, double* H,
+                  int64_t dim, int32_t m_max, const double* shifts, int32_t n_shifts, const double* sink,
+                  const double* beta, double* rot_c, double* rot_s, double* pi, double* gamma, double* green,
+                  double* resid, double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
+                  double breakdown_tol, void* stream) {
+  if (plans == nullptr || count < 1 || count > c3l_ortho::kMaxInstances || V == nullptr || H == nullptr || dim <= 0 ||
+      m_max <= 0 || n_shifts <= 0 || shifts == nullptr || sink == nullptr || beta == nullptr || rot_c == nullptr ||
+      rot_s == nullptr || pi == nullptr || gamma == nullptr || green == nullptr || resid == nullptr ||
+      scratch == nullptr || coef == nullptr || status == nullptr)
+    return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
+  for (int32_t i = 0; i < count; ++i)
+    if (plans[i] == nullptr || !(beta[i] > 0.0)) return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const Layout L = arnoldi_layout(dim, m_max, n_shifts);
+  // per-basis strides (doubles) of the caller's consecutive instance blocks
+  const int64_t sV = 2 * ((int64_t)m_max + 1) * dim, sH = 2 * (int64_t)m_max * ((int64_t)m_max + 1);
+  const int64_t sRot = (int64_t)n_shifts * m_max, sShift = n_shifts;
+  r->plans.assign(plans, plans + count);
+  r->count = count;
+  r->diag = diag;
+  r->shifts = reinterpret_cast<const double2*>(shifts);
+  r->status = status;
+  r->dim = dim;
+  r->m_max = m_max;
+  r->n_shifts = n_shifts;
+  r->min_dim = min_dim;
+  r->tolerance = tolerance;
+  r->breakdown = breakdown_tol;
+  r->stream = s;
+  for (int32_t i = 0; i < count; ++i) {
+    double* sc = scratch + i * L.scratch_total;
+    double* cf = coef + i * L.coef_total;
+    double2* Hz = reinterpret_cast<double2*>(H + i * sH);
+    double2* sink_v = reinterpret_cast<double2*>(cf + L.sink_v);
+    r->basis.push_back(c3l_ortho::Basis<double2>{reinterpret_cast<double2*>(V + i * sV), dim, dim, Hz,
+                                                 (int64_t)m_max + 1, sink_v,
+                                                 reinterpret_cast<const double2*>(sink + 2 * i * dim),
+                                                 reinterpret_cast<double2*>(sc + L.partial),
+                                                 reinterpret_cast<double2*>(cf + L.red),
+                                                 reinterpret_cast<double2*>(cf + L.ab),
+                                                 reinterpret_cast<double2*>(cf + L.sc), status + 4 * i, nullptr});
+    r->state.push_back(ShiftState{Hz, sink_v, beta[i], rot_c + i * sRot,
+                                  reinterpret_cast<double2*>(rot_s + 2 * i * sRot),
+                                  reinterpret_cast<double2*>(pi + 2 * i * sRot),
+                                  reinterpret_cast<double2*>(gamma + 2 * i * sShift),
+                                  reinterpret_cast<double2*>(green + 2 * i * sShift), resid + i * sShift,
+                                  sc + L.resid_hist, reinterpret_cast<double2*>(sc + L.green_hist), status + 4 * i});
+    r->w.push_back(sc + L.w);
+    r->V.push_back(V + i * sV);
+    r->live.push_back(i);
+    hipLaunchKernelGGL(init_shift_state_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s, r->state.back().gamma,
+                       r->state.back().green, beta[i], n_shifts, status + 4 * i);
+    D_CHECK(hipGetLastError());
+  }
+  D_CHECK(hipHostMalloc(reinterpret_cast<void**>(&r->pinned), 2 * 4 * sizeof(double) * (size_t)count));
+  for (hipEvent_t& e : r->copied) D_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return C3L_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
@@ -691,71 +899,77 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
                 double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
                 double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
                 int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream) {
-  if (plan == nullptr || V == nullptr || H == nullptr || dim <= 0 || m_max <= 0 || n_shifts <= 0 || shifts == nullptr ||
-      sink == nullptr || !(beta > 0.0) || rot_c == nullptr || rot_s == nullptr || pi == nullptr || gamma == nullptr ||
-      green == nullptr || resid == nullptr || scratch == nullptr || coef == nullptr || status == nullptr ||
-      m_out == nullptr || breakdown_out == nullptr || worst_out == nullptr)
-    return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const Layout L = arnoldi_layout(dim, m_max, n_shifts);
-  double2* Vz = reinterpret_cast<double2*>(V);
-  double2* Hz = reinterpret_cast<double2*>(H);
-  double2* w = reinterpret_cast<double2*>(scratch + L.w);
-  double2* sink_v = reinterpret_cast<double2*>(coef + L.sink_v);
-  c3l_ortho::Basis<double2> basis{Vz, dim, dim, Hz, (int64_t)m_max + 1, sink_v, reinterpret_cast<const double2*>(sink),
-                                  reinterpret_cast<double2*>(scratch + L.partial), reinterpret_cast<double2*>(coef + L.red),
-                                  reinterpret_cast<double2*>(coef + L.ab), reinterpret_cast<double2*>(coef + L.sc),
-                                  status};
-  DriverClock clock(0);
-  hipLaunchKernelGGL(init_shift_state_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s,
-                     reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), beta, n_shifts, status);
-  D_CHECK(hipGetLastError());
-  // status = [hn, worst, stop flag, m]; the host reads it every kCheck steps
-  constexpr int kCheck = kShiftBatch;
-  double host[4] = {0.0, INFINITY, 0.0, 0.0};
-  const int waves_per_block = 4;
-  const int64_t shift_blocks = (n_shifts + waves_per_block - 1) / waves_per_block;
-  double* resid_hist = scratch + L.resid_hist;
-  double2* green_hist = reinterpret_cast<double2*>(scratch + L.green_hist);
-  int c0 = 0;  // first column the shifts have not absorbed yet
-  // step j applies sigma to the once-projected v_j (none at j = m_max) and completes
-  // column j-1; every kShiftBatch columns (and at the end) the shifts absorb them
-  for (int32_t j = 0; j <= m_max; ++j) {
-    const bool has_w = j < m_max;
-    if (has_w) {
-      double* vj = V + (int64_t)2 * j * dim;
-      D_CALL(c3l_plan_load(plan, 2, vj, stream));
-      D_CALL(c3l_plan_apply(plan, 3, diag, vj, reinterpret_cast<double*>(w), stream));
-    }
-    D_CALL(c3l_ortho::step(basis, j, j == 0, has_w ? w : nullptr, breakdown_tol, s, &clock));
-    if (j == 0) continue;
-    if (j - c0 == kShiftBatch || j == m_max) {
-      const int nb = j - c0;
-      hipLaunchKernelGGL(multishift_batch_kernel, dim3(shift_blocks), dim3(64 * waves_per_block), 0, s, Hz, c0, nb,
-                         m_max, reinterpret_cast<const double2*>(shifts), n_shifts, sink_v, beta, rot_c,
-                         reinterpret_cast<double2*>(rot_s), reinterpret_cast<double2*>(pi),
-                         reinterpret_cast<double2*>(gamma), reinterpret_cast<double2*>(green), resid, resid_hist,
-                         green_hist, status);
-      D_CHECK(hipGetLastError());
-      hipLaunchKernelGGL(arnoldi_batch_status_kernel, dim3(1), dim3(1024), 0, s, resid_hist, green_hist,
-                         reinterpret_cast<double2*>(green), resid, n_shifts, status, Hz, m_max, c0, nb, min_dim,
-                         tolerance, breakdown_tol);
-      D_CHECK(hipGetLastError());
-      c0 = j;
-    }
-    if (j % kCheck == 0 || j == m_max) {
-      D_CHECK(hipMemcpyAsync(host, status, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-      D_CHECK(hipStreamSynchronize(s));
-      if (host[2] != 0.0) break;
+  if (plan == nullptr) return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
+  return c3l_arnoldi_batch(&plan, 1, diag, V, H, dim, m_max, shifts, n_shifts, sink, &beta, rot_c, rot_s, pi, gamma,
+                           green, resid, scratch, coef, status, tolerance, min_dim, breakdown_tol, m_out,
+                           breakdown_out, worst_out, stream);
+}
+
+int c3l_arnoldi_batch(c3l_plan* const* plans, int32_t count, const double* diag, double* V, double* H, int64_t dim,
+                      int32_t m_max, const double* shifts, int32_t n_shifts, const double* sink, const double* beta,
+                      double* rot_c, double* rot_s, double* pi, double* gamma, double* green, double* resid,
+                      double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
+                      double breakdown_tol, int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream) {
+  c3l_arnoldi_run* run = nullptr;
+  D_CALL(c3l_arnoldi_begin(plans, count, diag, V, H, dim, m_max, shifts, n_shifts, sink, beta, rot_c, rot_s, pi, gamma,
+                           green, resid, scratch, coef, status, tolerance, min_dim, breakdown_tol, stream, &run));
+  const int rc = c3l_arnoldi_drive(&run, 1);
+  const int rc_end = c3l_arnoldi_end(run, m_out, breakdown_out, worst_out);
+  return rc != C3L_OK ? rc : rc_end;
+}
+
+int c3l_arnoldi_begin(c3l_plan* const* plans, int32_t count, const double* diag, double* V, double* H, int64_t dim,
+                      int32_t m_max, const double* shifts, int32_t n_shifts, const double* sink, const double* beta,
+                      double* rot_c, double* rot_s, double* pi, double* gamma, double* green, double* resid,
+                      double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
+                      double breakdown_tol, void* stream, c3l_arnoldi_run** out) {
+  if (out == nullptr) return dfail(C3L_EINVAL, "c3l_arnoldi_begin: invalid argument");
+  *out = nullptr;
+  c3l_arnoldi_run* r = new c3l_arnoldi_run();
+  const int rc = arnoldi_setup(r, plans, count, diag, V, H, dim, m_max, shifts, n_shifts, sink, beta, rot_c, rot_s,
+                               pi, gamma, green, resid, scratch, coef, status, tolerance, min_dim, breakdown_tol,
+                               stream);
+  if (rc != C3L_OK) {
+    delete r;
+    return rc;
+  }
+  *out = r;
+  return C3L_OK;
+}
+
+int c3l_arnoldi_drive(c3l_arnoldi_run* const* runs, int32_t n_runs) {
+  if (runs == nullptr || n_runs < 0) return dfail(C3L_EINVAL, "c3l_arnoldi_drive: invalid argument");
+  for (int32_t r = 0; r < n_runs; ++r)
+    if (runs[r] == nullptr) return dfail(C3L_EINVAL, "c3l_arnoldi_drive: invalid argument");
+  // one step of every unfinished resolvent in turn, so that all streams stay fed
+  for (bool any = true; any;) {
+    any = false;
+    for (int32_t r = 0; r < n_runs; ++r) {
+      if (runs[r]->done()) continue;
+      D_CALL(runs[r]->advance());
+      any = true;
     }
   }
-  if (host[2] == 3.0) return dfail(C3L_EINVAL, "c3l_arnoldi: nonfinite Arnoldi recurrence");
-  const int32_t m = (int32_t)host[3];
-  const int32_t broke = host[2] == 2.0 ? 1 : 0;
-  const double worst = host[1];
-  *m_out = m;
-  *breakdown_out = broke;
-  *worst_out = worst;
+  return C3L_OK;
+}
+
+int c3l_arnoldi_end(c3l_arnoldi_run* run, int32_t* m_out, int32_t* breakdown_out, double* worst_out) {
+  if (run == nullptr) return dfail(C3L_EINVAL, "c3l_arnoldi_end: invalid argument");
+  std::unique_ptr<c3l_arnoldi_run> owned(run);
+  if (m_out == nullptr || breakdown_out == nullptr || worst_out == nullptr)
+    return dfail(C3L_EINVAL, "c3l_arnoldi_end: invalid argument");
+  // the final flags (frozen after each basis's stop), read after the stream has drained
+  std::vector<double> host((size_t)run->count * 4);
+  D_CHECK(hipMemcpyAsync(host.data(), run->status, sizeof(double) * host.size(), hipMemcpyDeviceToHost, run->stream));
+  D_CHECK(hipStreamSynchronize(run->stream));
+  for (int32_t i = 0; i < run->count; ++i)
+    if (host[(size_t)i * 4 + 2] == 3.0) return dfail(C3L_EINVAL, "c3l_arnoldi: nonfinite Arnoldi recurrence");
+  for (int32_t i = 0; i < run->count; ++i) {
+    const double* h = host.data() + (size_t)i * 4;
+    m_out[i] = (int32_t)h[3];
+    breakdown_out[i] = h[2] == 2.0 ? 1 : 0;
+    worst_out[i] = h[1];
+  }
   return C3L_OK;
 }
 

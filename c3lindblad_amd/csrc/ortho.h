@@ -59,4 +59,18 @@ struct Clock {
 template <typename T, typename S>
 int step(const Basis<T, S>& b, int j, bool fresh, const T* w, double breakdown, hipStream_t stream, Clock* clock);
 
+// Per-instance pointers of a kernel argument: launches over several bases of equal
+// shape index them with blockIdx.z.
+constexpr int kMaxInstances = 4;
+template <typename P>
+struct Many {
+  P p[kMaxInstances];
+};
+
+// Step j of `count` bases of equal dim, ld and ldh at once (one launch per kernel,
+// blockIdx.z = basis); w[i] belongs to basis i and is null for all or for none.
+template <typename T, typename S>
+int step_many(const Basis<T, S>* b, int count, int j, bool fresh, const T* const* w, double breakdown,
+              hipStream_t stream, Clock* clock);
+
 }  // namespace c3l_ortho

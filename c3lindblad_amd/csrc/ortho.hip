@@ -113,10 +113,12 @@ struct Shape<double2> {
 // is sink . q (unconjugated, 0 without a sink) and its w-dot |w|^2.
 // Partials: s-side output k at partial[k*chunks + chunk], w-side at
 // partial[(nv+1+k)*chunks + chunk].
+// Kernel bodies are device functions with restrict parameters, inlined into thin
+// kernels that pick the instance's pointers (blockIdx.z).
 template <typename T, typename S, int ROWS>
-__global__ __launch_bounds__(256) void reduce_kernel(const S* __restrict__ V, int64_t ld, int nv, int cpw,
-                                                     const T* __restrict__ w, const T* __restrict__ sink,
-                                                     int64_t dim, T* __restrict__ partial, int64_t chunks) {
+__device__ __forceinline__ void reduce_body(const S* __restrict__ V, int64_t ld, int nv, int cpw,
+                                            const T* __restrict__ w, const T* __restrict__ sink, int64_t dim,
+                                            T* __restrict__ partial, int64_t chunks) {
   using N = Num<T>;
   constexpr int kPer = ROWS / 64;
   const int lane = threadIdx.x & 63;
@@ -187,9 +189,19 @@ __global__ __launch_bounds__(256) void reduce_kernel(const S* __restrict__ V, in
   }
 }
 
+template <typename T, typename S, int ROWS>
+__global__ __launch_bounds__(256) void reduce_kernel(Many<const S*> V, int64_t ld, int nv, int cpw, Many<const T*> w,
+                                                     Many<const T*> sink, int64_t dim, Many<T*> partial,
+                                                     int64_t chunks) {
+  const int i = blockIdx.z;
+  reduce_body<T, S, ROWS>(V.p[i], ld, nv, cpw, w.p[i], sink.p[i], dim, partial.p[i], chunks);
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void finish_kernel(const T* __restrict__ partial, int64_t chunks, T* __restrict__ out) {
+__global__ __launch_bounds__(256) void finish_kernel(Many<const T*> partials, int64_t chunks, Many<T*> outs) {
   using N = Num<T>;
+  const T* __restrict__ partial = partials.p[blockIdx.z];
+  T* __restrict__ out = outs.p[blockIdx.z];
   __shared__ T part[4];
   const int o = blockIdx.x;
   T acc = N::zero();
@@ -236,11 +248,10 @@ __device__ __forceinline__ double2 take(const double (&v)[6], int at, double2) {
 // splitting l; threads of one l read consecutive rows.  `stage` (optional) gets the final
 // column j-1 followed by its h0, contiguous for one copy to the host.
 template <typename T>
-__global__ __launch_bounds__(256) void column_kernel(T* __restrict__ H, int64_t ldh, int j, int fresh, int has_w,
-                                                     const T* __restrict__ red, T* __restrict__ sink_v,
-                                                     T* __restrict__ ab, const T* __restrict__ sc_prev,
-                                                     T* __restrict__ sc_cur, double* __restrict__ status,
-                                                     T* __restrict__ stage, double breakdown) {
+__device__ __forceinline__ void column_body(T* __restrict__ H, int64_t ldh, int j, int fresh, int has_w,
+                                            const T* __restrict__ red, T* __restrict__ sink_v, T* __restrict__ ab,
+                                            const T* __restrict__ sc_prev, T* __restrict__ sc_cur,
+                                            double* __restrict__ status, T* __restrict__ stage, double breakdown) {
   using N = Num<T>;
   __shared__ double shared[4][6];
   __shared__ T part[256 / kGroupRows][kGroupRows];
@@ -331,11 +342,20 @@ __global__ __launch_bounds__(256) void column_kernel(T* __restrict__ H, int64_t 
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void column_kernel(Many<T*> H, int64_t ldh, int j, int fresh, int has_w,
+                                                     Many<const T*> red, Many<T*> sink_v, Many<T*> ab, Many<T*> sc,
+                                                     Many<double*> status, Many<T*> stage, double breakdown) {
+  const int i = blockIdx.z;
+  column_body<T>(H.p[i], ldh, j, fresh, has_w, red.p[i], sink_v.p[i], ab.p[i], sc.p[i] + 8 * ((j + 1) & 1),
+                 sc.p[i] + 8 * (j & 1), status.p[i], stage.p[i], breakdown);
+}
+
 // 64 rows per workgroup, WAVES waves splitting the columns, partial sums met in LDS.
 template <typename T, typename S, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void update_kernel(S* __restrict__ V, int64_t ld, int j, int fresh,
-                                                            const T* __restrict__ w, const T* __restrict__ ab,
-                                                            int64_t ldh, const T* __restrict__ sc, int64_t dim) {
+__device__ __forceinline__ void update_body(S* __restrict__ V, int64_t ld, int j, int fresh, const T* __restrict__ w,
+                                            const T* __restrict__ ab, int64_t ldh, const T* __restrict__ sc,
+                                            int64_t dim) {
   using N = Num<T>;
   __shared__ T pa[WAVES][64];
   __shared__ T pb[WAVES][64];
@@ -369,6 +389,14 @@ __global__ __launch_bounds__(64 * WAVES) void update_kernel(S* __restrict__ V, i
   }
 }
 
+template <typename T, typename S, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void update_kernel(Many<S*> V, int64_t ld, int j, int fresh, Many<const T*> w,
+                                                            Many<const T*> ab, int64_t ldh, Many<const T*> sc,
+                                                            int64_t dim) {
+  const int i = blockIdx.z;
+  update_body<T, S, WAVES>(V.p[i], ld, j, fresh, w.p[i], ab.p[i], ldh, sc.p[i] + 8 * (j & 1), dim);
+}
+
 int fail(const std::string& msg) { return c3l_internal_fail(C3L_EINVAL, msg.c_str()); }
 
 int check_launch(const char* what) {
@@ -387,45 +415,83 @@ int64_t partial_entries(bool complex_basis, int64_t dim, int32_t max_columns) {
 
 template <typename T, typename S>
 int step(const Basis<T, S>& b, int j, bool fresh, const T* w, double breakdown, hipStream_t s, Clock* clock) {
-  if (b.V == nullptr || b.H == nullptr || b.partial == nullptr || b.red == nullptr ||
-      b.ab == nullptr || b.sc == nullptr || b.dim <= 0 || b.ld < b.dim || j < 0 ||
-      (w != nullptr ? j + 2 > b.ldh : j + 1 > b.ldh) ||
-      (j == 0 && !fresh))
+  return step_many(&b, 1, j, fresh, &w, breakdown, s, clock);
+}
+
+template <typename T, typename S>
+int step_many(const Basis<T, S>* bs, int count, int j, bool fresh, const T* const* ws, double breakdown,
+              hipStream_t s, Clock* clock) {
+  if (bs == nullptr || ws == nullptr || count < 1 || count > kMaxInstances || j < 0 || (j == 0 && !fresh))
     return fail("Krylov orthogonalisation step: invalid argument");
+  const Basis<T, S>& b = bs[0];
+  const bool has_w = ws[0] != nullptr;
+  for (int i = 0; i < count; ++i) {
+    const Basis<T, S>& o = bs[i];
+    if (o.V == nullptr || o.H == nullptr || o.partial == nullptr || o.red == nullptr || o.ab == nullptr ||
+        o.sc == nullptr || o.dim <= 0 || o.ld < o.dim || (has_w ? j + 2 > o.ldh : j + 1 > o.ldh) ||
+        (ws[i] != nullptr) != has_w || o.dim != b.dim || o.ld != b.ld || o.ldh != b.ldh)
+      return fail("Krylov orthogonalisation step: invalid argument");
+  }
   constexpr int ROWS = Shape<T>::kRows;
   constexpr int WAVES = Shape<T>::kWaves;
+  Many<const S*> Vr{};
+  Many<S*> Vw{};
+  Many<const T*> w{}, sink{}, partial_r{}, red_r{}, ab_r{}, sc_r{};
+  Many<T*> partial{}, red{}, H{}, sink_v{}, ab{}, sc{}, stage{};
+  Many<double*> status{};
+  for (int i = 0; i < count; ++i) {
+    const Basis<T, S>& o = bs[i];
+    Vr.p[i] = o.V;
+    Vw.p[i] = o.V;
+    w.p[i] = ws[i];
+    sink.p[i] = o.sink;
+    partial.p[i] = o.partial;
+    partial_r.p[i] = o.partial;
+    red.p[i] = o.red;
+    red_r.p[i] = o.red;
+    H.p[i] = o.H;
+    sink_v.p[i] = o.sink_v;
+    ab.p[i] = o.ab;
+    ab_r.p[i] = o.ab;
+    sc.p[i] = o.sc;
+    sc_r.p[i] = o.sc;
+    stage.p[i] = o.stage;
+    status.p[i] = o.status;
+  }
   const int nv = j + 1;
   const int64_t chunks = (b.dim + ROWS - 1) / ROWS;
   const int cols = nv + 1;
   const int cpw = (int)std::min<int64_t>(8, std::max<int64_t>(1, (int64_t)cols * chunks / 4096));
   const int groups = (cols + 4 * cpw - 1) / (4 * cpw);
-  const double vec = (double)sizeof(T) * (double)b.dim;   // w, sink
-  const double col = (double)sizeof(S) * (double)b.dim;   // basis columns
+  const double vec = (double)sizeof(T) * (double)b.dim;  // w, sink
+  const double col = (double)sizeof(S) * (double)b.dim;  // basis columns
+  const unsigned z = (unsigned)count;
   if (clock) clock->begin(0, s);
-  hipLaunchKernelGGL((reduce_kernel<T, S, ROWS>), dim3(chunks, groups), dim3(256), 0, s, b.V, b.ld, nv, cpw, w, b.sink,
-                     b.dim, b.partial, chunks);
+  hipLaunchKernelGGL((reduce_kernel<T, S, ROWS>), dim3(chunks, groups, z), dim3(256), 0, s, Vr, b.ld, nv, cpw, w, sink,
+                     b.dim, partial, chunks);
   if (int rc = check_launch("reduce_kernel")) return rc;
-  if (clock) clock->end(0, col * nv + vec * ((w ? 1 : 0) + (b.sink ? 1 : 0)), s);
-  const int outs = w ? 2 * (nv + 1) : nv + 1;
-  hipLaunchKernelGGL((finish_kernel<T>), dim3(outs), dim3(256), 0, s, b.partial, chunks, b.red);
+  if (clock) clock->end(0, count * (col * nv + vec * ((has_w ? 1 : 0) + (b.sink ? 1 : 0))), s);
+  const int outs = has_w ? 2 * (nv + 1) : nv + 1;
+  hipLaunchKernelGGL((finish_kernel<T>), dim3(outs, 1, z), dim3(256), 0, s, partial_r, chunks, red);
   if (int rc = check_launch("finish_kernel")) return rc;
-  T* sc_cur = b.sc + 8 * (j & 1);
-  const T* sc_prev = b.sc + 8 * ((j + 1) & 1);
   const int groups_h = std::max(1, (j + kGroupRows - 1) / kGroupRows);
-  hipLaunchKernelGGL((column_kernel<T>), dim3(groups_h), dim3(256), 0, s, b.H, b.ldh, j, fresh ? 1 : 0, w ? 1 : 0,
-                     b.red, b.sink_v, b.ab, sc_prev, sc_cur, b.status, b.stage, breakdown);
+  hipLaunchKernelGGL((column_kernel<T>), dim3(groups_h, 1, z), dim3(256), 0, s, H, b.ldh, j, fresh ? 1 : 0,
+                     has_w ? 1 : 0, red_r, sink_v, ab, sc, status, stage, breakdown);
   if (int rc = check_launch("column_kernel")) return rc;
   if (clock) clock->begin(1, s);
-  hipLaunchKernelGGL((update_kernel<T, S, WAVES>), dim3((b.dim + 63) / 64), dim3(64 * WAVES), 0, s, b.V, b.ld, j,
-                     fresh ? 1 : 0, w, b.ab, b.ldh, sc_cur, b.dim);
+  hipLaunchKernelGGL((update_kernel<T, S, WAVES>), dim3((b.dim + 63) / 64, 1, z), dim3(64 * WAVES), 0, s, Vw, b.ld, j,
+                     fresh ? 1 : 0, w, ab_r, b.ldh, sc_r, b.dim);
   if (int rc = check_launch("update_kernel")) return rc;
-  if (clock) clock->end(1, col * (j + 1 + (w ? 1 : 0) + (fresh ? 0 : 1)) + vec * (w ? 1 : 0), s);
+  if (clock) clock->end(1, count * (col * (j + 1 + (has_w ? 1 : 0) + (fresh ? 0 : 1)) + vec * (has_w ? 1 : 0)), s);
   return C3L_OK;
 }
 
-template int step<double, double>(const Basis<double, double>&, int, bool, const double*, double, hipStream_t, Clock*);
-template int step<double, float>(const Basis<double, float>&, int, bool, const double*, double, hipStream_t, Clock*);
-template int step<double2, double2>(const Basis<double2, double2>&, int, bool, const double2*, double, hipStream_t,
-                                    Clock*);
+#define C3L_ORTHO_INSTANTIATE(T, S)                                                                            \
+  template int step<T, S>(const Basis<T, S>&, int, bool, const T*, double, hipStream_t, Clock*);             \
+  template int step_many<T, S>(const Basis<T, S>*, int, int, bool, const T* const*, double, hipStream_t, Clock*);
+C3L_ORTHO_INSTANTIATE(double, double)
+C3L_ORTHO_INSTANTIATE(double, float)
+C3L_ORTHO_INSTANTIATE(double2, double2)
+#undef C3L_ORTHO_INSTANTIATE
 
 }  // namespace c3l_ortho

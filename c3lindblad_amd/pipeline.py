@@ -45,7 +45,7 @@ class SolvePipeline:
     policy: NewtonKrylovPolicy
     rtol: float = 1e-10
     concurrent: bool = True
-    eom_streams: int | None = None  # streams for the EOM resolvents (None: one per resolvent)
+    eom_streams: int | None = None  # streams for the EOM resolvent tasks (None: one per task)
 
     @classmethod
     def prepare(cls, model, omega, eta=0.0, charges=(1, -1), policy=None, rtol=1e-10):

@@ -5,12 +5,15 @@ and ``arnoldi_resolvent`` (``src/c3l/green/krylov.py:284-417``).
 
 One Arnoldi basis per source serves every frequency, because the Krylov space
 of ``(z - σ)`` does not depend on ``z``.  The native driver (``c3l_arnoldi``)
-builds the basis with two-pass classical Gram--Schmidt and, for every shift at
-once, runs GMRES on it: a Givens QR of ``(z I - H)`` advanced one column per
+builds the basis with classical Gram--Schmidt with delayed reorthogonalisation
+(one reduction and one update pass per step) and, for every shift at once, runs
+GMRES on it: a Givens QR of ``(z I - H)`` advanced one column per
 step and the sink projection ``G(z) = sink·x_m(z)`` updated incrementally.
 Every shift's true relative residual ``||b - (z - σ) x_m(z)|| / ||b||`` is
 exact at every step, so the basis stops growing exactly when the worst shift
-meets the tolerance.  The reference solves the same systems with one restarted
+meets the tolerance.  ``c3l_arnoldi_batch`` runs the addition and removal
+bases of a charge block in lockstep on twin plans, one launch per kernel for
+both.  The reference solves the same systems with one restarted
 GMRES per frequency (``component_response.py:72-128``); both converge to the
 same ``G^R`` within the stated tolerance.
 """
@@ -180,6 +183,152 @@ def arnoldi_resolvent(
     )
 
 
+class _ArnoldiRun:
+    """Device buffers and native run (``c3l_arnoldi_begin`` .. ``c3l_arnoldi_end``) of one
+    ``arnoldi_resolvent_batch``, started on the current stream."""
+
+    def __init__(self, operators, rhs, shifts, *, maximum_dimension, residual_tolerance, sinks,
+                 minimum_dimension=16, breakdown_tolerance=1.0e-14):
+        torch = _lib.require_device()
+        count = len(operators)
+        if not 1 <= count <= 4 or len(rhs) != count or len(sinks) != count:
+            raise ValueError("between one and four operators, right-hand sides and sinks are required")
+        contexts = {id(op.context) for op in operators}
+        slots = [op.slot for op in operators]
+        if len(contexts) != 1 or len(set(slots)) != count:
+            raise ValueError("operators must share one context on distinct plan slots")
+        n = int(operators[0].dimension)
+        z = np.ascontiguousarray(np.asarray(shifts, dtype=np.complex128))
+        if z.ndim != 1 or z.size == 0 or not np.all(np.isfinite(z)):
+            raise ValueError("shifts must be a finite nonempty vector")
+        if (isinstance(maximum_dimension, bool) or not isinstance(maximum_dimension, int)
+                or not 0 < maximum_dimension <= n):
+            raise ValueError("maximum_dimension must lie in [1, operator.dimension]")
+        if not (np.isfinite(residual_tolerance) and residual_tolerance > 0.0):
+            raise ValueError("residual_tolerance must be finite and positive")
+        b = np.stack([np.asarray(v, dtype=np.complex128) for v in rhs])
+        snk = np.stack([np.asarray(v, dtype=np.complex128) for v in sinks])
+        if (b.shape != (count, n) or snk.shape != (count, n)
+                or not (np.all(np.isfinite(b)) and np.all(np.isfinite(snk)))):
+            raise ValueError("right-hand sides and sinks must be finite vectors matching the operator")
+        beta = np.array([float(np.linalg.norm(v)) for v in b])  # rounded as in arnoldi_resolvent
+        if np.any(beta == 0.0):
+            raise ValueError("right-hand sides must be nonzero")
+        self.started = perf_counter()
+        self.operators, self.count, self.z, self.beta = operators, count, z, beta
+        self.tolerance = float(residual_tolerance)
+        m_max = maximum_dimension
+        dev = _lib.DEVICE
+        c128, f64 = torch.complex128, torch.float64
+        ns = z.size
+        self.V = torch.empty(count, m_max + 1, n, dtype=c128, device=dev)  # columns written before read
+        self.V[:, 0].copy_(torch.from_numpy(b / beta[:, None]))
+        self.H = torch.zeros(count, m_max, m_max + 1, dtype=c128, device=dev)
+        zs = torch.from_numpy(z).to(dev)
+        sink_d = torch.from_numpy(np.ascontiguousarray(snk)).to(dev)
+        rot_c = torch.empty(count, ns * m_max, dtype=f64, device=dev)
+        rot_s = torch.empty(count, ns * m_max, dtype=c128, device=dev)
+        pi = torch.empty(count, ns * m_max, dtype=c128, device=dev)
+        gamma = torch.zeros(count, ns, dtype=c128, device=dev)
+        self.green = torch.zeros(count, ns, dtype=c128, device=dev)
+        self.resid = torch.zeros(count, ns, dtype=f64, device=dev)
+        scratch_size, coef_size = _lib.krylov_workspace(True, n, m_max, ns)
+        scratch = torch.empty(count, scratch_size, dtype=f64, device=dev)
+        coef = torch.zeros(count, coef_size, dtype=f64, device=dev)
+        status = torch.zeros(count, 4, dtype=f64, device=dev)
+        self._keep = (zs, sink_d, rot_c, rot_s, pi, gamma, scratch, coef, status)  # alive until the run ends
+        states = [op.device for op in operators]
+        for state in states:
+            if state.plan.loaded.get(0) != state.token:
+                state.token = state.plan.load("T", state.t)
+        plans = (ctypes.c_void_p * count)(*[state.plan.handle for state in states])
+        beta_h = (ctypes.c_double * count)(*beta.tolist())
+        self.handle = ctypes.c_void_p()
+        _lib.call(
+            "c3l_arnoldi_begin", plans, count, _lib.dptr(states[0].diag), _lib.dptr(self.V), _lib.dptr(self.H), n,
+            m_max, _lib.dptr(zs), ns, _lib.dptr(sink_d), beta_h, _lib.dptr(rot_c), _lib.dptr(rot_s), _lib.dptr(pi),
+            _lib.dptr(gamma), _lib.dptr(self.green), _lib.dptr(self.resid), _lib.dptr(scratch), _lib.dptr(coef),
+            _lib.dptr(status), self.tolerance, int(minimum_dimension), float(breakdown_tolerance),
+            _lib.stream_ptr(torch), ctypes.byref(self.handle),
+        )
+
+    def finish(self) -> list[DeviceArnoldi]:
+        """End the run (waits for its stream) and return one projection per right-hand side."""
+
+        count = self.count
+        m_out = (ctypes.c_int32 * count)()
+        broke = (ctypes.c_int32 * count)()
+        worst = (ctypes.c_double * count)()
+        handle, self.handle = self.handle, None
+        _lib.call("c3l_arnoldi_end", handle, m_out, broke, worst)
+        green_h = self.green.cpu().numpy()
+        resid_h = self.resid.cpu().numpy()
+        wall = perf_counter() - self.started
+        out = []
+        for i, op in enumerate(self.operators):
+            m = int(m_out[i])
+            op.action_calls += m
+            out.append(DeviceArnoldi(
+                basis=self.V[i, : m + 1],
+                hessenberg=self.H[i, :m, : m + 1].cpu().numpy().T.copy(),
+                source_norm=float(self.beta[i]),
+                breakdown=bool(broke[i]),
+                shifts=self.z,
+                green=green_h[i].copy(),
+                relative_residuals=resid_h[i].copy(),
+                converged=bool(broke[i] or np.max(resid_h[i]) <= self.tolerance),
+                construction_wall_seconds=wall,
+                action_calls=m,
+            ))
+        return out
+
+
+def _drive(runs) -> None:
+    """Advance every run from one host thread, one Krylov step of each in turn, until all stop."""
+
+    handles = (ctypes.c_void_p * len(runs))(*[r.handle.value for r in runs])
+    _lib.call("c3l_arnoldi_drive", handles, len(runs))
+
+
+def _drive_and_finish(runs) -> list:
+    """``_drive`` then ``finish`` every run; every run is ended even when driving fails."""
+
+    try:
+        _drive(runs)
+    except BaseException:
+        for r in runs:
+            try:
+                r.finish()
+            except RuntimeError:
+                pass
+        raise
+    return [r.finish() for r in runs]
+
+
+def arnoldi_resolvent_batch(
+    operators,
+    rhs,
+    shifts,
+    *,
+    maximum_dimension: int,
+    residual_tolerance: float,
+    sinks,
+    minimum_dimension: int = 16,
+    breakdown_tolerance: float = 1.0e-14,
+) -> list[DeviceArnoldi]:
+    """``arnoldi_resolvent`` for several right-hand sides in lockstep (``c3l_arnoldi_batch``).
+
+    ``operators[i]`` is the same sigma operator bound to distinct plan slots (twin plans);
+    every sigma and orthogonalisation kernel of a step runs once over all bases that have
+    not stopped.  Each basis stops on its own residual test.
+    """
+
+    run = _ArnoldiRun(operators, rhs, shifts, maximum_dimension=maximum_dimension,
+                      residual_tolerance=residual_tolerance, sinks=sinks, minimum_dimension=minimum_dimension,
+                      breakdown_tolerance=breakdown_tolerance)
+    return _drive_and_finish([run])[0]
+
+
 @dataclass(frozen=True)
 class ComponentShiftedSolution:
     solution: np.ndarray
@@ -312,10 +461,7 @@ def run_concurrently(tasks, streams: int | None = None):
     return results
 
 
-def _channel(operator, source, sink, z, cap, rtol, limit, name):
-    if not np.any(source):
-        return np.zeros(z.size, dtype=np.complex128), None, np.zeros(z.size)
-    proj = arnoldi_resolvent(operator, source, z, maximum_dimension=cap, residual_tolerance=rtol, sink=sink)
+def _checked(proj, limit, name):
     if np.max(proj.relative_residuals) > limit:
         raise RuntimeError(
             f"component EOM resolvent did not converge ({name}: max residual "
@@ -324,27 +470,80 @@ def _channel(operator, source, sink, z, cap, rtol, limit, name):
     return proj.green, proj, proj.relative_residuals
 
 
+def _channel(operator, source, sink, z, cap, rtol, limit, name):
+    if not np.any(source):
+        return np.zeros(z.size, dtype=np.complex128), None, np.zeros(z.size)
+    proj = arnoldi_resolvent(operator, source, z, maximum_dimension=cap, residual_tolerance=rtol, sink=sink)
+    return _checked(proj, limit, name)
+
+
+def _block_runs(pairs, z, rtol, limit, maximum_dimension, streams):
+    """Addition and removal resolvents of every block in lockstep on plan slots 0 and 1,
+    block b on stream b % k, all driven from this host thread; results in pair order."""
+
+    torch = _lib.require_device()
+    main = torch.cuda.current_stream()
+    k = len(pairs) if streams is None else max(1, min(int(streams), len(pairs)))
+    pool = [torch.cuda.Stream() for _ in range(k)]
+    results = [(np.zeros(z.size, dtype=np.complex128), None, np.zeros(z.size))] * (2 * len(pairs))
+    runs, owners = [], []
+    try:
+        for b, (operator, sources) in enumerate(pairs):
+            live = [(slot, name, src) for slot, (name, src) in
+                    enumerate((("addition", sources.addition), ("removal", sources.removal))) if np.any(src)]
+            if not live:
+                continue
+            cap = maximum_dimension or min(operator.dimension, 2048)
+            stream = pool[b % k]
+            stream.wait_stream(main)
+            with torch.cuda.stream(stream):
+                runs.append(_ArnoldiRun([operator.on_slot(slot) for slot, _, _ in live], [src for _, _, src in live],
+                                        z, maximum_dimension=cap, residual_tolerance=rtol,
+                                        sinks=[sources.sink] * len(live)))
+            owners.append((b, live))
+    except BaseException:
+        for r in runs:
+            try:
+                r.finish()
+            except RuntimeError:
+                pass
+        raise
+    finished = _drive_and_finish(runs)
+    for stream in pool:
+        main.wait_stream(stream)
+    for (b, live), projs in zip(owners, finished):
+        for (slot, name, _), proj in zip(live, projs):
+            results[2 * b + slot] = _checked(proj, limit, name)
+    return results
+
+
 def retarded_green_blocks(pairs, omega, eta: float, *, rtol: float = 1.0e-10, maximum_dimension: int | None = None,
-                          concurrent: bool = True, streams: int | None = None) -> list["ComponentRetardedGreen"]:
+                          concurrent: bool = True, streams: int | None = None,
+                          batched: bool = True) -> list["ComponentRetardedGreen"]:
     """``component_retarded_green`` for several ``(operator, sources)`` charge blocks at once.
 
-    The addition and removal resolvents of every block run concurrently (plan
-    slots 0 and 1 of each block) when ``concurrent``, on ``streams`` streams (one
-    per resolvent by default).
+    With ``concurrent`` the addition and removal resolvents of a block use plan slots 0
+    and 1 and the blocks run on concurrent streams (``streams`` of them, one per block or
+    task by default): with ``batched`` both resolvents of a block advance in lockstep (one
+    launch per kernel for both) and one host thread drives every block, step by step;
+    otherwise every resolvent is a task of its own host thread.
     """
 
     started = perf_counter()
     frequency = np.asarray(omega, dtype=np.float64)
     z = frequency + 1j * float(eta)
     limit = max(10.0 * rtol, 1.0e-12)
-    tasks = []
-    for operator, sources in pairs:
-        cap = maximum_dimension or min(operator.dimension, 2048)
-        for slot, (name, src) in enumerate((("addition", sources.addition), ("removal", sources.removal))):
-            op = operator.on_slot(slot) if concurrent else operator
-            tasks.append(lambda op=op, src=src, snk=sources.sink, cap=cap, name=name:
-                         _channel(op, src, snk, z, cap, rtol, limit, name))
-    results = run_concurrently(tasks, streams) if concurrent else [t() for t in tasks]
+    if concurrent and batched:
+        results = _block_runs(pairs, z, rtol, limit, maximum_dimension, streams)
+    else:
+        tasks = []
+        for operator, sources in pairs:
+            cap = maximum_dimension or min(operator.dimension, 2048)
+            for slot, (name, src) in enumerate((("addition", sources.addition), ("removal", sources.removal))):
+                op = operator.on_slot(slot) if concurrent else operator
+                tasks.append(lambda op=op, src=src, snk=sources.sink, cap=cap, name=name:
+                             _channel(op, src, snk, z, cap, rtol, limit, name))
+        results = run_concurrently(tasks, streams) if concurrent else [t() for t in tasks]
     out = []
     for b, (operator, _) in enumerate(pairs):
         (ga, pa, ra), (gr, pr, rr) = results[2 * b], results[2 * b + 1]

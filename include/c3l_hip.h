@@ -76,7 +76,7 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
 /* Register one output block of program `program` (0 <= program < 8): `count` coordinates
  * of degree `deg` (coord[count*deg] local indices < n <= 255, dest[count] output positions)
  * and `n_entries` wedge entries: entry t = coeff[t] * prod_{f in [fstart[t], fstart[t+1])}
- * ws[foffset[f] + flat_f], flat_f = sum_s coord[pos_s] n^(rank-1-s) with
+ * ws[foffset[f] + flat_f], flat_f = sum_s coord[pos_s] n^s (first index fastest) with
  * fcode[f] = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9 (rank <= 4).  All blocks of a
  * program are evaluated by one launch (blocks of a program must not read each other's
  * destinations); registering a block invalidates that program's device image. */
@@ -208,13 +208,48 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
  * the first m >= min_dim with every shift's residual <= tolerance, at a breakdown, or at m_max.
  * The shifts absorb completed columns eight at a time; the stop test after each batch finds
  * the exact first column that meets the criterion and freezes the shift state there.  The
- * host reads it after every batch, so up to 16 further basis vectors may be computed (and
- * ignored) after the stop. */
+ * host reads the stop flag asynchronously (at most one batch late), so up to 16 further basis
+ * vectors may be computed (and ignored) after the stop. */
 int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
                 const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
                 double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
                 double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
                 int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
+
+/* c3l_arnoldi for `count` (1..4) right-hand sides at once, in lockstep: basis i runs on
+ * plans[i], and every sigma and orthogonalisation kernel of a step is one launch over all
+ * bases that have not stopped (the addition and removal resolvents of a charge block,
+ * src/c3l/cc/component_response.py:180-245, each run as its own c3l_arnoldi in the
+ * reference's per-channel loop).  The plans must be twins: uploaded from the same lowered
+ * plan (equal structure, checked), each with its own workspace and the same T atoms loaded;
+ * diag, shifts and the scalars are shared.  Every device buffer of c3l_arnoldi holds `count`
+ * consecutive per-basis blocks of the single-basis size (V, H, sink, rot_c, rot_s, pi, gamma,
+ * green, resid, scratch, coef, status); beta, m_out, breakdown_out and worst_out are host
+ * arrays of `count`.  Each basis stops on its own criterion and is dropped from the launches
+ * after the check that sees its stop. */
+int c3l_arnoldi_batch(c3l_plan* const* plans, int32_t count, const double* diag, double* V, double* H, int64_t dim,
+                      int32_t m_max, const double* shifts, int32_t n_shifts, const double* sink, const double* beta,
+                      double* rot_c, double* rot_s, double* pi, double* gamma, double* green, double* resid,
+                      double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
+                      double breakdown_tol, int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
+
+/* c3l_arnoldi_batch in three calls, so that one host thread can drive the resolvents of
+ * several charge blocks, each on its own stream (the reference runs them one after the
+ * other, component_response.py:180-245).  c3l_arnoldi_begin validates, initialises the shift
+ * state on `stream` and returns the run; c3l_arnoldi_drive advances every given run one
+ * Krylov step at a time in turn until each has stopped; c3l_arnoldi_end waits for the run's
+ * stream, writes m_out / breakdown_out / worst_out (count entries) and frees the run (also
+ * after an error).  Stop flags are read asynchronously, at most one shift batch late, so a
+ * stopped basis may compute up to 16 further (ignored) basis vectors.  The buffers must
+ * stay valid until c3l_arnoldi_end. */
+typedef struct c3l_arnoldi_run c3l_arnoldi_run;
+int c3l_arnoldi_begin(c3l_plan* const* plans, int32_t count, const double* diag, double* V, double* H, int64_t dim,
+                      int32_t m_max, const double* shifts, int32_t n_shifts, const double* sink, const double* beta,
+                      double* rot_c, double* rot_s, double* pi, double* gamma, double* green, double* resid,
+                      double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
+                      double breakdown_tol, void* stream, c3l_arnoldi_run** out);
+int c3l_arnoldi_drive(c3l_arnoldi_run* const* runs, int32_t n_runs);
+int c3l_arnoldi_end(c3l_arnoldi_run* run, int32_t* m_out, int32_t* breakdown_out, double* worst_out);
 
 /* Workspace sizes (in doubles) of c3l_arnoldi (complex_basis = 1, dim complex rows, m_max =
  * max_columns, n_shifts shifts) and c3l_gmres (complex_basis = 0, real_dim rows, restart =

@@ -498,3 +498,48 @@ def test_arnoldi_basis_is_orthonormal_and_satisfies_the_arnoldi_relation(torch_g
     assert np.max(np.abs(defect)) / np.max(np.abs(proj.hessenberg)) < 1e-12
     # first basis vector is the normalised source, untouched by the driver
     assert np.max(np.abs(V[0].cpu().numpy() - b / np.linalg.norm(b))) < 1e-15
+
+
+def test_batched_arnoldi_equals_separate_bases(torch_gpu):
+    """Lockstep bases on twin plans (one launch per kernel for both) reproduce the
+    separate single-basis runs bit for bit, also when one basis stops long before the
+    other: its source spans six eigenvectors of sigma, so its shifts converge (or the
+    basis breaks down) within a few steps and it is dropped from later launches."""
+
+    import torch
+
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, eom_ccsd_problem
+    from c3lindblad_amd.response import ComponentSigmaOperator, arnoldi_resolvent, arnoldi_resolvent_batch
+
+    print("case batched arnoldi n=8", flush=True)
+    p, sp = _static(8, 3.0, 9)
+    rng = np.random.default_rng(23)
+    t = p.tilde.unpack(0.02 * rng.normal(size=p.num_variables))
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    op = ComponentSigmaOperator(component_eom_context(ep, t))
+    P = ep.dimension
+    eye = torch.eye(P, dtype=torch.complex128, device="cuda")
+    A = torch.empty(P, P, dtype=torch.complex128, device="cuda")
+    for k in range(P):
+        op.matvec_device(eye[k].contiguous(), A[k])
+    A = A.cpu().numpy().T
+    _, vecs = np.linalg.eig(A)
+    b1 = rng.normal(size=P) + 1j * rng.normal(size=P)
+    b2 = vecs[:, :6] @ (rng.normal(size=6) + 1j * rng.normal(size=6))
+    sink = rng.normal(size=P) + 1j * rng.normal(size=P)
+    z = np.linspace(-2.0, 2.0, 33) + 0.3j
+    kw = dict(maximum_dimension=400, residual_tolerance=1e-10, minimum_dimension=8)
+    pair = arnoldi_resolvent_batch([op.on_slot(0), op.on_slot(1)], [b1, b2], z, sinks=[sink, sink], **kw)
+    alone = [arnoldi_resolvent(op.on_slot(s), b, z, sink=sink, **kw) for s, b in ((0, b1), (1, b2))]
+    print("dimensions", [q.dimension for q in pair], [q.dimension for q in alone], flush=True)
+    assert pair[1].dimension < pair[0].dimension
+    for got, want in zip(pair, alone):
+        assert got.dimension == want.dimension and got.breakdown == want.breakdown
+        assert np.array_equal(got.hessenberg, want.hessenberg)
+        assert np.array_equal(got.green, want.green)
+        assert np.array_equal(got.relative_residuals, want.relative_residuals)
+        assert torch.equal(got.basis, want.basis)
+    assert pair[0].converged and np.max(pair[0].relative_residuals) <= 1e-10
+    # the converged source's Green function equals the dense resolvent
+    want = np.array([sink @ np.linalg.solve(zz * np.eye(P) - A, b1) for zz in z])
+    assert _rel(pair[0].green, want) < 1e-8
