@@ -243,9 +243,10 @@ def main() -> int:
         torch.cuda.synchronize()
         elapsed = time.perf_counter() - start
         timed_solves = args.steps
-    # Kernel timing: one extra solve with the EOM channels run one after the other, every
-    # plan program and every driver orthogonalisation call bracketed by HIP events on its
-    # stream (events of concurrent streams would also count the other streams' work).
+    # Kernel timing: one extra solve with every launch on one stream (the lockstep EOM
+    # basis pairs of both charge blocks interleaved on it, as in the timed solves but not
+    # overlapped), every plan program and every driver orthogonalisation call bracketed by
+    # HIP events (events of concurrent streams would also count the other streams' work).
     serial_phases = None
     if args.serial_probe:
         pipe.concurrent = False
@@ -257,13 +258,13 @@ def main() -> int:
         plan.set_timing(True)
     lib.c3l_krylov_set_timing(1)
     KernelTimer.enabled = True
-    pipe.concurrent = False
+    pipe.eom_streams = 1
     torch.cuda.synchronize()
     instrumented = time.perf_counter()
     pipe.solve()
     torch.cuda.synchronize()
     instrumented = time.perf_counter() - instrumented
-    pipe.concurrent = True
+    pipe.eom_streams = args.eom_streams
     KernelTimer.enabled = False
     kernels = {name: d for name, d in KernelTimer.summary().items()}
     for plan in plans:
@@ -344,7 +345,7 @@ def main() -> int:
             "avg_launch_us": 1e3 * dk["ms"] / dk["launches"],
             "launches": dk["launches"],
             "share_of_step": dk["ms"] * 1e-3 / instrumented,
-            "measured_on": "one extra instrumented solve after the timed steps (EOM channels serial)",
+            "measured_on": "one extra instrumented solve after the timed steps (all launches on one stream)",
         },
         "kernels": {
             k: {"ms": round(v["ms"], 3), "launches": v["launches"],

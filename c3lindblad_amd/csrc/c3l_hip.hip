@@ -745,6 +745,7 @@ struct ProgramExec {
 
 struct PendingTiming {
   int32_t program;
+  int32_t instances;  // twin plans evaluated by the timed launch
   hipEvent_t start;
   hipEvent_t stop;
 };
@@ -922,7 +923,7 @@ int c3l_plan_program_time(c3l_plan* plan, int32_t program, double* ms_total, int
     float ms = 0.f;
     C3L_HIP_CHECK(hipEventElapsedTime(&ms, p.start, p.stop));
     plan->program_ms[p.program] += ms;
-    plan->program_launches[p.program] += 1;
+    plan->program_launches[p.program] += p.instances;
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);
   }
@@ -1413,7 +1414,7 @@ int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t 
   if (internal && pe.max_dest >= plan->workspace)
     return fail(C3L_EINVAL, "c3l_plan_apply: workspace program writes outside the workspace");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  PendingTiming t{program, nullptr, nullptr};
+  PendingTiming t{program, count, nullptr, nullptr};
   if (plan->timing) {
     C3L_HIP_CHECK(hipEventCreate(&t.start));
     C3L_HIP_CHECK(hipEventCreate(&t.stop));

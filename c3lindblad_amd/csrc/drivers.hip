@@ -123,35 +123,32 @@ __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
 }
 
 // ---------------------------------------------------------------------------
-// Multi-shift GMRES on one shared Arnoldi basis, one 64-lane wave per shift.
+// Multi-shift GMRES on one shared Arnoldi basis.
 //
 // For shift z the (j+2)-entry column j of (z I~ - H) is a_k = [k == j] z - H[k, j].
-// The earlier Givens rotations act on it as the affine recurrence
-//   x_{k+1} = -conj(s_k) x_k + c_k a_{k+1},   R[k, j] = c_k x_k + s_k a_{k+1},
-// which the wave evaluates 64 rotations at a time with a shuffle scan of affine
-// maps.  The solution is tracked only through its sink projection:
+// The earlier Givens rotations act on it as the recurrence
+//   x_{k+1} = -conj(s_k) x_k + c_k a_{k+1},   R[k, j] = c_k x_k + s_k a_{k+1}.
+// The solution is tracked only through its sink projection:
 //   pi_j = (sink.v_j - sum_{k<j} R[k, j] pi_k) / R[j, j],  G += gamma_j pi_j,
 // where gamma_j is the rotated right-hand side entry (GMRES with incremental
 // solution update, x_m = x_{m-1} + gamma_m p_m, p = V R^{-1}).  The GMRES
 // residual |gamma~| / beta is exact for every shift at every step.
 // Per-shift state (row s, length m_max): rot_c, rot_s, pi.
-__device__ __forceinline__ double2 shfl_up2(double2 v, int off) {
-  return make_double2(__shfl_up(v.x, off, 64), __shfl_up(v.y, off, 64));
-}
 __device__ __forceinline__ double2 shfl2(double2 v, int lane) {
   return make_double2(__shfl(v.x, lane, 64), __shfl(v.y, lane, 64));
 }
 __device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
-// Multi-shift GMRES over a batch of nb <= kShiftBatch completed columns c0..c0+nb-1, one
-// 64-lane wave per shift.  The rotations 0..c0-1 computed by earlier launches act on
-// all nb columns in one pass over the shift's state: the affine maps of a rotation
-// share their multiplier -conj(s_k) across columns, so one scan of the multipliers
-// serves nb scans of the offsets.  Lane 0 then applies the rotations of the batch
-// itself, column after column, and forms each new rotation, pi and Green increment.
-// The residual and Green value after every column go to resid_hist / green_hist
-// ([b][shift]) so that the stop test can take the state of the exact stop column.
+// Multi-shift GMRES over a batch of nb <= kShiftBatch completed columns c0..c0+nb-1:
+// kShiftBatch lanes per shift, lane b of a group owning column c0+b.  Each lane applies
+// the shift's rotations 0..c0-1 (computed by earlier launches) to its column in order,
+// accumulating sum_k R[k, c0+b] pi_k on the way; the group's first lane then applies the
+// rotations of the batch itself, column after column, and forms each new rotation, pi
+// and Green increment.  The residual and Green value after every column go to
+// resid_hist / green_hist ([b][shift]) so that the stop test can take the state of the
+// exact stop column.
 constexpr int kShiftBatch = 8;
+constexpr int kShiftsPerBlock = 256 / kShiftBatch;
 
 __device__ __forceinline__ void multishift_batch_body(const double2* __restrict__ H, int c0, int nb, int m_max,
                                                       const double2* __restrict__ shifts, int n_shifts,
@@ -162,77 +159,39 @@ __device__ __forceinline__ void multishift_batch_body(const double2* __restrict_
                                                       double* __restrict__ resid_hist,
                                                       double2* __restrict__ green_hist,
                                                       const double* __restrict__ status) {
+  if (status[2] != 0.0) return;  // uniform: nothing after the stop
   const int lane = threadIdx.x & 63;
-  const int s = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (s >= n_shifts || status[2] != 0.0) return;  // whole wave exits together; nothing after the stop
+  const int mine = lane % kShiftBatch;
+  const int s_raw = blockIdx.x * kShiftsPerBlock + (int)(threadIdx.x / kShiftBatch);
+  const bool valid = s_raw < n_shifts;
+  const int s = valid ? s_raw : n_shifts - 1;  // groups past the end repeat the last shift and write nothing
   const double2 z = shifts[s];
   const int64_t row = (int64_t)s * m_max;
   const int64_t ldh = m_max + 1;
+  const int cm = c0 + (mine < nb ? mine : 0);  // lanes past nb repeat column c0 (unused)
+  const double2* __restrict__ hc = H + (int64_t)cm * ldh;
+  double2 xm = make_double2(-hc[0].x + (cm == 0 ? z.x : 0.0), -hc[0].y + (cm == 0 ? z.y : 0.0));
+  double2 am = make_double2(0.0, 0.0);
+#pragma unroll 4
+  for (int k = 0; k < c0; ++k) {
+    const double c = rot_c[row + k];
+    const double2 sn = rot_s[row + k];
+    const double2 pk = pi[row + k];
+    const double2 h = hc[k + 1];
+    const double2 a1 = make_double2(-h.x + (k + 1 == cm ? z.x : 0.0), -h.y + (k + 1 == cm ? z.y : 0.0));
+    const double2 R = zadd(make_double2(c * xm.x, c * xm.y), zmul(sn, a1));
+    am = zadd(am, zmul(R, pk));
+    xm = zadd(zmul(make_double2(-sn.x, sn.y), xm), make_double2(c * a1.x, c * a1.y));
+  }
+  // the group's first lane collects every column's rotated entry and partial sum
+  const int first = lane - mine;
   double2 x[kShiftBatch], acc[kShiftBatch];
 #pragma unroll
   for (int b = 0; b < kShiftBatch; ++b) {
-    const int cb = c0 + b;
-    const double2 h = b < nb ? H[(int64_t)cb * ldh] : make_double2(0.0, 0.0);
-    x[b] = make_double2(-h.x + (cb == 0 ? z.x : 0.0), -h.y + (cb == 0 ? z.y : 0.0));
-    acc[b] = make_double2(0.0, 0.0);
+    x[b] = shfl2(xm, first + b);
+    acc[b] = shfl2(am, first + b);
   }
-  for (int base = 0; base < c0; base += 64) {
-    const int k = base + lane;
-    const bool on = k < c0;
-    double2 A = make_double2(1.0, 0.0);
-    double c = 1.0;
-    double2 sn = make_double2(0.0, 0.0), pk = make_double2(0.0, 0.0);
-    double2 a1[kShiftBatch], B[kShiftBatch];
-    if (on) {
-      c = rot_c[row + k];
-      sn = rot_s[row + k];
-      pk = pi[row + k];
-      A = make_double2(-sn.x, sn.y);  // -conj(s)
-    }
-#pragma unroll
-    for (int b = 0; b < kShiftBatch; ++b) {
-      const int cb = c0 + b;
-      a1[b] = make_double2(0.0, 0.0);
-      if (on && b < nb) {
-        const double2 h = H[(int64_t)cb * ldh + k + 1];
-        a1[b] = make_double2(-h.x + (k + 1 == cb ? z.x : 0.0), -h.y + (k + 1 == cb ? z.y : 0.0));
-      }
-      B[b] = make_double2(c * a1[b].x, c * a1[b].y);
-    }
-    // inclusive scan: lane l holds map_l o ... o map_base for every column
-    for (int off = 1; off < 64; off <<= 1) {
-      const double2 Ap = shfl_up2(A, off);
-#pragma unroll
-      for (int b = 0; b < kShiftBatch; ++b) {
-        const double2 Bp = shfl_up2(B[b], off);
-        if (lane >= off) B[b] = zadd(zmul(A, Bp), B[b]);
-      }
-      if (lane >= off) A = zmul(A, Ap);
-    }
-    double2 Ae = shfl_up2(A, 1);
-    if (lane == 0) Ae = make_double2(1.0, 0.0);
-    const int last = (c0 - base) < 64 ? (c0 - base - 1) : 63;
-    const double2 Al = shfl2(A, last);
-#pragma unroll
-    for (int b = 0; b < kShiftBatch; ++b) {
-      double2 Be = shfl_up2(B[b], 1);
-      if (lane == 0) Be = make_double2(0.0, 0.0);
-      const double2 xk = zadd(zmul(Ae, x[b]), Be);
-      if (on) {
-        const double2 Rkj = zadd(make_double2(c * xk.x, c * xk.y), zmul(sn, a1[b]));
-        acc[b] = zadd(acc[b], zmul(Rkj, pk));
-      }
-      const double2 Bl = shfl2(B[b], last);
-      x[b] = zadd(zmul(Al, x[b]), Bl);
-    }
-  }
-#pragma unroll
-  for (int b = 0; b < kShiftBatch; ++b)
-    for (int off = 32; off > 0; off >>= 1) {
-      acc[b].x += __shfl_down(acc[b].x, off, 64);
-      acc[b].y += __shfl_down(acc[b].y, off, 64);
-    }
-  if (lane != 0) return;
+  if (mine != 0 || !valid) return;
   double cr[kShiftBatch];
   double2 sr[kShiftBatch], pr[kShiftBatch];
   double2 g = gamma[s], G = green[s];
@@ -755,10 +714,9 @@ struct c3l_arnoldi_run {
     D_CALL(c3l_ortho::step_many(lb, k, j, j == 0, wz, breakdown, stream, &clock));
     if (j > 0 && (j - c0 == kShiftBatch || j == m_max)) {
       const int nb = j - c0;
-      const int waves_per_block = 4;
-      const int64_t shift_blocks = (n_shifts + waves_per_block - 1) / waves_per_block;
-      hipLaunchKernelGGL(multishift_batch_kernel, dim3(shift_blocks, 1, k), dim3(64 * waves_per_block), 0, stream, ls,
-                         c0, nb, m_max, shifts, n_shifts);
+      const int64_t shift_blocks = (n_shifts + kShiftsPerBlock - 1) / kShiftsPerBlock;
+      hipLaunchKernelGGL(multishift_batch_kernel, dim3(shift_blocks, 1, k), dim3(256), 0, stream, ls, c0, nb, m_max,
+                         shifts, n_shifts);
       D_CHECK(hipGetLastError());
       hipLaunchKernelGGL(arnoldi_batch_status_kernel, dim3(1, 1, k), dim3(1024), 0, stream, ls, n_shifts, m_max, c0,
                          nb, min_dim, tolerance, breakdown);

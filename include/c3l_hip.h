@@ -106,7 +106,8 @@ int c3l_plan_set_precision(c3l_plan* plan, int32_t kind, int32_t program, int32_
 /* Measurement hooks (no counterpart in the reference, which times with perf_counter,
  * cc/component_jax.py:53-65): when enabled, every program launch of c3l_plan_apply is
  * bracketed by HIP events on its stream; c3l_plan_program_time waits for them and
- * returns the accumulated kernel time and launch count of one program.
+ * returns the accumulated kernel time and launch count of one program (a launch that
+ * evaluates k twin plans at once, inside c3l_arnoldi_batch, counts k).
  * c3l_plan_block_info returns the metadata of one registered output block. */
 int c3l_plan_set_timing(c3l_plan* plan, int32_t enable);
 int c3l_plan_program_time(c3l_plan* plan, int32_t program, double* ms_total, int64_t* launches);
