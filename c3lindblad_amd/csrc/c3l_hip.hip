@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -188,40 +189,39 @@ __global__ void contract_kernel(Twins<C*> wss, const C* __restrict__ vectors, co
   }
 }
 
-// All contraction levels of one atom kind in a single workgroup (small kinds such
-// as the EOM input): levels run in order with a workgroup barrier between them, so
-// one launch replaces one launch per level.  Jobs of level l are
-// jobs[level_start[l] .. level_start[l+1]).
+// Contraction trees, one workgroup each (grid.x = tree): a tree is a root job (whose
+// parent atom exists before the launch) followed by every job that depends on it,
+// ordered by level; barrier[t] marks the first job of a new level, whose parents the
+// workgroup itself has just written.  Trees are independent, so one launch replaces
+// one launch per level for the small levels that follow the large first ones.
 template <typename C>
-__global__ __launch_bounds__(1024) void contract_levels_kernel(Twins<C*> wss, const C* __restrict__ vectors,
-                                                               const ContractJob* __restrict__ jobs,
-                                                               const int32_t* __restrict__ level_start, int n_levels,
-                                                               int n) {
+__global__ __launch_bounds__(256) void contract_trees_kernel(Twins<C*> wss, const C* __restrict__ vectors,
+                                                             const ContractJob* __restrict__ jobs,
+                                                             const int32_t* __restrict__ tree_start,
+                                                             const uint8_t* __restrict__ barrier, int n) {
   C* __restrict__ ws = wss.p[blockIdx.z];
-  for (int lv = 0; lv < n_levels; ++lv) {
-    for (int jb = level_start[lv]; jb < level_start[lv + 1]; ++jb) {
-      const ContractJob job = jobs[jb];
-      const int r = job.par_rank - 1;
-      int64_t out_size = 1;
-      for (int k = 0; k < r; ++k) out_size *= n;
-      int64_t inner = 1;
-      for (int k = 0; k < job.slot; ++k) inner *= n;  // stride of the contracted slot
-      const C* __restrict__ par = ws + job.par_offset;
-      const C* __restrict__ vec = vectors + (int64_t)job.vec * n;
-      C* __restrict__ out = ws + job.out_offset;
-      for (int64_t J = threadIdx.x; J < out_size; J += blockDim.x) {
-        const int64_t hi = J / inner;
-        const int64_t lo = J - hi * inner;
-        const int64_t base = hi * n * inner + lo;
-        C acc = contract_one(par + base, vec, inner, n);
-        if (job.sign < 0) {
-          acc.x = -acc.x;
-          acc.y = -acc.y;
-        }
-        out[J] = acc;
+  for (int t = tree_start[blockIdx.x]; t < tree_start[blockIdx.x + 1]; ++t) {
+    if (barrier[t]) __syncthreads();  // the parents written by this workgroup are complete
+    const ContractJob job = jobs[t];
+    const int r = job.par_rank - 1;
+    int64_t out_size = 1;
+    for (int k = 0; k < r; ++k) out_size *= n;
+    int64_t inner = 1;
+    for (int k = 0; k < job.slot; ++k) inner *= n;  // stride of the contracted slot
+    const C* __restrict__ par = ws + job.par_offset;
+    const C* __restrict__ vec = vectors + (int64_t)job.vec * n;
+    C* __restrict__ out = ws + job.out_offset;
+    for (int64_t J = threadIdx.x; J < out_size; J += blockDim.x) {
+      const int64_t hi = J / inner;
+      const int64_t lo = J - hi * inner;
+      const int64_t base = hi * n * inner + lo;
+      C acc = contract_one(par + base, vec, inner, n);
+      if (job.sign < 0) {
+        acc.x = -acc.x;
+        acc.y = -acc.y;
       }
+      out[J] = acc;
     }
-    __syncthreads();  // this level's atoms are the next level's parents
   }
 }
 
@@ -696,17 +696,18 @@ struct KindExec {
   bool built = false;
   std::vector<BaseRun> runs;
   std::vector<void*> owned;
-  // small kinds: every level in one single-workgroup launch
-  bool single = false;
-  int32_t n_levels = 0;
-  ContractJob* jobs = nullptr;
-  int32_t* level_start = nullptr;
+  // levels before tree_level run one launch each; the jobs of the later levels run as
+  // contraction trees in one launch (tree_level = number of levels: no trees)
+  int32_t tree_level = 0;
+  int32_t n_trees = 0;
+  ContractJob* tree_jobs = nullptr;
+  int32_t* tree_start = nullptr;
+  uint8_t* tree_barrier = nullptr;
 };
 
-// Complex MACs below which one workgroup runs all levels of a kind (measured: one
-// workgroup takes ~150 us for the 2e5 MACs of the n = 25 EOM input, three separate
-// level launches ~33 us, so only tiny kinds qualify).
-constexpr int64_t kSingleLevelMacs = 1 << 15;
+// Complex MACs up to which one workgroup evaluates a contraction tree (a rank-3 root
+// of n = 25, 15625 MACs, and its descendants qualify; a rank-4 root does not).
+constexpr int64_t kTreeMacs = 1 << 16;
 
 struct PlanLevel {
   int32_t kind;
@@ -830,6 +831,64 @@ void mix(c3l_plan* plan, const void* data, size_t bytes) {
 template <typename T>
 void mix_value(c3l_plan* plan, T v) {
   mix(plan, &v, sizeof(T));
+}
+
+// Splits the contraction levels of `kind` into per-level launches (levels before
+// ke.tree_level) and contraction trees: the first level from which every tree (a job
+// whose parent is not produced from that level on, with all its dependants) stays
+// within kTreeMacs.  Uploads the tree jobs in tree order.
+int build_trees(c3l_plan* plan, int32_t kind, KindExec& ke) {
+  std::vector<const PlanLevel*> lv;
+  for (const PlanLevel& l : plan->levels)
+    if (l.kind == kind && l.n_jobs > 0) lv.push_back(&l);
+  ke.tree_level = (int32_t)lv.size();
+  for (size_t first = 0; first < lv.size(); ++first) {
+    struct Node {
+      ContractJob job;
+      int32_t level;
+      int32_t root;
+    };
+    std::vector<Node> nodes;
+    std::map<int64_t, int32_t> producer;  // atom offset -> node that writes it
+    std::vector<int64_t> macs;            // per node: MACs of the tree it roots
+    bool fits = true;
+    for (size_t l = first; l < lv.size() && fits; ++l) {
+      for (const ContractJob& j : lv[l]->host_jobs) {
+        const auto it = producer.find(j.par_offset);
+        const int32_t id = (int32_t)nodes.size();
+        const int32_t root = it == producer.end() ? id : nodes[(size_t)it->second].root;
+        nodes.push_back(Node{j, (int32_t)l, root});
+        macs.push_back(0);
+        macs[(size_t)root] += ipow(plan->n, j.par_rank);
+        if (macs[(size_t)root] > kTreeMacs) fits = false;
+        producer[j.out_offset] = id;
+      }
+    }
+    if (!fits) continue;
+    // trees in root order, each ordered by level (nodes are already in level order)
+    std::vector<std::vector<int32_t>> members(nodes.size());
+    for (size_t i = 0; i < nodes.size(); ++i) members[(size_t)nodes[i].root].push_back((int32_t)i);
+    std::vector<ContractJob> jobs;
+    std::vector<int32_t> start{0};
+    std::vector<uint8_t> barrier;
+    for (const std::vector<int32_t>& tree : members) {
+      if (tree.empty()) continue;
+      for (size_t k = 0; k < tree.size(); ++k) {
+        const Node& nd = nodes[(size_t)tree[k]];
+        jobs.push_back(nd.job);
+        barrier.push_back(k > 0 && nd.level != nodes[(size_t)tree[k - 1]].level ? 1 : 0);
+      }
+      start.push_back((int32_t)jobs.size());
+    }
+    int rc = upload_owned(ke.owned, &ke.tree_jobs, jobs.data(), jobs.size());
+    if (rc == C3L_OK) rc = upload_owned(ke.owned, &ke.tree_start, start.data(), start.size());
+    if (rc == C3L_OK) rc = upload_owned(ke.owned, &ke.tree_barrier, barrier.data(), barrier.size());
+    if (rc != C3L_OK) return rc;
+    ke.tree_level = (int32_t)first;
+    ke.n_trees = (int32_t)start.size() - 1;
+    break;
+  }
+  return C3L_OK;
 }
 
 }  // namespace
@@ -1128,25 +1187,8 @@ int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t k
       ke.runs.push_back(run);
       i = j;
     }
-    // all levels of this kind in one launch when the total work is small
-    std::vector<ContractJob> all;
-    std::vector<int32_t> starts{0};
-    int64_t macs = 0;
-    for (const PlanLevel& l : plan->levels) {
-      if (l.kind != kind || l.n_jobs == 0) continue;
-      for (const ContractJob& j : l.host_jobs) {
-        all.push_back(j);
-        macs += ipow(plan->n, j.par_rank);
-      }
-      starts.push_back((int32_t)all.size());
-    }
-    if (!all.empty() && macs <= kSingleLevelMacs) {
-      int rc = upload_owned(ke.owned, &ke.jobs, all.data(), all.size());
-      if (rc == C3L_OK) rc = upload_owned(ke.owned, &ke.level_start, starts.data(), starts.size());
-      if (rc != C3L_OK) return rc;
-      ke.single = true;
-      ke.n_levels = (int32_t)starts.size() - 1;
-    }
+    int rc = build_trees(plan, kind, ke);
+    if (rc != C3L_OK) return rc;
     ke.built = true;
   }
   const bool narrow = plan->kind_bits[kind] == 32;
@@ -1176,23 +1218,25 @@ int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t k
   }
   if (!narrow)
     for (int32_t i = 0; i < count; ++i) ++plans[i]->fixed_version;
-  if (ke.single) {
-    if (narrow)
-      hipLaunchKernelGGL(contract_levels_kernel<float2>, inst, dim3(1024), 0, s, ws32, plan->vectors32, ke.jobs,
-                         ke.level_start, ke.n_levels, plan->n);
-    else
-      hipLaunchKernelGGL(contract_levels_kernel<double2>, inst, dim3(1024), 0, s, ws, plan->vectors, ke.jobs,
-                         ke.level_start, ke.n_levels, plan->n);
-    C3L_HIP_CHECK(hipGetLastError());
-    return C3L_OK;
-  }
+  int32_t level = 0;
   for (const PlanLevel& l : plan->levels) {
     if (l.kind != kind || l.n_jobs == 0) continue;
+    if (level++ >= ke.tree_level) break;
     const dim3 grid(grid_for(l.max_out), l.n_jobs, count);
     if (narrow)
       hipLaunchKernelGGL(contract_kernel<float2>, grid, dim3(kThreads), 0, s, ws32, plan->vectors32, l.jobs, plan->n);
     else
       hipLaunchKernelGGL(contract_kernel<double2>, grid, dim3(kThreads), 0, s, ws, plan->vectors, l.jobs, plan->n);
+    C3L_HIP_CHECK(hipGetLastError());
+  }
+  if (ke.n_trees > 0) {
+    const dim3 grid(ke.n_trees, 1, count);
+    if (narrow)
+      hipLaunchKernelGGL(contract_trees_kernel<float2>, grid, dim3(kThreads), 0, s, ws32, plan->vectors32,
+                         ke.tree_jobs, ke.tree_start, ke.tree_barrier, plan->n);
+    else
+      hipLaunchKernelGGL(contract_trees_kernel<double2>, grid, dim3(kThreads), 0, s, ws, plan->vectors, ke.tree_jobs,
+                         ke.tree_start, ke.tree_barrier, plan->n);
     C3L_HIP_CHECK(hipGetLastError());
   }
   return C3L_OK;

@@ -48,12 +48,16 @@ def test_residual_and_jvp_match_polynomial_oracle(torch_gpu, n, U, seed):
     assert _rel(component_static_jvp(sp, t, dt).retained, J) < RTOL
 
 
-def test_residual_and_jvp_match_port_n7(torch_gpu):
+@pytest.mark.parametrize("n", [7, 17])
+def test_residual_and_jvp_match_port(torch_gpu, n):
+    """n = 17: rank-4 contractions exceed one workgroup's tree, so the first contraction
+    level runs as its own launch and the later levels as contraction trees."""
+
     from c3lindblad_amd.static import component_static_jvp, component_static_residual
     from oracle.port import PortPlan
 
-    print("case residual/jvp port n=7", flush=True)
-    p, sp = _static(7, 2.0, 5)
+    print(f"case residual/jvp port n={n}", flush=True)
+    p, sp = _static(n, 2.0, 5)
     rng = np.random.default_rng(12)
     t = p.tilde.unpack(0.02 * rng.normal(size=p.num_variables))
     dt = p.tilde.unpack(rng.normal(size=p.num_variables))
