@@ -33,6 +33,28 @@ namespace {
 
 constexpr int kGroupRows = 16;  // Hessenberg rows per column_kernel workgroup
 
+// Sum over the 64 lanes of a wave with DPP moves (no LDS traffic), valid in lane 63:
+// an inclusive scan inside each row of 16 lanes (row_shr 1, 2, 4, 8, zero-filled at
+// the row start), then the row totals carried forward with row_bcast 15 (rows 1, 3)
+// and row_bcast 31 (rows 2, 3).
+template <int CTRL, int ROW_MASK, bool ZERO_FILL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int2 b = __builtin_bit_cast(int2, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, b.x, CTRL, ROW_MASK, 0xf, ZERO_FILL);
+  const int hi = __builtin_amdgcn_update_dpp(0, b.y, CTRL, ROW_MASK, 0xf, ZERO_FILL);
+  return __builtin_bit_cast(double, make_int2(lo, hi));
+}
+
+__device__ __forceinline__ double wave_total(double v) {
+  v += dpp_move<0x111, 0xf, true>(v);  // row_shr:1
+  v += dpp_move<0x112, 0xf, true>(v);  // row_shr:2
+  v += dpp_move<0x114, 0xf, true>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xf, true>(v);  // row_shr:8
+  v += dpp_move<0x142, 0xa, false>(v);  // row_bcast:15 into rows 1 and 3 (others add 0)
+  v += dpp_move<0x143, 0xc, false>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 template <typename T>
 struct Num;
 
@@ -52,6 +74,7 @@ struct Num<double> {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     return v;
   }
+  static __device__ __forceinline__ double wave_total(double v) { return c3l_ortho::wave_total(v); }
 };
 
 template <>
@@ -78,6 +101,9 @@ struct Num<double2> {
       v.y += __shfl_down(v.y, off, 64);
     }
     return v;
+  }
+  static __device__ __forceinline__ double2 wave_total(double2 v) {
+    return make_double2(c3l_ortho::wave_total(v.x), c3l_ortho::wave_total(v.y));
   }
 };
 
@@ -157,11 +183,11 @@ __device__ __forceinline__ void reduce_body(const S* __restrict__ V, int64_t ld,
       as1 = N::cfma(v1[t], qv[t], as1);
       az1 = N::cfma(v1[t], wv[t], az1);
     }
-    as0 = N::wave_sum(as0);
-    az0 = N::wave_sum(az0);
-    as1 = N::wave_sum(as1);
-    az1 = N::wave_sum(az1);
-    if (lane == 0) {
+    as0 = N::wave_total(as0);
+    az0 = N::wave_total(az0);
+    as1 = N::wave_total(as1);
+    az1 = N::wave_total(az1);
+    if (lane == 63) {
       partial[(int64_t)k0 * chunks + blockIdx.x] = as0;
       if (w != nullptr) partial[(int64_t)(nv + 1 + k0) * chunks + blockIdx.x] = az0;
       if (two) {
@@ -180,9 +206,9 @@ __device__ __forceinline__ void reduce_body(const S* __restrict__ V, int64_t ld,
       if (sink != nullptr && i < dim) as = N::fma(sink[i], qv[t], as);
       az = N::cfma(wv[t], wv[t], az);
     }
-    as = N::wave_sum(as);
-    az = N::wave_sum(az);
-    if (lane == 0) {
+    as = N::wave_total(as);
+    az = N::wave_total(az);
+    if (lane == 63) {
       partial[(int64_t)nv * chunks + blockIdx.x] = as;
       if (w != nullptr) partial[(int64_t)(2 * nv + 1) * chunks + blockIdx.x] = az;
     }
