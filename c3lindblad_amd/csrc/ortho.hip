@@ -70,10 +70,6 @@ struct Num<double> {
   static __device__ __forceinline__ double abs2(double a) { return a * a; }
   static __device__ __forceinline__ double fma(double a, double b, double acc) { return acc + a * b; }   // acc + a b
   static __device__ __forceinline__ double cfma(double a, double b, double acc) { return acc + a * b; }  // acc + conj(a) b
-  static __device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-  }
   static __device__ __forceinline__ double wave_total(double v) { return c3l_ortho::wave_total(v); }
 };
 
@@ -94,13 +90,6 @@ struct Num<double2> {
   }
   static __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
     return make_double2(acc.x + a.x * b.x + a.y * b.y, acc.y + a.x * b.y - a.y * b.x);
-  }
-  static __device__ __forceinline__ double2 wave_sum(double2 v) {
-    for (int off = 32; off > 0; off >>= 1) {
-      v.x += __shfl_down(v.x, off, 64);
-      v.y += __shfl_down(v.y, off, 64);
-    }
-    return v;
   }
   static __device__ __forceinline__ double2 wave_total(double2 v) {
     return make_double2(c3l_ortho::wave_total(v.x), c3l_ortho::wave_total(v.y));
@@ -232,18 +221,17 @@ __global__ __launch_bounds__(256) void finish_kernel(Many<const T*> partials, in
   const int o = blockIdx.x;
   T acc = N::zero();
   for (int64_t c = threadIdx.x; c < chunks; c += 256) acc = N::add(acc, partial[(int64_t)o * chunks + c]);
-  acc = N::wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  acc = N::wave_total(acc);
+  if ((threadIdx.x & 63) == 63) part[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) out[o] = N::add(N::add(part[0], part[1]), N::add(part[2], part[3]));
 }
 
 // Sums of six doubles over the workgroup with one barrier (every thread gets them).
 __device__ void block_sum6(double (&v)[6], double (*shared)[6]) {
-  for (int q = 0; q < 6; ++q)
-    for (int off = 32; off > 0; off >>= 1) v[q] += __shfl_down(v[q], off, 64);
+  for (int q = 0; q < 6; ++q) v[q] = wave_total(v[q]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0)
+  if (lane == 63)
     for (int q = 0; q < 6; ++q) shared[wave][q] = v[q];
   __syncthreads();
   for (int q = 0; q < 6; ++q) {
