@@ -161,7 +161,37 @@ __device__ __forceinline__ C contract_one(const C* __restrict__ par, const C* __
   return acc;
 }
 
-// grid.y = job; threads cover the n^(R-1) outputs of that job.
+// Sum over aligned groups of 8 lanes with DPP moves, valid in the last lane of each
+// group: an inclusive scan inside rows of 16 (row_shr 1, 2, 4, zero filled at the row start).
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr(double v) {
+  const int2 b = __builtin_bit_cast(int2, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, b.x, CTRL, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, b.y, CTRL, 0xf, 0xf, true);
+  return __builtin_bit_cast(double, make_int2(lo, hi));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_shr(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
+template <typename R>
+__device__ __forceinline__ R group8_total(R v) {
+  v += dpp_shr<0x111>(v);  // row_shr:1
+  v += dpp_shr<0x112>(v);  // row_shr:2
+  v += dpp_shr<0x114>(v);  // row_shr:4
+  return v;
+}
+__device__ __forceinline__ double2 group8_total(double2 v) { return make_double2(group8_total(v.x), group8_total(v.y)); }
+__device__ __forceinline__ float2 group8_total(float2 v) { return make_float2(group8_total(v.x), group8_total(v.y)); }
+
+// Lanes per output of a contraction job: a job over the stride-1 slot (slot 0) reads n
+// consecutive elements per output, so eight lanes share one output (lane l takes the
+// terms q = l, l+8, ...) and the wave reads eight runs of consecutive elements instead of
+// 64 strided ones; other slots are read coalesced with one output per thread.
+__host__ __device__ __forceinline__ int contract_lanes(int slot, int n) { return slot == 0 && n >= 8 ? 8 : 1; }
+
+// grid.y = job; threads cover the n^(R-1) outputs of that job (eight threads per output
+// for slot-0 jobs).
 template <typename C>
 __global__ void contract_kernel(Twins<C*> wss, const C* __restrict__ vectors, const ContractJob* __restrict__ jobs,
                                 int n) {
@@ -175,6 +205,26 @@ __global__ void contract_kernel(Twins<C*> wss, const C* __restrict__ vectors, co
   const C* __restrict__ par = ws + job.par_offset;
   const C* __restrict__ vec = vectors + (int64_t)job.vec * n;
   C* __restrict__ out = ws + job.out_offset;
+  if (contract_lanes(job.slot, n) == 8) {  // uniform over the launch's job
+    // every lane of a group runs the same iterations (groups are whole: 8 | 8 out_size)
+    for (int64_t T = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; T < out_size * 8;
+         T += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t J = T >> 3;
+      const int l = (int)(T & 7);
+      const C* __restrict__ p = par + J * n;
+      C acc = czero<C>();
+      for (int q = l; q < n; q += 8) acc = cadd(acc, cmul(vec[q], p[q]));
+      acc = group8_total(acc);
+      if (l == 7) {
+        if (job.sign < 0) {
+          acc.x = -acc.x;
+          acc.y = -acc.y;
+        }
+        out[J] = acc;
+      }
+    }
+    return;
+  }
   for (int64_t J = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; J < out_size;
        J += (int64_t)gridDim.x * blockDim.x) {
     const int64_t hi = J / inner;
@@ -1037,7 +1087,7 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
                        const int32_t* sign) {
   if (plan == nullptr || n_jobs < 0) return fail(C3L_EINVAL, "c3l_plan_add_level: invalid argument");
   std::vector<ContractJob> jobs((size_t)n_jobs);
-  int64_t max_out = 1;
+  int64_t max_out = 1;  // threads the level's largest job needs
   for (int32_t j = 0; j < n_jobs; ++j) {
     if (par_rank[j] < 1 || par_rank[j] > 4 || slot[j] < 0 || slot[j] >= par_rank[j] || vec[j] < 0 ||
         vec[j] >= plan->n_vectors)
@@ -1047,7 +1097,8 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
         par_offset[j] + ipow(plan->n, par_rank[j]) > plan->workspace)
       return fail(C3L_EINVAL, "c3l_plan_add_level: job exceeds workspace");
     jobs[(size_t)j] = ContractJob{out_offset[j], par_offset[j], par_rank[j], slot[j], vec[j], sign[j]};
-    if (outsz > max_out) max_out = outsz;
+    const int64_t threads = outsz * contract_lanes(slot[j], plan->n);
+    if (threads > max_out) max_out = threads;
   }
   mix_value(plan, (int32_t)2);
   mix_value(plan, kind);
@@ -1144,6 +1195,21 @@ int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t k
                                 void* stream);
 int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t program, const double* diag,
                                  const double* const* x, double* const* out, void* stream);
+
+// 1 when every plan of the list was built from the same arrays (equal digests) and all
+// plans are FP64 in kind `kind` and program `program`, else 0.
+int c3l_internal_plans_are_twins(c3l_plan* const* plans, int32_t count, int32_t kind, int32_t program) {
+  if (plans == nullptr || count < 1 || count > kMaxTwins || kind < 0 || kind >= kKinds || program < 0 ||
+      program >= kPrograms)
+    return 0;
+  for (int32_t i = 0; i < count; ++i) {
+    if (plans[i] == nullptr) return 0;
+    if (count > 1 && (plans[i]->digest != plans[0]->digest || plans[i]->kind_bits[kind] != 64 ||
+                      plans[i]->program_bits[program] != 64))
+      return 0;
+  }
+  return 1;
+}
 
 int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
   return c3l_internal_plan_load_many(&plan, 1, kind, &v, stream);

@@ -40,6 +40,7 @@ extern "C" int c3l_internal_narrow_scale(const double* x, float* y, double a, in
 extern "C" int c3l_internal_rmultiaxpy_f32(const float* V, int64_t ld, int32_t ncols, const double* alpha,
                                            double scale, double* w, int64_t dim, void* stream);
 // Twin plans (equal digests) loaded / applied together, one launch per kernel.
+extern "C" int c3l_internal_plans_are_twins(c3l_plan* const* plans, int32_t count, int32_t kind, int32_t program);
 extern "C" int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t kind,
                                            const double* const* v, void* stream);
 extern "C" int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t program,
@@ -749,6 +750,8 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
     return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
   for (int32_t i = 0; i < count; ++i)
     if (plans[i] == nullptr || !(beta[i] > 0.0)) return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
+  if (!c3l_internal_plans_are_twins(plans, count, 2, 3))
+    return dfail(C3L_EINVAL, "c3l_arnoldi: the bases' plans must be FP64 twins (built from the same arrays)");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const Layout L = arnoldi_layout(dim, m_max, n_shifts);
   // per-basis strides (doubles) of the caller's consecutive instance blocks

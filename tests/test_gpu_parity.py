@@ -547,3 +547,51 @@ def test_batched_arnoldi_equals_separate_bases(torch_gpu):
     # the converged source's Green function equals the dense resolvent
     want = np.array([sink @ np.linalg.solve(zz * np.eye(P) - A, b1) for zz in z])
     assert _rel(pair[0].green, want) < 1e-8
+
+
+def test_batched_arnoldi_rejects_plans_that_are_not_twins(torch_gpu):
+    """The lockstep driver refuses bases whose plans were built from different arrays (two
+    EOM problems of equal dimension), before it launches anything."""
+
+    import ctypes
+
+    import torch
+
+    from c3lindblad_amd import _lib
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, eom_ccsd_problem
+
+    print("case batched arnoldi non-twin plans", flush=True)
+    ops = []
+    for seed in (31, 32):
+        p, sp = _static(3, 1.0, seed)
+        ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+        ops.append(component_eom_context(ep, np.zeros(p.num_amplitudes, complex)).device)
+    n = ops[0].dim
+    assert ops[1].dim == n and ops[0].plan.handle != ops[1].plan.handle
+    m, ns, count = 4, 3, 2
+    scratch_size, coef_size = _lib.krylov_workspace(True, n, m, ns)
+    dev = "cuda"
+    V = torch.zeros(count, m + 1, n, dtype=torch.complex128, device=dev)
+    V[:, 0, 0] = 1.0
+    H = torch.zeros(count, m, m + 1, dtype=torch.complex128, device=dev)
+    zs = torch.full((ns,), 1j, dtype=torch.complex128, device=dev)
+    sink = torch.ones(count, n, dtype=torch.complex128, device=dev)
+    rot_c = torch.zeros(count, ns * m, dtype=torch.float64, device=dev)
+    rot_s = torch.zeros(count, ns * m, dtype=torch.complex128, device=dev)
+    pi = torch.zeros(count, ns * m, dtype=torch.complex128, device=dev)
+    gamma = torch.zeros(count, ns, dtype=torch.complex128, device=dev)
+    green = torch.zeros(count, ns, dtype=torch.complex128, device=dev)
+    resid = torch.zeros(count, ns, dtype=torch.float64, device=dev)
+    scratch = torch.zeros(count, scratch_size, dtype=torch.float64, device=dev)
+    coef = torch.zeros(count, coef_size, dtype=torch.float64, device=dev)
+    status = torch.zeros(count, 4, dtype=torch.float64, device=dev)
+    plans = (ctypes.c_void_p * count)(*[op.plan.handle for op in ops])
+    beta = (ctypes.c_double * count)(1.0, 1.0)
+    m_out, broke, worst = (ctypes.c_int32 * count)(), (ctypes.c_int32 * count)(), (ctypes.c_double * count)()
+    with pytest.raises(ValueError, match="twins"):
+        _lib.call("c3l_arnoldi_batch", plans, count, _lib.dptr(ops[0].diag), _lib.dptr(V), _lib.dptr(H), n, m,
+                  _lib.dptr(zs), ns, _lib.dptr(sink), beta, _lib.dptr(rot_c), _lib.dptr(rot_s), _lib.dptr(pi),
+                  _lib.dptr(gamma), _lib.dptr(green), _lib.dptr(resid), _lib.dptr(scratch), _lib.dptr(coef),
+                  _lib.dptr(status), 1e-10, 1, 1e-14, m_out, broke, worst, _lib.stream_ptr(torch))
+    torch.cuda.synchronize()
+    assert float(status.abs().sum()) == 0.0  # nothing was launched
