@@ -312,7 +312,11 @@ def main() -> int:
             "variant": "fp64_gpu" if args.precision == "fp64" else "mixed_fp32_jvp",
             "newton_policy": {"forcing": args.forcing, "gmres_restart": args.restart,
                               "jacobian_precision": args.precision,
-                              "minimum_gmres_relative_tolerance": policy.minimum_gmres_relative_tolerance},
+                              "minimum_gmres_relative_tolerance": policy.minimum_gmres_relative_tolerance,
+                              "maximum_gmres_relative_tolerance": policy.maximum_gmres_relative_tolerance,
+                              "initial_gmres_relative_tolerance": policy.initial_gmres_relative_tolerance,
+                              "eisenstat_walker_gamma": policy.eisenstat_walker_gamma,
+                              "eisenstat_walker_exponent": policy.eisenstat_walker_exponent},
             "n_sites": model.n_sites,
             "ccsd_amplitudes": pipe.static.oracle.num_amplitudes,
             "eom_dimension_per_charge": {str(q): b.dimension for q, b in pipe.eom.items()},
