@@ -33,7 +33,12 @@ from .eom import (
     component_eom_sources,
     eom_ccsd_problem,
 )
-from .response import arnoldi_resolvent, component_retarded_green, solve_component_shifted
+from .response import (
+    arnoldi_resolvent,
+    component_retarded_green,
+    component_retarded_matrix_green_shifts,
+    solve_component_shifted,
+)
 from .observables import (
     ComponentObservableProblem,
     contact_currents,
@@ -58,6 +63,7 @@ __all__ = [
     "contact_currents",
     "factorized_ccsd_covariance",
     "component_retarded_green",
+    "component_retarded_matrix_green_shifts",
     "component_static_jvp",
     "component_static_linearization",
     "component_static_linearized_jvp",

@@ -595,3 +595,52 @@ def test_batched_arnoldi_rejects_plans_that_are_not_twins(torch_gpu):
                   _lib.dptr(status), 1e-10, 1, 1e-14, m_out, broke, worst, _lib.stream_ptr(torch))
     torch.cuda.synchronize()
     assert float(status.abs().sum()) == 0.0  # nothing was launched
+
+
+def test_matrix_green_at_complex_shifts_matches_dense_oracle(torch_gpu):
+    """Impurity Green matrix over two source sites at real-axis (omega + i eta) and
+    imaginary-axis (i omega_n) shifts sharing one Krylov space per source and channel,
+    against dense resolvents of the polynomial oracle's sigma matrix (1e-8 relative)."""
+
+    from c3lindblad_amd import component_retarded_matrix_green_shifts
+    from c3lindblad_amd.configs import readme_model
+    from c3lindblad_amd.eom import (
+        ComponentEOMProblem,
+        ComponentSigmaOperator,
+        component_eom_context,
+        component_eom_sources,
+        eom_ccsd_problem,
+    )
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.solver import solve_component_ccsd
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    print("case matrix green at complex shifts", flush=True)
+    p = factorized_cc_problem(readme_model())
+    sp = ComponentStaticProblem.compile(p)
+    sol = solve_component_ccsd(sp)
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    ctx = component_eom_context(ep, sol.amplitudes)
+    sources = [component_eom_sources(ctx, site=s) for s in (0, 1)]
+    beta = 10.0
+    z = np.concatenate([np.array([-1.0 + 0.2j, 0.3 + 0.5j, 2.5 + 0.05j]),
+                        1j * np.pi / beta * (2 * np.arange(4) + 1)])
+    op = ComponentSigmaOperator(ctx)
+    out = component_retarded_matrix_green_shifts(op, sources, z, maximum_krylov_dimension=ep.dimension)
+    masks = poly.coordinate_masks(ep.coordinates)
+    smasks = poly.coordinate_masks(p.coordinates)
+    A = solvers.dense_matrix(lambda v: poly.sigma(p, sol.amplitudes, masks, v, smasks), ep.dimension)
+    eye = np.eye(ep.dimension)
+    for name in ("addition", "removal"):
+        want = np.empty((z.size, 2, 2), dtype=complex)
+        for k, zk in enumerate(z):
+            X = np.linalg.solve(zk * eye - A, np.stack([getattr(s, name) for s in sources], axis=1))
+            want[k] = np.stack([s.sink for s in sources]) @ X
+        assert _rel(getattr(out, name), want) < 1e-8, name
+        assert np.max(getattr(out, f"{name}_relative_residuals")) < 1e-8
+    assert out.sites == (0, 1) and out.retarded.shape == (z.size, 2, 2)
+    assert out.action_calls == sum(p.dimension for p in out.addition_projections + out.removal_projections)
+    with pytest.raises(ValueError):
+        component_retarded_matrix_green_shifts(op, sources, np.array([0.5 + 0.0j]))  # Im z must be > 0
