@@ -37,6 +37,7 @@ from .response import (
     arnoldi_resolvent,
     component_retarded_green,
     component_retarded_matrix_green_shifts,
+    component_retarded_time,
     solve_component_shifted,
 )
 from .observables import (
@@ -64,6 +65,7 @@ __all__ = [
     "factorized_ccsd_covariance",
     "component_retarded_green",
     "component_retarded_matrix_green_shifts",
+    "component_retarded_time",
     "component_static_jvp",
     "component_static_linearization",
     "component_static_linearized_jvp",

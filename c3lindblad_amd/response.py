@@ -689,3 +689,43 @@ def component_retarded_matrix_green_shifts(
         wall_seconds=float(perf_counter() - started),
         peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
     )
+
+
+def component_retarded_time(
+    operator: ComponentSigmaOperator,
+    sources: ComponentEOMSources,
+    time,
+    *,
+    maximum_krylov_dimension: int = 160,
+    residual_tolerance: float = 1.0e-11,
+) -> np.ndarray:
+    """Positive-time impurity Green function ``G(t) = -i sink . exp(-i t sigma) (b_add + b_rem)``
+    in one Arnoldi space (``component_retarded_time``, ``component_response.py:628-661``).
+
+    The space is grown by the native driver with the reference's probe shift ``z = i`` and
+    tolerances; ``exp(-i t H_m)`` acts on ``beta e_1`` on the host for each time, and the
+    sink is projected on the basis on the device.
+    """
+
+    torch = _lib.require_device()
+    t = np.asarray(time, dtype=np.float64)
+    if t.ndim != 1 or not np.all(np.isfinite(t)):
+        raise ValueError("time must be a finite one-dimensional grid")
+    source = np.asarray(sources.addition, dtype=np.complex128) + np.asarray(sources.removal, dtype=np.complex128)
+    sink = np.asarray(sources.sink, dtype=np.complex128)
+    n = int(operator.dimension)
+    krylov = arnoldi_resolvent(operator, source, np.array([1j]), maximum_dimension=min(maximum_krylov_dimension, n),
+                               residual_tolerance=residual_tolerance, minimum_dimension=min(16, n), sink=sink)
+    m = krylov.dimension
+    H = krylov.hessenberg[:m, :m]
+    reduced_source = np.zeros(m, dtype=np.complex128)
+    reduced_source[0] = krylov.source_norm
+    red = Reducer(n, m, complex_=True)
+    sink_d = torch.from_numpy(np.ascontiguousarray(sink)).to(_lib.DEVICE)
+    sv = torch.empty(m, dtype=torch.complex128, device=_lib.DEVICE)
+    red.dot(krylov.basis, m, sink_d, sv, conjugate=False)
+    projected_sink = sv.cpu().numpy()
+    out = np.empty(t.size, dtype=np.complex128)
+    for i, value in enumerate(t):
+        out[i] = -1j * (projected_sink @ (scipy.linalg.expm(-1j * float(value) * H) @ reduced_source))
+    return out

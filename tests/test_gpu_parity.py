@@ -644,3 +644,47 @@ def test_matrix_green_at_complex_shifts_matches_dense_oracle(torch_gpu):
     assert out.action_calls == sum(p.dimension for p in out.addition_projections + out.removal_projections)
     with pytest.raises(ValueError):
         component_retarded_matrix_green_shifts(op, sources, np.array([0.5 + 0.0j]))  # Im z must be > 0
+
+
+def test_time_domain_green_function_matches_dense_exponential(torch_gpu):
+    """G(t) = -i sink . exp(-i t sigma)(b_add + b_rem) from one Arnoldi space against the dense
+    matrix exponential of the polynomial oracle's sigma (1e-8 relative)."""
+
+    import scipy.linalg
+
+    from c3lindblad_amd import component_retarded_time
+    from c3lindblad_amd.configs import readme_model
+    from c3lindblad_amd.eom import (
+        ComponentEOMProblem,
+        ComponentSigmaOperator,
+        component_eom_context,
+        component_eom_sources,
+        eom_ccsd_problem,
+    )
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.solver import solve_component_ccsd
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    print("case time-domain green", flush=True)
+    p = factorized_cc_problem(readme_model())
+    sp = ComponentStaticProblem.compile(p)
+    sol = solve_component_ccsd(sp)
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    ctx = component_eom_context(ep, sol.amplitudes)
+    sources = component_eom_sources(ctx)
+    t = np.array([0.0, 0.3, 1.0, 2.5, 6.0])
+    # a tolerance below rounding grows the space to all P = 39 directions, where the Krylov
+    # exponential is exact; the reference tolerance (1e-11) is exercised on top
+    got = component_retarded_time(ComponentSigmaOperator(ctx), sources, t, residual_tolerance=1e-15)
+    default = component_retarded_time(ComponentSigmaOperator(ctx), sources, t)
+    masks = poly.coordinate_masks(ep.coordinates)
+    smasks = poly.coordinate_masks(p.coordinates)
+    A = solvers.dense_matrix(lambda v: poly.sigma(p, sol.amplitudes, masks, v, smasks), ep.dimension)
+    b = sources.addition + sources.removal
+    want = np.array([-1j * sources.sink @ (scipy.linalg.expm(-1j * tk * A) @ b) for tk in t])
+    assert _rel(got, want) < 1e-8
+    assert _rel(default[:3], want[:3]) < 1e-6  # short times in the reference's converged space
+    # G(0) = -i sink . b exactly (the basis spans b)
+    assert abs(got[0] + 1j * (sources.sink @ b)) <= 1e-12 * max(1.0, abs(sources.sink @ b))
