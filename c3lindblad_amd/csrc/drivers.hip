@@ -662,6 +662,8 @@ struct c3l_arnoldi_run {
   bool done() const { return live.empty() || j > m_max; }
 
   ~c3l_arnoldi_run() {
+    // stop-flag copies into `pinned` may still be queued when a run ends early
+    if (stream != nullptr) (void)hipStreamSynchronize(stream);
     for (hipEvent_t& e : copied)
       if (e != nullptr) (void)hipEventDestroy(e);
     if (pinned != nullptr) (void)hipHostFree(pinned);
