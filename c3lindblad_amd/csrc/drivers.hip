@@ -17,8 +17,11 @@
 // the shared basis: its Givens QR of (z I - H) advances by one column per step
 // (one wave per shift, O(m) work) and the sink projection of its solution is
 // updated incrementally, so the Green function and the exact residual of every
-// shift are available at every step without any dense factorisation.  A device
-// stop flag is read by the host every few steps.
+// shift are available at every step without any dense factorisation.  Up to four
+// bases on twin plans advance in lockstep (one launch per kernel for all of them),
+// and one host thread can advance several such runs, each on its own stream; the
+// device stop flags reach the host through pinned memory and are polled, not waited
+// for.
 
 #include <hip/hip_runtime.h>
 
