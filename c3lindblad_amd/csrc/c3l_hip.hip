@@ -1,12 +1,14 @@
 // MI355X (gfx950) kernels and C ABI for the C3L coupled-cluster hot path.
 //
-// FP64 path.  Data layout in HBM:
+// FP64 path, with an FP32 (float2) mirror of each plan workspace for the mixed-precision
+// JVP.  Data layout in HBM:
 //   * Newton/GMRES vectors: real tilde coordinates, one contiguous double array.
 //   * Amplitudes / EOM vectors: complex (double2) in the reference's canonical
 //     string order (PH rank, then mask value).
 //   * Plan workspace: one double2 array holding every padded atom tensor
-//     (n^rank entries, slot-major, antisymmetric within a group) and the padded
-//     base blocks scattered from the current input vector.
+//     (n^rank entries, first index fastest, antisymmetric within a group) and the
+//     padded base blocks scattered from the current input vector.  Twin plans (built
+//     from the same arrays) each own a workspace and run in one launch per kernel.
 //   * Krylov bases: basis vectors stored contiguously, column k at V + k*ld.
 //
 // Every kernel is bandwidth/latency bound (GEMV-like contractions, gathers,
