@@ -1345,10 +1345,9 @@ int build_program(c3l_plan* plan, int32_t program) {
     d.lanes_log2 = lanes_log2_for(b);
     d.reps = 1;
     if (b.two_factors && d.lanes_log2 == 0) {
-      // coordinates per thread for large two-factor blocks (more independent gathers in flight)
-      const char* env = std::getenv("C3L_WEDGE_REPS");
-      int reps = env ? std::atoi(env) : 2;  // 2 measured best on MI355X for the n = 25 JVP
-      reps = reps < 1 ? 1 : (reps > kMaxReps ? kMaxReps : reps);
+      // coordinates per thread for large two-factor blocks (more independent gathers in
+      // flight; kMaxReps = 2 measured best on MI355X for the n = 25 JVP)
+      int reps = kMaxReps;
       while (reps > 1 && (b.count + 256 * reps - 1) / (256 * reps) < 512) reps >>= 1;
       d.reps = reps;
     }
