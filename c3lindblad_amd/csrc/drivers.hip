@@ -14,8 +14,8 @@
 // residual).  The last column of a cycle is completed by a step without operator.
 //
 // Arnoldi: the Hessenberg matrix stays on the device; every shift z runs GMRES on
-// the shared basis: its Givens QR of (z I - H) advances by one column per step
-// (one wave per shift, O(m) work) and the sink projection of its solution is
+// the shared basis: its Givens QR of (z I - H) advances by up to eight columns per
+// launch (eight lanes per shift, O(m) work per column) and the sink projection of its solution is
 // updated incrementally, so the Green function and the exact residual of every
 // shift are available at every step without any dense factorisation.  Up to four
 // bases on twin plans advance in lockstep (one launch per kernel for all of them),
