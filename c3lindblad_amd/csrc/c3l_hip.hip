@@ -757,9 +757,11 @@ struct KindExec {
   uint8_t* tree_barrier = nullptr;
 };
 
-// Complex MACs up to which one workgroup evaluates a contraction tree (a rank-3 root
-// of n = 25, 15625 MACs, and its descendants qualify; a rank-4 root does not).
-constexpr int64_t kTreeMacs = 1 << 16;
+// Complex MACs up to which one workgroup evaluates a contraction tree (a rank-2 root
+// of n = 25, 625 MACs, and its descendants qualify; a level of rank-3 roots, 15625 MACs
+// each, runs as one launch over the whole GPU: a workgroup per such tree leaves most
+// CUs idle and took longer than the extra launch).
+constexpr int64_t kTreeMacs = 1 << 13;
 
 struct PlanLevel {
   int32_t kind;

@@ -31,7 +31,7 @@ extern "C" int c3l_internal_fail(int code, const char* message);
 namespace c3l_ortho {
 namespace {
 
-constexpr int kGroupRows = 16;  // Hessenberg rows per column_kernel workgroup
+constexpr int kGroupRows = 4;  // Hessenberg rows per column_kernel workgroup (one wave each)
 
 // Sum over the 64 lanes of a wave with DPP moves (no LDS traffic), valid in lane 63:
 // an inclusive scan inside each row of 16 lanes (row_shr 1, 2, 4, 8, zero-filled at
@@ -258,8 +258,8 @@ __device__ __forceinline__ double2 take(const double (&v)[6], int at, double2) {
 //    tentative sub-diagonal nu of column j]
 // (h0 = |w|/r, the norm of A v before projection).  Row k < j: final
 // H[k, j-1] = H[k, j-1] + nu s_k and tentative H[k, j] = (z_k - (H s)_k)/r with
-// (H s)_k = sum_{l >= k-1} H~[k, l] s_l (H~: column j-1 completed), the row's 16 lanes
-// splitting l; threads of one l read consecutive rows.  `stage` (optional) gets the final
+// (H s)_k = sum_{l >= k-1} H~[k, l] s_l (H~: column j-1 completed), one wave per row,
+// its 64 lanes splitting l (all loads of a lane independent).  `stage` (optional) gets the final
 // column j-1 followed by its h0, contiguous for one copy to the host.
 template <typename T>
 __device__ __forceinline__ void column_body(T* __restrict__ H, int64_t ldh, int j, int fresh, int has_w,
@@ -268,7 +268,6 @@ __device__ __forceinline__ void column_body(T* __restrict__ H, int64_t ldh, int 
                                             double* __restrict__ status, T* __restrict__ stage, double breakdown) {
   using N = Num<T>;
   __shared__ double shared[4][6];
-  __shared__ T part[256 / kGroupRows][kGroupRows];
   const int nv = j + 1;
   const T* S = red;
   const T* Z = red + nv + 1;
@@ -302,22 +301,22 @@ __device__ __forceinline__ void column_body(T* __restrict__ H, int64_t ldh, int 
   }
   T* prev = H + (int64_t)(j - 1) * ldh;  // column j-1 (touched only when !fresh)
   T* col = H + (int64_t)j * ldh;         // column j (written only when has_w)
-  const int row = tid % kGroupRows, lr = tid / kGroupRows;
-  const int k = blockIdx.x * kGroupRows + row;
+  const int lane = tid & 63;
+  const int k = blockIdx.x * kGroupRows + (tid >> 6);  // uniform per wave
   T hs = N::zero();
   if (!fresh && has_w && k < j) {
     const T sk = S[k];
 #pragma unroll 4
-    for (int l = lr; l < j; l += 256 / kGroupRows) {
+    for (int l = max(k - 1, 0) + lane; l < j; l += 64) {
       T h = H[(int64_t)l * ldh + k];
       if (l == j - 1) h = N::fma(N::make(nu), sk, h);
-      hs = l >= k - 1 ? N::fma(h, S[l], hs) : hs;
+      hs = N::fma(h, S[l], hs);
     }
   }
-  part[lr][row] = hs;
-  __syncthreads();  // every read of this workgroup's rows of column j-1 precedes the writes below
-  if (lr == 0 && k < j) {
-    for (int u = 1; u < 256 / kGroupRows; ++u) hs = N::add(hs, part[u][row]);
+  // The row's only reader is this wave: its reads of column j-1 feed the sum that the
+  // writes below wait for, so no barrier is needed.
+  hs = N::wave_total(hs);
+  if (lane == 63 && k < j) {
     const T s = fresh ? N::zero() : S[k];
     if (!fresh) {
       const T h = N::fma(N::make(nu), s, prev[k]);
