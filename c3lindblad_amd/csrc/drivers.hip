@@ -153,6 +153,7 @@ __device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_doub
 // exact stop column.
 constexpr int kShiftBatch = 8;
 constexpr int kShiftsPerBlock = 256 / kShiftBatch;
+constexpr int kTileK = 32;  // rotations staged in LDS per tile
 
 __device__ __forceinline__ void multishift_batch_body(const double2* __restrict__ H, int c0, int nb, int m_max,
                                                       const double2* __restrict__ shifts, int n_shifts,
@@ -176,16 +177,46 @@ __device__ __forceinline__ void multishift_batch_body(const double2* __restrict_
   const double2* __restrict__ hc = H + (int64_t)cm * ldh;
   double2 xm = make_double2(-hc[0].x + (cm == 0 ? z.x : 0.0), -hc[0].y + (cm == 0 ? z.y : 0.0));
   double2 am = make_double2(0.0, 0.0);
+  // The rotations of the block's shifts and the lanes' H entries pass through LDS in tiles
+  // of kTileK steps: one round of coalesced loads serves kTileK links of the chain (rows
+  // padded by one entry so that the eight shifts of a wave read different banks).
+  static_assert(kShiftBatch * kTileK == 256, "one H entry per thread and tile");
+  __shared__ double s_c[kShiftsPerBlock][kTileK + 1];
+  __shared__ double2 s_s[kShiftsPerBlock][kTileK + 1];
+  __shared__ double2 s_p[kShiftsPerBlock][kTileK + 1];
+  __shared__ double2 s_h[kShiftBatch][kTileK + 1];
+  const int local = (int)(threadIdx.x / kShiftBatch);
+  for (int t0 = 0; t0 < c0; t0 += kTileK) {
+    const int len = min(kTileK, c0 - t0);
+    __syncthreads();  // the previous tile is consumed
+    for (int e = threadIdx.x; e < kShiftsPerBlock * kTileK; e += 256) {
+      const int sh = e / kTileK, kk = e % kTileK;
+      if (kk < len) {
+        const int se = min((int)blockIdx.x * kShiftsPerBlock + sh, n_shifts - 1);
+        const int64_t at = (int64_t)se * m_max + t0 + kk;
+        s_c[sh][kk] = rot_c[at];
+        s_s[sh][kk] = rot_s[at];
+        s_p[sh][kk] = pi[at];
+      }
+    }
+    {
+      const int b = (int)(threadIdx.x / kTileK), kk = (int)(threadIdx.x % kTileK);
+      const int cb = c0 + (b < nb ? b : 0);
+      if (kk < len) s_h[b][kk] = H[(int64_t)cb * ldh + t0 + kk + 1];
+    }
+    __syncthreads();
 #pragma unroll 4
-  for (int k = 0; k < c0; ++k) {
-    const double c = rot_c[row + k];
-    const double2 sn = rot_s[row + k];
-    const double2 pk = pi[row + k];
-    const double2 h = hc[k + 1];
-    const double2 a1 = make_double2(-h.x + (k + 1 == cm ? z.x : 0.0), -h.y + (k + 1 == cm ? z.y : 0.0));
-    const double2 R = zadd(make_double2(c * xm.x, c * xm.y), zmul(sn, a1));
-    am = zadd(am, zmul(R, pk));
-    xm = zadd(zmul(make_double2(-sn.x, sn.y), xm), make_double2(c * a1.x, c * a1.y));
+    for (int kk = 0; kk < len; ++kk) {
+      const int k = t0 + kk;
+      const double c = s_c[local][kk];
+      const double2 sn = s_s[local][kk];
+      const double2 pk = s_p[local][kk];
+      const double2 h = s_h[mine][kk];
+      const double2 a1 = make_double2(-h.x + (k + 1 == cm ? z.x : 0.0), -h.y + (k + 1 == cm ? z.y : 0.0));
+      const double2 R = zadd(make_double2(c * xm.x, c * xm.y), zmul(sn, a1));
+      am = zadd(am, zmul(R, pk));
+      xm = zadd(zmul(make_double2(-sn.x, sn.y), xm), make_double2(c * a1.x, c * a1.y));
+    }
   }
   // the group's first lane collects every column's rotated entry and partial sum
   const int first = lane - mine;
