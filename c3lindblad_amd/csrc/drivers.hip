@@ -218,6 +218,18 @@ __device__ __forceinline__ void multishift_batch_body(const double2* __restrict_
       xm = zadd(zmul(make_double2(-sn.x, sn.y), xm), make_double2(c * a1.x, c * a1.y));
     }
   }
+  // the batch's own Hessenberg entries H[c0+r+1, c0+b] (r <= b) and sink projections,
+  // loaded once by the block instead of one dependent load after another by the tail
+  __shared__ double2 s_hb[kShiftBatch][kShiftBatch];
+  __shared__ double2 s_sv[kShiftBatch];
+  if (threadIdx.x < kShiftBatch * kShiftBatch) {
+    const int b = (int)(threadIdx.x / kShiftBatch), r = (int)(threadIdx.x % kShiftBatch);
+    if (b < nb && r <= b) s_hb[b][r] = H[(int64_t)(c0 + b) * ldh + c0 + r + 1];
+  } else if (threadIdx.x < kShiftBatch * kShiftBatch + kShiftBatch) {
+    const int b = (int)threadIdx.x - kShiftBatch * kShiftBatch;
+    if (b < nb) s_sv[b] = sink_v[c0 + b];
+  }
+  __syncthreads();
   // the group's first lane collects every column's rotated entry and partial sum
   const int first = lane - mine;
   double2 x[kShiftBatch], acc[kShiftBatch];
@@ -235,16 +247,16 @@ __device__ __forceinline__ void multishift_batch_body(const double2* __restrict_
   for (int b = 0; b < kShiftBatch; ++b) {
     if (b >= nb) break;
     const int cb = c0 + b;
-    const double2* col = H + (int64_t)cb * ldh;
     double2 xb = x[b], ab = acc[b];
     for (int bb = 0; bb < b; ++bb) {  // rotations of this batch
       const int k = c0 + bb;
-      const double2 a1 = make_double2(-col[k + 1].x + (k + 1 == cb ? z.x : 0.0), -col[k + 1].y + (k + 1 == cb ? z.y : 0.0));
+      const double2 hk = s_hb[b][bb];  // H[k+1, cb]
+      const double2 a1 = make_double2(-hk.x + (k + 1 == cb ? z.x : 0.0), -hk.y + (k + 1 == cb ? z.y : 0.0));
       const double2 R = zadd(make_double2(cr[bb] * xb.x, cr[bb] * xb.y), zmul(sr[bb], a1));
       ab = zadd(ab, zmul(R, pr[bb]));
       xb = zadd(zmul(make_double2(-sr[bb].x, sr[bb].y), xb), make_double2(cr[bb] * a1.x, cr[bb] * a1.y));
     }
-    const double2 bsub = make_double2(-col[cb + 1].x, -col[cb + 1].y);
+    const double2 bsub = make_double2(-s_hb[b][b].x, -s_hb[b][b].y);
     const double ax = hypot(xb.x, xb.y);
     const double bx = hypot(bsub.x, bsub.y);
     double cn;
@@ -265,7 +277,7 @@ __device__ __forceinline__ void multishift_batch_body(const double2* __restrict_
     cr[b] = cn;
     sr[b] = snn;
     // pi_cb = (sink.v_cb - sum_k R[k, cb] pi_k) / R[cb, cb]
-    const double2 num = make_double2(sink_v[cb].x - ab.x, sink_v[cb].y - ab.y);
+    const double2 num = make_double2(s_sv[b].x - ab.x, s_sv[b].y - ab.y);
     const double den = r.x * r.x + r.y * r.y;
     const double2 p = den > 0.0 ? make_double2((num.x * r.x + num.y * r.y) / den, (num.y * r.x - num.x * r.y) / den)
                                 : make_double2(0.0, 0.0);
@@ -314,9 +326,12 @@ __device__ __forceinline__ void arnoldi_batch_status_body(const double* __restri
                                                           const double2* __restrict__ H, int m_max, int c0, int nb,
                                                           int min_dim, double tolerance, double breakdown) {
   __shared__ double part[kShiftBatch][16];
+  __shared__ double sub[kShiftBatch];
   __shared__ int stop_b;
   if (status[2] != 0.0) return;  // uniform: frozen after an earlier stop
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t ldh = m_max + 1;
+  if ((int)threadIdx.x < nb) sub[threadIdx.x] = H[(int64_t)(c0 + threadIdx.x) * ldh + c0 + threadIdx.x + 1].x;
   double m[kShiftBatch];
 #pragma unroll
   for (int b = 0; b < kShiftBatch; ++b) m[b] = 0.0;
@@ -340,12 +355,11 @@ __device__ __forceinline__ void arnoldi_batch_status_body(const double* __restri
   if (threadIdx.x == 0) {
     stop_b = -1;
     const int waves = (int)(blockDim.x >> 6);
-    const int64_t ldh = m_max + 1;
     for (int b = 0; b < nb; ++b) {
       double worst = 0.0;
       for (int u = 0; u < waves; ++u) worst = part[b][u] > worst ? part[b][u] : worst;
       const int c = c0 + b;
-      const double hn = H[(int64_t)c * ldh + c + 1].x;
+      const double hn = sub[b];
       status[0] = hn;
       status[1] = worst;
       status[3] = (double)(c + 1);
