@@ -335,13 +335,24 @@ __device__ __forceinline__ void arnoldi_batch_status_body(const double* __restri
   double m[kShiftBatch];
 #pragma unroll
   for (int b = 0; b < kShiftBatch; ++b) m[b] = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  // Four rows of every column per thread and pass, all loads unconditional: indices past
+  // the end repeat the last shift (a duplicate cannot change a maximum) and columns past
+  // nb repeat column nb-1 (their maxima are never read).
+  constexpr int kRows = 4;
+  for (int i0 = threadIdx.x; i0 < n; i0 += kRows * (int)blockDim.x) {
+    double v[kRows][kShiftBatch];
 #pragma unroll
-    for (int b = 0; b < kShiftBatch; ++b) {
-      if (b >= nb) break;
-      const double v = resid_hist[(int64_t)b * n + i];
-      m[b] = (v != v) ? INFINITY : (v > m[b] ? v : m[b]);  // NaN counts as unconverged
-    }
+    for (int r = 0; r < kRows; ++r)
+#pragma unroll
+      for (int b = 0; b < kShiftBatch; ++b)
+        v[r][b] = resid_hist[(int64_t)min(b, nb - 1) * n + min(i0 + r * (int)blockDim.x, n - 1)];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+#pragma unroll
+      for (int b = 0; b < kShiftBatch; ++b) {
+        const double x = v[r][b];
+        m[b] = (x != x) ? INFINITY : (x > m[b] ? x : m[b]);  // NaN counts as unconverged
+      }
   }
 #pragma unroll
   for (int b = 0; b < kShiftBatch; ++b) {
