@@ -537,6 +537,11 @@ def test_batched_arnoldi_equals_separate_bases(torch_gpu):
     alone = [arnoldi_resolvent(op.on_slot(s), b, z, sink=sink, **kw) for s, b in ((0, b1), (1, b2))]
     print("dimensions", [q.dimension for q in pair], [q.dimension for q in alone], flush=True)
     assert pair[1].dimension < pair[0].dimension
+    # the multi-shift Green values of the long basis against dense resolvents: at this Krylov
+    # dimension the shift state runs through many 32-step rotation tiles and eight-column batches
+    assert pair[0].dimension > 96
+    want_g = np.array([sink @ np.linalg.solve(zz * np.eye(P) - A, b1) for zz in z])
+    assert _rel(pair[0].green, want_g) < 1e-8
     for got, want in zip(pair, alone):
         assert got.dimension == want.dimension and got.breakdown == want.breakdown
         assert np.array_equal(got.hessenberg, want.hessenberg)
