@@ -59,10 +59,11 @@ def _measured_traffic(kernel: str):
     return None, None
 
 
-def _algorithmic_program_bytes(lowered, program: int) -> float:
-    """Compulsory bytes of one program launch: per coordinate its output, destination,
-    packed coordinate and (residual/JVP/sigma) diagonal and input value, plus every
-    distinct atom tensor the program reads once."""
+def _algorithmic_program_bytes(lowered, program: int, atom_bytes: int = 16) -> float:
+    """Compulsory bytes of one program launch: per coordinate its output (complex128),
+    destination, packed coordinate and (residual/JVP/sigma) diagonal and input value, plus
+    every distinct atom tensor the program reads once, at ``atom_bytes`` per element (16 for
+    FP64 atoms, 8 for the complex64 atoms of an FP32 program)."""
 
     n = lowered.n
     total = 0.0
@@ -74,7 +75,7 @@ def _algorithmic_program_bytes(lowered, program: int) -> float:
         total += dest.size * (16 + 8 + 4 + diag)
         for off, code in zip(foff, fcode):
             atoms[int(off)] = n ** (int(code) & 7)
-    return total + 16.0 * sum(atoms.values())
+    return total + float(atom_bytes) * sum(atoms.values())
 
 
 def _plain_fp64_estimate(pipe, ref, runs=((0, 4), (1000, 4), (2000, 4), (3000, 4))) -> dict:
@@ -135,8 +136,47 @@ def _plain_fp64_estimate(pipe, ref, runs=((0, 4), (1000, 4), (2000, 4), (3000, 4
     }
 
 
-def _cpu_baseline(pipe, result, seconds_budget: float = 30.0) -> dict:
-    """Time the NumPy port (single thread) on one residual, one JVP and one sigma per charge."""
+def _host_cgs_seconds(rows: int, columns: list, dtype, threads: int) -> float:
+    """Host time of two-pass classical Gram-Schmidt (``c = V^H w; w -= V c`` twice) against
+    ``j`` columns, for every ``j`` in ``columns``: measured with NumPy BLAS at three column
+    counts and integrated with the linear fit (the cost is affine in ``j``)."""
+
+    from threadpoolctl import threadpool_limits
+
+    if not columns:
+        return 0.0
+    top = max(columns)
+    probe = sorted({max(1, top // 8), max(1, top // 2), max(1, top)})
+    rng = np.random.default_rng(1)
+    V = np.asarray(rng.normal(size=(top, rows)), dtype=dtype)
+    if np.iscomplexobj(V):
+        V = V + 1j * np.asarray(rng.normal(size=(top, rows)), dtype=np.float64)
+    w0 = V[0].copy()
+    times = []
+    with threadpool_limits(limits=threads):
+        for j in probe:
+            Vj = V[:j]
+            best = np.inf
+            for _ in range(2):
+                w = w0.copy()
+                t0 = time.perf_counter()
+                for _ in range(2):
+                    c = (Vj @ w.conj()).conj()  # V^H w without a conjugated copy of V
+                    w -= c @ Vj
+                best = min(best, time.perf_counter() - t0)
+            times.append(best)
+    slope, intercept = np.polyfit(np.asarray(probe, dtype=float), np.asarray(times), 1) if len(probe) > 1 else (
+        times[0] / probe[0], 0.0)
+    return float(sum(max(intercept + slope * j, 0.0) for j in columns))
+
+
+def _cpu_baseline(pipe, result) -> dict:
+    """Estimated whole solve of the CPU restatement on this host: the NumPy port of the same
+    plans timed on one residual, one JVP and one sigma per charge (single thread: the port's
+    gathers do not thread), scaled by the GPU solve's operator counts, plus the Krylov
+    orthogonalisation of the same solve (two-pass CGS in NumPy BLAS on every host thread the
+    box grants, timed at three column counts and integrated over the solve's columns).  The
+    small shifted solves are left out (optimistic for the CPU)."""
 
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle.port import PortPlan
@@ -166,17 +206,27 @@ def _cpu_baseline(pipe, result, seconds_budget: float = 30.0) -> dict:
     t_res, t_jvp = t1 - t0, t2 - t1
     n_sigma = sum(sum(d) for d in result.krylov_dimensions.values())
     per_sigma = float(np.mean(list(sigma.values())))
-    solve_seconds = result.residual_evaluations * t_res + result.jvp_evaluations * t_jvp + n_sigma * per_sigma
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    restart = pipe.policy.gmres_restart
+    gmres_columns = [j % restart + 1 for its in result.gmres_iterations for j in range(its)]
+    arnoldi_columns = [j + 1 for dims in result.krylov_dimensions.values() for m in dims for j in range(m)]
+    P = max(b.dimension for b in pipe.eom.values())
+    t_gmres = _host_cgs_seconds(p.num_variables, gmres_columns, np.float64, threads)
+    t_arnoldi = _host_cgs_seconds(P, arnoldi_columns, np.complex128, threads)
+    solve_seconds = (result.residual_evaluations * t_res + result.jvp_evaluations * t_jvp + n_sigma * per_sigma
+                     + t_gmres + t_arnoldi)
     return {
         "value": 1.0 / solve_seconds,
         "unit": "solves/sec",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
         "sample": (
-            f"NumPy port of the same plans on this host: 1 residual ({t_res:.2f}s), 1 JVP ({t_jvp:.2f}s), "
-            f"1 sigma per charge ({per_sigma:.3f}s) at n={p.model.n_sites}; scaled by one GPU solve's counts "
-            f"({result.residual_evaluations} residuals, {result.jvp_evaluations} JVPs, {n_sigma} sigmas); "
-            "Krylov orthogonalisation and shifted solves excluded (optimistic for the CPU)"
+            f"NumPy port of the same plans on this host, single thread: 1 residual ({t_res:.2f}s), 1 JVP "
+            f"({t_jvp:.2f}s), 1 sigma per charge ({per_sigma:.3f}s) at n={p.model.n_sites}, scaled by one GPU "
+            f"solve's counts ({result.residual_evaluations} residuals, {result.jvp_evaluations} JVPs, {n_sigma} "
+            f"sigmas); plus two-pass CGS of the same solve in NumPy BLAS on {threads} threads ({len(gmres_columns)} "
+            f"GMRES steps: {t_gmres:.1f}s, {len(arnoldi_columns)} Arnoldi steps: {t_arnoldi:.1f}s, each timed at three "
+            "column counts and integrated); shifted solves excluded (optimistic for the CPU)"
         ),
     }
 
@@ -335,7 +385,8 @@ def main() -> int:
             d = kernels.setdefault(f"wedge_program[{name}]", {"ms": 0.0, "launches": 0, "bytes": 0.0})
             d["ms"] += ms
             d["launches"] += launches
-            d["bytes"] += launches * _algorithmic_program_bytes(plan.lowered, pid)
+            atom_bytes = 8 if (name == "jvp" and args.precision == "fp32") else 16
+            d["bytes"] += launches * _algorithmic_program_bytes(plan.lowered, pid, atom_bytes)
         plan.set_timing(False)
     import ctypes
 
@@ -370,6 +421,13 @@ def main() -> int:
         "config": {
             "workload": workload,
             "variant": "fp64_gpu" if args.precision == "fp64" else "mixed_fp32_jvp",
+            "layers_in_timed_region": {
+                "mixed_precision": args.precision == "fp32",
+                "batched": "every frequency on one Krylov basis per channel; addition and removal bases of a "
+                           "charge block in lockstep; charge blocks on concurrent streams"
+                           + (f"; {len(pipes)} instances on concurrent streams" if len(pipes) > 1 else ""),
+                "learned_warm_start": False,
+            },
             "newton_policy": {"forcing": args.forcing, "gmres_restart": args.restart,
                               "jacobian_precision": args.precision,
                               "minimum_gmres_relative_tolerance": policy.minimum_gmres_relative_tolerance,
@@ -405,7 +463,8 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "traffic_source": traffic_source,
+            "traffic_source": (f"committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same kernel and workload "
+                               f"({traffic_source}), not measured in this run") if traffic_source else None,
             "avg_launch_us": 1e3 * dk["ms"] / dk["launches"],
             "launches": dk["launches"],
             "share_of_step": dk["ms"] * 1e-3 / instrumented,
@@ -428,8 +487,9 @@ def main() -> int:
         torch.cuda.synchronize()
         t_ref = time.perf_counter() - t_ref
         line["config"]["fp64_reference_policy"] = {
-            "policy": "FP64 JVPs, legacy forcing, GMRES restart 40 (reference defaults)",
+            "policy": "FP64 JVPs, legacy forcing, GMRES restart 40 (reference defaults), same shared-basis resolvent",
             "solves_per_sec": 1.0 / t_ref,
+            "speedup_of_this_variant": line["value"] * t_ref,
             "jvp_evaluations": ref.jvp_evaluations,
             "packed_residual_norm": ref.packed_residual_norm,
             "gr_rel_error_of_this_variant": max(

@@ -24,7 +24,7 @@ from .static import (
     component_static_linearized_jvp,
     component_static_residual,
 )
-from .solver import NewtonKrylovPolicy, solve_component_ccsd
+from .solver import ComponentCCSolution, NewtonKrylovPolicy, RapidityPreconditionerDiagnostics, solve_component_ccsd
 from .eom import (
     ComponentEOMProblem,
     ComponentSigmaOperator,
@@ -34,12 +34,17 @@ from .eom import (
     eom_ccsd_problem,
 )
 from .response import (
+    ArnoldiResolvent,
     arnoldi_resolvent,
     component_retarded_green,
+    component_retarded_matrix_green,
+    component_retarded_matrix_green_arnoldi,
     component_retarded_matrix_green_shifts,
+    component_retarded_matrix_time,
     component_retarded_time,
     solve_component_shifted,
 )
+from .matsubara import MatsubaraGrid, component_matsubara_green, matsubara_grid
 from .observables import (
     ComponentObservableProblem,
     contact_currents,
@@ -48,6 +53,15 @@ from .observables import (
 )
 
 __all__ = [
+    "ArnoldiResolvent",
+    "ComponentCCSolution",
+    "MatsubaraGrid",
+    "RapidityPreconditionerDiagnostics",
+    "component_matsubara_green",
+    "component_retarded_matrix_green",
+    "component_retarded_matrix_green_arnoldi",
+    "component_retarded_matrix_time",
+    "matsubara_grid",
     "ComponentEOMProblem",
     "ComponentObservableProblem",
     "ComponentSigmaOperator",

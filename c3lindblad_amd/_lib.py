@@ -70,7 +70,7 @@ SIGNATURES = {
     "c3l_krylov_timing": ([_I32, _P, _P, _P], ctypes.c_int),
     "c3l_arnoldi": (
         [_P, _P, _P, _P, _I64, _I32, _P, _I32, _P, ctypes.c_double, _P, _P, _P, _P, _P, _P, _P, _P, _P,
-         ctypes.c_double, _I32, ctypes.c_double, _P, _P, _P, _P],
+         ctypes.c_double, _I32, ctypes.c_double, _I32, _P, _P, _P, _P],
         ctypes.c_int,
     ),
     "c3l_arnoldi_batch": (

@@ -163,7 +163,7 @@ __device__ __forceinline__ void multishift_batch_body(const double2* __restrict_
                                                       double2* __restrict__ green, double* __restrict__ resid,
                                                       double* __restrict__ resid_hist,
                                                       double2* __restrict__ green_hist,
-                                                      const double* __restrict__ status) {
+                                                      const double* __restrict__ status, int galerkin) {
   if (status[2] != 0.0) return;  // uniform: nothing after the stop
   const int lane = threadIdx.x & 63;
   const int mine = lane % kShiftBatch;
@@ -284,9 +284,12 @@ __device__ __forceinline__ void multishift_batch_body(const double2* __restrict_
     pi[row + cb] = p;
     pr[b] = p;
     const double2 gj = make_double2(cn * g.x, cn * g.y);
+    // Galerkin (FOM) residual |h_{c+1,c}| |y_c| / beta of the square system: the last
+    // coordinate is g / x_b before this rotation, i.e. the GMRES residual divided by cn.
+    const double fom = ax > 0.0 ? bx * hypot(g.x, g.y) / (ax * beta) : INFINITY;
     g = zmul(make_double2(-snn.x, snn.y), g);  // residual entry after this rotation
     G = zadd(G, zmul(gj, p));
-    res = hypot(g.x, g.y) / beta;
+    res = galerkin ? fom : hypot(g.x, g.y) / beta;
     resid_hist[(int64_t)b * n_shifts + s] = res;
     green_hist[(int64_t)b * n_shifts + s] = G;
   }
@@ -409,6 +412,7 @@ struct ShiftState {
   double* resid_hist;
   double2* green_hist;
   double* status;
+  int galerkin;  // 1: stop on the Galerkin (FOM) residual, 0: on the GMRES residual
 };
 using ShiftStates = c3l_ortho::Many<ShiftState>;
 
@@ -416,7 +420,7 @@ __global__ __launch_bounds__(256) void multishift_batch_kernel(ShiftStates state
                                                                const double2* __restrict__ shifts, int n_shifts) {
   const ShiftState& t = states.p[blockIdx.z];
   multishift_batch_body(t.H, c0, nb, m_max, shifts, n_shifts, t.sink_v, t.beta, t.rot_c, t.rot_s, t.pi, t.gamma,
-                        t.green, t.resid, t.resid_hist, t.green_hist, t.status);
+                        t.green, t.resid, t.resid_hist, t.green_hist, t.status, t.galerkin);
 }
 
 __global__ __launch_bounds__(1024) void arnoldi_batch_status_kernel(ShiftStates states, int n, int m_max, int c0,
@@ -425,6 +429,10 @@ __global__ __launch_bounds__(1024) void arnoldi_batch_status_kernel(ShiftStates 
   const ShiftState& t = states.p[blockIdx.z];
   arnoldi_batch_status_body(t.resid_hist, t.green_hist, t.green, t.resid, n, t.status, t.H, m_max, c0, nb, min_dim,
                             tolerance, breakdown);
+}
+
+bool out_invalid(const int32_t* m_out, const int32_t* breakdown_out, const double* worst_out) {
+  return m_out == nullptr || breakdown_out == nullptr || worst_out == nullptr;
 }
 
 int64_t blocks_for(int64_t n, int t = 256) {
@@ -803,7 +811,8 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
                   int64_t dim, int32_t m_max, const double* shifts, int32_t n_shifts, const double* sink,
                   const double* beta, double* rot_c, double* rot_s, double* pi, double* gamma, double* green,
                   double* resid, double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
-                  double breakdown_tol, void* stream) {
+                  double breakdown_tol, int32_t criterion, void* stream) {
+  if (criterion != 0 && criterion != 1) return dfail(C3L_EINVAL, "c3l_arnoldi: criterion must be 0 or 1");
   if (plans == nullptr || count < 1 || count > c3l_ortho::kMaxInstances || V == nullptr || H == nullptr || dim <= 0 ||
       m_max <= 0 || n_shifts <= 0 || shifts == nullptr || sink == nullptr || beta == nullptr || rot_c == nullptr ||
       rot_s == nullptr || pi == nullptr || gamma == nullptr || green == nullptr || resid == nullptr ||
@@ -847,7 +856,8 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
                                   reinterpret_cast<double2*>(pi + 2 * i * sRot),
                                   reinterpret_cast<double2*>(gamma + 2 * i * sShift),
                                   reinterpret_cast<double2*>(green + 2 * i * sShift), resid + i * sShift,
-                                  sc + L.resid_hist, reinterpret_cast<double2*>(sc + L.green_hist), status + 4 * i});
+                                  sc + L.resid_hist, reinterpret_cast<double2*>(sc + L.green_hist), status + 4 * i,
+                                  (int)criterion});
     r->w.push_back(sc + L.w);
     r->V.push_back(V + i * sV);
     r->live.push_back(i);
@@ -920,11 +930,20 @@ int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_
                 const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
                 double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
                 double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
-                int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream) {
-  if (plan == nullptr) return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
-  return c3l_arnoldi_batch(&plan, 1, diag, V, H, dim, m_max, shifts, n_shifts, sink, &beta, rot_c, rot_s, pi, gamma,
-                           green, resid, scratch, coef, status, tolerance, min_dim, breakdown_tol, m_out,
-                           breakdown_out, worst_out, stream);
+                int32_t criterion, int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream) {
+  if (plan == nullptr || out_invalid(m_out, breakdown_out, worst_out))
+    return dfail(C3L_EINVAL, "c3l_arnoldi: invalid argument");
+  c3l_arnoldi_run* run = new c3l_arnoldi_run();
+  const int rc = arnoldi_setup(run, &plan, 1, diag, V, H, dim, m_max, shifts, n_shifts, sink, &beta, rot_c, rot_s, pi,
+                               gamma, green, resid, scratch, coef, status, tolerance, min_dim, breakdown_tol,
+                               criterion, stream);
+  if (rc != C3L_OK) {
+    delete run;
+    return rc;
+  }
+  const int rc_drive = c3l_arnoldi_drive(&run, 1);
+  const int rc_end = c3l_arnoldi_end(run, m_out, breakdown_out, worst_out);
+  return rc_drive != C3L_OK ? rc_drive : rc_end;
 }
 
 int c3l_arnoldi_batch(c3l_plan* const* plans, int32_t count, const double* diag, double* V, double* H, int64_t dim,
@@ -949,7 +968,7 @@ int c3l_arnoldi_begin(c3l_plan* const* plans, int32_t count, const double* diag,
   *out = nullptr;
   c3l_arnoldi_run* r = new c3l_arnoldi_run();
   const int rc = arnoldi_setup(r, plans, count, diag, V, H, dim, m_max, shifts, n_shifts, sink, beta, rot_c, rot_s,
-                               pi, gamma, green, resid, scratch, coef, status, tolerance, min_dim, breakdown_tol,
+                               pi, gamma, green, resid, scratch, coef, status, tolerance, min_dim, breakdown_tol, 0,
                                stream);
   if (rc != C3L_OK) {
     delete r;

@@ -154,6 +154,16 @@ class ComponentEOMContext:
         return self._devices[slot]
 
 
+def _check_device_amplitudes(device_amplitudes, shape) -> None:
+    """The plan reads the amplitudes as contiguous complex128: anything else is rejected on the host."""
+
+    if (getattr(device_amplitudes, "is_cuda", None) is None or tuple(device_amplitudes.shape) != tuple(shape)
+            or str(device_amplitudes.dtype) != "torch.complex128" or not device_amplitudes.is_contiguous()):
+        raise ValueError("device_amplitudes must be a contiguous complex128 GPU vector matching the CCSD basis")
+    if not device_amplitudes.is_cuda:
+        raise ValueError("device_amplitudes must be a contiguous complex128 GPU vector matching the CCSD basis")
+
+
 def component_eom_context(problem: ComponentEOMProblem, amplitudes, *, device_amplitudes=None) -> ComponentEOMContext:
     """``device_amplitudes`` (a GPU copy of the same amplitudes, e.g. from the solver) skips the upload."""
 
@@ -162,8 +172,7 @@ def component_eom_context(problem: ComponentEOMProblem, amplitudes, *, device_am
         raise ValueError("amplitudes must be finite and match the CCSD basis")
     context = ComponentEOMContext(problem, t)
     if device_amplitudes is not None:
-        if tuple(device_amplitudes.shape) != t.shape or not device_amplitudes.is_cuda:
-            raise ValueError("device_amplitudes must be a GPU vector matching the CCSD basis")
+        _check_device_amplitudes(device_amplitudes, t.shape)
         context.__dict__["_device_t"] = device_amplitudes
     context.device  # make the amplitude atoms resident on plan slot 0
     return context

@@ -292,9 +292,77 @@ class FermionicLindbladModel:
         return bool(self.interaction.terms)
 
     def to_dict(self) -> dict[str, Any]:
-        return {
-            "E": np.asarray(self.E).tolist() if not np.iscomplexobj(self.E) else [[complex(v) for v in r] for r in self.E],
-            "U": self.U,
+        """The reference's canonical model record (``models.py:428-452``)."""
+
+        result: dict[str, Any] = {
+            "model_type": "FermionicLindbladModel",
+            "symmetry": self.symmetry,
+            "E": _matrix_to_dict(self.E),
+            "gamma_loss": _matrix_to_dict(self.gamma_loss),
+            "gamma_gain": _matrix_to_dict(self.gamma_gain),
+            "U": float(self.U),
             "impurity_site": self.impurity_site,
-            "interaction_sites": list(self.interaction_sites),
+            "units": self.units,
+            "input_tolerance": {"absolute": float(self.input_tolerance.absolute),
+                                "relative": float(self.input_tolerance.relative)},
+            "corrections": [],
         }
+        if self.interaction_sites != (self.impurity_site,):
+            result["interaction_sites"] = list(self.interaction_sites)
+        if self.density_interaction is not None:
+            result["density_interaction"] = np.asarray(self.density_interaction).tolist()
+        if self.two_body_interaction is not None:
+            result["two_body_interaction"] = {
+                "n_spin_orbitals": self.two_body_interaction.n_spin_orbitals,
+                "terms": [{"indices": [p, q, r, s], "coefficient": [float(v.real), float(v.imag)]}
+                          for p, q, r, s, v in self.two_body_interaction.terms],
+            }
+        return result
+
+    @classmethod
+    def from_dict(cls, data) -> "FermionicLindbladModel":
+        """Read the reference's model record (``models.py:454-501``).
+
+        Matrices are ``{"real": ..., "imag": ...}`` records (plain nested lists are accepted
+        too); ``corrections`` are derived from the matrices and ignored on input.
+        """
+
+        model_type = data.get("model_type", "FermionicLindbladModel")
+        if model_type != "FermionicLindbladModel":
+            raise ValueError(f"unsupported model_type {model_type!r}")
+        tolerance_data = data.get("input_tolerance")
+        tolerance = (TolerancePolicy.first().input_abs_rel if tolerance_data is None
+                     else AbsRelTolerance(float(tolerance_data["absolute"]), float(tolerance_data["relative"])))
+        sites = data.get("interaction_sites")
+        two_body_data = data.get("two_body_interaction")
+        two_body = None
+        if two_body_data is not None:
+            two_body = SparseTwoBodyInteraction(
+                n_spin_orbitals=int(two_body_data["n_spin_orbitals"]),
+                terms=tuple((*(int(i) for i in record["indices"]), complex(*record["coefficient"]))
+                            for record in two_body_data["terms"]),
+            )
+        return cls(
+            E=_matrix_from_dict(data["E"]),
+            gamma_loss=_matrix_from_dict(data["gamma_loss"]),
+            gamma_gain=_matrix_from_dict(data["gamma_gain"]),
+            U=data["U"],
+            impurity_site=int(data.get("impurity_site", 0)),
+            units=str(data.get("units", "Gamma")),
+            input_tolerance=tolerance,
+            interaction_sites=None if sites is None else tuple(int(site) for site in sites),
+            density_interaction=data.get("density_interaction"),
+            two_body_interaction=two_body,
+            symmetry=data.get("symmetry", "spin"),
+        )
+
+
+def _matrix_to_dict(matrix) -> dict[str, list]:
+    m = np.asarray(matrix, dtype=np.complex128)
+    return {"real": m.real.tolist(), "imag": m.imag.tolist()}
+
+
+def _matrix_from_dict(data) -> np.ndarray:
+    if isinstance(data, dict):
+        return np.asarray(data["real"], dtype=np.float64) + 1j * np.asarray(data["imag"], dtype=np.float64)
+    return np.asarray(data, dtype=np.complex128)

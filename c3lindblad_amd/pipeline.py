@@ -33,6 +33,7 @@ class SolveResult:
     max_shift_residual: float
     seconds: dict = field(default_factory=dict)
     amplitudes: np.ndarray | None = None
+    gmres_iterations: tuple = ()  # per Newton step
 
 
 @dataclass
@@ -95,6 +96,7 @@ class SolvePipeline:
             max_shift_residual=worst,
             seconds={"ccsd": t1 - t0, "eom_green": t2 - t1, "eom_contexts_sources": t_sources - t1},
             amplitudes=sol.amplitudes,
+            gmres_iterations=tuple(step.gmres_iterations for step in sol.history),
         )
 
 

@@ -34,11 +34,21 @@ from .eom import ComponentEOMSources, ComponentSigmaOperator
 
 
 @dataclass
-class DeviceArnoldi:
-    """Krylov projection ``σ V_m = V_{m+1} H`` with the basis kept on the GPU.
+class ArnoldiResolvent:
+    """Krylov projection ``σ V_m = V_m H_m + h v_{m+1} e_m^T`` with the basis kept on the GPU.
 
-    ``green``/``relative_residuals`` hold the multi-shift GMRES results for
-    the sink and shifts the basis was built with.
+    The reference's ``ArnoldiResolvent`` (``green/krylov.py:28-254``): the same fields and
+    methods (``coefficients``, ``residual_norms``, ``relative_residuals``, ``solutions``,
+    ``time_coefficients``, ``time_residual_norms``, ``relative_time_residuals``, ``transfer``,
+    ``to_dict``).  ``basis`` is a torch ``[m+1, P]`` complex128 device tensor whose rows are
+    the basis vectors (the reference stores them as columns).  The reduced solves use the
+    Schur form of ``H_m`` on the host, as the reference does; projections of sinks on the
+    basis and reconstructed solutions run on the device.  ``shifts``/``green``/
+    ``shift_relative_residuals`` are the native multi-shift results for the shifts and sink
+    the basis was built with (GMRES or Galerkin, following ``criterion``).
+    ``action_wall_seconds`` holds the construction time spread evenly over the actions (the
+    native driver does not time actions one by one); ``check_dimensions`` holds the final
+    dimension only.
     """
 
     basis: object  # torch [m+1, P] complex128 (rows are basis vectors)
@@ -47,31 +57,187 @@ class DeviceArnoldi:
     breakdown: bool
     shifts: np.ndarray
     green: np.ndarray
-    relative_residuals: np.ndarray
+    shift_relative_residuals: np.ndarray
     converged: bool
     construction_wall_seconds: float
     action_calls: int
+    residual_tolerance: float = 0.0
+    criterion: str = "gmres"
+    peak_rss_kib: int = 0
 
     @property
     def dimension(self) -> int:
         return int(self.hessenberg.shape[1])
 
-    def project(self, sink, shifts):
-        """``sink·V_m (z - H_m)^{-1} β e_1`` for other sinks/shifts via the Schur form (host O(m^3))."""
+    @property
+    def vector_dimension(self) -> int:
+        return int(self.basis.shape[1])
+
+    @property
+    def terminal_coupling(self) -> complex:
+        m = self.dimension
+        return complex(self.hessenberg[m, m - 1])
+
+    @property
+    def action_wall_seconds(self) -> np.ndarray:
+        m = max(self.dimension, 1)
+        return np.full(self.dimension, float(self.construction_wall_seconds) / m)
+
+    @property
+    def check_dimensions(self) -> tuple:
+        return (self.dimension,)
+
+    @property
+    def check_maximum_relative_residuals(self) -> np.ndarray:
+        return np.asarray([float(np.max(self.shift_relative_residuals))])
+
+    @property
+    def orthogonality_defect(self) -> float:
+        """``||V_m^H V_m - I||_F / m`` (``krylov.py:395-402``), one Gram product on the device."""
+
+        cached = self.__dict__.get("_orthogonality")
+        if cached is None:
+            m = self.dimension
+            V = self.basis[:m].contiguous()
+            gram = (V.conj() @ V.transpose(0, 1).contiguous()).cpu().numpy()
+            cached = float(np.linalg.norm(gram - np.eye(m), ord="fro") / max(float(m), 1.0))
+            self.__dict__["_orthogonality"] = cached
+        return cached
+
+    def _dimension(self, dimension) -> int:
+        m = self.dimension if dimension is None else dimension
+        if isinstance(m, bool) or not isinstance(m, (int, np.integer)):
+            raise TypeError("dimension must be an integer")
+        if not 1 <= m <= self.dimension:
+            raise ValueError("dimension must lie in [1, krylov_dimension]")
+        return int(m)
+
+    def _schur(self, m: int):
+        key = ("_schur", m)
+        cached = self.__dict__.get(key)
+        if cached is None:
+            T, Q = scipy.linalg.schur(self.hessenberg[:m, :m], output="complex", check_finite=False)
+            cached = (T, Q)
+            self.__dict__[key] = cached
+        return cached
+
+    def coefficients(self, shifts, *, dimension=None) -> np.ndarray:
+        """Galerkin coordinates ``y(z) = (z - H_m)^{-1} beta e_1`` (``krylov.py:66-93``), shifts x m."""
+
+        z = np.asarray(shifts, dtype=np.complex128)
+        if z.ndim != 1 or z.size == 0 or not np.all(np.isfinite(z)):
+            raise ValueError("shifts must be a finite nonempty vector")
+        m = self._dimension(dimension)
+        T, Q = self._schur(m)
+        reduced_rhs = Q.conj().T[:, 0] * self.source_norm
+        identity = np.eye(m, dtype=np.complex128)
+        reduced = np.empty((z.size, m), dtype=np.complex128)
+        for p, shift in enumerate(z):
+            reduced[p] = Q @ scipy.linalg.solve_triangular(shift * identity - T, reduced_rhs, lower=False,
+                                                           check_finite=False)
+        return reduced
+
+    def residual_norms(self, shifts, *, dimension=None) -> np.ndarray:
+        m = self._dimension(dimension)
+        reduced = self.coefficients(shifts, dimension=m)
+        return np.asarray(np.abs(self.hessenberg[m, m - 1] * reduced[:, -1]), dtype=np.float64)
+
+    def relative_residuals(self, shifts, *, dimension=None) -> np.ndarray:
+        return np.asarray(self.residual_norms(shifts, dimension=dimension)
+                          / max(float(self.source_norm), np.finfo(np.float64).tiny), dtype=np.float64)
+
+    def _device_combination(self, reduced: np.ndarray, m: int):
+        torch = _lib.require_device()
+        y = torch.from_numpy(np.ascontiguousarray(reduced)).to(_lib.DEVICE)
+        return y @ self.basis[:m].contiguous()
+
+    def solutions(self, shifts, *, dimension=None) -> np.ndarray:
+        """``V_m y(z)`` for every shift (shifts x P), formed on the device."""
+
+        m = self._dimension(dimension)
+        reduced = self.coefficients(shifts, dimension=m)
+        return self._device_combination(reduced, m).cpu().numpy()
+
+    def projected_sink(self, sink, m: int) -> np.ndarray:
+        """``sink · V_m`` (no conjugation), one reduction over the basis on the device."""
 
         torch = _lib.require_device()
-        m = self.dimension
-        T, Q = scipy.linalg.schur(self.hessenberg[:m, :m], output="complex")
-        r = Q.conj().T[:, 0] * self.source_norm
-        red = Reducer(self.basis.shape[1], m, complex_=True)
-        sink_d = torch.from_numpy(np.ascontiguousarray(sink, dtype=np.complex128)).to(_lib.DEVICE)
+        row = np.asarray(sink, dtype=np.complex128)
+        if row.shape != (self.vector_dimension,) or not np.all(np.isfinite(row)):
+            raise ValueError("sink must match the full vector dimension")
+        red = Reducer(self.vector_dimension, m, complex_=True)
+        sink_d = torch.from_numpy(np.ascontiguousarray(row)).to(_lib.DEVICE)
         sv = torch.empty(m, dtype=torch.complex128, device=_lib.DEVICE)
         red.dot(self.basis, m, sink_d, sv, conjugate=False)
-        w = sv.cpu().numpy() @ Q
+        return sv.cpu().numpy()
+
+    def transfer(self, shifts, sink, *, dimension=None) -> np.ndarray:
+        """``sink · V_m y(z)`` (``krylov.py:214-227``)."""
+
+        m = self._dimension(dimension)
+        projected = self.projected_sink(sink, m)
+        return np.asarray(self.coefficients(shifts, dimension=m) @ projected, dtype=np.complex128)
+
+    def time_coefficients(self, time, *, dimension=None) -> np.ndarray:
+        """``exp(-i H_m t) beta e_1`` on a time grid (``krylov.py:136-172``)."""
+
+        t = np.asarray(time, dtype=np.float64)
+        if t.ndim != 1 or t.size == 0 or not np.all(np.isfinite(t)):
+            raise ValueError("time must be a finite nonempty vector")
+        m = self._dimension(dimension)
+        # exp(-i t H_m) = Q exp(-i t T) Q^H with the Schur form H_m = Q T Q^H
+        T, Q = self._schur(m)
+        r = Q.conj().T[:, 0] * self.source_norm
+        out = np.empty((t.size, m), dtype=np.complex128)
+        for i, value in enumerate(t):
+            out[i] = Q @ (scipy.linalg.expm(-1j * float(value) * T) @ r)
+        return out
+
+    def time_residual_norms(self, time, *, dimension=None) -> np.ndarray:
+        """``|h_{m+1,m} e_m^T exp(-i H_m t) beta e_1|`` (``krylov.py:174-196``)."""
+
+        m = self._dimension(dimension)
+        reduced = self.time_coefficients(time, dimension=m)
+        return np.asarray(np.abs(self.hessenberg[m, m - 1] * reduced[:, -1]), dtype=np.float64)
+
+    def relative_time_residuals(self, time, *, dimension=None) -> np.ndarray:
+        return np.asarray(self.time_residual_norms(time, dimension=dimension)
+                          / max(float(self.source_norm), np.finfo(np.float64).tiny), dtype=np.float64)
+
+    def project(self, sink, shifts):
+        """``sink·V_m (z - H_m)^{-1} β e_1`` for other sinks/shifts (batched triangular solves on the
+        device) and the Galerkin residual estimate ``|h_{m+1,m} e_m^T y| / β`` of every shift."""
+
+        m = self.dimension
+        T, Q = self._schur(m)
+        r = Q.conj().T[:, 0] * self.source_norm
+        w = self.projected_sink(sink, m) @ Q
         z = np.ascontiguousarray(np.asarray(shifts, dtype=np.complex128))
         green, last = shifted_green(T, r, w, Q[m - 1, :], z)
         residual = np.abs(self.hessenberg[m, m - 1] * last) / max(self.source_norm, np.finfo(float).tiny)
         return green, residual
+
+    def to_dict(self) -> dict:
+        """The reference's record (``krylov.py:229-254``)."""
+
+        return {
+            "krylov_dimension": self.dimension,
+            "full_vector_dimension": self.vector_dimension,
+            "source_norm": float(self.source_norm),
+            "terminal_coupling_absolute": float(abs(self.terminal_coupling)),
+            "orthogonality_defect": float(self.orthogonality_defect),
+            "action_wall_seconds": [float(v) for v in self.action_wall_seconds],
+            "action_wall_seconds_total": float(np.sum(self.action_wall_seconds)),
+            "construction_wall_seconds": float(self.construction_wall_seconds),
+            "peak_rss_kib": int(self.peak_rss_kib),
+            "breakdown": bool(self.breakdown),
+            "check_dimensions": list(self.check_dimensions),
+            "check_maximum_relative_residuals": [float(v) for v in self.check_maximum_relative_residuals],
+            "maximum_relative_residual": float(self.check_maximum_relative_residuals[-1]),
+            "residual_tolerance": float(self.residual_tolerance),
+            "converged": bool(self.converged),
+            "method": "delayed_reorthogonalisation_CGS_shift_invariant_resolvent_" + self.criterion,
+        }
 
 
 def shifted_green(T, r, w, q, z):
@@ -97,6 +263,19 @@ def shifted_green(T, r, w, q, z):
     return out
 
 
+def _krylov_dimension_checks(maximum_dimension, minimum_dimension, check_interval, breakdown_tolerance, n):
+    if isinstance(maximum_dimension, bool) or not isinstance(maximum_dimension, (int, np.integer)) \
+            or not 0 < maximum_dimension <= n:
+        raise ValueError("maximum_dimension must lie in [1, operator.dimension]")
+    if isinstance(minimum_dimension, bool) or not isinstance(minimum_dimension, (int, np.integer)) \
+            or not 1 <= minimum_dimension <= maximum_dimension:
+        raise ValueError("minimum_dimension must lie in [1, maximum_dimension]")
+    if isinstance(check_interval, bool) or not isinstance(check_interval, (int, np.integer)) or check_interval <= 0:
+        raise ValueError("check_interval must be a positive integer")
+    if not (np.isfinite(breakdown_tolerance) and breakdown_tolerance > 0.0):
+        raise ValueError("breakdown_tolerance must be finite and positive")
+
+
 def arnoldi_resolvent(
     operator: ComponentSigmaOperator,
     rhs,
@@ -108,15 +287,23 @@ def arnoldi_resolvent(
     check_interval: int = 8,
     breakdown_tolerance: float = 1.0e-14,
     sink=None,
-) -> DeviceArnoldi:
-    """Build one shift-invariant Krylov space for ``(zI - σ)^{-1} b`` with the native driver.
+    criterion: str = "galerkin",
+) -> ArnoldiResolvent:
+    """Build one shift-invariant Krylov space for ``(zI - σ)^{-1} b`` with the native driver
+    (``arnoldi_resolvent``, ``green/krylov.py:284-417``).
 
-    Every shift's exact residual is tested at every step (``check_interval``
-    is accepted for signature compatibility).  With ``sink`` the returned
-    ``green`` holds ``sink·x(z)`` for every shift.
+    The basis stops growing at the first dimension ``m >= minimum_dimension`` at which every
+    shift's Galerkin residual ``|h_{m+1,m} e_m^T y_m(z)| / ||b||`` is at most
+    ``residual_tolerance`` (the reference's rule, tested here after every column rather than
+    every ``check_interval`` columns, so the dimension is the smallest one that passes), at a
+    breakdown, or at ``maximum_dimension``.  ``criterion="gmres"`` stops on the GMRES residual
+    instead (the per-frequency solves of ``component_retarded_green``).  With ``sink`` the
+    result's ``green`` holds the native GMRES value ``sink·x(z)`` for every shift.
     """
 
     torch = _lib.require_device()
+    if not hasattr(operator, "dimension"):
+        raise TypeError("operator must expose dimension and matvec")
     n = int(operator.dimension)
     b = np.asarray(rhs, dtype=np.complex128)
     z = np.ascontiguousarray(np.asarray(shifts, dtype=np.complex128))
@@ -124,10 +311,11 @@ def arnoldi_resolvent(
         raise ValueError("rhs must be a finite vector matching the operator")
     if z.ndim != 1 or z.size == 0 or not np.all(np.isfinite(z)):
         raise ValueError("shifts must be a finite nonempty vector")
-    if isinstance(maximum_dimension, bool) or not isinstance(maximum_dimension, int) or not 0 < maximum_dimension <= n:
-        raise ValueError("maximum_dimension must lie in [1, operator.dimension]")
+    _krylov_dimension_checks(maximum_dimension, minimum_dimension, check_interval, breakdown_tolerance, n)
     if not (np.isfinite(residual_tolerance) and residual_tolerance > 0.0):
         raise ValueError("residual_tolerance must be finite and positive")
+    if criterion not in ("galerkin", "gmres"):
+        raise ValueError("criterion must be 'galerkin' or 'gmres'")
     sink_h = np.zeros(n, dtype=np.complex128) if sink is None else np.asarray(sink, dtype=np.complex128)
     if sink_h.shape != (n,) or not np.all(np.isfinite(sink_h)):
         raise ValueError("sink must be a finite vector matching the operator")
@@ -135,7 +323,7 @@ def arnoldi_resolvent(
     if beta == 0.0:
         raise ValueError("rhs must be nonzero")
     started = perf_counter()
-    m_max = maximum_dimension
+    m_max = int(maximum_dimension)
     dev = _lib.DEVICE
     c128, f64 = torch.complex128, torch.float64
     ns = z.size
@@ -163,23 +351,26 @@ def arnoldi_resolvent(
         "c3l_arnoldi", state.plan.handle, _lib.dptr(state.diag), _lib.dptr(V), _lib.dptr(H), n, m_max, _lib.dptr(zs), ns,
         _lib.dptr(sink_d), beta, _lib.dptr(rot_c), _lib.dptr(rot_s), _lib.dptr(pi), _lib.dptr(gamma), _lib.dptr(green),
         _lib.dptr(resid), _lib.dptr(scratch), _lib.dptr(coef), _lib.dptr(status), float(residual_tolerance),
-        int(minimum_dimension), float(breakdown_tolerance), ctypes.byref(m_out), ctypes.byref(broke), ctypes.byref(worst),
-        _lib.stream_ptr(torch),
+        int(minimum_dimension), float(breakdown_tolerance), 1 if criterion == "galerkin" else 0, ctypes.byref(m_out),
+        ctypes.byref(broke), ctypes.byref(worst), _lib.stream_ptr(torch),
     )
     m = m_out.value
     operator.action_calls += m
     residuals = resid.cpu().numpy()
-    return DeviceArnoldi(
+    return ArnoldiResolvent(
         basis=V[: m + 1],
         hessenberg=H[:m, : m + 1].cpu().numpy().T.copy(),
         source_norm=beta,
         breakdown=bool(broke.value),
         shifts=z,
         green=green.cpu().numpy(),
-        relative_residuals=residuals,
+        shift_relative_residuals=residuals,
         converged=bool(broke.value or np.max(residuals) <= residual_tolerance),
         construction_wall_seconds=perf_counter() - started,
         action_calls=m,
+        residual_tolerance=float(residual_tolerance),
+        criterion=criterion,
+        peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
     )
 
 
@@ -252,7 +443,7 @@ class _ArnoldiRun:
             _lib.stream_ptr(torch), ctypes.byref(self.handle),
         )
 
-    def finish(self) -> list[DeviceArnoldi]:
+    def finish(self) -> list[ArnoldiResolvent]:
         """End the run (waits for its stream) and return one projection per right-hand side."""
 
         count = self.count
@@ -268,17 +459,20 @@ class _ArnoldiRun:
         for i, op in enumerate(self.operators):
             m = int(m_out[i])
             op.action_calls += m
-            out.append(DeviceArnoldi(
+            out.append(ArnoldiResolvent(
                 basis=self.V[i, : m + 1],
                 hessenberg=self.H[i, :m, : m + 1].cpu().numpy().T.copy(),
                 source_norm=float(self.beta[i]),
                 breakdown=bool(broke[i]),
                 shifts=self.z,
                 green=green_h[i].copy(),
-                relative_residuals=resid_h[i].copy(),
+                shift_relative_residuals=resid_h[i].copy(),
                 converged=bool(broke[i] or np.max(resid_h[i]) <= self.tolerance),
                 construction_wall_seconds=wall,
                 action_calls=m,
+                residual_tolerance=self.tolerance,
+                criterion="gmres",
+                peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
             ))
         return out
 
@@ -315,7 +509,7 @@ def arnoldi_resolvent_batch(
     sinks,
     minimum_dimension: int = 16,
     breakdown_tolerance: float = 1.0e-14,
-) -> list[DeviceArnoldi]:
+) -> list[ArnoldiResolvent]:
     """``arnoldi_resolvent`` for several right-hand sides in lockstep (``c3l_arnoldi_batch``).
 
     ``operators[i]`` is the same sigma operator bound to distinct plan slots (twin plans);
@@ -387,7 +581,8 @@ def solve_component_shifted(
     if np.linalg.norm(r0) > 0.0:
         cap = min(n, restart * maxiter, 2048)
         proj = arnoldi_resolvent(operator, r0, np.array([z]), maximum_dimension=cap,
-                                 residual_tolerance=rtol * bnorm / float(np.linalg.norm(r0)))
+                                 residual_tolerance=rtol * bnorm / float(np.linalg.norm(r0)),
+                                 minimum_dimension=min(16, cap), criterion="gmres")
         m = proj.dimension
         Hz = -proj.hessenberg[: m + 1, :m].copy()
         Hz[np.arange(m), np.arange(m)] += z
@@ -398,7 +593,7 @@ def solve_component_shifted(
         corr = (yd @ proj.basis[:m]).cpu().numpy()
         x = x0 + corr
         iterations = m
-        history = tuple(float(v) for v in proj.relative_residuals)
+        history = tuple(float(v) for v in proj.shift_relative_residuals)
         info = 0 if proj.converged else int(maxiter)
     true = float(np.linalg.norm(b - apply(x)) / bnorm)
     return ComponentShiftedSolution(
@@ -419,8 +614,8 @@ class ComponentRetardedGreen:
     addition: np.ndarray
     removal: np.ndarray
     retarded: np.ndarray
-    addition_projection: DeviceArnoldi | None
-    removal_projection: DeviceArnoldi | None
+    addition_projection: ArnoldiResolvent | None
+    removal_projection: ArnoldiResolvent | None
     addition_relative_residuals: np.ndarray
     removal_relative_residuals: np.ndarray
     wall_seconds: float
@@ -462,18 +657,19 @@ def run_concurrently(tasks, streams: int | None = None):
 
 
 def _checked(proj, limit, name):
-    if np.max(proj.relative_residuals) > limit:
+    if np.max(proj.shift_relative_residuals) > limit:
         raise RuntimeError(
             f"component EOM resolvent did not converge ({name}: max residual "
-            f"{np.max(proj.relative_residuals):.2e} at m={proj.dimension})"
+            f"{np.max(proj.shift_relative_residuals):.2e} at m={proj.dimension})"
         )
-    return proj.green, proj, proj.relative_residuals
+    return proj.green, proj, proj.shift_relative_residuals
 
 
 def _channel(operator, source, sink, z, cap, rtol, limit, name):
     if not np.any(source):
         return np.zeros(z.size, dtype=np.complex128), None, np.zeros(z.size)
-    proj = arnoldi_resolvent(operator, source, z, maximum_dimension=cap, residual_tolerance=rtol, sink=sink)
+    proj = arnoldi_resolvent(operator, source, z, maximum_dimension=cap, residual_tolerance=rtol, sink=sink,
+                             minimum_dimension=min(16, cap), criterion="gmres")
     return _checked(proj, limit, name)
 
 
@@ -595,7 +791,8 @@ def component_retarded_green(
 
 @dataclass(frozen=True)
 class ComponentShiftedMatrixArnoldiGreen:
-    """``G^R(z)[sink_i, source_j]`` over many complex shifts (axes shift, sink, source)."""
+    """``G^R(z)[sink_i, source_j]`` over many complex shifts (axes shift, sink, source)
+    (``component_response.py:449-465``)."""
 
     shifts: np.ndarray
     sites: tuple
@@ -604,11 +801,140 @@ class ComponentShiftedMatrixArnoldiGreen:
     retarded: np.ndarray
     addition_projections: tuple
     removal_projections: tuple
-    addition_relative_residuals: np.ndarray  # (source, shift) Galerkin residual estimates
+    addition_relative_residuals: np.ndarray  # (source, shift) Galerkin residuals
     removal_relative_residuals: np.ndarray
     action_calls: int
     wall_seconds: float
     peak_rss_kib: int
+
+
+@dataclass(frozen=True)
+class ComponentRetardedMatrixArnoldiGreen:
+    """Real-axis Green matrix over many frequencies (``component_response.py:162-178``)."""
+
+    omega: np.ndarray
+    eta: float
+    sites: tuple
+    addition: np.ndarray
+    removal: np.ndarray
+    retarded: np.ndarray
+    addition_projections: tuple
+    removal_projections: tuple
+    addition_relative_residuals: np.ndarray
+    removal_relative_residuals: np.ndarray
+    component_action_calls: int
+    component_action_wall_seconds: float
+    wall_seconds: float
+    peak_rss_kib: int
+
+
+@dataclass(frozen=True)
+class ComponentRetardedMatrixGreen:
+    """Green matrix with one record per source, channel and frequency (``component_response.py:146-159``).
+
+    The records are ``ComponentShiftedSolution`` objects whose ``solution`` is ``None`` (the
+    vectors stay in the device Krylov space; ``*_projections`` reconstruct any of them) and whose
+    ``true_relative_residual`` is the exact residual of that frequency's solution.
+    """
+
+    omega: np.ndarray
+    eta: float
+    sites: tuple
+    addition: np.ndarray
+    removal: np.ndarray
+    retarded: np.ndarray
+    addition_solutions: tuple
+    removal_solutions: tuple
+    addition_projections: tuple
+    removal_projections: tuple
+    wall_seconds: float
+    peak_rss_kib: int
+
+
+def _validated_source_columns(operator, sources):
+    """Same checks and messages as ``component_response.py:496-517``; sources shorter than the
+    operator (S13 sources in a larger space) are zero-padded."""
+
+    columns = tuple(sources)
+    if not columns:
+        raise ValueError("at least one impurity source is required")
+    if len({item.site for item in columns}) != len(columns):
+        raise ValueError("impurity source sites must be unique")
+    if any(item.charge != columns[0].charge for item in columns):
+        raise ValueError("all impurity sources must use one response charge")
+    retained = columns[0].addition.size
+    if any(item.addition.shape != (retained,) or item.removal.shape != (retained,) or item.sink.shape != (retained,)
+           for item in columns):
+        raise ValueError("impurity sources and sinks must share one retained basis")
+    if int(operator.dimension) < retained:
+        raise ValueError("response operator is smaller than the retained sources")
+    return columns, retained
+
+
+def _validated_krylov_controls(operator, maximum_krylov_dimension, minimum_krylov_dimension, check_interval,
+                               residual_tolerance):
+    """``component_response.py:520-547``."""
+
+    for value in (maximum_krylov_dimension, minimum_krylov_dimension, check_interval):
+        if isinstance(value, bool) or not isinstance(value, (int, np.integer)):
+            raise TypeError("Krylov dimensions and check_interval must be integers")
+    if (maximum_krylov_dimension <= 0 or minimum_krylov_dimension <= 0 or check_interval <= 0
+            or not np.isfinite(residual_tolerance) or residual_tolerance <= 0.0):
+        raise ValueError("Krylov controls and residual tolerance must be positive")
+    maximum = min(int(maximum_krylov_dimension), int(operator.dimension))
+    return maximum, min(int(minimum_krylov_dimension), maximum)
+
+
+def _pad(vector, padding):
+    v = np.asarray(vector, dtype=np.complex128)
+    return v if padding == 0 else np.concatenate((v, np.zeros(padding, dtype=np.complex128)))
+
+
+def _matrix_arnoldi_response(operator, columns, retained, shifts, *, maximum_dimension, minimum_dimension,
+                             residual_tolerance, check_interval) -> ComponentShiftedMatrixArnoldiGreen:
+    """One Galerkin-stopped Krylov space per source and channel; every sink contracted from it
+    (``component_response.py:550-625``)."""
+
+    started = perf_counter()
+    before = operator.action_calls
+    padding = int(operator.dimension) - retained
+    sinks = [_pad(item.sink, padding) for item in columns]
+    shape = (shifts.size, len(columns), len(columns))
+    values = {"addition": np.empty(shape, dtype=np.complex128), "removal": np.empty(shape, dtype=np.complex128)}
+    projections = {"addition": [], "removal": []}
+    residuals = {"addition": [], "removal": []}
+    for j, column in enumerate(columns):
+        for name in ("addition", "removal"):
+            proj = arnoldi_resolvent(operator, _pad(getattr(column, name), padding), shifts,
+                                     maximum_dimension=maximum_dimension, residual_tolerance=residual_tolerance,
+                                     minimum_dimension=minimum_dimension, check_interval=check_interval,
+                                     sink=sinks[0], criterion="galerkin")
+            if not proj.converged:
+                raise RuntimeError("a component EOM matrix Arnoldi projection failed")
+            worst = np.zeros(shifts.size)
+            for i, sink in enumerate(sinks):
+                green, estimate = proj.project(sink, shifts)
+                values[name][:, i, j] = green
+                worst = np.maximum(worst, estimate)
+            projections[name].append(proj)
+            residuals[name].append(worst)
+    expected = sum(p.dimension for p in projections["addition"] + projections["removal"])
+    if operator.action_calls - before != expected:
+        raise RuntimeError("component action count disagrees with Arnoldi dimensions")
+    return ComponentShiftedMatrixArnoldiGreen(
+        shifts=np.asarray(shifts, dtype=np.complex128),
+        sites=tuple(item.site for item in columns),
+        addition=values["addition"],
+        removal=values["removal"],
+        retarded=values["addition"] + values["removal"],
+        addition_projections=tuple(projections["addition"]),
+        removal_projections=tuple(projections["removal"]),
+        addition_relative_residuals=np.asarray(residuals["addition"]),
+        removal_relative_residuals=np.asarray(residuals["removal"]),
+        action_calls=int(expected),
+        wall_seconds=float(perf_counter() - started),
+        peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
+    )
 
 
 def component_retarded_matrix_green_shifts(
@@ -625,67 +951,126 @@ def component_retarded_matrix_green_shifts(
     (``component_retarded_matrix_green_shifts``, ``component_response.py:449-493``).
 
     Imaginary-axis points ``z = i omega_n`` and real-axis points ``omega + i eta`` share one
-    Krylov space per source and channel.  Each space is built by the native driver, which
-    stops when every shift's exact residual meets ``residual_tolerance``; the value for
-    every sink then comes from the Schur form of the Hessenberg matrix (Galerkin
-    coefficients, as the reference's ``ArnoldiResolvent.coefficients``,
-    ``green/krylov.py:81-91``), with its residual estimate ``|h_{m+1,m} e_m^T y| / beta``.
-    ``check_interval`` is accepted for signature compatibility.
+    Krylov space per source and channel.  Each space is built by the native driver and stops
+    when every shift's Galerkin residual meets ``residual_tolerance`` (the reference's rule);
+    the value for every sink comes from the Schur form of the Hessenberg matrix (Galerkin
+    coefficients, as ``ArnoldiResolvent.coefficients``, ``green/krylov.py:66-93``), with its
+    residual ``|h_{m+1,m} e_m^T y| / beta`` returned per source, channel and shift.
     """
 
-    started = perf_counter()
-    before = operator.action_calls
-    columns = tuple(sources)
-    if not columns:
-        raise ValueError("at least one impurity source is required")
-    if len({item.site for item in columns}) != len(columns):
-        raise ValueError("impurity source sites must be unique")
-    if any(item.charge != columns[0].charge for item in columns):
-        raise ValueError("all impurity sources must use one response charge")
-    n = int(operator.dimension)
-    if any(item.addition.shape != (n,) or item.removal.shape != (n,) or item.sink.shape != (n,) for item in columns):
-        raise ValueError("impurity sources and sinks must match the response operator")
+    columns, retained = _validated_source_columns(operator, sources)
     z = np.asarray(shifts, dtype=np.complex128)
     if z.ndim != 1 or z.size == 0 or not np.all(np.isfinite(z)) or np.any(z.imag <= 0.0):
         raise ValueError("shifts must be finite with positive imaginary parts")
-    for value in (maximum_krylov_dimension, minimum_krylov_dimension, check_interval):
-        if isinstance(value, bool) or not isinstance(value, int) or value <= 0:
-            raise ValueError("Krylov controls must be positive integers")
-    if minimum_krylov_dimension > maximum_krylov_dimension:
-        raise ValueError("minimum Krylov dimension exceeds the maximum")
-    if not (np.isfinite(residual_tolerance) and residual_tolerance > 0.0):
-        raise ValueError("residual_tolerance must be finite and positive")
-    cap = min(maximum_krylov_dimension, n)
-    sinks = [np.asarray(item.sink, dtype=np.complex128) for item in columns]
-    shape = (z.size, len(columns), len(columns))
-    values = {"addition": np.empty(shape, dtype=np.complex128), "removal": np.empty(shape, dtype=np.complex128)}
-    projections = {"addition": [], "removal": []}
-    residuals = {"addition": [], "removal": []}
-    for j, column in enumerate(columns):
-        for name in ("addition", "removal"):
-            proj = arnoldi_resolvent(operator, getattr(column, name), z, maximum_dimension=cap,
-                                     residual_tolerance=residual_tolerance,
-                                     minimum_dimension=min(minimum_krylov_dimension, cap), sink=sinks[0])
-            if not proj.converged:
-                raise RuntimeError("a component EOM matrix Arnoldi projection failed")
-            worst = np.zeros(z.size)
-            for i, sink in enumerate(sinks):
-                green, estimate = proj.project(sink, z)
-                values[name][:, i, j] = green
-                worst = np.maximum(worst, estimate)
-            projections[name].append(proj)
-            residuals[name].append(worst)
-    return ComponentShiftedMatrixArnoldiGreen(
-        shifts=z,
-        sites=tuple(item.site for item in columns),
-        addition=values["addition"],
-        removal=values["removal"],
-        retarded=values["addition"] + values["removal"],
-        addition_projections=tuple(projections["addition"]),
-        removal_projections=tuple(projections["removal"]),
-        addition_relative_residuals=np.asarray(residuals["addition"]),
-        removal_relative_residuals=np.asarray(residuals["removal"]),
-        action_calls=int(operator.action_calls - before),
+    maximum, minimum = _validated_krylov_controls(operator, maximum_krylov_dimension, minimum_krylov_dimension,
+                                                  check_interval, residual_tolerance)
+    return _matrix_arnoldi_response(operator, columns, retained, z, maximum_dimension=maximum,
+                                    minimum_dimension=minimum, residual_tolerance=float(residual_tolerance),
+                                    check_interval=int(check_interval))
+
+
+def _real_axis(omega, eta):
+    frequency = np.asarray(omega, dtype=np.float64)
+    broadening = float(eta)
+    if (frequency.ndim != 1 or frequency.size == 0 or not np.all(np.isfinite(frequency))
+            or not np.isfinite(broadening) or broadening <= 0.0):
+        raise ValueError("omega and positive eta must be finite")
+    return frequency, broadening
+
+
+def component_retarded_matrix_green_arnoldi(
+    operator: ComponentSigmaOperator,
+    sources,
+    omega,
+    eta: float,
+    *,
+    maximum_krylov_dimension: int = 320,
+    residual_tolerance: float = 1.0e-10,
+    minimum_krylov_dimension: int = 16,
+    check_interval: int = 8,
+) -> ComponentRetardedMatrixArnoldiGreen:
+    """Green matrix on ``omega + i eta`` from one Krylov space per source and channel
+    (``component_response.py:356-446``; ``eta > 0`` as the reference requires)."""
+
+    columns, retained = _validated_source_columns(operator, sources)
+    frequency, broadening = _real_axis(omega, eta)
+    maximum, minimum = _validated_krylov_controls(operator, maximum_krylov_dimension, minimum_krylov_dimension,
+                                                  check_interval, residual_tolerance)
+    before_seconds = operator.action_wall_seconds
+    core = _matrix_arnoldi_response(operator, columns, retained, frequency.astype(np.complex128) + 1j * broadening,
+                                    maximum_dimension=maximum, minimum_dimension=minimum,
+                                    residual_tolerance=float(residual_tolerance), check_interval=int(check_interval))
+    return ComponentRetardedMatrixArnoldiGreen(
+        omega=frequency,
+        eta=broadening,
+        sites=core.sites,
+        addition=core.addition,
+        removal=core.removal,
+        retarded=core.retarded,
+        addition_projections=core.addition_projections,
+        removal_projections=core.removal_projections,
+        addition_relative_residuals=core.addition_relative_residuals,
+        removal_relative_residuals=core.removal_relative_residuals,
+        component_action_calls=core.action_calls,
+        component_action_wall_seconds=float(operator.action_wall_seconds - before_seconds),
+        wall_seconds=core.wall_seconds,
+        peak_rss_kib=core.peak_rss_kib,
+    )
+
+
+def component_retarded_matrix_green(
+    operator: ComponentSigmaOperator,
+    sources,
+    omega,
+    eta: float,
+    *,
+    rtol: float = 1.0e-10,
+    restart: int = 40,
+    maxiter: int = 200,
+) -> ComponentRetardedMatrixGreen:
+    """Green matrix with every frequency's shifted systems solved to ``rtol``
+    (``component_retarded_matrix_green``, ``component_response.py:248-353``).
+
+    The reference runs one warm-started restarted GMRES per source, channel and frequency.
+    Here one Krylov space per source and channel serves every frequency: it grows until the
+    Galerkin residual of every frequency is at most ``rtol`` (the exact residual of that
+    frequency's solution ``V_m y(z)``, by the Arnoldi relation), so every returned value
+    meets the reference's acceptance ``true <= max(10 rtol, 1e-12)``; a space that cannot
+    reach it within ``min(P, restart*maxiter, 2048)`` columns raises ``RuntimeError`` as the
+    reference does for a failed shifted solve.
+    """
+
+    columns, retained = _validated_source_columns(operator, sources)
+    frequency, broadening = _real_axis(omega, eta)
+    if not (np.isfinite(rtol) and rtol > 0.0):
+        raise ValueError("rtol must be finite and positive")
+    cap = min(int(operator.dimension), int(restart) * int(maxiter), 2048)
+    started = perf_counter()
+    z = frequency.astype(np.complex128) + 1j * broadening
+    core = _matrix_arnoldi_response(operator, columns, retained, z, maximum_dimension=cap,
+                                    minimum_dimension=min(16, cap), residual_tolerance=float(rtol), check_interval=8)
+    limit = max(10.0 * rtol, 1.0e-12)
+    records = {}
+    for name in ("addition", "removal"):
+        worst = getattr(core, f"{name}_relative_residuals")
+        if np.max(worst) > limit:
+            raise RuntimeError("a component EOM matrix shifted solve failed")
+        records[name] = tuple(
+            tuple(ComponentShiftedSolution(None, True, 0, proj.dimension, float(worst[j, k]), (), 0.0)
+                  for k in range(z.size))
+            for j, proj in enumerate(getattr(core, f"{name}_projections"))
+        )
+    return ComponentRetardedMatrixGreen(
+        omega=frequency,
+        eta=broadening,
+        sites=core.sites,
+        addition=core.addition,
+        removal=core.removal,
+        retarded=core.retarded,
+        addition_solutions=records["addition"],
+        removal_solutions=records["removal"],
+        addition_projections=core.addition_projections,
+        removal_projections=core.removal_projections,
         wall_seconds=float(perf_counter() - started),
         peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
     )
@@ -703,29 +1088,120 @@ def component_retarded_time(
     in one Arnoldi space (``component_retarded_time``, ``component_response.py:628-661``).
 
     The space is grown by the native driver with the reference's probe shift ``z = i`` and
-    tolerances; ``exp(-i t H_m)`` acts on ``beta e_1`` on the host for each time, and the
-    sink is projected on the basis on the device.
+    tolerances (Galerkin stop rule); ``exp(-i t H_m)`` acts on ``beta e_1`` on the host for each
+    time, and the sink is projected on the basis on the device.
     """
 
-    torch = _lib.require_device()
     t = np.asarray(time, dtype=np.float64)
     if t.ndim != 1 or not np.all(np.isfinite(t)):
         raise ValueError("time must be a finite one-dimensional grid")
-    source = np.asarray(sources.addition, dtype=np.complex128) + np.asarray(sources.removal, dtype=np.complex128)
-    sink = np.asarray(sources.sink, dtype=np.complex128)
+    p = np.asarray(sources.addition).size
+    padding = int(operator.dimension) - p
+    source = _pad(np.asarray(sources.addition, dtype=np.complex128) + np.asarray(sources.removal, dtype=np.complex128),
+                  padding)
+    sink = _pad(sources.sink, padding)
     n = int(operator.dimension)
     krylov = arnoldi_resolvent(operator, source, np.array([1j]), maximum_dimension=min(maximum_krylov_dimension, n),
-                               residual_tolerance=residual_tolerance, minimum_dimension=min(16, n), sink=sink)
+                               residual_tolerance=residual_tolerance, minimum_dimension=min(16, n),
+                               check_interval=min(8, n), sink=sink)
+    if t.size == 0:
+        return np.empty(0, dtype=np.complex128)
     m = krylov.dimension
-    H = krylov.hessenberg[:m, :m]
-    reduced_source = np.zeros(m, dtype=np.complex128)
-    reduced_source[0] = krylov.source_norm
-    red = Reducer(n, m, complex_=True)
-    sink_d = torch.from_numpy(np.ascontiguousarray(sink)).to(_lib.DEVICE)
-    sv = torch.empty(m, dtype=torch.complex128, device=_lib.DEVICE)
-    red.dot(krylov.basis, m, sink_d, sv, conjugate=False)
-    projected_sink = sv.cpu().numpy()
-    out = np.empty(t.size, dtype=np.complex128)
-    for i, value in enumerate(t):
-        out[i] = -1j * (projected_sink @ (scipy.linalg.expm(-1j * float(value) * H) @ reduced_source))
+    return -1j * (krylov.time_coefficients(t) @ krylov.projected_sink(sink, m))
+
+
+@dataclass(frozen=True)
+class ComponentRetardedMatrixTime:
+    """Direct-time impurity Green matrix (``component_response.py:664-681``), axes (time, sink, source)."""
+
+    time: np.ndarray
+    sites: tuple
+    addition: np.ndarray
+    removal: np.ndarray
+    retarded: np.ndarray
+    channel_sum_defect: float
+    combined_projections: tuple
+    addition_projections: tuple
+    removal_projections: tuple
+    combined_relative_time_residuals: np.ndarray
+    addition_relative_time_residuals: np.ndarray
+    removal_relative_time_residuals: np.ndarray
+    wall_seconds: float
+    peak_rss_kib: int
+
+
+def _matrix_time_from_projections(projections, sinks, time) -> np.ndarray:
+    out = np.empty((time.size, len(sinks), len(projections)), dtype=np.complex128)
+    for j, proj in enumerate(projections):
+        m = proj.dimension
+        projected = np.stack([proj.projected_sink(sink, m) for sink in sinks])  # (sink, m)
+        out[:, :, j] = -1j * (proj.time_coefficients(time) @ projected.T)
     return out
+
+
+def component_retarded_matrix_time(
+    operator: ComponentSigmaOperator,
+    sources,
+    time,
+    validation_shifts,
+    *,
+    maximum_krylov_dimension: int = 160,
+    residual_tolerance: float = 1.0e-11,
+    include_channels: bool = True,
+) -> ComponentRetardedMatrixTime:
+    """Positive-time Green matrix from one Krylov space per source orbital (and, with
+    ``include_channels``, per channel) (``component_retarded_matrix_time``,
+    ``component_response.py:702-815``).  Each space stops on the Galerkin residual at
+    ``validation_shifts``; ``G_ab(t) = -i sink_a · V_m exp(-i t H_m) beta e_1``."""
+
+    columns, retained = _validated_source_columns(operator, sources)
+    t = np.asarray(time, dtype=np.float64)
+    z = np.asarray(validation_shifts, dtype=np.complex128)
+    if (t.ndim != 1 or t.size == 0 or np.any(t < 0.0) or not np.all(np.isfinite(t)) or z.ndim != 1 or z.size == 0
+            or not np.all(np.isfinite(z))):
+        raise ValueError("time and validation shifts must be finite nonempty vectors")
+    padding = int(operator.dimension) - retained
+    sinks = [_pad(item.sink, padding) for item in columns]
+    n = int(operator.dimension)
+    maximum = min(int(maximum_krylov_dimension), n)
+
+    def project(vector):
+        return arnoldi_resolvent(operator, _pad(vector, padding), z, maximum_dimension=maximum,
+                                 residual_tolerance=residual_tolerance, minimum_dimension=min(16, n),
+                                 check_interval=min(8, n), sink=sinks[0])
+
+    started = perf_counter()
+    combined = tuple(project(np.asarray(item.addition) + np.asarray(item.removal)) for item in columns)
+    retarded = _matrix_time_from_projections(combined, sinks, t)
+    combined_res = np.asarray([p.relative_time_residuals(t) for p in combined])
+    if include_channels:
+        add_p = tuple(project(item.addition) for item in columns)
+        rem_p = tuple(project(item.removal) for item in columns)
+        addition = _matrix_time_from_projections(add_p, sinks, t)
+        removal = _matrix_time_from_projections(rem_p, sinks, t)
+        add_res = np.asarray([p.relative_time_residuals(t) for p in add_p])
+        rem_res = np.asarray([p.relative_time_residuals(t) for p in rem_p])
+        defect = float(np.max(np.abs(retarded - addition - removal)))
+    else:
+        add_p = rem_p = ()
+        add_res = np.empty((0, t.size))
+        rem_res = np.empty((0, t.size))
+        addition = np.full_like(retarded, np.nan + 1j * np.nan)
+        removal = np.full_like(retarded, np.nan + 1j * np.nan)
+        defect = float("nan")
+    return ComponentRetardedMatrixTime(
+        time=t,
+        sites=tuple(item.site for item in columns),
+        addition=addition,
+        removal=removal,
+        retarded=retarded,
+        channel_sum_defect=defect,
+        combined_projections=combined,
+        addition_projections=add_p,
+        removal_projections=rem_p,
+        combined_relative_time_residuals=combined_res,
+        addition_relative_time_residuals=add_res,
+        removal_relative_time_residuals=rem_res,
+        wall_seconds=float(perf_counter() - started),
+        peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
+    )

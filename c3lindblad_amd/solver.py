@@ -114,8 +114,46 @@ class NewtonKrylovStep:
     accepted_residual_norm: float
 
 
+@dataclass(frozen=True)
+class RapidityPreconditionerDiagnostics:
+    """Conditioning of the rapidity diagonal (``cc/newton_krylov.py:127-134``), not of the full Jacobian."""
+
+    minimum_magnitude: float
+    maximum_magnitude: float
+    magnitude_condition: float
+    regularization_threshold: float
+    regularized_amplitudes: int
+
+
+def rapidity_preconditioner(problem: ComponentStaticProblem):
+    """``1 / (-i Lambda)`` with the 1e-12 regularisation and its diagnostics (``newton_krylov.py:158-192``)."""
+
+    diag = -1j * problem.oracle.rapidity_sums()
+    mags = np.abs(diag)
+    threshold = 1.0e-12 * max(float(np.max(mags, initial=0.0)), 1.0)
+    small = mags < threshold
+    diag[small] = threshold * np.exp(1j * np.angle(diag[small] + 1e-300))
+    finite = mags[~small]
+    minimum = float(np.min(finite)) if finite.size else threshold
+    maximum = float(np.max(mags, initial=threshold))
+    diagnostics = RapidityPreconditionerDiagnostics(
+        minimum_magnitude=minimum,
+        maximum_magnitude=maximum,
+        magnitude_condition=maximum / max(minimum, threshold),
+        regularization_threshold=threshold,
+        regularized_amplitudes=int(np.count_nonzero(small)),
+    )
+    return 1.0 / diag, diagnostics
+
+
 @dataclass
 class ComponentCCSolution:
+    """Result of ``solve_component_ccsd`` with the reference's fields (``cc/component_solver.py:37-54``).
+
+    ``rank6`` is always ``None``: the rank-six truncation diagnostic is not built (it does not enter
+    the CCSD equations or their convergence test).
+    """
+
     problem: ComponentStaticProblem
     variables: np.ndarray
     amplitudes: np.ndarray
@@ -130,6 +168,8 @@ class ComponentCCSolution:
     history: tuple
     wall_seconds: float
     peak_rss_kib: int
+    rank6: object = None
+    preconditioner: RapidityPreconditionerDiagnostics | None = None
     device_seconds: dict = field(default_factory=dict)
     device_amplitudes: object = field(default=None, repr=False)  # torch complex128 copy on the GPU
     warm_start_fallback: bool = False
@@ -162,14 +202,10 @@ class _DeviceNewton:
         self.gmres_coef = torch.zeros(coef, dtype=f64, device=_lib.DEVICE)
         cached = problem.__dict__.get("_inverse_preconditioner")
         if cached is None:
-            diag = -1j * problem.oracle.rapidity_sums()
-            mags = np.abs(diag)
-            threshold = 1.0e-12 * max(float(np.max(mags, initial=0.0)), 1.0)
-            small = mags < threshold
-            diag[small] = threshold * np.exp(1j * np.angle(diag[small] + 1e-300))
-            cached = torch.from_numpy(1.0 / diag).to(_lib.DEVICE)
+            inverse, diagnostics = rapidity_preconditioner(problem)
+            cached = (torch.from_numpy(inverse).to(_lib.DEVICE), diagnostics)
             problem.__dict__["_inverse_preconditioner"] = cached
-        self.inv_diag = cached
+        self.inv_diag, self.preconditioner = cached
         self.jvp_calls = 0
         self.residual_calls = 0
 
@@ -234,26 +270,43 @@ def solve_component_ccsd(
     initial_amplitudes=None,
     *,
     policy: NewtonKrylovPolicy | None = None,
+    rank6_tile_size: int = 4096,
+    compute_rank6: bool = False,
     accepted_step_callback=None,
     initial_variables=None,
 ) -> ComponentCCSolution:
     """Solve the tilde-real CCSD equations to ``||pack R|| <= policy.tolerance`` on the GPU.
 
+    Signature of the reference (``cc/component_solver.py:103-111``).  ``rank6_tile_size`` is
+    accepted and unused; ``compute_rank6=True`` (the optional rank-six truncation diagnostic,
+    outside the CCSD equations) raises ``NotImplementedError``.
+
     ``initial_variables`` (real tilde variables, NumPy or a GPU tensor, e.g. from a learned
-    warm start) replaces the initial amplitudes; when Newton does not converge from it the
-    solve restarts from the zero guess and ``warm_start_fallback`` is set.
+    warm start) replaces the initial amplitudes; when Newton does not converge from it, or the
+    linear solver rejects a Krylov vector built from it, the solve restarts from the zero guess
+    and ``warm_start_fallback`` is set.
     """
 
+    if compute_rank6:
+        raise NotImplementedError("compute_rank6=True: the rank-six truncation diagnostic is not built")
+    if isinstance(rank6_tile_size, bool) or not isinstance(rank6_tile_size, (int, np.integer)) or rank6_tile_size <= 0:
+        raise ValueError("rank6_tile_size must be a positive integer")
     if initial_variables is not None:
         if initial_amplitudes is not None:
             raise ValueError("give initial_amplitudes or initial_variables, not both")
-        first = _solve(problem, None, policy, accepted_step_callback, initial_variables)
-        if first.converged:
+        try:
+            first = _solve(problem, None, policy, accepted_step_callback, initial_variables)
+        except ValueError as error:
+            if "initial_variables" in str(error):
+                raise
+            first = None  # the native linear solver rejected a vector (nonfinite / singular)
+        if first is not None and first.converged:
             return first
         second = _solve(problem, None, policy, accepted_step_callback, None)
         second.warm_start_fallback = True
-        second.jvp_evaluations += first.jvp_evaluations
-        second.residual_evaluations += first.residual_evaluations
+        if first is not None:
+            second.jvp_evaluations += first.jvp_evaluations
+            second.residual_evaluations += first.residual_evaluations
         return second
     return _solve(problem, initial_amplitudes, policy, accepted_step_callback, None)
 
@@ -266,6 +319,18 @@ def _solve(problem, initial_amplitudes, policy, accepted_step_callback, initial_
     torch = state.torch
     bits = 32 if controls.jacobian_precision == "fp32" else 64
     state.dev.plan.set_precision("D", "jvp", bits)
+    try:
+        return _newton(problem, controls, state, bits, started, initial_amplitudes, accepted_step_callback,
+                       initial_variables)
+    finally:
+        if bits != 64:
+            state.dev.plan.set_precision("D", "jvp", 64)  # the JVP entry points stay FP64
+
+
+def _newton(problem, controls, state, bits, started, initial_amplitudes, accepted_step_callback,
+            initial_variables) -> ComponentCCSolution:
+    oracle = problem.oracle
+    torch = state.torch
     if initial_variables is not None:
         x0 = None
     elif initial_amplitudes is None:
@@ -337,8 +402,6 @@ def _solve(problem, initial_amplitudes, policy, accepted_step_callback, initial_
         norm = accepted_norm
     if not current:
         state.residual(x, state.r)
-    if bits != 64:
-        state.dev.plan.set_precision("D", "jvp", 64)  # the JVP entry points stay FP64
     dev = state.dev
     xh = x.cpu().numpy()
     amplitudes = dev.t.cpu().numpy()
@@ -360,5 +423,7 @@ def _solve(problem, initial_amplitudes, policy, accepted_step_callback, initial_
         history=tuple(history),
         wall_seconds=float(perf_counter() - started),
         peak_rss_kib=int(resource.getrusage(resource.RUSAGE_SELF).ru_maxrss),
+        rank6=None,
+        preconditioner=state.preconditioner,
         device_amplitudes=dev.t.clone(),
     )

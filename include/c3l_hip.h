@@ -210,12 +210,15 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
  * The shifts absorb completed columns eight at a time; the stop test after each batch finds
  * the exact first column that meets the criterion and freezes the shift state there.  The
  * host reads the stop flag asynchronously (at most one batch late), so up to 16 further basis
- * vectors may be computed (and ignored) after the stop. */
+ * vectors may be computed (and ignored) after the stop.  criterion = 0 tests the GMRES
+ * residual (the per-frequency GMRES of component_retarded_green); criterion = 1 tests the
+ * Galerkin residual |h_{m+1,m} e_m^T (z - H_m)^{-1} beta e_1| / beta (the stop rule of
+ * arnoldi_resolvent, green/krylov.py:257-281,380-390), and resid then holds it. */
 int c3l_arnoldi(c3l_plan* plan, const double* diag, double* V, double* H, int64_t dim, int32_t m_max,
                 const double* shifts, int32_t n_shifts, const double* sink, double beta, double* rot_c,
                 double* rot_s, double* pi, double* gamma, double* green, double* resid, double* scratch,
                 double* coef, double* status, double tolerance, int32_t min_dim, double breakdown_tol,
-                int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
+                int32_t criterion, int32_t* m_out, int32_t* breakdown_out, double* worst_out, void* stream);
 
 /* c3l_arnoldi for `count` (1..4) right-hand sides at once, in lockstep: basis i runs on
  * plans[i], and every sigma and orthogonalisation kernel of a step is one launch over all
