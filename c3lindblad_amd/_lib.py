@@ -45,6 +45,7 @@ SIGNATURES = {
     "c3l_rmultidot": ([_P, _I64, _I32, _P, _I64, _P, _P, _P], ctypes.c_int),
     "c3l_rmultiaxpy": ([_P, _I64, _I32, _P, ctypes.c_double, _P, _I64, _P], ctypes.c_int),
     "c3l_zmultidot": ([_P, _I64, _I32, _P, _I64, _I32, _P, _P, _P], ctypes.c_int),
+    "c3l_zmultidot_noalloc": ([_P, _I64, _I32, _P, _I64, _I32, _P, _P], ctypes.c_int),
     "c3l_zmultiaxpy": ([_P, _I64, _I32, _P, ctypes.c_double, _P, _I64, _P], ctypes.c_int),
     "c3l_tilde_precondition": ([_P, _P, _P, _P, _P, _P, _P, _P, _I64, _P], ctypes.c_int),
     "c3l_rscale": ([_P, _P, ctypes.c_double, _I64, _P], ctypes.c_int),

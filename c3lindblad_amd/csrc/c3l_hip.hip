@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1470,6 +1471,39 @@ int narrow_fixed_atoms(c3l_plan* plan, hipStream_t s) {
 
 }  // namespace
 
+// ---- allocation-free complex multidot (c3l_zmultidot_noalloc) --------------------------
+// Its partial sums live in this file-scope device array: one copy per GPU, held until the
+// process ends.  In host code `scratch` is only a symbol: the device address of each GPU's
+// copy comes from hipGetSymbolAddress, taken once per device with that device current (on
+// the entry point's first call there) and kept in scratch_ptr[].
+constexpr int64_t kNoAllocScratchDoubles = int64_t(1) << 20;  // 8 MiB per GPU
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+
+constexpr int kMaxNoAllocDevices = 64;
+std::mutex scratch_mu;
+double* scratch_ptr[kMaxNoAllocDevices] = {};
+
+// Device address of the current GPU's copy of `scratch` (looked up once per device).
+int current_scratch(double** out) {
+  int device = -1;
+  C3L_HIP_CHECK(hipGetDevice(&device));
+  if (device < 0 || device >= kMaxNoAllocDevices)
+    return fail(C3L_EINVAL, "c3l_zmultidot_noalloc: device index beyond the supported range");
+  std::lock_guard<std::mutex> guard(scratch_mu);
+  if (scratch_ptr[device] == nullptr) {
+    void* ptr = nullptr;
+    C3L_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (ptr == nullptr) return fail(C3L_EHIP, "c3l_zmultidot_noalloc: no device address for the scratch array");
+    scratch_ptr[device] = static_cast<double*>(ptr);
+  }
+  *out = scratch_ptr[device];
+  return C3L_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int c3l_plan_set_precision(c3l_plan* plan, int32_t kind, int32_t program, int32_t bits) {
@@ -1733,6 +1767,34 @@ int c3l_zmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, i
   });
   C3L_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(zfinish_kernel, dim3(ncols), dim3(kThreads), 0, s, reinterpret_cast<const double2*>(scratch), chunks,
+                     reinterpret_cast<double2*>(out));
+  C3L_HIP_CHECK(hipGetLastError());
+  return C3L_OK;
+}
+
+// c3l_zmultidot with its partial sums in the file-scope `scratch` array instead of a caller
+// buffer: no allocation of any kind.  Every partial entry read by the finishing kernel is
+// written by the partial kernel of the same call, so nothing needs clearing between calls.
+int c3l_zmultidot_noalloc(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim,
+                          int32_t conjugate, double* out, void* stream) {
+  if (ncols <= 0 || dim <= 0 || ld < dim || V == nullptr || w == nullptr || out == nullptr)
+    return fail(C3L_EINVAL, "c3l_zmultidot_noalloc: invalid argument");
+  const int rows = dot_rows_for(ncols);
+  const int64_t chunks = (dim + rows - 1) / rows;
+  if ((int64_t)ncols * chunks * 2 > kNoAllocScratchDoubles)
+    return fail(C3L_EINVAL, "c3l_zmultidot_noalloc: ncols * ceil(dim / rows) partial sums exceed the " +
+                                std::to_string(kNoAllocScratchDoubles / 2) + " complex entries of the scratch array");
+  double* partial = nullptr;
+  const int rc = current_scratch(&partial);
+  if (rc != C3L_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dispatch_rows(rows, [&](auto R) {
+    hipLaunchKernelGGL((zmultidot_partial_kernel<decltype(R)::value>), dim3(chunks, (ncols + 3) / 4), dim3(kThreads), 0,
+                       s, reinterpret_cast<const double2*>(V), ld, ncols, (const double2*)nullptr,
+                       reinterpret_cast<const double2*>(w), dim, conjugate, reinterpret_cast<double2*>(partial), chunks);
+  });
+  C3L_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(zfinish_kernel, dim3(ncols), dim3(kThreads), 0, s, reinterpret_cast<const double2*>(partial), chunks,
                      reinterpret_cast<double2*>(out));
   C3L_HIP_CHECK(hipGetLastError());
   return C3L_OK;

@@ -140,6 +140,15 @@ int c3l_rmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alp
 /* Complex: out[k] = sum_i conj?(V[k*ld+i]) * w[i] (conj when conjugate != 0). */
 int c3l_zmultidot(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim, int32_t conjugate,
                   double* scratch, double* out, void* stream);
+/* c3l_zmultidot without the scratch argument and without any allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Its partial sums live in a
+ * library-owned device array of 2^20 doubles (8 MiB) per GPU, held until the process ends:
+ * calls with ncols * ceil(dim / rows) > 524288 complex partials (rows = 512 for ncols <= 64,
+ * 1024 for ncols <= 256, else 2048) are rejected with C3L_EINVAL.  Every call on one GPU shares
+ * that array, so calls that overlap on one GPU (concurrent streams or threads) race. */
+int c3l_zmultidot_noalloc(const double* V, int64_t ld, int32_t ncols, const double* w, int64_t dim,
+                          int32_t conjugate, double* out, void* stream);
 /* Complex: w[i] += scale * sum_k alpha[k] * V[k*ld + i] */
 int c3l_zmultiaxpy(const double* V, int64_t ld, int32_t ncols, const double* alpha, double scale, double* w,
                    int64_t dim, void* stream);
