@@ -74,6 +74,12 @@ def config2_grid() -> np.ndarray:
     return np.linspace(-8.0, 8.0, 4001)
 
 
+def config3_modes(instance: int) -> int:
+    """Pseudomode count of one instance of the uneven-size batch (uniform on 4..32)."""
+
+    return int(np.random.default_rng(SEED_BASE + 1000 * 3 + instance).integers(4, 33))
+
+
 def config3_model(instance: int) -> FermionicLindbladModel:
     """One instance of the uneven-size SIAM batch (4..32 modes, U in [0, 6])."""
 
@@ -83,3 +89,137 @@ def config3_model(instance: int) -> FermionicLindbladModel:
     eps = -0.5 * U + float(rng.normal(0.0, 0.5))
     return siam_model(modes, U, seed=SEED_BASE + 1000 * 3 + 100000 + instance, epsilon_d=eps, span=5.0, hop=0.2,
                       gamma_range=(0.01, 0.6))
+
+
+CONFIG3_INSTANCES = 512
+
+
+def config3_grid() -> np.ndarray:
+    """The batch's 2048 frequencies per instance."""
+
+    return np.linspace(-8.0, 8.0, 2048)
+
+
+def _symmetric_orbit_values(orbitals: int, rng, scale: float) -> np.ndarray:
+    """Real ``V[p,q,r,s]`` with the 8-fold permutational symmetry of real orbitals
+    (``(pq|rs) = (qp|rs) = (pq|sr) = (rs|pq) = ...``): one ``N(0, scale^2)`` value per orbit of
+    index tuples, drawn in lexicographic order of the orbit's smallest member."""
+
+    def orbit(p, q, r, s):
+        return {(p, q, r, s), (q, p, r, s), (p, q, s, r), (q, p, s, r),
+                (r, s, p, q), (s, r, p, q), (r, s, q, p), (s, r, q, p)}
+
+    V = np.zeros((orbitals,) * 4)
+    seen = set()
+    for index in np.ndindex(*V.shape):
+        if index in seen:
+            continue
+        members = orbit(*index)
+        value = float(rng.normal(0.0, scale))
+        for m in members:
+            V[m] = value
+        seen |= members
+    return V
+
+
+def multiorbital_model(
+    orbitals: int,
+    modes_per_orbital,
+    *,
+    seed: int,
+    U: float = 3.0,
+    J: float = 0.5,
+    perturbation: float = 0.1,
+    span: float = 6.0,
+    hop: float = 0.2,
+    gamma_range: tuple[float, float] = (0.02, 0.5),
+    bias: float = 0.5,
+    epsilon: float | None = None,
+    impurity_hopping: float = 0.0,
+) -> FermionicLindbladModel:
+    """Multi-orbital impurity with a general two-body interaction and Lindblad pseudomodes.
+
+    Interaction (spatial, chemists' order, ``H = 1/2 sum (ab|cd) c†_{aσ} c†_{cσ'} c_{dσ'} c_{bσ}``):
+    Kanamori ``(aa|aa) = U``, ``(aa|bb) = U' = U - 2J``, ``(ab|ba) = (ab|ab) = J`` (so opposite-spin
+    density ``U - 2J``, same-spin ``U - 3J``, spin flip and pair hopping ``J``) plus a random
+    real tensor with the 8-fold symmetry, ``N(0, perturbation^2)`` per symmetry orbit.  It enters
+    the model as a ``two_body_interaction`` on the impurity orbitals ``0..orbitals-1`` through the
+    spatial-to-spin-orbital map of ``interaction_from_spatial_tensor``
+    (``cc/_static_schema.py:77-91``).  Impurity levels sit at the half-filling point
+    ``-(U + 2(K-1)U' - (K-1)J)/2`` unless ``epsilon`` is given.  Orbital ``a`` couples to its own
+    ``modes_per_orbital[a]`` pseudomodes (an int applies to every orbital): energies uniform on
+    ``[-span, span]``, hoppings ``N(0, hop^2)``, loss and gain uniform on ``gamma_range``; the first
+    half of each orbital's modes belongs to a left lead at chemical potential ``+bias`` and the
+    rest to a right lead at ``-bias``, the larger rate being the loss above the lead's potential
+    and the gain below it.
+    """
+
+    from .model import SparseTwoBodyInteraction
+
+    rng = np.random.default_rng(seed)
+    K = int(orbitals)
+    counts = [int(modes_per_orbital)] * K if np.isscalar(modes_per_orbital) else [int(m) for m in modes_per_orbital]
+    if len(counts) != K or K < 1 or min(counts) < 0:
+        raise ValueError("modes_per_orbital must give a non-negative count per orbital")
+    n = K + sum(counts)
+    Up = U - 2.0 * J
+    level = -(U + 2 * (K - 1) * Up - (K - 1) * J) / 2.0 if epsilon is None else float(epsilon)
+    E = np.zeros((n, n))
+    loss = np.zeros(n)
+    gain = np.zeros(n)
+    for a in range(K):
+        E[a, a] = level
+        for b in range(a + 1, K):
+            E[a, b] = E[b, a] = impurity_hopping
+    mode = K
+    for a, count in enumerate(counts):
+        for k in range(count):
+            mu = bias if k < (count + 1) // 2 else -bias
+            E[mode, mode] = rng.uniform(-span, span)
+            E[a, mode] = E[mode, a] = rng.normal(0.0, hop)
+            x, y = rng.uniform(*gamma_range, 2)
+            big, small = max(x, y), min(x, y)
+            loss[mode], gain[mode] = (big, small) if E[mode, mode] > mu else (small, big)
+            mode += 1
+    C = np.zeros((K,) * 4)
+    for a in range(K):
+        C[a, a, a, a] = U
+        for b in range(K):
+            if a != b:
+                C[a, a, b, b] = Up
+                C[a, b, b, a] = J
+                C[a, b, a, b] = J
+    C = C + _symmetric_orbit_values(K, rng, perturbation)
+    terms = []
+    for w in range(K):
+        for x in range(K):
+            for y in range(K):
+                for z in range(K):
+                    value = 0.5 * C[w, z, x, y]  # T[w,x,y,z] of interaction_from_spatial_tensor
+                    if value == 0.0:
+                        continue
+                    for s1 in (0, 1):
+                        for s2 in (0, 1):
+                            creation = (2 * w + s1, 2 * x + s2)
+                            annihilation = (2 * z + s1, 2 * y + s2)
+                            if creation[0] != creation[1] and annihilation[0] != annihilation[1]:
+                                terms.append((*creation, *annihilation, complex(value)))
+    return FermionicLindbladModel(
+        E=E,
+        gamma_loss=np.diag(loss),
+        gamma_gain=np.diag(gain),
+        U=0.0,
+        interaction_sites=tuple(range(K)),
+        two_body_interaction=SparseTwoBodyInteraction(2 * n, tuple(terms)),
+    )
+
+
+def config5_model(instance: int = 0) -> FermionicLindbladModel:
+    """Three Kanamori orbitals (U = 3, J = 0.5, U' = 2) + V_pqrs ~ N(0, 0.1^2), 12 pseudomodes each
+    (n = 39), lead bias +-0.5."""
+
+    return multiorbital_model(3, 12, seed=SEED_BASE + 1000 * 5 + instance)
+
+
+def config5_grid() -> np.ndarray:
+    return np.linspace(-8.0, 8.0, 8001)

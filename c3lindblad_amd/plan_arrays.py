@@ -174,15 +174,25 @@ def lower_plan(
         if targets.intersection(int(a) for a in bp.factor_atom):
             raise ValueError("a creation tensor reads another creation tensor: one program launch cannot order them")
     blocks = []
+    dense_coords: dict[int, tuple[np.ndarray, np.ndarray]] = {}  # every index tuple of a padded rank-deg tensor
+
+    def dense(deg: int):
+        if deg not in dense_coords:
+            flat = np.arange(n**deg, dtype=np.int64)
+            coord = (np.stack([(flat // n**d) % n for d in range(deg)], axis=1).astype(np.int32) if deg
+                     else np.zeros((1, 0), np.int32))
+            flat.setflags(write=False)
+            coord.setflags(write=False)
+            dense_coords[deg] = (flat, coord)
+        return dense_coords[deg]
+
     for name, programs in plan.programs.items():
         coords = out_sets.get(name)
         for bp in programs:
             deg = sum(bp.key)
             if bp.target_atom >= 0:
                 # creation tensor: every index tuple of its padded storage, written in place
-                flat = np.arange(n**deg, dtype=np.int64)
-                coord = (np.stack([(flat // n**d) % n for d in range(deg)], axis=1).astype(np.int32) if deg
-                         else np.zeros((1, 0), np.int32))
+                flat, coord = dense(deg)
                 dest = plan.atom_offsets[bp.target_atom] + flat
             elif deg == 0:
                 coord = np.zeros((1, 0), dtype=np.int32)
@@ -201,7 +211,7 @@ def lower_plan(
                 (
                     PROGRAMS[name],
                     deg,
-                    np.ascontiguousarray(coord),
+                    coord if coord.flags.c_contiguous else np.ascontiguousarray(coord),
                     dest,
                     bp.coefficients,
                     bp.factor_start,

@@ -208,11 +208,36 @@ def ph_neutral_spectrum(problem) -> np.ndarray:
     return np.linalg.eigvals(K)
 
 
+def lindblad_superoperator_sparse(model):
+    """``L_even`` of ``lindblad_generators`` (same column-major vec and the same terms), built
+    from sparse Kronecker products, without the steady state; returns ``(L, dim)``."""
+
+    import scipy.sparse as sparse
+
+    N = model.n_spin_orbitals
+    c = [sparse.csr_matrix(m) for m in physical_annihilators(N)]
+    H = sparse.csr_matrix(physical_hamiltonian(model, [m.toarray() for m in c]))
+    loss = np.asarray(model.physical_gamma_loss)
+    gain = np.asarray(model.physical_gamma_gain)
+    dim = H.shape[0]
+    I = sparse.identity(dim, dtype=np.complex128, format="csr")
+    L = -1j * (sparse.kron(I, H) - sparse.kron(H.T, I))
+    for i in range(N):
+        ci_dag = c[i].conj().T
+        for j in range(N):
+            if loss[i, j] != 0.0:
+                ln = ci_dag @ c[j]
+                L = L + loss[i, j] * (2.0 * sparse.kron(c[i].conj(), c[j]) - sparse.kron(I, ln) - sparse.kron(ln.T, I))
+            if gain[i, j] != 0.0:
+                gn = c[j] @ ci_dag
+                L = L + gain[i, j] * (2.0 * sparse.kron(c[j].T, ci_dag) - sparse.kron(I, gn) - sparse.kron(gn.T, I))
+    return L.tocsr(), dim
+
+
 def lindblad_neutral_spectrum(model) -> np.ndarray:
     """Eigenvalues of ``L`` on operators ``|a><b|`` with equal up and down particle numbers."""
 
-    _, L, _, c = lindblad_generators(model)
-    dim = c[0].shape[0]
+    L, dim = lindblad_superoperator_sparse(model)
     states = np.arange(dim)
     n_up = np.array([bin(int(s) & 0x5555555555555555).count("1") for s in states])
     n_dn = np.array([bin(int(s) & 0xAAAAAAAAAAAAAAAA).count("1") for s in states])
@@ -220,7 +245,7 @@ def lindblad_neutral_spectrum(model) -> np.ndarray:
     a, b = np.meshgrid(states, states, indexing="ij")
     keep = (n_up[a] == n_up[b]) & (n_dn[a] == n_dn[b])
     sel = np.flatnonzero(keep.reshape(-1, order="F"))
-    return np.linalg.eigvals(L[np.ix_(sel, sel)])
+    return np.linalg.eigvals(L[sel][:, sel].toarray())
 
 
 def exact_covariance(model) -> np.ndarray:

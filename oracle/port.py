@@ -24,7 +24,7 @@ class PortPlan:
         for off, vec in zip(lowered.const_offset, lowered.const_vec):
             self.ws[off : off + self.n] = lowered.vectors[vec]
 
-    def load(self, kind: int, v: np.ndarray) -> None:
+    def load(self, kind: int, v: np.ndarray, *, creation: bool = True) -> None:
         n = self.n
         v = np.asarray(v, dtype=np.complex128)
         for k, off, deg, src, sign in self.lp.bases:
@@ -40,8 +40,39 @@ class PortPlan:
                 parent = self.ws[p : p + n**r].reshape((n,) * r, order="F")  # first index fastest
                 out = np.tensordot(parent, self.lp.vectors[vi], axes=([s], [0]))
                 self.ws[o : o + n ** (r - 1)] = sg * np.ravel(out, order="F")
-        if kind == 0:  # amplitudes changed: rebuild the creation tensors that depend on them
+        if kind == 0 and creation:  # amplitudes changed: rebuild the creation tensors that depend on them
             self.apply(4, None)
+
+    def apply_sampled(self, program: int, per_block: int, seed: int = 0):
+        """Program values at up to ``per_block`` seeded random coordinates of every output block
+        (all coordinates of smaller blocks), without the diagonal term: ``(dest, values)``.
+
+        The sample costs ``per_block`` coordinates per block instead of the whole output, so
+        residuals can be spot-checked where the full evaluation would take too long.
+        """
+
+        n = self.n
+        rng = np.random.default_rng(seed)
+        dests, values = [], []
+        for pid, deg, coord, dest, coeff, fstart, foff, fcode in self.lp.blocks:
+            if pid != program:
+                continue
+            rows = np.arange(dest.size) if dest.size <= per_block else np.sort(
+                rng.choice(dest.size, per_block, replace=False))
+            sub = coord[rows]
+            acc = np.zeros(rows.size, dtype=np.complex128)
+            for t in range(coeff.size):
+                prod = np.full(rows.size, coeff[t], dtype=np.complex128)
+                for f in range(fstart[t], fstart[t + 1]):
+                    code = int(fcode[f])
+                    flat = np.zeros(rows.size, dtype=np.int64)
+                    for slot in range((code & 7) - 1, -1, -1):
+                        flat = flat * n + sub[:, (code >> (3 + 2 * slot)) & 3]
+                    prod *= self.ws[foff[f] + flat]
+                acc += prod
+            dests.append(dest[rows])
+            values.append(acc)
+        return np.concatenate(dests), np.concatenate(values)
 
     def apply(self, program: int, out: np.ndarray | None, diag=None, x=None) -> np.ndarray:
         """Run one program; ``out=None`` writes a workspace (creation) program in place."""

@@ -19,10 +19,21 @@ from tests.conftest import random_model
 
 
 def _setup(n, U, seed):
+    """``n = "kanamori3"`` / ``"kanamori4"``: two Kanamori orbitals with exchange, pair hopping and a
+    random 8-fold symmetric V_pqrs (a multi-site general ``two_body_interaction``) plus one or
+    two pseudomodes; otherwise a random single-impurity model."""
+
+    from c3lindblad_amd.configs import multiorbital_model
     from c3lindblad_amd.lowering import factorized_cc_problem
     from c3lindblad_amd.static import ComponentStaticProblem
 
-    p = factorized_cc_problem(random_model(n, U, seed))
+    if n == "kanamori3":
+        model = multiorbital_model(2, (1, 0), seed=3, U=2.0, J=0.3, perturbation=0.1, impurity_hopping=0.3)
+    elif n == "kanamori4":
+        model = multiorbital_model(2, (1, 1), seed=1, U=2.0, J=0.3, perturbation=0.1, impurity_hopping=0.2)
+    else:
+        model = random_model(n, U, seed)
+    p = factorized_cc_problem(model)
     return p, ComponentStaticProblem.compile(p)
 
 
@@ -30,7 +41,7 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
-@pytest.mark.parametrize("n,U,seed", [(1, 1.3, 1), (2, 0.9, 2), (3, 1.2, 3)])
+@pytest.mark.parametrize("n,U,seed", [(1, 1.3, 1), (2, 0.9, 2), (3, 1.2, 3), ("kanamori3", 0, 3)])
 def test_polynomial_residual_equals_fock_brute_force(n, U, seed):
     from oracle import polynomial as poly
     from oracle.fock import PHFockSpace
@@ -42,16 +53,16 @@ def test_polynomial_residual_equals_fock_brute_force(n, U, seed):
     assert _rel(PHFockSpace(p).residual(p.coordinates, t), R) < 1e-13
 
 
-@pytest.mark.parametrize("n,charge", [(1, 1), (2, 1), (2, -1)])
+@pytest.mark.parametrize("n,charge", [(1, 1), (2, 1), (2, -1), (3, 1), (3, -1), ("kanamori3", 1), ("kanamori3", -1)])
 def test_polynomial_sigma_equals_fock_brute_force(n, charge):
     from c3lindblad_amd.lowering import response_coordinates
     from oracle import polynomial as poly
     from oracle.fock import PHFockSpace
 
-    p, _ = _setup(n, 0.8, 30 + n)
-    rng = np.random.default_rng(n)
+    p, _ = _setup(n, 0.8, 30 + (n if isinstance(n, int) else 3))
+    rng = np.random.default_rng(n if isinstance(n, int) else 5)
     t = p.tilde.unpack(0.1 * rng.normal(size=p.num_variables))
-    rc = response_coordinates(n, charge)
+    rc = response_coordinates(p.model.n_sites, charge)
     x = rng.normal(size=rc.dimension) + 1j * rng.normal(size=rc.dimension)
     want = PHFockSpace(p).sigma(p.coordinates, t, rc, x)
     assert _rel(poly.sigma(p, t, poly.coordinate_masks(rc), x), want) < 1e-13
@@ -69,7 +80,8 @@ def test_polynomial_jvp_equals_finite_difference():
     assert np.max(np.abs(poly.jvp(p, t, d) - fd)) < 1e-9
 
 
-@pytest.mark.parametrize("n,U,seed", [(1, 1.3, 1), (2, 0.9, 2), (3, 1.2, 3), (4, 2.5, 4)])
+@pytest.mark.parametrize("n,U,seed", [(1, 1.3, 1), (2, 0.9, 2), (3, 1.2, 3), (4, 2.5, 4), ("kanamori3", 0, 3),
+                                     ("kanamori4", 0, 4)])
 def test_compiled_plan_equals_polynomial_oracle(n, U, seed):
     from oracle import polynomial as poly
     from oracle.port import PortPlan
@@ -90,14 +102,15 @@ def test_compiled_plan_equals_polynomial_oracle(n, U, seed):
     assert p.tilde.defect(R) < 1e-13 * max(1.0, np.max(np.abs(R)))
 
 
-@pytest.mark.parametrize("n,charge", [(2, 1), (3, -1), (4, 1)])
+@pytest.mark.parametrize("n,charge", [(2, 1), (3, -1), (4, 1), ("kanamori3", 1), ("kanamori4", -1)])
 def test_compiled_sigma_and_sources_equal_polynomial_oracle(n, charge):
     from c3lindblad_amd.eom import ComponentEOMProblem, _sink, channel_operators, eom_ccsd_problem
     from c3lindblad_amd.lowering import physical_mode_operator
     from oracle import polynomial as poly
     from oracle.port import PortPlan
 
-    p, sp = _setup(n, 1.7, 40 + n)
+    p, sp = _setup(n, 1.7, 40 + (n if isinstance(n, int) else 0))
+    n = p.model.n_sites
     rng = np.random.default_rng(n + 7)
     t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
     ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, charge)
@@ -216,3 +229,20 @@ def test_covariance_at_zero_amplitudes_is_the_reference_covariance(n, U, seed):
     p, _ = _setup(n, U, seed)
     got = poly.covariance(p, np.zeros(p.num_amplitudes, dtype=np.complex128))
     assert np.max(np.abs(got - p.reference.gamma)) < 1e-13
+
+
+def test_port_sampled_program_equals_full_program():
+    """``PortPlan.apply_sampled`` (the spot check used where the full port residual is too slow)
+    returns exactly the full program's values at its coordinates."""
+
+    from oracle.port import PortPlan
+
+    p, sp = _setup(4, 2.5, 4)
+    rng = np.random.default_rng(9)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    port = PortPlan(sp.lowered)
+    port.load(0, t, creation=False)
+    full = port.apply(0, np.zeros(p.num_amplitudes, complex))
+    dest, values = port.apply_sampled(0, 7, seed=1)
+    assert dest.size == len({int(d) for d in dest}) and dest.size < p.num_amplitudes
+    assert np.array_equal(values, full[dest])
