@@ -45,6 +45,7 @@ from .response import (
     solve_component_shifted,
 )
 from .matsubara import MatsubaraGrid, component_matsubara_green, matsubara_grid
+from .checkpoint import CheckpointedArnoldiResult, checkpointed_arnoldi_resolvent, eom_checkpoint_identity
 from .observables import (
     ComponentObservableProblem,
     contact_currents,
@@ -54,6 +55,9 @@ from .observables import (
 
 __all__ = [
     "ArnoldiResolvent",
+    "CheckpointedArnoldiResult",
+    "checkpointed_arnoldi_resolvent",
+    "eom_checkpoint_identity",
     "ComponentCCSolution",
     "MatsubaraGrid",
     "RapidityPreconditionerDiagnostics",

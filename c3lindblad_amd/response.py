@@ -85,11 +85,13 @@ class ArnoldiResolvent:
 
     @property
     def check_dimensions(self) -> tuple:
-        return (self.dimension,)
+        checks = self.__dict__.get("_checks")
+        return checks[0] if checks is not None else (self.dimension,)
 
     @property
     def check_maximum_relative_residuals(self) -> np.ndarray:
-        return np.asarray([float(np.max(self.shift_relative_residuals))])
+        checks = self.__dict__.get("_checks")
+        return checks[1] if checks is not None else np.asarray([float(np.max(self.shift_relative_residuals))])
 
     @property
     def orthogonality_defect(self) -> float:

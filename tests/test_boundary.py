@@ -166,3 +166,13 @@ def test_arnoldi_resolvent_reduced_solves_on_the_host():
 
     assert np.allclose(c[1], scipy.linalg.expm(-0.4j * H[:m, :m]) @ e1, rtol=1e-10, atol=1e-12)
     assert np.allclose(c[0], e1)
+
+
+def test_checkpoint_resource_guard_and_identity():
+    from c3lindblad_amd.checkpoint import require_resources
+
+    require_resources(required_memory_bytes=10, required_disk_bytes=10, available_memory_bytes=10, free_disk_bytes=10)
+    with pytest.raises(MemoryError):
+        require_resources(required_memory_bytes=11, required_disk_bytes=0, available_memory_bytes=10, free_disk_bytes=0)
+    with pytest.raises(OSError):
+        require_resources(required_memory_bytes=0, required_disk_bytes=11, available_memory_bytes=0, free_disk_bytes=10)

@@ -228,3 +228,41 @@ def test_matsubara_green_matches_dense_resolvents(readme):
     want = np.where((grid.indices < 0)[:, None, None], upper.conj().swapaxes(1, 2), upper)
     assert _rel(out.green, want) < 1e-8
     assert out.max_relative_residual <= 1e-10
+
+
+def test_checkpointed_arnoldi_resumes_exactly(siam6, tmp_path):
+    """An interrupted (maximum_new_steps) and resumed disk-backed run equals the uninterrupted run bit
+    for bit (``tests/green/test_checkpointed_arnoldi.py:67-135`` checks in-memory against
+    checkpointed the same way); its transfer matches dense resolvents; a checkpoint refuses other
+    inputs."""
+
+    from c3lindblad_amd import checkpointed_arnoldi_resolvent, eom_checkpoint_identity
+
+    print("case checkpointed arnoldi n=6", flush=True)
+    A, P, sources = siam6["A"], siam6["P"], siam6["sources"]
+    b = sources[0].addition
+    sinks = np.stack([sources[0].sink, np.roll(sources[0].sink, 3)])
+    z = np.linspace(-3.0, 3.0, 13) + 0.1j
+    ident = eom_checkpoint_identity(siam6["ctx"])
+    kw = dict(maximum_dimension=160, checkpoint_dimensions=(20, 40, 60, 80, 100, 120, 140, 160),
+              residual_tolerance=1e-10, provenance=ident)
+    whole = checkpointed_arnoldi_resolvent(siam6["op"](), b, sinks, z, tmp_path / "whole", **kw)
+    part = checkpointed_arnoldi_resolvent(siam6["op"](), b, sinks, z, tmp_path / "split", maximum_new_steps=33, **kw)
+    assert part.projection.dimension == 33 and not part.projection.converged
+    done = checkpointed_arnoldi_resolvent(siam6["op"](), b, sinks, z, tmp_path / "split", **kw)
+    m = whole.projection.dimension
+    print("dimension", m, whole.projection.check_dimensions, flush=True)
+    assert whole.projection.converged and done.projection.dimension == m
+    assert np.array_equal(done.projection.hessenberg, whole.projection.hessenberg)
+    assert np.array_equal(done.sink_projections, whole.sink_projections)
+    assert np.array_equal(done.transfer(z), whole.transfer(z))
+    mirror = np.load(tmp_path / "split" / "basis.npy", mmap_mode="r")
+    assert np.array_equal(mirror[:, : m + 1], whole.projection.basis.cpu().numpy().T)
+    eye = np.eye(P)
+    want = np.stack([sinks @ np.linalg.solve(zk * eye - A, b) for zk in z])
+    assert _rel(whole.transfer(z), want) < 1e-8
+    with pytest.raises(ValueError, match="do not match"):
+        checkpointed_arnoldi_resolvent(siam6["op"](), b, sinks, z, tmp_path / "split",
+                                       **{**kw, "residual_tolerance": 1e-9})
+    with pytest.raises(ValueError, match="zero new steps"):
+        checkpointed_arnoldi_resolvent(siam6["op"](), b, sinks, z, tmp_path / "fresh", maximum_new_steps=0, **kw)
