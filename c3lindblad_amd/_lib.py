@@ -32,6 +32,7 @@ SIGNATURES = {
     "c3l_plan_add_base": ([_P, _I32, _I64, _I32, _P, _P], ctypes.c_int),
     "c3l_plan_add_level": ([_P, _I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "c3l_plan_add_block": ([_P, _I32, _I32, _I64, _P, _P, _I32, _P, _P, _I32, _P, _P], ctypes.c_int),
+    "c3l_plan_add_dense_block": ([_P, _I32, _I32, _I64, _I32, _P, _P, _I32, _P, _P], ctypes.c_int),
     "c3l_plan_load": ([_P, _I32, _P, _P], ctypes.c_int),
     "c3l_plan_apply": ([_P, _I32, _P, _P, _P, _P], ctypes.c_int),
     "c3l_plan_set_timing": ([_P, _I32], ctypes.c_int),
@@ -55,6 +56,10 @@ SIGNATURES = {
     "c3l_gmres": (
         [_P, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _P, ctypes.c_double, ctypes.c_double, _I32, _I32, _P, _P, _P,
          _P, _P],
+        ctypes.c_int,
+    ),
+    "c3l_gmres_group": (
+        [_P, _P, _I32, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P, _P, _P, _P],
         ctypes.c_int,
     ),
     "c3l_bf16_gemm": ([_P, _I64, _I64, _P, _I64, _I64, _P, _I64, _I32, _I32, _I32, ctypes.c_float, ctypes.c_float, _P],
@@ -81,6 +86,11 @@ SIGNATURES = {
     ),
     "c3l_arnoldi_begin": (
         [_P, _I32, _P, _P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+         ctypes.c_double, _I32, ctypes.c_double, _P, _P],
+        ctypes.c_int,
+    ),
+    "c3l_arnoldi_group_begin": (
+        [_P, _I32, _I32, _P, _P, _P, _P, _I64, _I32, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
          ctypes.c_double, _I32, ctypes.c_double, _P, _P],
         ctypes.c_int,
     ),

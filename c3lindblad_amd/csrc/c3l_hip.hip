@@ -307,14 +307,16 @@ struct WedgeEntry {
 
 struct WedgeBlockDesc {
   int64_t count;
-  const uint32_t* coord;  // packed indices, one word per coordinate
-  const int64_t* dest;
+  const uint32_t* coord;  // packed indices, one word per coordinate (null: dense block)
+  const int64_t* dest;    // output positions (null: dense block)
   const WedgeEntry* entries;
   const WedgeFactor* factors;
   int32_t n_entries;
   int32_t lanes_log2;
-  int32_t reps;  // coordinates per thread (two-factor kernel, lanes_log2 == 0 only)
-  int32_t pad;
+  int32_t reps;     // coordinates per thread (two-factor kernel, lanes_log2 == 0 only)
+  int32_t dense_n;  // dense block: coordinate e is the index tuple of flat index e (first index
+                    // fastest, base dense_n) and writes position dense_base + e
+  int64_t dense_base;
 };
 
 constexpr int kMaxReps = 2;
@@ -361,11 +363,22 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
     act[r] = r < d.reps && e[r] < d.count;
     i0[r] = i1[r] = i2[r] = i3[r] = 0;
     if (act[r]) {
-      const uint32_t c = d.coord[e[r]];
-      i0[r] = c & 255u;
-      i1[r] = (c >> 8) & 255u;
-      i2[r] = (c >> 16) & 255u;
-      i3[r] = c >> 24;
+      if (d.coord != nullptr) {
+        const uint32_t c = d.coord[e[r]];
+        i0[r] = c & 255u;
+        i1[r] = (c >> 8) & 255u;
+        i2[r] = (c >> 16) & 255u;
+        i3[r] = c >> 24;
+      } else {  // dense: e < dense_n^deg, so the unused higher indices come out 0
+        const int n = d.dense_n;
+        int f = (int)e[r];
+        i0[r] = f % n;
+        f /= n;
+        i1[r] = f % n;
+        f /= n;
+        i2[r] = f % n;
+        i3[r] = f / n;
+      }
     }
     acc[r] = czero<C>();
   }
@@ -418,7 +431,7 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       if (act[r]) {
-        const int64_t dd = d.dest[e[r]];
+        const int64_t dd = d.dest != nullptr ? d.dest[e[r]] : d.dense_base + e[r];
         double2 v = to_d2(acc[r]);
         if (diag != nullptr) v = cadd(v, cmul(diag[dd], x[dd]));
         out[dd] = v;
@@ -780,6 +793,8 @@ struct PlanBlock {
   int32_t n_factors;
   int64_t max_dest = -1;
   bool two_factors = true;
+  bool dense = false;  // every index tuple of a padded rank-deg tensor, written at dense_base + flat
+  int64_t dense_base = 0;
   // host copies, concatenated into the program's device arrays when it is built
   std::vector<uint32_t> coord;
   std::vector<int64_t> dest;
@@ -1125,19 +1140,15 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
   return C3L_OK;
 }
 
-int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
-                       const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
-                       int32_t n_factors, const int64_t* foffset, const int32_t* fcode) {
-  if (plan == nullptr || program < 0 || program >= kPrograms || deg < 0 || deg > 4 || count < 0 ||
-      count > INT32_MAX || n_entries < 0 || n_factors < 0)
-    return fail(C3L_EINVAL, "c3l_plan_add_block: invalid argument");
-  if (plan->n > 255) return fail(C3L_EINVAL, "c3l_plan_add_block: more than 255 sites per group");
+}  // extern "C"
+
+namespace {
+
+// Entries and factors of a block being registered (shared by the listed and dense forms).
+int parse_block_entries(c3l_plan* plan, PlanBlock& b, int32_t deg, int32_t n_entries, const double* coeff,
+                        const int32_t* fstart, int32_t n_factors, const int64_t* foffset, const int32_t* fcode) {
   if (n_entries > 0 && (fstart[0] != 0 || fstart[n_entries] != n_factors))
     return fail(C3L_EINVAL, "c3l_plan_add_block: fstart mismatch");
-  PlanBlock b;
-  b.program = program;
-  b.deg = deg;
-  b.count = count;
   b.n_entries = n_entries;
   b.n_factors = n_factors;
   b.factors.resize((size_t)n_factors);
@@ -1166,6 +1177,34 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
     if (en.nf != 2) b.two_factors = false;
     b.entries[(size_t)t] = en;
   }
+  return C3L_OK;
+}
+
+void finish_block(c3l_plan* plan, PlanBlock&& b) {
+  plan->blocks.push_back(std::move(b));
+  plan->ranges_built = false;
+  ProgramExec& pe = plan->programs[plan->blocks.back().program];
+  for (void* p : pe.owned) (void)hipFree(p);
+  pe = ProgramExec();  // rebuilt at the next apply
+}
+
+}  // namespace
+
+extern "C" {
+
+int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
+                       const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
+                       int32_t n_factors, const int64_t* foffset, const int32_t* fcode) {
+  if (plan == nullptr || program < 0 || program >= kPrograms || deg < 0 || deg > 4 || count < 0 ||
+      count > INT32_MAX || n_entries < 0 || n_factors < 0)
+    return fail(C3L_EINVAL, "c3l_plan_add_block: invalid argument");
+  if (plan->n > 255) return fail(C3L_EINVAL, "c3l_plan_add_block: more than 255 sites per group");
+  PlanBlock b;
+  b.program = program;
+  b.deg = deg;
+  b.count = count;
+  int rc = parse_block_entries(plan, b, deg, n_entries, coeff, fstart, n_factors, foffset, fcode);
+  if (rc != C3L_OK) return rc;
   b.coord.resize((size_t)count);
   b.dest.assign(dest, dest + count);
   for (int64_t k = 0; k < count; ++k) {
@@ -1188,11 +1227,37 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
   if (n_entries > 0) mix(plan, fstart, sizeof(int32_t) * ((size_t)n_entries + 1));
   mix(plan, foffset, sizeof(int64_t) * (size_t)n_factors);
   mix(plan, fcode, sizeof(int32_t) * (size_t)n_factors);
-  plan->blocks.push_back(std::move(b));
-  plan->ranges_built = false;
-  ProgramExec& pe = plan->programs[program];
-  for (void* p : pe.owned) (void)hipFree(p);
-  pe = ProgramExec();  // rebuilt at the next apply
+  finish_block(plan, std::move(b));
+  return C3L_OK;
+}
+
+int c3l_plan_add_dense_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t base, int32_t n_entries,
+                             const double* coeff, const int32_t* fstart, int32_t n_factors, const int64_t* foffset,
+                             const int32_t* fcode) {
+  if (plan == nullptr || program < 0 || program >= kPrograms || deg < 0 || deg > 4 || base < 0 || n_entries < 0 ||
+      n_factors < 0)
+    return fail(C3L_EINVAL, "c3l_plan_add_dense_block: invalid argument");
+  if (plan->n > 255) return fail(C3L_EINVAL, "c3l_plan_add_dense_block: more than 255 sites per group");
+  const int64_t count = ipow(plan->n, deg);
+  if (count > INT32_MAX) return fail(C3L_EINVAL, "c3l_plan_add_dense_block: block too large");
+  PlanBlock b;
+  b.program = program;
+  b.deg = deg;
+  b.count = count;
+  b.dense = true;
+  b.dense_base = base;
+  b.max_dest = base + count - 1;
+  int rc = parse_block_entries(plan, b, deg, n_entries, coeff, fstart, n_factors, foffset, fcode);
+  if (rc != C3L_OK) return rc;
+  mix_value(plan, (int32_t)4);
+  mix_value(plan, program);
+  mix_value(plan, deg);
+  mix(plan, &base, sizeof(base));
+  mix(plan, coeff, sizeof(double) * 2 * (size_t)n_entries);
+  if (n_entries > 0) mix(plan, fstart, sizeof(int32_t) * ((size_t)n_entries + 1));
+  mix(plan, foffset, sizeof(int64_t) * (size_t)n_factors);
+  mix(plan, fcode, sizeof(int32_t) * (size_t)n_factors);
+  finish_block(plan, std::move(b));
   return C3L_OK;
 }
 
@@ -1354,13 +1419,17 @@ int build_program(c3l_plan* plan, int32_t program) {
       while (reps > 1 && (b.count + 256 * reps - 1) / (256 * reps) < 512) reps >>= 1;
       d.reps = reps;
     }
-    d.pad = 0;
+    d.dense_n = b.dense ? plan->n : 0;
+    d.dense_base = b.dense ? b.dense_base : 0;
     uint32_t* coord = nullptr;
     int64_t* dest = nullptr;
     WedgeEntry* entries = nullptr;
     WedgeFactor* factors = nullptr;
-    int rc = upload_owned(pe.owned, &coord, b.coord.data(), b.coord.size());
-    if (rc == C3L_OK) rc = upload_owned(pe.owned, &dest, b.dest.data(), b.dest.size());
+    int rc = C3L_OK;
+    if (!b.dense) {
+      rc = upload_owned(pe.owned, &coord, b.coord.data(), b.coord.size());
+      if (rc == C3L_OK) rc = upload_owned(pe.owned, &dest, b.dest.data(), b.dest.size());
+    }
     if (rc == C3L_OK) rc = upload_owned(pe.owned, &entries, b.entries.data(), b.entries.size());
     if (rc == C3L_OK) rc = upload_owned(pe.owned, &factors, b.factors.data(), b.factors.size());
     if (rc != C3L_OK) return rc;

@@ -30,6 +30,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/c3l_hip.h"
@@ -152,6 +153,7 @@ __device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_doub
 // resid_hist / green_hist ([b][shift]) so that the stop test can take the state of the
 // exact stop column.
 constexpr int kShiftBatch = 8;
+constexpr int kMaxGroupBases = 64;  // bases of one group run (c3l_arnoldi_group_begin)
 constexpr int kShiftsPerBlock = 256 / kShiftBatch;
 constexpr int kTileK = 32;  // rotations staged in LDS per tile
 
@@ -697,6 +699,278 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   return C3L_OK;
 }
 
+// ---- GMRES for a group of independent systems in lockstep (c3l_gmres_group) --------------
+//
+// `count` systems of equal real dimension rd share one union plan (their contraction plans
+// side by side, one launch per program for all of them) and one set of union tilde maps
+// (dimension count * rd).  Every vector is the systems' vectors side by side; the Krylov basis
+// column j holds column j of every system (system i's basis is V + i*rd with leading
+// dimension count*rd).  Each system runs its own restarted GMRES (gmres_run's algorithm:
+// SciPy's tolerance control, delayed-reorthogonalisation CGS, host Givens one step behind the
+// device) in lockstep with the others: one union Jacobian product per step serves them all,
+// the orthogonalisation runs over the systems still iterating (up to four per launch), a
+// system that meets its inner test leaves the cycle, and one that meets its target leaves
+// the solve.  Cycles end together: the union product of the updated steps gives every
+// system's true residual at once.
+struct GroupSystem {
+  c3l_ortho::Basis<double, double> b64;
+  c3l_ortho::Basis<double, float> b32;
+  double* scratch = nullptr;
+  double* coef = nullptr;
+  bool done = false, stop = false, breakdown = false, trivial = false;
+  double bnrm = 0.0, atol = 0.0, ptol_factor = 1.0, ptol = 0.0, presid = 0.0, rn = 0.0;
+  int used = 0, absorbed = 0;
+  std::vector<double> H, cs, sn, S, y;
+};
+
+template <typename Store>
+int gmres_group_run(c3l_plan* plan, const c3l_tilde_maps* tm, int32_t count, const double* jvp_diag, double* t_work,
+                    double* out_work, const double* b, double* x, Store* V, double* w, double* tmp, double* scratch,
+                    double* coef, const double* rtol, const double* atol_in, const int32_t* active, int32_t restart,
+                    int32_t cycles, int32_t* info, int32_t* iterations, double* true_rel, int32_t* jvp_calls,
+                    void* stream) {
+  if (plan == nullptr || tm == nullptr || b == nullptr || x == nullptr || V == nullptr || w == nullptr ||
+      tmp == nullptr || scratch == nullptr || coef == nullptr || rtol == nullptr || atol_in == nullptr ||
+      active == nullptr || info == nullptr || iterations == nullptr || true_rel == nullptr || jvp_calls == nullptr ||
+      count < 1 || restart <= 0 || cycles <= 0 || tm->real_dim <= 0 || tm->real_dim % count != 0)
+    return dfail(C3L_EINVAL, "c3l_gmres_group: invalid argument");
+  for (int32_t i = 0; i < count; ++i)
+    if (active[i] && (!(rtol[i] > 0.0) || !(atol_in[i] >= 0.0)))
+      return dfail(C3L_EINVAL, "c3l_gmres_group: invalid tolerance");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t rd = tm->real_dim / count;
+  const int64_t ldv = tm->real_dim;
+  const int R = restart;
+  const Layout L = gmres_layout(rd, R);
+  const double eps = 2.220446049250313e-16;
+  Gmres g{plan, tm, jvp_diag, t_work, out_work, scratch, coef, stream, tm->real_dim, nullptr};
+  std::vector<GroupSystem> sys((size_t)count);
+  for (int32_t i = 0; i < count; ++i) {
+    GroupSystem& q = sys[(size_t)i];
+    q.scratch = scratch + i * L.scratch_total;
+    q.coef = coef + i * L.coef_total;
+    q.b64 = c3l_ortho::Basis<double, double>{reinterpret_cast<double*>(V) + i * rd, ldv, rd, q.coef + L.H, R + 1,
+                                              nullptr, nullptr, q.scratch + L.partial, q.coef + L.red,
+                                              q.coef + L.ab, q.coef + L.sc, nullptr, q.coef + L.stage};
+    q.b32 = c3l_ortho::Basis<double, float>{reinterpret_cast<float*>(V) + i * rd, ldv, rd, q.coef + L.H, R + 1,
+                                             nullptr, nullptr, q.scratch + L.partial, q.coef + L.red,
+                                             q.coef + L.ab, q.coef + L.sc, nullptr, q.coef + L.stage};
+    q.H.assign((size_t)(R + 1) * R, 0.0);
+    q.cs.assign(R, 0.0);
+    q.sn.assign(R, 0.0);
+    q.S.assign(R + 1, 0.0);
+    q.y.assign(R, 0.0);
+    q.done = !active[i];
+    iterations[i] = 0;
+    info[i] = active[i] ? 1 : 0;
+    true_rel[i] = 0.0;
+    jvp_calls[i] = 0;
+  }
+  // per-system norms of a union vector: one reduction per live system, one copy, one wait
+  std::vector<double> norms((size_t)count, 0.0);
+  auto segment_norms = [&](const double* v, std::vector<int32_t> const& which) -> int {
+    for (int32_t i : which)
+      D_CALL(c3l_rmultidot(v + i * rd, rd, 1, v + i * rd, rd, sys[(size_t)i].scratch,
+                           sys[(size_t)i].coef + L.y, stream));
+    for (int32_t i : which)
+      D_CHECK(hipMemcpyAsync(&norms[(size_t)i], sys[(size_t)i].coef + L.y, sizeof(double), hipMemcpyDeviceToHost, s));
+    D_CHECK(hipStreamSynchronize(s));
+    for (int32_t i : which) norms[(size_t)i] = std::sqrt(norms[(size_t)i] > 0.0 ? norms[(size_t)i] : 0.0);
+    return C3L_OK;
+  };
+  auto live_list = [&]() {
+    std::vector<int32_t> out;
+    for (int32_t i = 0; i < count; ++i)
+      if (!sys[(size_t)i].done) out.push_back(i);
+    return out;
+  };
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(ldv)), dim3(256), 0, s, x, 0.0, ldv);
+  D_CHECK(hipGetLastError());
+  std::vector<int32_t> live = live_list();
+  D_CALL(segment_norms(b, live));
+  for (int32_t i : live) {
+    sys[(size_t)i].bnrm = norms[(size_t)i];
+    if (sys[(size_t)i].bnrm == 0.0) {
+      sys[(size_t)i].done = true;
+      info[i] = 0;
+    }
+  }
+  live = live_list();
+  D_CALL(g.precondition(b, tmp));
+  D_CALL(segment_norms(tmp, live));
+  D_CHECK(hipMemcpyAsync(w, b, sizeof(double) * ldv, hipMemcpyDeviceToDevice, s));  // r = b - A 0
+  for (int32_t i : live) {
+    GroupSystem& q = sys[(size_t)i];
+    q.atol = std::max(atol_in[i], rtol[i] * q.bnrm);
+    q.ptol = norms[(size_t)i] * std::min(q.ptol_factor, q.atol / q.bnrm);
+    q.rn = q.bnrm;
+    if (q.bnrm < q.atol) {  // x = 0 already meets an absolute target
+      q.done = q.trivial = true;
+      info[i] = 0;
+      true_rel[i] = 1.0;
+    }
+  }
+  // pinned Hessenberg columns: two slots per system
+  HostColumns pinned;
+  pinned.stride = R + 2;
+  D_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned.host), sizeof(double) * 2 * pinned.stride * count, 0));
+  std::vector<hipEvent_t> events((size_t)2 * count, nullptr);
+  struct EventGuard {
+    std::vector<hipEvent_t>& e;
+    ~EventGuard() {
+      for (hipEvent_t ev : e)
+        if (ev != nullptr) {
+          (void)hipEventSynchronize(ev);
+          (void)hipEventDestroy(ev);
+        }
+    }
+  } guard{events};
+  for (hipEvent_t& e : events) D_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  auto slot = [&](int32_t i, int c) { return pinned.host + ((int64_t)i * 2 + (c % 2)) * pinned.stride; };
+  auto absorb = [&](int32_t i, int c) -> int {
+    GroupSystem& q = sys[(size_t)i];
+    const double* col = slot(i, c);
+    D_CHECK(hipEventSynchronize(events[(size_t)i * 2 + (c % 2)]));
+    auto h = [&](int r, int k) -> double& { return q.H[(size_t)r * R + k]; };
+    for (int k = 0; k <= c; ++k) h(k, c) = col[k];
+    const double hn = col[c + 1];
+    const double h0 = col[c + 2];
+    if (!std::isfinite(hn)) return dfail(C3L_EINVAL, "c3l_gmres_group: nonfinite Krylov vector");
+    if (hn <= eps * h0) {
+      h(c + 1, c) = 0.0;
+      q.breakdown = true;
+    } else {
+      h(c + 1, c) = hn;
+    }
+    for (int k = 0; k < c; ++k) {
+      const double a = h(k, c), bb = h(k + 1, c);
+      h(k, c) = q.cs[k] * a + q.sn[k] * bb;
+      h(k + 1, c) = -q.sn[k] * a + q.cs[k] * bb;
+    }
+    const double a = h(c, c), bb = h(c + 1, c);
+    const double r = std::hypot(a, bb);
+    q.cs[c] = r == 0.0 ? 1.0 : a / r;
+    q.sn[c] = r == 0.0 ? 0.0 : bb / r;
+    h(c, c) = r;
+    h(c + 1, c) = 0.0;
+    q.S[c + 1] = -q.sn[c] * q.S[c];
+    q.S[c] = q.cs[c] * q.S[c];
+    q.presid = std::fabs(q.S[c + 1]);
+    ++iterations[i];
+    q.used = c + 1;
+    q.stop = q.presid <= q.ptol || q.breakdown;
+    return C3L_OK;
+  };
+  // orthogonalisation step over a list of systems, at most four per launch
+  auto step_over = [&](std::vector<int32_t> const& which, int j, bool has_w) -> int {
+    for (size_t k = 0; k < which.size(); k += c3l_ortho::kMaxInstances) {
+      const int n = (int)std::min<size_t>(c3l_ortho::kMaxInstances, which.size() - k);
+      const double* wz[c3l_ortho::kMaxInstances];
+      c3l_ortho::Basis<double, Store> bs[c3l_ortho::kMaxInstances];
+      for (int u = 0; u < n; ++u) {
+        const int32_t i = which[k + (size_t)u];
+        wz[u] = has_w ? w + i * rd : nullptr;
+        if constexpr (std::is_same<Store, float>::value) bs[u] = sys[(size_t)i].b32;
+        else bs[u] = sys[(size_t)i].b64;
+      }
+      D_CALL(c3l_ortho::step_many(bs, n, j, j == 0, wz, 0.0, s, nullptr));
+    }
+    return C3L_OK;
+  };
+  for (int32_t cycle = 0; cycle < cycles; ++cycle) {
+    live = live_list();
+    if (live.empty()) break;
+    D_CALL(g.precondition(w, tmp));
+    D_CALL(segment_norms(tmp, live));
+    for (int32_t i : live) {
+      GroupSystem& q = sys[(size_t)i];
+      const double beta = norms[(size_t)i];
+      if (!(beta > 0.0) || !std::isfinite(beta))
+        return dfail(C3L_EINVAL, "c3l_gmres_group: singular preconditioned residual");
+      if constexpr (std::is_same<Store, float>::value)
+        D_CALL(c3l_internal_narrow_scale(tmp + i * rd, q.b32.V, 1.0 / beta, rd, stream));
+      else
+        D_CALL(c3l_rscale(tmp + i * rd, q.b64.V, 1.0 / beta, rd, stream));
+      std::fill(q.H.begin(), q.H.end(), 0.0);
+      std::fill(q.S.begin(), q.S.end(), 0.0);
+      q.S[0] = beta;
+      q.used = 0;
+      q.absorbed = 0;
+      q.stop = q.breakdown = false;
+      q.presid = 0.0;
+    }
+    for (int j = 0; j <= R; ++j) {
+      std::vector<int32_t> iterating;
+      for (int32_t i : live)
+        if (!sys[(size_t)i].stop) iterating.push_back(i);
+      if (iterating.empty()) break;
+      const bool has_w = j < R;
+      if (has_w) {
+        D_CALL(g.apply_preconditioned(V + (int64_t)j * ldv, w));
+        for (int32_t i : iterating) ++jvp_calls[i];
+      }
+      D_CALL(step_over(iterating, j, has_w));
+      if (j == 0) continue;
+      const int c = j - 1;
+      for (int32_t i : iterating) {
+        D_CHECK(hipMemcpyAsync(slot(i, c), sys[(size_t)i].coef + L.stage, sizeof(double) * (c + 3),
+                               hipMemcpyDeviceToHost, s));
+        D_CHECK(hipEventRecord(events[(size_t)i * 2 + (c % 2)], s));
+      }
+      if (c >= 1)
+        for (int32_t i : iterating) {
+          D_CALL(absorb(i, c - 1));
+          sys[(size_t)i].absorbed = c;
+        }
+    }
+    for (int32_t i : live) {  // columns still in flight at the end of the cycle
+      GroupSystem& q = sys[(size_t)i];
+      while (!q.stop && q.absorbed < R) {
+        D_CALL(absorb(i, q.absorbed));
+        ++q.absorbed;
+      }
+    }
+    for (int32_t i : live) {
+      GroupSystem& q = sys[(size_t)i];
+      auto h = [&](int r, int k) -> double { return q.H[(size_t)r * R + k]; };
+      const int used = q.used;
+      if (used == 0) continue;
+      if (h(used - 1, used - 1) == 0.0) q.S[used - 1] = 0.0;
+      for (int k = used - 1; k >= 0; --k) {
+        double acc = q.S[k];
+        for (int l = k + 1; l < used; ++l) acc -= h(k, l) * q.y[l];
+        q.y[k] = h(k, k) == 0.0 ? 0.0 : acc / h(k, k);
+      }
+      D_CHECK(hipMemcpyAsync(q.coef + L.y, q.y.data(), sizeof(double) * used, hipMemcpyHostToDevice, s));
+      if constexpr (std::is_same<Store, float>::value)
+        D_CALL(c3l_internal_rmultiaxpy_f32(q.b32.V, ldv, used, q.coef + L.y, 1.0, x + i * rd, rd, stream));
+      else
+        D_CALL(c3l_rmultiaxpy(q.b64.V, ldv, used, q.coef + L.y, 1.0, x + i * rd, rd, stream));
+    }
+    D_CALL(g.apply(x, w));
+    for (int32_t i : live) ++jvp_calls[i];
+    D_CALL(c3l_raxpby(1.0, b, -1.0, w, ldv, stream));
+    D_CALL(segment_norms(w, live));  // synchronises: y may be reused by the next cycle
+    for (int32_t i : live) {
+      GroupSystem& q = sys[(size_t)i];
+      q.rn = norms[(size_t)i];
+      if (q.rn <= q.atol || q.breakdown) {
+        q.done = true;
+        continue;
+      }
+      q.ptol_factor = q.presid <= q.ptol ? std::max(eps, 0.25 * q.ptol_factor) : std::min(1.0, 1.5 * q.ptol_factor);
+      q.ptol = q.presid * std::min(q.ptol_factor, q.atol / q.rn);
+    }
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    GroupSystem& q = sys[(size_t)i];
+    if (!active[i] || q.bnrm == 0.0 || q.trivial) continue;
+    true_rel[i] = q.rn / q.bnrm;
+    info[i] = q.rn <= q.atol ? 0 : cycles;
+  }
+  return C3L_OK;
+}
+
 }  // namespace
 
 // One batched Arnoldi resolvent in flight (c3l_arnoldi_begin / _drive / _end): the
@@ -725,6 +999,14 @@ struct c3l_arnoldi_run {
   int32_t batches = 0;     // status copies enqueued so far
   int32_t flags_read = 0;  // status copies read so far
   DriverClock clock{0};
+  // Group runs (c3l_arnoldi_group_begin): bases share plan slots.  Basis i belongs to slot
+  // slot_of[i] at row offset row_of[i] (complex entries) of the slot's vectors; the slot's
+  // columns have slot_ld complex entries and its sigma output is slot_w.  Empty otherwise.
+  std::vector<int32_t> slot_of;
+  std::vector<int64_t> row_of;
+  std::vector<c3l_plan*> slot_plan;
+  std::vector<double*> slot_V, slot_w;
+  int64_t slot_ld = 0;
 
   bool done() const { return live.empty() || j > m_max; }
 
@@ -757,9 +1039,66 @@ struct c3l_arnoldi_run {
     return C3L_OK;
   }
 
+  // Step j of a group run: one sigma launch over the slots that still have a live basis,
+  // the orthogonalisation and shift updates over the live bases, four per launch.
+  int advance_group() {
+    const bool has_w = j < m_max;
+    std::vector<int32_t> slots;
+    for (int32_t i : live)
+      if (std::find(slots.begin(), slots.end(), slot_of[(size_t)i]) == slots.end()) slots.push_back(slot_of[(size_t)i]);
+    if (has_w) {
+      c3l_plan* lp[c3l_ortho::kMaxInstances];
+      const double* vj[c3l_ortho::kMaxInstances];
+      double* wj[c3l_ortho::kMaxInstances];
+      const int ns = (int)slots.size();
+      for (int u = 0; u < ns; ++u) {
+        lp[u] = slot_plan[(size_t)slots[(size_t)u]];
+        vj[u] = slot_V[(size_t)slots[(size_t)u]] + (int64_t)2 * j * slot_ld;
+        wj[u] = slot_w[(size_t)slots[(size_t)u]];
+      }
+      D_CALL(c3l_internal_plan_load_many(lp, ns, 2, vj, stream));
+      D_CALL(c3l_internal_plan_apply_many(lp, ns, 3, diag, vj, wj, stream));
+    }
+    const bool batch = j > 0 && (j - c0 == kShiftBatch || j == m_max);
+    const int nb = j - c0;
+    for (size_t k0 = 0; k0 < live.size(); k0 += c3l_ortho::kMaxInstances) {
+      const int k = (int)std::min<size_t>(c3l_ortho::kMaxInstances, live.size() - k0);
+      const double2* wz[c3l_ortho::kMaxInstances];
+      c3l_ortho::Basis<double2> lb[c3l_ortho::kMaxInstances];
+      ShiftStates ls{};
+      for (int u = 0; u < k; ++u) {
+        const int32_t i = live[k0 + (size_t)u];
+        lb[u] = basis[(size_t)i];
+        wz[u] = has_w ? reinterpret_cast<const double2*>(slot_w[(size_t)slot_of[(size_t)i]]) + row_of[(size_t)i] : nullptr;
+        ls.p[u] = state[(size_t)i];
+      }
+      D_CALL(c3l_ortho::step_many(lb, k, j, j == 0, wz, breakdown, stream, &clock));
+      if (batch) {
+        const int64_t shift_blocks = (n_shifts + kShiftsPerBlock - 1) / kShiftsPerBlock;
+        hipLaunchKernelGGL(multishift_batch_kernel, dim3(shift_blocks, 1, k), dim3(256), 0, stream, ls, c0, nb, m_max,
+                           shifts, n_shifts);
+        D_CHECK(hipGetLastError());
+        hipLaunchKernelGGL(arnoldi_batch_status_kernel, dim3(1, 1, k), dim3(1024), 0, stream, ls, n_shifts, m_max, c0,
+                           nb, min_dim, tolerance, breakdown);
+        D_CHECK(hipGetLastError());
+      }
+    }
+    if (batch) {
+      c0 = j;
+      const int slot = batches & 1;
+      D_CHECK(hipMemcpyAsync(pinned + (size_t)slot * 4 * count, status, 4 * sizeof(double) * (size_t)count,
+                             hipMemcpyDeviceToHost, stream));
+      D_CHECK(hipEventRecord(copied[slot], stream));
+      ++batches;
+    }
+    ++j;
+    return poll();
+  }
+
   // Step j: sigma on the once-projected v_j of every live basis (none at j = m_max),
   // one orthogonalisation step, and every kShiftBatch columns the shift update.
   int advance() {
+    if (!slot_of.empty()) return advance_group();
     const int k = (int)live.size();
     const bool has_w = j < m_max;
     c3l_plan* lp[c3l_ortho::kMaxInstances];
@@ -870,9 +1209,107 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
   return C3L_OK;
 }
 
+// Sets up the bases of c3l_arnoldi_group_begin: slot s (0 <= s < n_slots) holds per_slot bases;
+// basis i = s * per_slot + p reads rows [p dim, (p+1) dim) of the slot's vectors.
+int arnoldi_group_setup(c3l_arnoldi_run* r, c3l_plan* const* slot_plans, int32_t n_slots, int32_t per_slot,
+                        const double* diag, double* V, double* W, double* H, int64_t dim, int32_t m_max,
+                        const double* shifts, int32_t n_shifts, const double* sink, const double* beta,
+                        double* rot_c, double* rot_s, double* pi, double* gamma, double* green, double* resid,
+                        double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
+                        double breakdown_tol, void* stream) {
+  const int32_t count = n_slots * per_slot;
+  if (slot_plans == nullptr || n_slots < 1 || n_slots > c3l_ortho::kMaxInstances || per_slot < 1 ||
+      count > kMaxGroupBases || V == nullptr || W == nullptr || H == nullptr || dim <= 0 || m_max <= 0 ||
+      n_shifts <= 0 || shifts == nullptr || sink == nullptr || beta == nullptr || rot_c == nullptr ||
+      rot_s == nullptr || pi == nullptr || gamma == nullptr || green == nullptr || resid == nullptr ||
+      scratch == nullptr || coef == nullptr || status == nullptr)
+    return dfail(C3L_EINVAL, "c3l_arnoldi_group_begin: invalid argument");
+  for (int32_t i = 0; i < n_slots; ++i)
+    if (slot_plans[i] == nullptr) return dfail(C3L_EINVAL, "c3l_arnoldi_group_begin: invalid argument");
+  for (int32_t i = 0; i < count; ++i)
+    if (!(beta[i] > 0.0)) return dfail(C3L_EINVAL, "c3l_arnoldi_group_begin: every right-hand side must be nonzero");
+  if (!c3l_internal_plans_are_twins(slot_plans, n_slots, 2, 3))
+    return dfail(C3L_EINVAL, "c3l_arnoldi_group_begin: the slots' plans must be FP64 twins (built from the same arrays)");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const Layout L = arnoldi_layout(dim, m_max, n_shifts);
+  const int64_t ld = (int64_t)per_slot * dim;                      // complex entries per slot column
+  const int64_t sV = 2 * ((int64_t)m_max + 1) * ld;                 // doubles per slot basis block
+  const int64_t sH = 2 * (int64_t)m_max * ((int64_t)m_max + 1);
+  const int64_t sRot = (int64_t)n_shifts * m_max, sShift = n_shifts;
+  r->plans.clear();
+  r->count = count;
+  r->diag = diag;
+  r->shifts = reinterpret_cast<const double2*>(shifts);
+  r->status = status;
+  r->dim = dim;
+  r->m_max = m_max;
+  r->n_shifts = n_shifts;
+  r->min_dim = min_dim;
+  r->tolerance = tolerance;
+  r->breakdown = breakdown_tol;
+  r->stream = s;
+  r->slot_ld = ld;
+  for (int32_t q = 0; q < n_slots; ++q) {
+    r->slot_plan.push_back(slot_plans[q]);
+    r->slot_V.push_back(V + q * sV);
+    r->slot_w.push_back(W + 2 * q * ld);
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    const int32_t q = i / per_slot, p = i % per_slot;
+    double* sc = scratch + i * L.scratch_total;
+    double* cf = coef + i * L.coef_total;
+    double2* Hz = reinterpret_cast<double2*>(H + i * sH);
+    double2* sink_v = reinterpret_cast<double2*>(cf + L.sink_v);
+    double2* Vi = reinterpret_cast<double2*>(V + q * sV) + (int64_t)p * dim;
+    r->basis.push_back(c3l_ortho::Basis<double2>{Vi, ld, dim, Hz, (int64_t)m_max + 1, sink_v,
+                                                 reinterpret_cast<const double2*>(sink) + (int64_t)i * dim,
+                                                 reinterpret_cast<double2*>(sc + L.partial),
+                                                 reinterpret_cast<double2*>(cf + L.red),
+                                                 reinterpret_cast<double2*>(cf + L.ab),
+                                                 reinterpret_cast<double2*>(cf + L.sc), status + 4 * i, nullptr});
+    r->state.push_back(ShiftState{Hz, sink_v, beta[i], rot_c + i * sRot,
+                                  reinterpret_cast<double2*>(rot_s + 2 * i * sRot),
+                                  reinterpret_cast<double2*>(pi + 2 * i * sRot),
+                                  reinterpret_cast<double2*>(gamma + 2 * i * sShift),
+                                  reinterpret_cast<double2*>(green + 2 * i * sShift), resid + i * sShift,
+                                  sc + L.resid_hist, reinterpret_cast<double2*>(sc + L.green_hist), status + 4 * i, 0});
+    r->slot_of.push_back(q);
+    r->row_of.push_back((int64_t)p * dim);
+    r->w.push_back(W + 2 * (q * ld + (int64_t)p * dim));
+    r->V.push_back(reinterpret_cast<double*>(Vi));
+    r->live.push_back(i);
+    hipLaunchKernelGGL(init_shift_state_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s, r->state.back().gamma,
+                       r->state.back().green, beta[i], n_shifts, status + 4 * i);
+    D_CHECK(hipGetLastError());
+  }
+  D_CHECK(hipHostMalloc(reinterpret_cast<void**>(&r->pinned), 2 * 4 * sizeof(double) * (size_t)count));
+  for (hipEvent_t& e : r->copied) D_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return C3L_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int c3l_arnoldi_group_begin(c3l_plan* const* slot_plans, int32_t n_slots, int32_t per_slot, const double* diag,
+                            double* V, double* W, double* H, int64_t dim, int32_t m_max, const double* shifts,
+                            int32_t n_shifts, const double* sink, const double* beta, double* rot_c, double* rot_s,
+                            double* pi, double* gamma, double* green, double* resid, double* scratch, double* coef,
+                            double* status, double tolerance, int32_t min_dim, double breakdown_tol, void* stream,
+                            c3l_arnoldi_run** out) {
+  if (out == nullptr) return dfail(C3L_EINVAL, "c3l_arnoldi_group_begin: invalid argument");
+  *out = nullptr;
+  c3l_arnoldi_run* r = new c3l_arnoldi_run();
+  const int rc = arnoldi_group_setup(r, slot_plans, n_slots, per_slot, diag, V, W, H, dim, m_max, shifts, n_shifts,
+                                     sink, beta, rot_c, rot_s, pi, gamma, green, resid, scratch, coef, status,
+                                     tolerance, min_dim, breakdown_tol, stream);
+  if (rc != C3L_OK) {
+    delete r;
+    return rc;
+  }
+  *out = r;
+  return C3L_OK;
+}
 
 int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
               const double* b, double* x, void* V, int32_t basis_bits, double* w, double* tmp, double* scratch,
@@ -885,6 +1322,20 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
     return gmres_run(plan, tm, jvp_diag, t_work, out_work, b, x, static_cast<float*>(V), w, tmp, scratch, coef, rtol,
                      atol, restart, cycles, info, iterations, true_rel, jvp_calls, stream);
   return dfail(C3L_EINVAL, "c3l_gmres: basis_bits must be 32 or 64");
+}
+
+int c3l_gmres_group(c3l_plan* plan, const c3l_tilde_maps* tm, int32_t count, const double* jvp_diag, double* t_work,
+                    double* out_work, const double* b, double* x, void* V, int32_t basis_bits, double* w, double* tmp,
+                    double* scratch, double* coef, const double* rtol, const double* atol, const int32_t* active,
+                    int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations, double* true_rel,
+                    int32_t* jvp_calls, void* stream) {
+  if (basis_bits == 64)
+    return gmres_group_run(plan, tm, count, jvp_diag, t_work, out_work, b, x, static_cast<double*>(V), w, tmp, scratch,
+                           coef, rtol, atol, active, restart, cycles, info, iterations, true_rel, jvp_calls, stream);
+  if (basis_bits == 32)
+    return gmres_group_run(plan, tm, count, jvp_diag, t_work, out_work, b, x, static_cast<float*>(V), w, tmp, scratch,
+                           coef, rtol, atol, active, restart, cycles, info, iterations, true_rel, jvp_calls, stream);
+  return dfail(C3L_EINVAL, "c3l_gmres_group: basis_bits must be 32 or 64");
 }
 
 int c3l_krylov_workspace(int32_t complex_basis, int64_t dim, int32_t max_columns, int32_t n_shifts,

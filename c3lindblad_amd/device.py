@@ -12,7 +12,7 @@ import itertools
 import numpy as np
 
 from . import _lib
-from .plan_arrays import KINDS, PROGRAMS, LoweredPlan
+from .plan_arrays import KINDS, PROGRAMS, DenseDest, LoweredPlan
 
 _tokens = itertools.count(1)
 
@@ -60,12 +60,19 @@ class DevicePlan:
             ]
             _lib.call("c3l_plan_add_level", handle, kind, arrays[0].size, *[_lib.host_ptr(a) for a in arrays])
         for program, deg, coord, dest, coeff, fstart, foff, fcode in lowered.blocks:
-            coord = _c(coord, np.int32)
-            dest = _c(dest, np.int64)
             coeff = _c(coeff, np.complex128)
             fstart = _c(fstart, np.int32)
             foff = _c(foff, np.int64)
             fcode = _c(fcode, np.int32)
+            if isinstance(dest, DenseDest):
+                _lib.call(
+                    "c3l_plan_add_dense_block", handle, program, deg, dest.start, coeff.size,
+                    _lib.host_ptr(coeff) if coeff.size else None, _lib.host_ptr(fstart), foff.size,
+                    _lib.host_ptr(foff) if foff.size else None, _lib.host_ptr(fcode) if fcode.size else None,
+                )
+                continue
+            coord = _c(coord, np.int32)
+            dest = _c(dest, np.int64)
             _lib.call(
                 "c3l_plan_add_block",
                 handle,

@@ -45,6 +45,26 @@ PROGRAMS = {
 }
 
 
+class DenseDest:
+    """Destinations ``start .. start+size-1`` of a dense block (every index tuple of a padded
+    tensor, in flat order) without storing them; converts to the index array on demand."""
+
+    __slots__ = ("start", "size")
+
+    def __init__(self, start: int, size: int):
+        self.start = int(start)
+        self.size = int(size)
+
+    def __add__(self, shift) -> "DenseDest":
+        return DenseDest(self.start + int(shift), self.size)
+
+    def __array__(self, dtype=None, copy=None):
+        return np.arange(self.start, self.start + self.size, dtype=dtype or np.int64)
+
+    def max(self) -> int:
+        return self.start + self.size - 1
+
+
 def encode_factors(ranks: np.ndarray, positions: np.ndarray) -> np.ndarray:
     """``fcode = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9`` (rank <= 4, positions < 4)."""
 
@@ -193,7 +213,7 @@ def lower_plan(
             if bp.target_atom >= 0:
                 # creation tensor: every index tuple of its padded storage, written in place
                 flat, coord = dense(deg)
-                dest = plan.atom_offsets[bp.target_atom] + flat
+                dest = DenseDest(plan.atom_offsets[bp.target_atom], flat.size)
             elif deg == 0:
                 coord = np.zeros((1, 0), dtype=np.int32)
                 dest = np.full(1, max(bp.scalar_index, 0), dtype=np.int64)
@@ -229,4 +249,64 @@ def lower_plan(
         levels=levels,
         blocks=blocks,
         n_terms=dict(plan.n_terms),
+    )
+
+
+WORKSPACE_PROGRAMS = (PROGRAMS["creation"],)
+
+
+def union_plan(parts: list[LoweredPlan], in_sizes: dict[int, int], out_sizes: dict[int, int]) -> LoweredPlan:
+    """One plan evaluating several independent plans of equal ``n`` at once.
+
+    ``parts[i]`` reads its input vectors at ``[i * in_sizes[kind], (i+1) * in_sizes[kind])`` of the
+    concatenated input of that kind and writes program outputs at
+    ``[i * out_sizes[program], ...)`` of the concatenated output (workspace programs write their
+    own shifted workspace).  Workspaces, vector tables and constant atoms are concatenated; the
+    contraction levels of every kind are merged depth by depth, so each level launch builds that
+    depth's atoms of every part; blocks keep their per-part coefficients.  No part reads another
+    part's atoms, so the union's results are exactly the parts' results side by side.
+    """
+
+    if not parts:
+        raise ValueError("a union needs at least one plan")
+    n = parts[0].n
+    if any(p.n != n for p in parts):
+        raise ValueError("united plans must share the site count")
+    ws_base = np.cumsum([0] + [p.workspace for p in parts])
+    vec_base = np.cumsum([0] + [p.vectors.shape[0] for p in parts])
+    bases, blocks = [], []
+    const_offset, const_vec = [], []
+    per_kind_levels: dict[int, list[list]] = {}
+    for i, p in enumerate(parts):
+        w0, v0 = int(ws_base[i]), int(vec_base[i])
+        const_offset.append(np.asarray(p.const_offset, dtype=np.int64) + w0)
+        const_vec.append(np.asarray(p.const_vec, dtype=np.int32) + v0)
+        for kind, offset, deg, src, sign in p.bases:
+            shift = i * in_sizes[kind]
+            bases.append((kind, int(offset) + w0, deg, np.where(src >= 0, src + shift, -1).astype(np.int64), sign))
+        depth: dict[int, int] = {}
+        for kind, out_off, par_off, par_rank, slot, vec, sign in p.levels:
+            d = depth.get(kind, 0)
+            depth[kind] = d + 1
+            levels = per_kind_levels.setdefault(kind, [])
+            while len(levels) <= d:
+                levels.append([])
+            levels[d].append((out_off + w0, par_off + w0, par_rank, slot, vec + v0, sign))
+        for program, deg, coord, dest, coeff, fstart, foff, fcode in p.blocks:
+            shift = w0 if program in WORKSPACE_PROGRAMS else i * out_sizes[program]
+            blocks.append((program, deg, coord, dest + shift, coeff, fstart, foff + w0, fcode))
+    levels = []
+    for kind in sorted(per_kind_levels):
+        for jobs in per_kind_levels[kind]:
+            levels.append((kind, *[np.concatenate([j[k] for j in jobs]) for k in range(6)]))
+    return LoweredPlan(
+        n=n,
+        workspace=int(ws_base[-1]),
+        vectors=np.concatenate([p.vectors for p in parts], axis=0),
+        const_offset=np.concatenate(const_offset),
+        const_vec=np.concatenate(const_vec),
+        bases=bases,
+        levels=levels,
+        blocks=blocks,
+        n_terms={},
     )

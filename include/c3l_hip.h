@@ -84,6 +84,14 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
                        const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
                        int32_t n_factors, const int64_t* foffset, const int32_t* fcode);
 
+/* A dense output block: every index tuple of a padded rank-deg tensor (n^deg coordinates, flat
+ * index e = sum_s i_s n^s, first index fastest), coordinate e written at position base + e (the
+ * creation tensors of a workspace program).  Entries and factors as in c3l_plan_add_block; no
+ * coordinate or destination arrays are stored. */
+int c3l_plan_add_dense_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t base, int32_t n_entries,
+                             const double* coeff, const int32_t* fstart, int32_t n_factors, const int64_t* foffset,
+                             const int32_t* fcode);
+
 /* Scatter the complex device vector v into the base blocks of `kind` and run that
  * kind's contraction levels (atoms of that kind are then current). */
 int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream);
@@ -205,6 +213,24 @@ int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
               double* coef, double rtol, double atol, int32_t restart, int32_t cycles, int32_t* info,
               int32_t* iterations, double* true_rel, int32_t* jvp_calls, void* stream);
 
+/* c3l_gmres for `count` independent systems of equal real dimension at once (the Newton
+ * corrections of a group of problem instances, each solved by the reference's
+ * solve_component_ccsd, src/c3l/cc/component_solver.py:181-191, one after the other).  `plan` is
+ * the union plan of the systems (their plans side by side, one launch per program for all) and
+ * `tm` its union tilde maps (real_dim = count * rd).  b, x, w, tmp are the systems' vectors side
+ * by side; V holds (restart+1) columns of count*rd entries, column j being every system's basis
+ * column j; scratch and coef hold `count` consecutive blocks of the c3l_gmres sizes for rd.
+ * Host arrays of `count`: rtol, atol (per system), active (0 = skip the system: its x is set to
+ * zero and nothing else is reported), and the results info, iterations, true_rel, jvp_calls.
+ * Each system follows c3l_gmres exactly (same tolerance control and stopping rules); the systems
+ * advance in lockstep: one union Jacobian product per step serves all, the orthogonalisation
+ * runs over the systems still iterating, and restart cycles end together. */
+int c3l_gmres_group(c3l_plan* plan, const c3l_tilde_maps* tm, int32_t count, const double* jvp_diag, double* t_work,
+                    double* out_work, const double* b, double* x, void* V, int32_t basis_bits, double* w, double* tmp,
+                    double* scratch, double* coef, const double* rtol, const double* atol, const int32_t* active,
+                    int32_t restart, int32_t cycles, int32_t* info, int32_t* iterations, double* true_rel,
+                    int32_t* jvp_calls, void* stream);
+
 /* Shift-invariant Krylov resolvent on the sigma program (replaces the per-frequency GMRES
  * loop of component_retarded_green, src/c3l/cc/component_response.py:180-245, and the Arnoldi
  * resolvent arnoldi_resolvent, src/c3l/green/krylov.py:284-417).  One Arnoldi basis (classical
@@ -262,6 +288,24 @@ int c3l_arnoldi_begin(c3l_plan* const* plans, int32_t count, const double* diag,
                       double* scratch, double* coef, double* status, double tolerance, int32_t min_dim,
                       double breakdown_tol, void* stream, c3l_arnoldi_run** out);
 int c3l_arnoldi_drive(c3l_arnoldi_run* const* runs, int32_t n_runs);
+/* c3l_arnoldi_begin for a group of problem instances of equal dimension whose sigma plans are
+ * side by side in one union plan (the resolvents of component_retarded_green,
+ * src/c3l/cc/component_response.py:180-245, of every instance in the group).  n_slots (1..4)
+ * twin copies of the union plan each serve per_slot bases (n_slots * per_slot <= 64): basis
+ * i = s * per_slot + p is instance p's resolvent on slot s (e.g. slot 0 addition, slot 1
+ * removal).  V holds n_slots blocks of (m_max+1) columns of per_slot*dim complex entries (column
+ * j of slot s: every instance's basis column j side by side, V[0] of basis i = its b/beta); W is
+ * the sigma output, n_slots blocks of per_slot*dim; diag the union diagonal (per_slot*dim).  H,
+ * sink (dim each), beta and every per-basis workspace of c3l_arnoldi_batch hold n_slots*per_slot
+ * consecutive blocks in basis order.  One sigma launch per step serves every slot that still
+ * has a live basis; each basis stops on its own criterion.  Driven and ended like
+ * c3l_arnoldi_begin's runs (c3l_arnoldi_drive, c3l_arnoldi_end with count = n_slots*per_slot). */
+int c3l_arnoldi_group_begin(c3l_plan* const* slot_plans, int32_t n_slots, int32_t per_slot, const double* diag,
+                            double* V, double* W, double* H, int64_t dim, int32_t m_max, const double* shifts,
+                            int32_t n_shifts, const double* sink, const double* beta, double* rot_c, double* rot_s,
+                            double* pi, double* gamma, double* green, double* resid, double* scratch, double* coef,
+                            double* status, double tolerance, int32_t min_dim, double breakdown_tol, void* stream,
+                            c3l_arnoldi_run** out);
 int c3l_arnoldi_end(c3l_arnoldi_run* run, int32_t* m_out, int32_t* breakdown_out, double* worst_out);
 
 /* Workspace sizes (in doubles) of c3l_arnoldi (complex_basis = 1, dim complex rows, m_max =

@@ -57,6 +57,7 @@ class PortPlan:
         for pid, deg, coord, dest, coeff, fstart, foff, fcode in self.lp.blocks:
             if pid != program:
                 continue
+            dest = np.asarray(dest)  # dense blocks list their destinations on demand
             rows = np.arange(dest.size) if dest.size <= per_block else np.sort(
                 rng.choice(dest.size, per_block, replace=False))
             sub = coord[rows]
@@ -82,13 +83,16 @@ class PortPlan:
         for pid, deg, coord, dest, coeff, fstart, foff, fcode in self.lp.blocks:
             if pid != program:
                 continue
-            acc = np.zeros(dest.size, dtype=np.complex128)
+            if not isinstance(dest, np.ndarray):  # a dense block: consecutive destinations
+                dest = slice(dest.start, dest.start + dest.size)
+            count = coord.shape[0]
+            acc = np.zeros(count, dtype=np.complex128)
             for t in range(coeff.size):
-                prod = np.full(dest.size, coeff[t], dtype=np.complex128)
+                prod = np.full(count, coeff[t], dtype=np.complex128)
                 for f in range(fstart[t], fstart[t + 1]):
                     code = int(fcode[f])
                     rank = code & 7
-                    flat = np.zeros(dest.size, dtype=np.int64)
+                    flat = np.zeros(count, dtype=np.int64)
                     for s in range(rank - 1, -1, -1):  # slot s has stride n**s
                         flat = flat * n + coord[:, (code >> (3 + 2 * s)) & 3]
                     prod *= self.ws[foff[f] + flat]

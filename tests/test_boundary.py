@@ -176,3 +176,17 @@ def test_checkpoint_resource_guard_and_identity():
         require_resources(required_memory_bytes=11, required_disk_bytes=0, available_memory_bytes=10, free_disk_bytes=0)
     with pytest.raises(OSError):
         require_resources(required_memory_bytes=0, required_disk_bytes=11, available_memory_bytes=0, free_disk_bytes=10)
+
+
+def test_batch_skips_instances_without_a_full_hf_reference():
+    """The reference raises for a model whose default full-HF reference does not converge
+    (``cc/factorized.py:304-318``); the batch set-up leaves such an instance out and names it."""
+
+    from c3lindblad_amd.batch import _compile_job
+    from c3lindblad_amd.configs import multiorbital_model, siam_model
+
+    bad = multiorbital_model(3, 1, seed=0)
+    inst, why = _compile_job((7, bad, (1,)))
+    assert inst is None and why[0] == 7 and "full-HF" in why[1]
+    inst, why = _compile_job((3, siam_model(3, 2.0, seed=1), (1, -1)))
+    assert why is None and inst.index == 3 and set(inst.eom) == {1, -1}

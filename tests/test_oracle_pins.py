@@ -246,3 +246,45 @@ def test_port_sampled_program_equals_full_program():
     dest, values = port.apply_sampled(0, 7, seed=1)
     assert dest.size == len({int(d) for d in dest}) and dest.size < p.num_amplitudes
     assert np.array_equal(values, full[dest])
+
+
+def test_union_plan_equals_its_parts():
+    """A union of three plans of equal n (different models, one with U = 0 and so fewer products)
+    executed by the port gives every part's residual, JVP and sigma side by side."""
+
+    from c3lindblad_amd.eom import ComponentEOMProblem, eom_ccsd_problem
+    from c3lindblad_amd.plan_arrays import PROGRAMS, union_plan
+    from oracle.port import PortPlan
+
+    parts = [_setup(3, U, seed) for U, seed in ((1.5, 11), (0.0, 12), (2.2, 13))]
+    D = parts[0][0].num_amplitudes
+    rng = np.random.default_rng(4)
+    ts = [p.tilde.unpack(0.05 * rng.normal(size=D)) for p, _ in parts]
+    dts = [p.tilde.unpack(rng.normal(size=D)) for p, _ in parts]
+    u = union_plan([sp.lowered for _, sp in parts], {0: D, 1: D}, {0: D, 1: 1, 2: D})
+    port = PortPlan(u)
+    port.load(0, np.concatenate(ts))
+    port.load(1, np.concatenate(dts))
+    R = port.apply(0, np.zeros(3 * D, complex))
+    ref = port.apply(1, np.zeros(3, complex))
+    J = port.apply(2, np.zeros(3 * D, complex))
+    eoms = [ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, -1) for p, sp in parts]
+    P = eoms[0].dimension
+    xs = [rng.normal(size=P) + 1j * rng.normal(size=P) for _ in parts]
+    ue = union_plan([e.lowered for e in eoms], {0: D, 2: P}, {PROGRAMS["sigma"]: P})
+    eport = PortPlan(ue)
+    eport.load(0, np.concatenate(ts))
+    eport.load(2, np.concatenate(xs))
+    S = eport.apply(3, np.zeros(3 * P, complex))
+    for i, ((p, sp), e) in enumerate(zip(parts, eoms)):
+        one = PortPlan(sp.lowered)
+        one.load(0, ts[i])
+        one.load(1, dts[i])
+        assert np.array_equal(R[i * D:(i + 1) * D], one.apply(0, np.zeros(D, complex)))
+        assert ref[i] == one.apply(1, np.zeros(1, complex))[0]
+        assert np.array_equal(J[i * D:(i + 1) * D], one.apply(2, np.zeros(D, complex)))
+        single = PortPlan(e.lowered)
+        single.load(0, ts[i])
+        single.load(2, xs[i])
+        assert np.array_equal(S[i * P:(i + 1) * P], single.apply(3, np.zeros(P, complex)))
+    assert len(parts[1][1].lowered.blocks[0][4]) < len(parts[0][1].lowered.blocks[0][4])  # U = 0: fewer entries
