@@ -231,17 +231,20 @@ def _cpu_baseline(pipe, result) -> dict:
     }
 
 
-def _jvp_probe(jvps: int) -> int:
-    """Counter workload: siam24 residual, ``jvps`` JVPs, then a 256 MiB copy for calibration."""
+def _jvp_probe(jvps: int, config: str = "siam24", precision: str = "fp64") -> int:
+    """Counter workload: one residual of ``config`` (siam24 or kanamori39), ``jvps`` JVPs in
+    ``precision``, then a 256 MiB copy for calibration."""
 
     import torch
 
-    from c3lindblad_amd.configs import config2_model
+    from c3lindblad_amd.configs import config2_model, config5_model
     from c3lindblad_amd.lowering import factorized_cc_problem
     from c3lindblad_amd.static import ComponentStaticProblem
 
-    p = factorized_cc_problem(config2_model())
+    p = factorized_cc_problem(config5_model(0) if config == "kanamori39" else config2_model())
     dev = ComponentStaticProblem.compile(p).device
+    if precision == "fp32":
+        dev.plan.set_precision("D", "jvp", 32)
     rng = np.random.default_rng(0)
     x = torch.from_numpy(0.01 * rng.normal(size=p.num_variables)).cuda()
     r = torch.empty_like(x)
@@ -256,12 +259,104 @@ def _jvp_probe(jvps: int) -> int:
     return 0
 
 
+def _bench_batch(args) -> int:
+    """The uneven-size batch: ``--instances`` instances of the 4..32-mode SIAM distribution (512 by
+    default), each to ||pack R|| <= 1e-10 and both charges' G^R on 2048 frequencies, solved as
+    groups of equal site count in lockstep (``c3lindblad_amd.batch``).  The instances go through in
+    chunks of ``--batch-chunk``, in order of site count so that the groups fill: each chunk's host set-up (full HF, lowering, compilation, upload)
+    and one untimed warm-up solve of its groups (device program images) precede its timed solve;
+    ``value`` = instances solved / the sum of the timed regions."""
+
+    import torch
+
+    from c3lindblad_amd.batch import prepare_batch, solve_group
+    from c3lindblad_amd.configs import config3_grid, config3_model, config3_modes
+    from c3lindblad_amd.solver import NewtonKrylovPolicy
+
+    omega = config3_grid()
+    policy = NewtonKrylovPolicy.mixed_precision(forcing_strategy="eisenstat_walker", gmres_restart=args.restart)
+    # chunks of consecutive site counts, so that the groups inside a chunk are full
+    ids = sorted(range(args.instances), key=lambda i: (config3_modes(i), i))
+    timed, setup = 0.0, 0.0
+    solutions, skipped, group_sizes = [], [], []
+    steps = max(1, args.steps)
+    for c0 in range(0, len(ids), args.batch_chunk):
+        chunk = ids[c0:c0 + args.batch_chunk]
+        prep = prepare_batch([config3_model(i) for i in chunk], indices=chunk, release_host=True,
+                             max_group=args.max_group)
+        setup += prep.setup_seconds
+        skipped += prep.skipped
+        group_sizes += [(g.n, g.count) for g in prep.groups]
+        for g in prep.groups:  # warm-up: program images are built at their first launch
+            solve_group(g, omega, policy=policy)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            chunk_solutions = []
+            for g in prep.groups:
+                chunk_solutions += solve_group(g, omega, policy=policy)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        timed += dt
+        solutions += chunk_solutions
+        print(f"[bench] chunk {c0 // args.batch_chunk}: {len(chunk_solutions)} instances in {len(prep.groups)} groups, "
+              f"setup {prep.setup_seconds:.1f}s, timed {dt:.2f}s for {steps} solve(s) of each", file=sys.stderr,
+              flush=True)
+        del prep
+        torch.cuda.empty_cache()
+    solved = len(solutions)
+    n_of = {s.index: config3_modes(s.index) + 1 for s in solutions}
+    line = {
+        "metric": "CCSD+EOM G^R(ω) solves/sec, 1 MI355X: mixed+batched+learned vs FP64 @1e-8",
+        "value": solved * steps / timed,
+        "unit": "solves/sec",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 1,
+        "ms_per_step": 1e3 * timed / steps,  # one step = one solve of every instance
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (JVP f32)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"siam_uneven_batch_{args.instances}_instances_4to32_modes_ccsd_eom_s13_gr_2048w",
+            "variant": "mixed_fp32_jvp_grouped_lockstep",
+            "jvp_matrix_cores": args.matrix_cores,
+            "instances_requested": args.instances,
+            "instances_solved": solved,
+            "instances_skipped_no_full_hf": [i for i, _ in skipped],
+            "groups": len(group_sizes),
+            "group_sizes_by_n": sorted(group_sizes),
+            "chunk": args.batch_chunk,
+            "timed_seconds": timed,
+            "setup_seconds_total": setup,
+            "n_sites_range": [min(n_of.values()), max(n_of.values())] if n_of else None,
+            "omega_points": int(omega.size),
+            "charges": [1, -1],
+            "packed_residual_max": max(s.packed_residual_norm for s in solutions),
+            "max_shift_relative_residual": max(s.max_shift_residual for s in solutions),
+            "jvp_evaluations_total": int(sum(s.jvp_evaluations for s in solutions)),
+            "newton_iterations_max": int(max(s.newton_iterations for s in solutions)),
+            "layers_in_timed_region": {"mixed_precision": True,
+                                       "batched": "instances of equal site count in lockstep groups (union plans); "
+                                                  "every frequency on one Krylov basis per channel",
+                                       "learned_warm_start": False},
+        },
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="siam24", choices=("siam24", "readme"))
+    ap.add_argument("--config", default="siam24", choices=("siam24", "readme", "kanamori39", "batch512"))
+    ap.add_argument("--instances", type=int, default=512, help="batch512: instances of the uneven-size batch")
+    ap.add_argument("--batch-chunk", type=int, default=64, help="batch512: instances set up and solved per chunk")
+    ap.add_argument("--max-group", type=int, default=8, help="batch512: instances per lockstep group")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forcing", default="eisenstat_walker", choices=("legacy", "eisenstat_walker"),
                     help="inexact-Newton forcing strategy of the CCSD solve (the reference's two options)")
@@ -280,12 +375,19 @@ def main() -> int:
                     help="also time one solve with the EOM channels on one stream (no instrumentation)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="end with one 256 MiB device copy (known bytes for hardware-counter calibration)")
+    ap.add_argument("--matrix-cores", default="auto", choices=("auto", "on", "off"),
+                    help="FP32 JVPs: rank-4 two-atom blocks on the matrix cores where they gain (auto), "
+                         "always (on) or never (off)")
     ap.add_argument("--jvp-probe", type=int, default=0, metavar="K",
-                    help="profiling workload only: one residual and K JVPs of siam24, then a 256 MiB copy")
+                    help="profiling workload only: one residual and K JVPs (--config siam24 or kanamori39, "
+                         "--precision), then a 256 MiB copy")
     args = ap.parse_args()
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from c3lindblad_amd.device import DevicePlan
+
+    DevicePlan.matrix_cores = {"off": 0, "auto": 1, "on": 2}[args.matrix_cores]
     if args.jvp_probe:
-        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-        return _jvp_probe(args.jvp_probe)
+        return _jvp_probe(args.jvp_probe, args.config, args.precision)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(
@@ -294,7 +396,8 @@ def main() -> int:
             file=sys.stderr,
         )
         return 2
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    if args.config == "batch512":
+        return _bench_batch(args)
     import torch
 
     from c3lindblad_amd.configs import config2_grid, config2_model, readme_grid, readme_model
@@ -305,6 +408,11 @@ def main() -> int:
     if args.config == "siam24":
         model, grid, charges = config2_model(), config2_grid(), (1, -1)
         workload = "siam_24_pseudomodes_U4_ccsd_eom_s13_gr_4001w"
+    elif args.config == "kanamori39":
+        from c3lindblad_amd.configs import config5_grid, config5_model
+
+        model, grid, charges = config5_model(0), config5_grid(), (1, -1)
+        workload = "kanamori_3orb_U3_J0.5_Vpqrs_36_pseudomodes_ccsd_eom_s13_gr_8001w"
     else:
         model, grid, charges = readme_model(), readme_grid(), (1,)
         workload = "readme_3site_U1.2_ccsd_eom_s13_gr_81w"
@@ -421,6 +529,7 @@ def main() -> int:
         "config": {
             "workload": workload,
             "variant": "fp64_gpu" if args.precision == "fp64" else "mixed_fp32_jvp",
+            "jvp_matrix_cores": args.matrix_cores if args.precision == "fp32" else "off",
             "layers_in_timed_region": {
                 "mixed_precision": args.precision == "fp32",
                 "batched": "every frequency on one Krylov basis per channel; addition and removal bases of a "

@@ -78,10 +78,40 @@ class InstanceGroup:
 
     @property
     def D(self) -> int:
-        return self.members[0].static.oracle.num_amplitudes
+        if "D" not in self._dev:
+            self._dev["D"] = self.members[0].static.oracle.num_amplitudes
+        return self._dev["D"]
+
+    @property
+    def n(self) -> int:
+        if "n" not in self._dev:
+            self._dev["n"] = self.members[0].static.oracle.model.n_sites
+        return self._dev["n"]
 
     def P(self, charge: int) -> int:
-        return self.members[0].eom[charge].dimension
+        if ("P", charge) not in self._dev:
+            self._dev[("P", charge)] = self.members[0].eom[charge].dimension
+        return self._dev[("P", charge)]
+
+    def release_host(self) -> None:
+        """Drop the host copies of the instances' problems and plans once the group is on the GPU
+        (the device plans, maps and the small per-instance source plans stay); the group can then
+        only be solved, not re-uploaded."""
+
+        self.D, self.n  # noqa: B018  (cached before the problems go)
+        for q in self.charges:
+            self.P(q)
+        for m in self.members:
+            m.static.lowered = None
+            m.static.oracle = None
+            for e in m.eom.values():
+                e.lowered = None
+                e.static = None
+                e.oracle = None
+        for key, value in self._dev.items():
+            for plan in (value if isinstance(value, list) else [value]):
+                if isinstance(plan, DevicePlan):
+                    plan.lowered = None
 
     def upload(self) -> None:
         """Union plans (static; per charge two twin copies of the sigma plan), union tilde maps,
@@ -411,13 +441,15 @@ class PreparedBatch:
         return sum(g.count for g in self.groups)
 
 
-def prepare_batch(models, charges=(1, -1), *, max_group: int = 8, workers: int = 1) -> PreparedBatch:
+def prepare_batch(models, charges=(1, -1), *, max_group: int = 8, workers: int = 1, indices=None,
+                  release_host: bool = False) -> PreparedBatch:
     """Host set-up of every instance (in ``workers`` processes when > 1; no GPU is touched there),
     grouping by site count and upload of the groups' union plans.  Instances whose full-HF reference
-    does not converge are skipped and listed."""
+    does not converge are skipped and listed.  ``indices`` names the instances (default 0..)."""
 
     started = perf_counter()
-    jobs = [(i, m, tuple(charges)) for i, m in enumerate(models)]
+    names = range(len(models)) if indices is None else indices
+    jobs = [(int(i), m, tuple(charges)) for i, m in zip(names, models)]
     if workers > 1:
         import multiprocessing as mp
         from concurrent.futures import ProcessPoolExecutor
@@ -428,7 +460,10 @@ def prepare_batch(models, charges=(1, -1), *, max_group: int = 8, workers: int =
         results = [_compile_job(j) for j in jobs]
     instances = [inst for inst, _ in results if inst is not None]
     skipped = [why for _, why in results if why is not None]
+    del results
     groups = [InstanceGroup(members, tuple(charges)) for members in group_by_size(instances, max_group)]
     for g in groups:
         g.upload()
+        if release_host:
+            g.release_host()
     return PreparedBatch(groups, skipped, perf_counter() - started)

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -440,6 +441,257 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
   }
 }
 
+// ------------------------------------------------------- matrix-core tile program
+//
+// FP32 programs: rank-4 output blocks whose entries are products of two atoms that split the
+// four positions between them (or of a rank-4 atom and a scalar atom) run on the matrix cores
+// instead of the per-coordinate gathers.  A workgroup owns a 16x4x4x4 tile of the padded
+// output (16 along position 0).  An entry A * B is taken with A the atom that covers position
+// 0 and c the lowest position of B; (pa, pb) are the other two positions.  Wave w owns the
+// slab pb = w of the tile and evaluates the entry as one v_mfma_f32_4x4x1_16b_f32 outer
+// product per 16 blocks: block b = (position 0 / 4, pa), rows = position 0 % 4, columns = c,
+// D[b][i][j] += (coeff A)[b][i] B[b][j], so 16 consecutive lanes read 16 consecutive
+// position-0 elements of A (whole 128-byte lines) and no product is wasted.  Complex
+// products use the 3M split (Higham): with A' = coeff A, T1 = Re A' Re B, T2 = Im A' Im B,
+// T3 = (Re A' + Im A')(Re B + Im B), Re = T1 - T2, Im = T3 - T1 - T2, three real MFMAs (a real
+// coefficient, the only kind the CCSD JVP has, scales A by two multiplies instead of four).
+// The three choices of c are the three views.  Operand lanes past the tensor read on into
+// the next atom or the zeroed tail of the workspace mirror; they feed only outputs outside
+// the tensor, which are never written.  The views meet in LDS, and the tile's outputs go out
+// 16 consecutive destinations per 16 lanes, with the rank-4 x scalar entries and the diagonal.
+// MFMA 4x4x1 (16 blocks) operand layout: lane l holds A[l/4][l%4] (block l/4, row l%4) and
+// B[l/4][l%4] (column l%4); accumulator register i of lane l holds D[l/4][i][l%4].
+
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+
+constexpr int kTileViews = 3;
+constexpr int kTileMaxRules = 32;
+// Below this many entries per block the per-coordinate gathers are as fast (measured: 44
+// entries at n = 25, 38 us on the tiles against 36 us gathered; 516 entries at n = 39, 0.87 ms
+// against 3.8 ms).
+constexpr int kTileMinEntries = 128;
+constexpr int kBufferWord3 = 0x00020000;  // raw buffer descriptor word 3 on gfx9-family parts
+
+// Elements past the FP32 workspace that tile operand lanes can reach: at most 15 indices past
+// the tensor at position 0 and 3 at the others, times strides of at most n^3 each.
+inline int64_t tile_tail_elements(int n) { return 32 * (int64_t)n * n * n + 64; }
+
+// view v: c = v + 1, (pa, pb) the other two of positions 1..3 in increasing order
+__host__ __device__ constexpr int view_pa(int v) { return v == 0 ? 2 : 1; }
+__host__ __device__ constexpr int view_pb(int v) { return v == 2 ? 2 : 3; }
+
+struct TileRule {  // one operand layout: lane l indexes position pos[0] by l % 16 (A) or l % 4
+                   // (B), pos[1] by l / 16; wave w indexes pos[2] by w
+  int32_t pos[3];
+  uint32_t stride[3];  // bytes per unit of the three positions (0 where the atom does not cover one)
+  int32_t b_side;      // 1 for B (l % 4)
+};
+
+struct TileEntry {  // coeff * A * B in one view of the FP32 workspace (16 bytes: one scalar load)
+  uint32_t a_off, b_off;  // atom offsets (bytes)
+  uint32_t lds;           // byte offsets of the A (bits 0-15) and B (bits 16-31) layouts in a wave's table
+  float coeff;            // real part; the imaginary parts are a separate table
+};
+
+struct TileScalarEntry {  // coeff * A[p0, p1, p2, p3] * s, element offsets
+  float2 coeff;
+  uint32_t a_off;
+  int32_t a_stride[4];
+  uint32_t scalar;
+};
+
+struct TileBlockDesc {
+  int32_t n;
+  int32_t per_dim;                    // 4-wide tiles per position
+  int32_t view_start[kTileViews + 1];  // entries of view v: [view_start[v], view_start[v + 1])
+  int32_t rule_start, rule_end;       // operand layouts of the block
+  int32_t scalar_start, scalar_end;   // rank-4 x scalar entries
+  const int32_t* dest;                // n^4 (first index fastest): destination of a tuple, -1 if none
+};
+
+// Entry tables are kernel arguments so that their wave-uniform reads become scalar loads.
+// tiles[blockIdx.x] = (block, g): g = g0 + G0 (t1 + T (t2 + T t3)) with G0 = ceil(n / 16).
+template <bool kReal>
+__global__ __launch_bounds__(256) void tile_program_mfma_kernel(const float2* __restrict__ ws,
+                                                                double2* __restrict__ out,
+                                                                const TileBlockDesc* __restrict__ descs,
+                                                                const TileRule* __restrict__ rules,
+                                                                const TileEntry* __restrict__ entries,
+                                                                const float* __restrict__ coeff_im,
+                                                                const TileScalarEntry* __restrict__ scalars,
+                                                                const int2* __restrict__ tiles,
+                                                                const double2* __restrict__ diag,
+                                                                const double2* __restrict__ x) {
+  __shared__ float2 part[1024];  // the workgroup's 16x4x4x4 outputs, position 0 fastest
+  // per wave, layout and lane: byte offset of the lane's operand element from the atom origin
+  __shared__ uint32_t lane_off[4][kTileMaxRules][64];
+  const int2 tile = tiles[blockIdx.x];
+  const TileBlockDesc& d = descs[tile.x];
+  const int n = d.n, T = d.per_dim, G0 = (n + 15) / 16;
+  const int l = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int P0 = 16 * (tile.y % G0), P1 = 4 * ((tile.y / G0) % T), P2 = 4 * ((tile.y / (G0 * T)) % T),
+            P3 = 4 * (tile.y / (G0 * T * T));
+  auto origin_of = [&](int k) {  // branch-free selection of the tile origin at position k
+    return P0 * (k == 0) + P1 * (k == 1) + P2 * (k == 2) + P3 * (k == 3);
+  };
+  const uint32_t u15 = l & 15, u3 = l & 3, u4 = l >> 4;
+  // the wave's operand offsets for every layout of the block (written and read by this wave
+  // only; LDS executes a wave's accesses in order)
+  for (int r = d.rule_start; r < d.rule_end; ++r) {
+    const TileRule R = rules[r];
+    lane_off[wave][r - d.rule_start][l] = (origin_of(R.pos[0]) + (R.b_side ? u3 : u15)) * R.stride[0] +
+                                          (origin_of(R.pos[1]) + u4) * R.stride[1] +
+                                          (origin_of(R.pos[2]) + wave) * R.stride[2];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const char* offs = reinterpret_cast<const char*>(&lane_off[wave][0][l]);
+  // raw buffer loads: the atom offset rides in the scalar offset, the lane offset in the vector
+  // offset; every address is inside the mirror and its tail by construction
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc((void*)ws, (short)0, -1, kBufferWord3);
+#pragma unroll
+  for (int v = 0; v < kTileViews; ++v) {
+    v4f_t acc[3] = {v4f_t{0.f, 0.f, 0.f, 0.f}, v4f_t{0.f, 0.f, 0.f, 0.f}, v4f_t{0.f, 0.f, 0.f, 0.f}};
+    if (origin_of(view_pb(v)) + wave < n) {  // wave-uniform: a slab past the tensor has no outputs
+      auto operands = [&](const TileEntry& E, float2& a, float2& b) {
+        const uint32_t ao = *reinterpret_cast<const uint32_t*>(offs + (E.lds & 0xffffu));
+        const uint32_t bo = *reinterpret_cast<const uint32_t*>(offs + (E.lds >> 16));
+        a = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)ao, (int)E.a_off, 0));
+        b = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)bo, (int)E.b_off, 0));
+      };
+      auto mac = [&](const float2 c, const float2 a, const float2 b) {  // c: coefficient
+        float ar = c.x * a.x, ai = c.x * a.y;
+        if constexpr (!kReal) {
+          ar -= c.y * a.y;
+          ai += c.y * a.x;
+        }
+        acc[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(ar, b.x, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(ai, b.y, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(ar + ai, b.x + b.y, acc[2], 0, 0, 0);
+      };
+      const int end = d.view_start[v + 1];
+      int e = d.view_start[v];
+      // batches of eight entries, software-pipelined: the next batch's operand loads are in
+      // flight while the MFMAs of the current one run
+      constexpr int kBatch = 8;
+      struct Batch {
+        float2 a[kBatch], b[kBatch], c[kBatch];
+      };
+      auto issue = [&](int e0, Batch& B) {
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) {
+          const TileEntry E = entries[e0 + q];
+          operands(E, B.a[q], B.b[q]);
+          B.c[q] = make_float2(E.coeff, kReal ? 0.f : coeff_im[e0 + q]);
+        }
+      };
+      auto consume = [&](const Batch& B) {
+#pragma unroll
+        for (int q = 0; q < kBatch; ++q) mac(B.c[q], B.a[q], B.b[q]);
+      };
+      if (e + kBatch <= end) {
+        Batch X, Y;
+        issue(e, X);
+        e += kBatch;
+        while (true) {
+          if (e + kBatch > end) {
+            consume(X);
+            break;
+          }
+          issue(e, Y);
+          e += kBatch;
+          consume(X);
+          if (e + kBatch > end) {
+            consume(Y);
+            break;
+          }
+          issue(e, X);
+          e += kBatch;
+          consume(Y);
+        }
+      }
+      for (; e < end; ++e) {
+        const TileEntry E = entries[e];
+        float2 f, g;
+        operands(E, f, g);
+        mac(make_float2(E.coeff, kReal ? 0.f : coeff_im[e]), f, g);
+      }
+    }
+    // accumulator register i of lane l: position 0 = 4 ((l / 4) % 4) + i, pa = l / 16, c = l % 4,
+    // pb = wave; the four waves write disjoint slabs of the workgroup tile
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      int loc[4];
+      loc[0] = 4 * (int)((l >> 2) & 3) + i;
+      loc[v + 1] = (int)u3;
+      loc[view_pa(v)] = (int)u4;
+      loc[view_pb(v)] = wave;
+      const int t = loc[0] + 16 * loc[1] + 64 * loc[2] + 256 * loc[3];
+      const float t1 = acc[0][i], t2 = acc[1][i], t3 = acc[2][i];
+      float2 val = make_float2(t1 - t2, t3 - t1 - t2);
+      if (v > 0) {
+        const float2 prev = part[t];
+        val.x += prev.x;
+        val.y += prev.y;
+      }
+      part[t] = val;
+    }
+    __syncthreads();
+  }
+  // the workgroup's outputs, four per thread (16 consecutive position-0 indices per 16 lanes)
+  int32_t dd[4];
+  float2 sum[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = threadIdx.x + 256 * k;
+    const int p0 = P0 + (t & 15), p1 = P1 + ((t >> 4) & 3), p2 = P2 + ((t >> 6) & 3), p3 = P3 + (t >> 8);
+    const bool inside = p0 < n && p1 < n && p2 < n && p3 < n;
+    dd[k] = inside ? d.dest[p0 + n * (p1 + n * (p2 + n * p3))] : -1;
+    sum[k] = part[t];
+  }
+  // rank-4 x scalar entries, four entries' loads in flight; tuples without a destination read
+  // index 0 of the atom
+  auto scalar_terms = [&](int e0, int count) {
+    float2 cs[4], av[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q >= count) break;
+      const TileScalarEntry S = scalars[e0 + q];
+      const float2 sv = ws[S.scalar];
+      cs[q] = make_float2(S.coeff.x * sv.x - S.coeff.y * sv.y, S.coeff.x * sv.y + S.coeff.y * sv.x);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int t = threadIdx.x + 256 * k;
+        const int p0 = P0 + (t & 15), p1 = P1 + ((t >> 4) & 3), p2 = P2 + ((t >> 6) & 3), p3 = P3 + (t >> 8);
+        const int64_t idx = dd[k] >= 0 ? (int64_t)p0 * S.a_stride[0] + (int64_t)p1 * S.a_stride[1] +
+                                             (int64_t)p2 * S.a_stride[2] + (int64_t)p3 * S.a_stride[3]
+                                       : 0;
+        av[q][k] = ws[(int64_t)S.a_off + idx];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q >= count) break;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        sum[k].x += cs[q].x * av[q][k].x - cs[q].y * av[q][k].y;
+        sum[k].y += cs[q].x * av[q][k].y + cs[q].y * av[q][k].x;
+      }
+    }
+  };
+  int e = d.scalar_start;
+  for (; e + 4 <= d.scalar_end; e += 4) scalar_terms(e, 4);
+  if (e < d.scalar_end) scalar_terms(e, d.scalar_end - e);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (dd[k] < 0) continue;
+    double2 v = make_double2((double)sum[k].x, (double)sum[k].y);
+    if (diag != nullptr) v = cadd(v, cmul(diag[dd[k]], x[dd[k]]));
+    out[dd[k]] = v;
+  }
+}
+
 __global__ void fill_const_kernel(double2* __restrict__ ws, const double2* __restrict__ vectors,
                                   const int64_t* __restrict__ offset, const int32_t* __restrict__ vec, int n) {
   const int k = blockIdx.x;
@@ -812,6 +1064,21 @@ struct ProgramExec {
   WedgeBlockDesc* descs = nullptr;
   int2* tiles = nullptr;  // two-factor tiles first, then general tiles
   std::vector<void*> owned;
+  std::vector<int32_t> desc_block;  // plan block of every descriptor
+  std::vector<int2> host_tiles;
+  // FP32 matrix-core image (built at the program's first FP32 launch): rank-4 two-atom blocks
+  // go to tile_program_mfma_kernel, the rest keep the gather kernels
+  bool mfma_built = false;
+  int32_t mfma_tiles = 0;
+  bool mfma_real = false;  // every matrix-core entry has a real coefficient
+  TileBlockDesc* mfma_descs = nullptr;
+  TileRule* mfma_rules = nullptr;
+  TileEntry* mfma_entries = nullptr;
+  float* mfma_coeff_im = nullptr;
+  TileScalarEntry* mfma_scalars = nullptr;
+  int2* mfma_tile_list = nullptr;
+  int32_t tiles32_two = 0, tiles32_general = 0;
+  int2* tiles32 = nullptr;
 };
 
 struct PendingTiming {
@@ -841,7 +1108,11 @@ struct c3l_plan {
   int32_t kind_bits[kKinds] = {64, 64, 64};
   int32_t program_bits[kPrograms] = {64, 64, 64, 64, 64, 64, 64, 64};
   float2* ws32 = nullptr;
+  int64_t ws32_tail = 0;  // zeroed elements allocated past the FP32 workspace
   float2* vectors32 = nullptr;
+  // FP32 programs: rank-4 two-atom blocks on the matrix cores: 0 never, 1 blocks with at least
+  // kTileMinEntries entries (below that the gather kernels are as fast), 2 every block
+  int32_t mfma = 1;
   int64_t fixed_version = 0;
   int64_t narrowed_version = -1;
   bool ranges_built = false;
@@ -1441,6 +1712,7 @@ int build_program(c3l_plan* plan, int32_t program) {
     std::vector<int2>& tiles = b.two_factors ? two : general;
     for (int64_t first = 0; first < b.count; first += per) tiles.push_back(make_int2((int)descs.size(), (int)first));
     descs.push_back(d);
+    pe.desc_block.push_back((int32_t)i);
     if (b.max_dest > pe.max_dest) pe.max_dest = b.max_dest;
   }
   // Workgroup w runs on XCD w % 8 (round-robin dispatch).  Each block's tiles are cut
@@ -1481,7 +1753,175 @@ int build_program(c3l_plan* plan, int32_t program) {
   int rc = upload_owned(pe.owned, &pe.descs, descs.data(), descs.size());
   if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.tiles, tiles.data(), tiles.size());
   if (rc != C3L_OK) return rc;
+  pe.host_tiles = tiles;
   pe.built = true;
+  return C3L_OK;
+}
+
+// Splits the entries of a rank-4 two-atom block into the views of tile_program_mfma_kernel
+// with the block's operand layouts (byte strides in the FP32 workspace), and collects its
+// rank-4 x scalar entries; false when an entry's atoms do not split the four positions
+// between them or the block has more layouts than the kernel holds.
+bool tile_entries_for(const PlanBlock& b, std::vector<TileRule>& rules, std::vector<TileEntry>& entries,
+                      std::vector<float2>& coeffs, int32_t (&view_start)[kTileViews + 1],
+                      std::vector<TileScalarEntry>& scalars) {
+  auto positions = [](const WedgeFactor& f) {
+    int m = 0;
+    for (int k = 0; k < 4; ++k)
+      if (f.stride[k] != 0) m |= 1 << k;
+    return m;
+  };
+  std::map<std::array<uint32_t, 7>, int32_t> rule_of;
+  auto rule = [&](const int (&pos)[3], int b_side, const WedgeFactor& f) -> int32_t {
+    std::array<uint32_t, 7> key{};
+    for (int k = 0; k < 3; ++k) {
+      key[(size_t)k] = (uint32_t)pos[k];
+      key[(size_t)k + 3] = 8u * (uint32_t)f.stride[pos[k]];
+    }
+    key[6] = (uint32_t)b_side;
+    auto it = rule_of.find(key);
+    if (it != rule_of.end()) return it->second;
+    if ((int)rules.size() == kTileMaxRules) return -1;
+    rules.push_back(TileRule{{pos[0], pos[1], pos[2]}, {key[3], key[4], key[5]}, b_side});
+    return rule_of[key] = (int32_t)rules.size() - 1;
+  };
+  std::vector<TileEntry> by_view[kTileViews];
+  std::vector<float2> coeff_by_view[kTileViews];
+  for (int32_t t = 0; t < b.n_entries; ++t) {
+    const WedgeEntry& e = b.entries[(size_t)t];
+    const WedgeFactor* f[2] = {&b.factors[(size_t)e.first], &b.factors[(size_t)e.first + 1]};
+    const int m[2] = {positions(*f[0]), positions(*f[1])};
+    if ((m[0] & m[1]) != 0 || (m[0] | m[1]) != 15) return false;
+    const float2 coeff = make_float2((float)e.coeff.x, (float)e.coeff.y);
+    if (m[0] == 0 || m[1] == 0) {
+      const WedgeFactor& a = m[0] == 0 ? *f[1] : *f[0];
+      const WedgeFactor& sc = m[0] == 0 ? *f[0] : *f[1];
+      TileScalarEntry se{};
+      se.coeff = coeff;
+      se.a_off = (uint32_t)a.offset;
+      for (int k = 0; k < 4; ++k) se.a_stride[k] = a.stride[k];
+      se.scalar = (uint32_t)sc.offset;
+      scalars.push_back(se);
+      continue;
+    }
+    const int ai = m[0] & 1 ? 0 : 1;  // A covers position 0
+    const WedgeFactor& A = *f[ai];
+    const WedgeFactor& B = *f[1 - ai];
+    const int c = __builtin_ctz((unsigned)m[1 - ai]);
+    const int v = c - 1, pa = view_pa(v), pb = view_pb(v);
+    const int apos[3] = {0, pa, pb}, bpos[3] = {c, pa, pb};
+    const int32_t ra = rule(apos, 0, A), rb = rule(bpos, 1, B);
+    if (ra < 0 || rb < 0) return false;
+    by_view[v].push_back(TileEntry{8u * (uint32_t)A.offset, 8u * (uint32_t)B.offset,
+                                   256u * (uint32_t)ra | (256u * (uint32_t)rb << 16), coeff.x});
+    coeff_by_view[v].push_back(coeff);
+  }
+  for (int v = 0; v < kTileViews; ++v) {
+    entries.insert(entries.end(), by_view[v].begin(), by_view[v].end());
+    coeffs.insert(coeffs.end(), coeff_by_view[v].begin(), coeff_by_view[v].end());
+    view_start[v + 1] = (int32_t)entries.size();
+  }
+  return true;
+}
+
+// The FP32 matrix-core image of a built program (see ProgramExec): rank-4 two-atom blocks
+// whose coordinates fill at least a quarter of the 16x4x4x4 workgroup tiles they touch.
+int build_program_mfma(c3l_plan* plan, int32_t program) {
+  ProgramExec& pe = plan->programs[program];
+  if (pe.mfma_built) return C3L_OK;
+  const int n = plan->n;
+  const int64_t n4 = (int64_t)n * n * n * n;
+  const int T = (n + 3) / 4, G0 = (n + 15) / 16;
+  const int64_t tiles_per_block = (int64_t)G0 * T * T * T;
+  std::vector<char> on_tiles(pe.desc_block.size(), 0);
+  std::vector<TileBlockDesc> descs;
+  std::vector<TileRule> all_rules;
+  std::vector<TileEntry> all_entries;
+  std::vector<float> all_im;
+  std::vector<TileScalarEntry> all_scalars;
+  std::vector<int2> tile_list;
+  // byte offsets into the mirror (atom offsets, and within a rank-4 atom) are 32-bit, destinations
+  // and tuple indices 31-bit; operand lanes past the tensor need the zeroed tail of the mirror
+  const bool fits = plan->mfma != 0 && plan->ws32_tail >= tile_tail_elements(n) &&
+                    8 * (plan->workspace + plan->ws32_tail) < (int64_t(1) << 32) && 16 * n4 < (int64_t(1) << 32);
+  for (size_t di = 0; fits && di < pe.desc_block.size(); ++di) {
+    const PlanBlock& b = plan->blocks[(size_t)pe.desc_block[di]];
+    if (b.dense || b.deg != 4 || !b.two_factors || b.n_entries == 0 || b.max_dest >= (int64_t(1) << 31)) continue;
+    if (plan->mfma == 1 && b.n_entries < kTileMinEntries) continue;
+    std::vector<TileRule> rules;
+    std::vector<TileEntry> entries;
+    std::vector<float2> coeffs;
+    std::vector<TileScalarEntry> scalars;
+    TileBlockDesc d{};
+    if (!tile_entries_for(b, rules, entries, coeffs, d.view_start, scalars)) continue;
+    std::vector<int32_t> dest((size_t)n4, -1);
+    std::vector<char> used((size_t)tiles_per_block, 0);
+    for (int64_t k = 0; k < b.count; ++k) {
+      const uint32_t c = b.coord[(size_t)k];
+      const int i0 = c & 255u, i1 = (c >> 8) & 255u, i2 = (c >> 16) & 255u, i3 = c >> 24;
+      dest[(size_t)(i0 + (int64_t)n * (i1 + (int64_t)n * (i2 + (int64_t)n * i3)))] = (int32_t)b.dest[(size_t)k];
+      used[(size_t)(i0 / 16 + (int64_t)G0 * (i1 / 4 + (int64_t)T * (i2 / 4 + (int64_t)T * (i3 / 4))))] = 1;
+    }
+    int64_t volume = 0;  // index tuples inside the tensor of the touched tiles
+    for (int64_t t = 0; t < tiles_per_block; ++t) {
+      if (!used[(size_t)t]) continue;
+      const int64_t g0 = t % G0, t1 = (t / G0) % T, t2 = (t / (G0 * T)) % T, t3 = t / ((int64_t)G0 * T * T);
+      volume += std::min<int64_t>(16, n - 16 * g0) * std::min<int64_t>(4, n - 4 * t1) *
+                std::min<int64_t>(4, n - 4 * t2) * std::min<int64_t>(4, n - 4 * t3);
+    }
+    if (4 * b.count < volume) continue;  // too sparse for whole tiles
+    int32_t* dst = nullptr;
+    int rc = upload_owned(pe.owned, &dst, dest.data(), dest.size());
+    if (rc != C3L_OK) return rc;
+    d.n = n;
+    d.per_dim = T;
+    for (int v = 0; v <= kTileViews; ++v) d.view_start[v] += (int32_t)all_entries.size();
+    all_entries.insert(all_entries.end(), entries.begin(), entries.end());
+    for (const float2& c : coeffs) all_im.push_back(c.y);
+    d.rule_start = (int32_t)all_rules.size();
+    all_rules.insert(all_rules.end(), rules.begin(), rules.end());
+    d.rule_end = (int32_t)all_rules.size();
+    d.scalar_start = (int32_t)all_scalars.size();
+    all_scalars.insert(all_scalars.end(), scalars.begin(), scalars.end());
+    d.scalar_end = (int32_t)all_scalars.size();
+    d.dest = dst;
+    // workgroups go round-robin to the eight XCDs: give each XCD one contiguous eighth of the
+    // block's touched tiles in memory order (neighbouring tiles share atom slices in its L2)
+    std::vector<int32_t> seq;
+    for (int64_t t = 0; t < tiles_per_block; ++t)
+      if (used[(size_t)t]) seq.push_back((int32_t)t);
+    constexpr int kXcds = 8;
+    const int64_t count = (int64_t)seq.size();
+    const int64_t chunk = (count + kXcds - 1) / kXcds;
+    for (int64_t q = 0; q < chunk; ++q)
+      for (int xcd = 0; xcd < kXcds; ++xcd) {
+        const int64_t t = (int64_t)xcd * chunk + q;
+        if (t < count) tile_list.push_back(make_int2((int)descs.size(), seq[(size_t)t]));
+      }
+    descs.push_back(d);
+    on_tiles[di] = 1;
+  }
+  std::vector<int2> two, general;
+  for (int32_t k = 0; k < pe.tiles_two + pe.tiles_general; ++k) {
+    const int2 t = pe.host_tiles[(size_t)k];
+    if (on_tiles[(size_t)t.x]) continue;
+    (k < pe.tiles_two ? two : general).push_back(t);
+  }
+  pe.tiles32_two = (int32_t)two.size();
+  pe.tiles32_general = (int32_t)general.size();
+  two.insert(two.end(), general.begin(), general.end());
+  int rc = upload_owned(pe.owned, &pe.tiles32, two.data(), two.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_descs, descs.data(), descs.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_tile_list, tile_list.data(), tile_list.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_rules, all_rules.data(), all_rules.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_entries, all_entries.data(), all_entries.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_coeff_im, all_im.data(), all_im.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_scalars, all_scalars.data(), all_scalars.size());
+  if (rc != C3L_OK) return rc;
+  pe.mfma_tiles = (int32_t)tile_list.size();
+  pe.mfma_real = true;
+  for (float im : all_im) pe.mfma_real = pe.mfma_real && im == 0.f;
+  pe.mfma_built = true;
   return C3L_OK;
 }
 
@@ -1575,15 +2015,36 @@ int current_scratch(double** out) {
 
 extern "C" {
 
+int c3l_plan_set_matrix_cores(c3l_plan* plan, int32_t mode) {
+  if (plan == nullptr || mode < 0 || mode > 2) return fail(C3L_EINVAL, "c3l_plan_set_matrix_cores: invalid argument");
+  if (mode != plan->mfma) {
+    plan->mfma = mode;
+    for (ProgramExec& pe : plan->programs) pe.mfma_built = false;  // rebuilt at the next FP32 launch
+  }
+  return C3L_OK;
+}
+
+int c3l_plan_matrix_core_tiles(c3l_plan* plan, int32_t program, int32_t* tiles) {
+  if (plan == nullptr || tiles == nullptr || program < 0 || program >= kPrograms)
+    return fail(C3L_EINVAL, "c3l_plan_matrix_core_tiles: invalid argument");
+  const ProgramExec& pe = plan->programs[program];
+  *tiles = pe.mfma_built ? pe.mfma_tiles : 0;
+  return C3L_OK;
+}
+
 int c3l_plan_set_precision(c3l_plan* plan, int32_t kind, int32_t program, int32_t bits) {
   if (plan == nullptr || kind < 0 || kind >= kKinds || program < 0 || program >= kPrograms || (bits != 32 && bits != 64))
     return fail(C3L_EINVAL, "c3l_plan_set_precision: invalid argument");
   if (bits == 32 && plan->ws32 == nullptr) {
     void* p = nullptr;
-    if (hipMalloc(&p, sizeof(float2) * (size_t)(plan->workspace > 0 ? plan->workspace : 1)) != hipSuccess)
+    // a zeroed tail past the mirror: the matrix-core tiles' operand lanes past the tensor read it
+    const int64_t tail = tile_tail_elements(plan->n);
+    if (hipMalloc(&p, sizeof(float2) * (size_t)(plan->workspace + tail)) != hipSuccess)
       return fail(C3L_ENOMEM, "c3l_plan_set_precision: FP32 workspace allocation failed");
     plan->owned.push_back(p);
     plan->ws32 = static_cast<float2*>(p);
+    C3L_HIP_CHECK(hipMemset(plan->ws32 + plan->workspace, 0, sizeof(float2) * (size_t)tail));
+    plan->ws32_tail = tail;
     const int64_t nv = (int64_t)plan->n_vectors * plan->n;
     if (nv > 0) {
       if (hipMalloc(&p, sizeof(float2) * (size_t)nv) != hipSuccess)
@@ -1650,14 +2111,27 @@ int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t 
     rc = narrow_fixed_atoms(plan, s);
     if (rc != C3L_OK) return rc;
     const Twins<const float2*> ws32 = twin<const float2*>(plan->ws32);
-    if (pe.tiles_two > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<true, float2>), two, dim3(kThreads), 0, s, ws32, o, pe.descs, pe.tiles,
-                         dg, xx);
+    rc = build_program_mfma(plan, program);
+    if (rc != C3L_OK) return rc;
+    if (pe.mfma_tiles > 0) {
+      if (pe.mfma_real)
+        hipLaunchKernelGGL(tile_program_mfma_kernel<true>, dim3(pe.mfma_tiles), dim3(kThreads), 0, s, plan->ws32, o.p[0],
+                           pe.mfma_descs, pe.mfma_rules, pe.mfma_entries, pe.mfma_coeff_im, pe.mfma_scalars,
+                           pe.mfma_tile_list, dg, xx.p[0]);
+      else
+        hipLaunchKernelGGL(tile_program_mfma_kernel<false>, dim3(pe.mfma_tiles), dim3(kThreads), 0, s, plan->ws32,
+                           o.p[0], pe.mfma_descs, pe.mfma_rules, pe.mfma_entries, pe.mfma_coeff_im, pe.mfma_scalars,
+                           pe.mfma_tile_list, dg, xx.p[0]);
       C3L_HIP_CHECK(hipGetLastError());
     }
-    if (pe.tiles_general > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<false, float2>), general, dim3(kThreads), 0, s, ws32, o, pe.descs,
-                         pe.tiles + pe.tiles_two, dg, xx);
+    if (pe.tiles32_two > 0) {
+      hipLaunchKernelGGL((wedge_program_kernel<true, float2>), dim3(pe.tiles32_two), dim3(kThreads), 0, s, ws32, o,
+                         pe.descs, pe.tiles32, dg, xx);
+      C3L_HIP_CHECK(hipGetLastError());
+    }
+    if (pe.tiles32_general > 0) {
+      hipLaunchKernelGGL((wedge_program_kernel<false, float2>), dim3(pe.tiles32_general), dim3(kThreads), 0, s, ws32,
+                         o, pe.descs, pe.tiles32 + pe.tiles32_two, dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
   } else {

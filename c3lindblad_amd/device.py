@@ -24,9 +24,12 @@ def _c(array: np.ndarray, dtype) -> np.ndarray:
 class DevicePlan:
     """A ``LoweredPlan`` uploaded to the GPU (``c3l_plan_*``)."""
 
+    matrix_cores = 1  # matrix-core mode of the FP32 programs of plans created from now on
+
     def __init__(self, lowered: LoweredPlan):
         self.torch = _lib.require_device()
         self.lowered = lowered
+        self._has_creation = any(b[0] == PROGRAMS["creation"] for b in lowered.blocks)
         self.loaded: dict[int, int] = {}
         handle = ctypes.c_void_p()
         vectors = _c(lowered.vectors, np.complex128)
@@ -45,6 +48,8 @@ class DevicePlan:
             ctypes.byref(handle),
         )
         self.handle = handle
+        if DevicePlan.matrix_cores != 1:
+            self.set_matrix_cores(DevicePlan.matrix_cores)
         for kind, offset, deg, src, sign in lowered.bases:
             s = _c(src, np.int64)
             g = _c(sign, np.int8)
@@ -114,12 +119,23 @@ class DevicePlan:
 
     @property
     def has_creation(self) -> bool:
-        return any(b[0] == PROGRAMS["creation"] for b in self.lowered.blocks)
+        return self._has_creation
 
     def set_precision(self, kind: str, program: str, bits: int) -> None:
         """Build ``kind`` atoms and evaluate ``program`` in FP32 (``bits=32``) or FP64."""
 
         _lib.call("c3l_plan_set_precision", self.handle, KINDS[kind], PROGRAMS[program], int(bits))
+
+    def set_matrix_cores(self, mode) -> None:
+        """FP32 programs: rank-4 two-atom blocks on the matrix cores -- ``False``/0 never,
+        ``True``/1 blocks large enough to gain (the default), 2 every such block."""
+
+        _lib.call("c3l_plan_set_matrix_cores", self.handle, int(mode))
+
+    def matrix_core_tiles(self, program: str) -> int:
+        tiles = ctypes.c_int32()
+        _lib.call("c3l_plan_matrix_core_tiles", self.handle, PROGRAMS[program], ctypes.byref(tiles))
+        return int(tiles.value)
 
     def set_timing(self, enable: bool) -> None:
         _lib.call("c3l_plan_set_timing", self.handle, 1 if enable else 0)

@@ -23,6 +23,7 @@ ascending modes sorts exactly like the integer mask within one rank.
 
 from __future__ import annotations
 
+from functools import lru_cache
 from dataclasses import dataclass, field
 from itertools import combinations
 from math import comb
@@ -283,10 +284,20 @@ def coordinate_set(n: int, keys_by_rank: list[list[BlockKey]]) -> CoordinateSet:
     return cs
 
 
-def static_coordinates(n: int) -> CoordinateSet:
-    """CCSD strings: PH rank 1 ((1,1,0,0),(0,0,1,1)) then rank 2 (``ci/basis.py:149-178``)."""
+def _frozen_set(cs: CoordinateSet) -> CoordinateSet:
+    for a in (cs.modes, cs.degrees, *[x for b in cs.blocks for x in (b.positions, b.indices)]):
+        a.setflags(write=False)
+    return cs
 
-    return coordinate_set(n, [[(1, 1, 0, 0), (0, 0, 1, 1)], [(2, 2, 0, 0), (1, 1, 1, 1), (0, 0, 2, 2)]])
+
+@lru_cache(maxsize=64)
+def static_coordinates(n: int) -> CoordinateSet:
+    """CCSD strings: PH rank 1 ((1,1,0,0),(0,0,1,1)) then rank 2 (``ci/basis.py:149-178``).
+
+    The set depends on ``n`` only: it is built once per site count and shared (read-only) by
+    every problem of that size."""
+
+    return _frozen_set(coordinate_set(n, [[(1, 1, 0, 0), (0, 0, 1, 1)], [(2, 2, 0, 0), (1, 1, 1, 1), (0, 0, 2, 2)]]))
 
 
 def response_keys(charge: int) -> list[list[BlockKey]]:
@@ -299,11 +310,14 @@ def response_keys(charge: int) -> list[list[BlockKey]]:
     raise ValueError("response charge must be +1 or -1")
 
 
+@lru_cache(maxsize=128)
 def response_coordinates(n: int, charge: int) -> CoordinateSet:
+    """S13 strings of one charge block (built once per site count and charge, read-only)."""
+
     keys = response_keys(charge)
     if n < 2:
         keys = [keys[0], [k for k in keys[1] if max(k) <= n]]
-    return coordinate_set(n, keys)
+    return _frozen_set(coordinate_set(n, keys))
 
 
 # ----------------------------------------------------------------------------- amplitude tilde map

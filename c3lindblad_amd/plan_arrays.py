@@ -92,9 +92,25 @@ class LoweredPlan:
         return int(sum(b[4].size for b in self.blocks if b[0] == pid))
 
 
-def padded_scatter(coords: CoordinateSet, key: BlockKey) -> tuple[np.ndarray, np.ndarray]:
-    """Index/sign maps from a vector to the padded antisymmetric tensor of one block."""
+_scatter_cache: dict = {}
 
+
+def padded_scatter(coords: CoordinateSet, key: BlockKey) -> tuple[np.ndarray, np.ndarray]:
+    """Index/sign maps from a vector to the padded antisymmetric tensor of one block (shared,
+    read-only, per coordinate set and key)."""
+
+    cache_key = (id(coords), key)
+    hit = _scatter_cache.get(cache_key)
+    if hit is not None and hit[0] is coords:
+        return hit[1], hit[2]
+    src, sign = _padded_scatter(coords, key)
+    src.setflags(write=False)
+    sign.setflags(write=False)
+    _scatter_cache[cache_key] = (coords, src, sign)
+    return src, sign
+
+
+def _padded_scatter(coords: CoordinateSet, key: BlockKey) -> tuple[np.ndarray, np.ndarray]:
     n = coords.n
     block = coords.block(key)
     deg = sum(key)

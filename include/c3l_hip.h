@@ -111,6 +111,18 @@ int c3l_plan_apply(c3l_plan* plan, int32_t program, const double* diag, const do
  * Workspace (creation) programs always run in FP64. */
 int c3l_plan_set_precision(c3l_plan* plan, int32_t kind, int32_t program, int32_t bits);
 
+/* Matrix cores for FP32 programs (no counterpart in the reference): output blocks of rank 4
+ * whose entries are products of two atoms splitting the four positions between them (or of a
+ * rank-4 atom and a scalar atom) run as 16x4x4x4 output tiles on v_mfma_f32_4x4x1_16b_f32
+ * (complex products by the 3M split) instead of the per-coordinate gather kernels.  mode 0:
+ * never; 1 (the default): blocks with at least 128 entries; 2: every such block.  FP64
+ * programs are not affected. */
+int c3l_plan_set_matrix_cores(c3l_plan* plan, int32_t mode);
+
+/* Matrix-core tiles of `program` at its last FP32 launch (0 before one, or when no block
+ * qualifies). */
+int c3l_plan_matrix_core_tiles(c3l_plan* plan, int32_t program, int32_t* tiles);
+
 /* Measurement hooks (no counterpart in the reference, which times with perf_counter,
  * cc/component_jax.py:53-65): when enabled, every program launch of c3l_plan_apply is
  * bracketed by HIP events on its stream; c3l_plan_program_time waits for them and
