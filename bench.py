@@ -283,6 +283,7 @@ def _bench_batch(args) -> int:
     for c0 in range(0, len(ids), args.batch_chunk):
         chunk = ids[c0:c0 + args.batch_chunk]
         prep = prepare_batch([config3_model(i) for i in chunk], indices=chunk, release_host=True,
+                             workers=args.setup_workers,
                              max_group=args.max_group)
         setup += prep.setup_seconds
         skipped += prep.skipped
@@ -357,6 +358,8 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=512, help="batch512: instances of the uneven-size batch")
     ap.add_argument("--batch-chunk", type=int, default=64, help="batch512: instances set up and solved per chunk")
     ap.add_argument("--max-group", type=int, default=8, help="batch512: instances per lockstep group")
+    ap.add_argument("--setup-workers", type=int, default=min(16, os.cpu_count() or 1),
+                    help="batch512: host processes for the instances' set-up (outside the timed region)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--forcing", default="eisenstat_walker", choices=("legacy", "eisenstat_walker"),
                     help="inexact-Newton forcing strategy of the CCSD solve (the reference's two options)")

@@ -421,8 +421,15 @@ def solve_group(group: InstanceGroup, omega, eta: float = 0.0, policy=None, rtol
 
 
 def _compile_job(args):
-    index, model, charges = args
+    index, model, charges = args[:3]
     try:
+        if len(args) > 3 and args[3]:
+            # worker process: the host set-up is many small dense factorisations (Sylvester
+            # solves of 2n x 2n), which run fastest on one BLAS thread per worker
+            from threadpoolctl import threadpool_limits
+
+            with threadpool_limits(limits=1):
+                return compile_instance(index, model, charges), None
         return compile_instance(index, model, charges), None
     except RuntimeError as err:
         if "full-HF" not in str(err):
@@ -449,7 +456,7 @@ def prepare_batch(models, charges=(1, -1), *, max_group: int = 8, workers: int =
 
     started = perf_counter()
     names = range(len(models)) if indices is None else indices
-    jobs = [(int(i), m, tuple(charges)) for i, m in zip(names, models)]
+    jobs = [(int(i), m, tuple(charges), workers > 1) for i, m in zip(names, models)]
     if workers > 1:
         import multiprocessing as mp
         from concurrent.futures import ProcessPoolExecutor

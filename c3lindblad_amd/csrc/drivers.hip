@@ -444,21 +444,20 @@ int64_t blocks_for(int64_t n, int t = 256) {
 
 // Workspace layouts in doubles.
 // GMRES (restart R, real basis of rd rows):
-//   coef:    red [2(R+2)] | ab [2(R+1)] | sc [16] | H [R(R+1)] | y [R] | stage [R+2]
+//   coef:    red [2(R+2)] | sc [16] | H [R(R+1)] | y [R] | stage [R+2]
 //   scratch: partial   (also the norms' partials)
 // Arnoldi (m_max columns, complex basis of dim rows, n_shifts shifts), complex entries doubled:
-//   coef:    red [2(m+2)] | ab [2(m+1)] | sc [16] | sink_v [m+1]
+//   coef:    red [2(m+2)] | sc [16] | sink_v [m+1]
 //   scratch: w [dim] | partial | green_hist [kShiftBatch n_shifts] | resid_hist (doubles)
 struct Layout {
-  int64_t red, ab, sc, H, y, stage, sink_v, coef_total;
+  int64_t red, sc, H, y, stage, sink_v, coef_total;
   int64_t w, partial, green_hist, resid_hist, scratch_total;
 };
 
 Layout gmres_layout(int64_t rd, int32_t R) {
   Layout L{};
   L.red = 0;
-  L.ab = L.red + 2 * ((int64_t)R + 2);
-  L.sc = L.ab + 2 * ((int64_t)R + 1);
+  L.sc = L.red + 2 * ((int64_t)R + 2);
   L.H = L.sc + 16;
   L.y = L.H + (int64_t)R * (R + 1);
   L.stage = L.y + R;
@@ -472,8 +471,7 @@ Layout gmres_layout(int64_t rd, int32_t R) {
 Layout arnoldi_layout(int64_t dim, int32_t m, int32_t n_shifts) {
   Layout L{};
   L.red = 0;
-  L.ab = L.red + 2 * 2 * ((int64_t)m + 2);
-  L.sc = L.ab + 2 * 2 * ((int64_t)m + 1);
+  L.sc = L.red + 2 * 2 * ((int64_t)m + 2);
   L.sink_v = L.sc + 2 * 16;
   L.coef_total = L.sink_v + 2 * ((int64_t)m + 1);
   L.w = 0;
@@ -602,7 +600,7 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   const int R = restart;
   const Layout L = gmres_layout(rd, R);
   c3l_ortho::Basis<double, Store> basis{V, rd, rd, coef + L.H, R + 1, nullptr, nullptr, scratch + L.partial,
-                                 coef + L.red, coef + L.ab, coef + L.sc, nullptr, coef + L.stage};
+                                 coef + L.red, coef + L.sc, nullptr, coef + L.stage};
   DriverClock clock(2);
   HostColumns pinned;
   pinned.stride = R + 2;
@@ -751,10 +749,10 @@ int gmres_group_run(c3l_plan* plan, const c3l_tilde_maps* tm, int32_t count, con
     q.coef = coef + i * L.coef_total;
     q.b64 = c3l_ortho::Basis<double, double>{reinterpret_cast<double*>(V) + i * rd, ldv, rd, q.coef + L.H, R + 1,
                                               nullptr, nullptr, q.scratch + L.partial, q.coef + L.red,
-                                              q.coef + L.ab, q.coef + L.sc, nullptr, q.coef + L.stage};
+                                              q.coef + L.sc, nullptr, q.coef + L.stage};
     q.b32 = c3l_ortho::Basis<double, float>{reinterpret_cast<float*>(V) + i * rd, ldv, rd, q.coef + L.H, R + 1,
                                              nullptr, nullptr, q.scratch + L.partial, q.coef + L.red,
-                                             q.coef + L.ab, q.coef + L.sc, nullptr, q.coef + L.stage};
+                                             q.coef + L.sc, nullptr, q.coef + L.stage};
     q.H.assign((size_t)(R + 1) * R, 0.0);
     q.cs.assign(R, 0.0);
     q.sn.assign(R, 0.0);
@@ -1188,7 +1186,6 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
                                                  reinterpret_cast<const double2*>(sink + 2 * i * dim),
                                                  reinterpret_cast<double2*>(sc + L.partial),
                                                  reinterpret_cast<double2*>(cf + L.red),
-                                                 reinterpret_cast<double2*>(cf + L.ab),
                                                  reinterpret_cast<double2*>(cf + L.sc), status + 4 * i, nullptr});
     r->state.push_back(ShiftState{Hz, sink_v, beta[i], rot_c + i * sRot,
                                   reinterpret_cast<double2*>(rot_s + 2 * i * sRot),
@@ -1265,7 +1262,6 @@ int arnoldi_group_setup(c3l_arnoldi_run* r, c3l_plan* const* slot_plans, int32_t
                                                  reinterpret_cast<const double2*>(sink) + (int64_t)i * dim,
                                                  reinterpret_cast<double2*>(sc + L.partial),
                                                  reinterpret_cast<double2*>(cf + L.red),
-                                                 reinterpret_cast<double2*>(cf + L.ab),
                                                  reinterpret_cast<double2*>(cf + L.sc), status + 4 * i, nullptr});
     r->state.push_back(ShiftState{Hz, sink_v, beta[i], rot_c + i * sRot,
                                   reinterpret_cast<double2*>(rot_s + 2 * i * sRot),

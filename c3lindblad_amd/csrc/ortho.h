@@ -32,7 +32,6 @@ struct Basis {
   const T* sink;   // dotted unconjugated with every basis vector (nullable)
   T* partial;      // reduction partials: 2*(ldh+1)*chunks entries
   T* red;          // finished reductions: 2*(ldh+1) entries
-  T* ab;           // update coefficients: a at ab, b at ab + ldh
   T* sc;           // step scalars, two alternating blocks of 8 (16 entries)
   double* status;  // status[0] = final H[j, j-1] when set (nullable)
   T* stage;        // when set: final column j-1 rows 0..j, then its h0 (j+2 entries; nullable)

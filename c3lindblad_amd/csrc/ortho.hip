@@ -1,15 +1,16 @@
 // One-reduction Gram-Schmidt step with delayed reorthogonalisation (see ortho.h).
 //
-// Per step four launches on the caller's stream:
-//   reduce_kernel  one pass over basis columns 0..j (column j = q) against q and w,
-//                  one wave per column per chunk of rows, q and w held in registers:
-//                  partials of s = V^H q (s_j = |q|^2), z = V^H w (z_j = q^H w),
-//                  |w|^2 and sink . q;
-//   finish_kernel  one workgroup per reduced number;
-//   column_kernel  r, final column j-1, sink . v_j, H s, tentative column j and
-//                  the update coefficients, one workgroup per 16 Hessenberg rows;
-//   update_kernel  one pass over columns 0..j-1: v_j = q/r - V a and
-//                  q_{j+1} = w/g - (c/(r g)) q - V b.
+// Per step three launches on the caller's stream:
+//   reduce_kernel         one pass over basis columns 0..j (column j = q) against q and w,
+//                         one wave per column per chunk of rows, q and w held in registers:
+//                         partials of s = V^H q (s_j = |q|^2), z = V^H w (z_j = q^H w),
+//                         |w|^2 and sink . q;
+//   finish_kernel         one workgroup per reduced number;
+//   update_column_kernel  every workgroup derives the step scalars (r, c, g) from the
+//                         reduced numbers; the first ones write the final column j-1,
+//                         sink . v_j, H s and the tentative column j (one wave per
+//                         Hessenberg row), the others make one pass over columns 0..j-1:
+//                         v_j = (q - V s)/r and q_{j+1} = w/g - (c/(r g)) q - V (z - (c/r) s)/g.
 // Algebra (columns 0..j-1 orthonormal, q the once-projected v_j, nu its tentative
 // sub-diagonal, A v_{j-1} = V h + nu q):
 //   r = sqrt(|q|^2 - |s|^2), v_j = (q - V s)/r, H[:j, j-1] = h + nu s, H[j, j-1] = nu r;
@@ -30,8 +31,6 @@ extern "C" int c3l_internal_fail(int code, const char* message);
 
 namespace c3l_ortho {
 namespace {
-
-constexpr int kGroupRows = 4;  // Hessenberg rows per column_kernel workgroup (one wave each)
 
 // Sum over the 64 lanes of a wave with DPP moves (no LDS traffic), valid in lane 63:
 // an inclusive scan inside each row of 16 lanes (row_shr 1, 2, 4, 8, zero-filled at
@@ -249,34 +248,56 @@ __device__ __forceinline__ void put(double (&v)[6], int at, double2 x) {
 __device__ __forceinline__ double take(const double (&v)[6], int at, double) { return v[at]; }
 __device__ __forceinline__ double2 take(const double (&v)[6], int at, double2) { return make_double2(v[at], v[at + 1]); }
 
-// Step scalars and Hessenberg columns: one workgroup of 256 threads per kGroupRows rows
-// (at least one workgroup).  red = [s_0..s_{j-1}, |q|^2, sink.q, z_0..z_{j-1}, q^H w, |w|^2].
-// Every workgroup recomputes the step scalars (four sums over j numbers).  The scalar
-// block alternates with the step parity, so no workgroup overwrites what another still
-// reads: sc_prev (step j-1) is only read, sc_cur (step j) is written by workgroup 0:
-//   [1/r, 1/g, c/(r g), final H[j, j-1], its h0, h0 of the tentative column j,
-//    tentative sub-diagonal nu of column j]
-// (h0 = |w|/r, the norm of A v before projection).  Row k < j: final
-// H[k, j-1] = H[k, j-1] + nu s_k and tentative H[k, j] = (z_k - (H s)_k)/r with
-// (H s)_k = sum_{l >= k-1} H~[k, l] s_l (H~: column j-1 completed), one wave per row,
-// its 64 lanes splitting l (all loads of a lane independent).  `stage` (optional) gets the final
-// column j-1 followed by its h0, contiguous for one copy to the host.
+// Step scalars of step j from the finished reductions red = [s_0..s_{j-1}, |q|^2, sink.q,
+// z_0..z_{j-1}, q^H w, |w|^2] and the previous step's scalar block; every workgroup of the
+// fused pass computes them (four sums over j numbers, one barrier), so the update rows need
+// no coefficient table written by another workgroup.
 template <typename T>
-__device__ __forceinline__ void column_body(T* __restrict__ H, int64_t ldh, int j, int fresh, int has_w,
-                                            const T* __restrict__ red, T* __restrict__ sink_v, T* __restrict__ ab,
-                                            const T* __restrict__ sc_prev, T* __restrict__ sc_cur,
-                                            double* __restrict__ status, T* __restrict__ stage, double breakdown) {
+struct StepScalars {
+  double nu, h0_prev, r, hn, ir, omega, g, ig;
+  T c, sv;
+};
+
+// Scalars from the sums ss = |s|^2, zz = |z|^2, sz = s^H z over k < j.
+template <typename T>
+__device__ __forceinline__ StepScalars<T> finish_scalars(double ss, double zz, T sz, T sv, const T* __restrict__ red,
+                                                         int j, int fresh, int has_w, const T* __restrict__ sc_prev,
+                                                         double breakdown) {
   using N = Num<T>;
-  __shared__ double shared[4][6];
   const int nv = j + 1;
   const T* S = red;
   const T* Z = red + nv + 1;
-  const int tid = threadIdx.x;
-  const double nu = fresh ? 0.0 : N::re(sc_prev[6]);
-  const double h0_prev = N::re(sc_prev[5]);
+  StepScalars<T> o;
+  o.nu = fresh ? 0.0 : N::re(sc_prev[6]);
+  o.h0_prev = N::re(sc_prev[5]);
+  o.sv = sv;
+  o.r = fresh ? 1.0 : sqrt(fmax(N::re(S[j]) - ss, 0.0));
+  o.hn = o.nu * o.r;
+  o.ir = o.r > 0.0 ? 1.0 / o.r : 0.0;
+  if (!fresh && !(o.hn > breakdown)) o.ir = 0.0;  // breakdown: v_j = 0
+  o.c = N::zero();
+  o.omega = 0.0;
+  o.g = 0.0;
+  o.ig = 0.0;
+  if (has_w) {
+    o.c = N::scale(N::sub(Z[j], sz), o.ir);
+    o.omega = N::re(Z[nv]);
+    o.g = sqrt(fmax(o.omega - zz - N::abs2(o.c), 0.0));
+    o.ig = o.g > 0.0 ? 1.0 / o.g : 0.0;
+  }
+  return o;
+}
+
+template <typename T>
+__device__ __forceinline__ StepScalars<T> step_scalars(const T* __restrict__ red, const T* __restrict__ sink_v, int j,
+                                                       int fresh, int has_w, const T* __restrict__ sc_prev,
+                                                       double breakdown, double (*shared)[6]) {
+  using N = Num<T>;
+  const T* S = red;
+  const T* Z = red + j + 2;
   // [|s|^2, |z|^2, s^H z, s . sink_v] (complex sums in two slots; a real basis leaves slots 3, 5 at 0)
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int k = tid; k < j; k += 256) {
+  for (int k = threadIdx.x; k < j; k += blockDim.x) {
     const T s = fresh ? N::zero() : S[k];
     const T z = has_w ? Z[k] : N::zero();
     acc[0] += N::abs2(s);
@@ -285,31 +306,39 @@ __device__ __forceinline__ void column_body(T* __restrict__ H, int64_t ldh, int 
     if (sink_v != nullptr) put(acc, 4, N::fma(s, sink_v[k], N::zero()));
   }
   block_sum6(acc, shared);
-  const double ss = acc[0], zz = acc[1];
-  const T sz = take(acc, 2, T{}), sv = take(acc, 4, T{});
-  const double r = fresh ? 1.0 : sqrt(fmax(N::re(S[j]) - ss, 0.0));
-  const double hn = nu * r;
-  double ir = r > 0.0 ? 1.0 / r : 0.0;
-  if (!fresh && !(hn > breakdown)) ir = 0.0;  // breakdown: v_j = 0
-  T c = N::zero();
-  double omega = 0.0, g = 0.0, ig = 0.0;
-  if (has_w) {
-    c = N::scale(N::sub(Z[j], sz), ir);
-    omega = N::re(Z[nv]);
-    g = sqrt(fmax(omega - zz - N::abs2(c), 0.0));
-    ig = g > 0.0 ? 1.0 / g : 0.0;
-  }
+  return finish_scalars<T>(acc[0], acc[1], take(acc, 2, T{}), take(acc, 4, T{}), red, j, fresh, has_w, sc_prev,
+                           breakdown);
+}
+
+// Hessenberg columns of step j, one wave per row k (WAVES rows per workgroup):
+// final H[k, j-1] = H[k, j-1] + nu s_k and tentative H[k, j] = (z_k - (H s)_k)/r with
+// (H s)_k = sum_{l >= k-1} H~[k, l] s_l (H~: column j-1 completed), the wave's 64 lanes
+// splitting l (all loads of a lane independent).  Workgroup 0 also writes the rows j, j+1,
+// sink . v_j and the scalar block of step j:
+//   [1/r, 1/g, c/(r g), final H[j, j-1], its h0, h0 of the tentative column j,
+//    tentative sub-diagonal nu of column j]
+// (h0 = |w|/r, the norm of A v before projection).  `stage` (optional) gets the final column
+// j-1 followed by its h0, contiguous for one copy to the host.
+template <typename T, int WAVES>
+__device__ __forceinline__ void column_part(int blk, T* __restrict__ H, int64_t ldh, int j, int fresh, int has_w,
+                                            const T* __restrict__ red, T* __restrict__ sink_v, T* __restrict__ sc_cur,
+                                            double* __restrict__ status, T* __restrict__ stage,
+                                            const StepScalars<T>& o) {
+  using N = Num<T>;
+  const int nv = j + 1;
+  const T* S = red;
+  const T* Z = red + nv + 1;
   T* prev = H + (int64_t)(j - 1) * ldh;  // column j-1 (touched only when !fresh)
   T* col = H + (int64_t)j * ldh;         // column j (written only when has_w)
-  const int lane = tid & 63;
-  const int k = blockIdx.x * kGroupRows + (tid >> 6);  // uniform per wave
+  const int lane = threadIdx.x & 63;
+  const int k = blk * WAVES + (int)(threadIdx.x >> 6);  // uniform per wave
   T hs = N::zero();
   if (!fresh && has_w && k < j) {
     const T sk = S[k];
-#pragma unroll 4
+#pragma unroll 2
     for (int l = max(k - 1, 0) + lane; l < j; l += 64) {
       T h = H[(int64_t)l * ldh + k];
-      if (l == j - 1) h = N::fma(N::make(nu), sk, h);
+      if (l == j - 1) h = N::fma(N::make(o.nu), sk, h);
       hs = N::fma(h, S[l], hs);
     }
   }
@@ -319,95 +348,132 @@ __device__ __forceinline__ void column_body(T* __restrict__ H, int64_t ldh, int 
   if (lane == 63 && k < j) {
     const T s = fresh ? N::zero() : S[k];
     if (!fresh) {
-      const T h = N::fma(N::make(nu), s, prev[k]);
+      const T h = N::fma(N::make(o.nu), s, prev[k]);
       prev[k] = h;
       if (stage != nullptr) stage[k] = h;
     }
-    ab[k] = N::scale(s, ir);
-    if (has_w) {
-      const T z = Z[k];
-      col[k] = N::scale(N::sub(z, hs), ir);
-      ab[ldh + k] = N::scale(N::sub(z, N::mul(N::scale(c, ir), s)), ig);
-    }
+    if (has_w) col[k] = N::scale(N::sub(Z[k], hs), o.ir);
   }
-  if (blockIdx.x == 0 && tid == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     if (!fresh) {
-      prev[j] = N::make(hn);
+      prev[j] = N::make(o.hn);
       if (stage != nullptr) {
-        stage[j] = N::make(hn);
-        stage[j + 1] = N::make(h0_prev);
+        stage[j] = N::make(o.hn);
+        stage[j + 1] = N::make(o.h0_prev);
       }
     }
     if (has_w) {
-      const T hsj = fresh ? N::zero() : N::scale(S[j - 1], hn);  // (H s)_j = H[j, j-1] s_{j-1}
-      col[j] = N::scale(N::sub(c, hsj), ir);
-      col[j + 1] = N::make(g * ir);
+      const T hsj = fresh ? N::zero() : N::scale(S[j - 1], o.hn);  // (H s)_j = H[j, j-1] s_{j-1}
+      col[j] = N::scale(N::sub(o.c, hsj), o.ir);
+      col[j + 1] = N::make(o.g * o.ir);
     }
-    if (sink_v != nullptr) sink_v[j] = N::scale(N::sub(S[nv], sv), ir);
-    sc_cur[0] = N::make(ir);
-    sc_cur[1] = N::make(ig);
-    sc_cur[2] = N::scale(c, ir * ig);
-    sc_cur[3] = N::make(hn);
-    sc_cur[4] = N::make(h0_prev);
-    sc_cur[5] = N::make(has_w ? sqrt(fmax(omega, 0.0)) * ir : 0.0);
-    sc_cur[6] = N::make(has_w ? g * ir : 0.0);
-    if (status != nullptr && !fresh) status[0] = hn;
+    if (sink_v != nullptr) sink_v[j] = N::scale(N::sub(S[nv], o.sv), o.ir);
+    sc_cur[0] = N::make(o.ir);
+    sc_cur[1] = N::make(o.ig);
+    sc_cur[2] = N::scale(o.c, o.ir * o.ig);
+    sc_cur[3] = N::make(o.hn);
+    sc_cur[4] = N::make(o.h0_prev);
+    sc_cur[5] = N::make(has_w ? sqrt(fmax(o.omega, 0.0)) * o.ir : 0.0);
+    sc_cur[6] = N::make(has_w ? o.g * o.ir : 0.0);
+    if (status != nullptr && !fresh) status[0] = o.hn;
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void column_kernel(Many<T*> H, int64_t ldh, int j, int fresh, int has_w,
-                                                     Many<const T*> red, Many<T*> sink_v, Many<T*> ab, Many<T*> sc,
-                                                     Many<double*> status, Many<T*> stage, double breakdown) {
-  const int i = blockIdx.z;
-  column_body<T>(H.p[i], ldh, j, fresh, has_w, red.p[i], sink_v.p[i], ab.p[i], sc.p[i] + 8 * ((j + 1) & 1),
-                 sc.p[i] + 8 * (j & 1), status.p[i], stage.p[i], breakdown);
-}
-
-// 64 rows per workgroup, WAVES waves splitting the columns, partial sums met in LDS.
+// 64 rows per workgroup, WAVES waves splitting the columns, partial sums met in LDS:
+// with A' = V s and Z' = V z over columns 0..j-1,
+//   v_j = q/r - A'/r,  q_{j+1} = w/g - (c/(r g)) q - (Z' - (c/r) A')/g.
+// The scalars r, c, g need |s|^2, |z|^2 and s^H z: each wave sums them over its own
+// columns while it streams them (the coefficients are wave-uniform), and they meet in LDS
+// with the row sums, so the pass starts its loads at once and has one barrier.
 template <typename T, typename S, int WAVES>
-__device__ __forceinline__ void update_body(S* __restrict__ V, int64_t ld, int j, int fresh, const T* __restrict__ w,
-                                            const T* __restrict__ ab, int64_t ldh, const T* __restrict__ sc,
-                                            int64_t dim) {
+__device__ __forceinline__ void update_part(int blk, S* __restrict__ V, int64_t ld, int j, int fresh,
+                                            const T* __restrict__ w, const T* __restrict__ red, int64_t dim,
+                                            const T* __restrict__ sc_prev, double breakdown) {
   using N = Num<T>;
   __shared__ T pa[WAVES][64];
   __shared__ T pb[WAVES][64];
+  __shared__ double ps[WAVES][4];
+  const T* Sv = red;
+  const T* Zv = red + j + 2;
+  const int has_w = w != nullptr ? 1 : 0;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int64_t i = blockIdx.x * (int64_t)64 + lane;
-  T A = N::zero(), B = N::zero();
-  if (i < dim) {
-    const T* a = ab;
-    const T* b = ab + ldh;
+  const int64_t i = blk * (int64_t)64 + lane;
+  const int64_t ic = i < dim ? i : dim - 1;  // rows past the end read the last row, discarded
+  T A = N::zero(), B = N::zero(), sz = N::zero();
+  double ss = 0.0, zz = 0.0;
 #pragma unroll 4
-    for (int k = wave; k < j; k += WAVES) {
-      const T v = widen(V[(int64_t)k * ld + i]);
-      A = N::fma(a[k], v, A);
-      B = N::fma(b[k], v, B);
-    }
+  for (int k = wave; k < j; k += WAVES) {
+    const T v = widen(V[(int64_t)k * ld + ic]);
+    const T sk = fresh ? N::zero() : Sv[k];
+    const T zk = has_w ? Zv[k] : N::zero();
+    A = N::fma(sk, v, A);
+    B = N::fma(zk, v, B);
+    ss += N::abs2(sk);
+    zz += N::abs2(zk);
+    sz = N::cfma(sk, zk, sz);
   }
   pa[wave][lane] = A;
   pb[wave][lane] = B;
+  if (lane == 0) {
+    double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    put(t, 2, sz);
+    ps[wave][0] = ss;
+    ps[wave][1] = zz;
+    ps[wave][2] = t[2];
+    ps[wave][3] = t[3];
+  }
   __syncthreads();
-  if (wave == 0 && i < dim) {
-    for (int u = 1; u < WAVES; ++u) {
-      A = N::add(A, pa[u][lane]);
-      B = N::add(B, pb[u][lane]);
+  if (wave == 0) {
+    double tss = 0.0, tzz = 0.0, t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int u = 0; u < WAVES; ++u) {
+      tss += ps[u][0];
+      tzz += ps[u][1];
+      t[2] += ps[u][2];
+      t[3] += ps[u][3];
     }
-    S* q = V + (int64_t)j * ld;
-    const T qi = widen(q[i]);
-    if (!fresh) q[i] = narrow_to<S>(N::sub(N::scale(qi, N::re(sc[0])), A));
-    if (w != nullptr)
-      V[(int64_t)(j + 1) * ld + i] = narrow_to<S>(N::sub(N::sub(N::scale(w[i], N::re(sc[1])), N::mul(sc[2], qi)), B));
+    const StepScalars<T> o =
+        finish_scalars<T>(tss, tzz, take(t, 2, T{}), N::zero(), red, j, fresh, has_w, sc_prev, breakdown);
+    if (i < dim) {
+      for (int u = 1; u < WAVES; ++u) {
+        A = N::add(A, pa[u][lane]);
+        B = N::add(B, pb[u][lane]);
+      }
+      S* q = V + (int64_t)j * ld;
+      const T qi = widen(q[i]);
+      if (!fresh) q[i] = narrow_to<S>(N::sub(N::scale(qi, o.ir), N::scale(A, o.ir)));
+      if (has_w) {
+        const T ci = N::scale(o.c, o.ir);
+        const T b = N::scale(N::sub(B, N::mul(ci, A)), o.ig);
+        V[(int64_t)(j + 1) * ld + i] =
+            narrow_to<S>(N::sub(N::sub(N::scale(w[i], o.ig), N::mul(N::scale(ci, o.ig), qi)), b));
+      }
+    }
   }
 }
 
+// The step's second pass: workgroups [0, col_blocks) write the Hessenberg columns and the
+// scalar block, the others update 64 basis rows each.  The scalar block alternates with the
+// step parity (sc_prev: step j-1, only read; sc_cur: step j, written by workgroup 0).
 template <typename T, typename S, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void update_kernel(Many<S*> V, int64_t ld, int j, int fresh, Many<const T*> w,
-                                                            Many<const T*> ab, int64_t ldh, Many<const T*> sc,
-                                                            int64_t dim) {
+__global__ __launch_bounds__(64 * WAVES) void update_column_kernel(Many<S*> V, int64_t ld, int j, int fresh,
+                                                                   Many<const T*> w, Many<T*> H, int64_t ldh,
+                                                                   Many<const T*> red, Many<T*> sink_v, Many<T*> sc,
+                                                                   Many<double*> status, Many<T*> stage,
+                                                                   double breakdown, int64_t dim, int col_blocks) {
   const int i = blockIdx.z;
-  update_body<T, S, WAVES>(V.p[i], ld, j, fresh, w.p[i], ab.p[i], ldh, sc.p[i] + 8 * (j & 1), dim);
+  const int has_w = w.p[i] != nullptr ? 1 : 0;
+  T* sc_prev = sc.p[i] + 8 * ((j + 1) & 1);
+  T* sc_cur = sc.p[i] + 8 * (j & 1);
+  if ((int)blockIdx.x < col_blocks) {
+    __shared__ double shared[WAVES][6];
+    const StepScalars<T> o = step_scalars<T>(red.p[i], sink_v.p[i], j, fresh, has_w, sc_prev, breakdown, shared);
+    column_part<T, WAVES>((int)blockIdx.x, H.p[i], ldh, j, fresh, has_w, red.p[i], sink_v.p[i], sc_cur, status.p[i],
+                          stage.p[i], o);
+  } else {
+    update_part<T, S, WAVES>((int)blockIdx.x - col_blocks, V.p[i], ld, j, fresh, w.p[i], red.p[i], dim, sc_prev,
+                             breakdown);
+  }
 }
 
 int fail(const std::string& msg) { return c3l_internal_fail(C3L_EINVAL, msg.c_str()); }
@@ -440,7 +506,7 @@ int step_many(const Basis<T, S>* bs, int count, int j, bool fresh, const T* cons
   const bool has_w = ws[0] != nullptr;
   for (int i = 0; i < count; ++i) {
     const Basis<T, S>& o = bs[i];
-    if (o.V == nullptr || o.H == nullptr || o.partial == nullptr || o.red == nullptr || o.ab == nullptr ||
+    if (o.V == nullptr || o.H == nullptr || o.partial == nullptr || o.red == nullptr ||
         o.sc == nullptr || o.dim <= 0 || o.ld < o.dim || (has_w ? j + 2 > o.ldh : j + 1 > o.ldh) ||
         (ws[i] != nullptr) != has_w || o.dim != b.dim || o.ld != b.ld || o.ldh != b.ldh)
       return fail("Krylov orthogonalisation step: invalid argument");
@@ -449,8 +515,8 @@ int step_many(const Basis<T, S>* bs, int count, int j, bool fresh, const T* cons
   constexpr int WAVES = Shape<T>::kWaves;
   Many<const S*> Vr{};
   Many<S*> Vw{};
-  Many<const T*> w{}, sink{}, partial_r{}, red_r{}, ab_r{}, sc_r{};
-  Many<T*> partial{}, red{}, H{}, sink_v{}, ab{}, sc{}, stage{};
+  Many<const T*> w{}, sink{}, partial_r{}, red_r{};
+  Many<T*> partial{}, red{}, H{}, sink_v{}, sc{}, stage{};
   Many<double*> status{};
   for (int i = 0; i < count; ++i) {
     const Basis<T, S>& o = bs[i];
@@ -464,10 +530,7 @@ int step_many(const Basis<T, S>* bs, int count, int j, bool fresh, const T* cons
     red_r.p[i] = o.red;
     H.p[i] = o.H;
     sink_v.p[i] = o.sink_v;
-    ab.p[i] = o.ab;
-    ab_r.p[i] = o.ab;
     sc.p[i] = o.sc;
-    sc_r.p[i] = o.sc;
     stage.p[i] = o.stage;
     status.p[i] = o.status;
   }
@@ -487,14 +550,12 @@ int step_many(const Basis<T, S>* bs, int count, int j, bool fresh, const T* cons
   const int outs = has_w ? 2 * (nv + 1) : nv + 1;
   hipLaunchKernelGGL((finish_kernel<T>), dim3(outs, 1, z), dim3(256), 0, s, partial_r, chunks, red);
   if (int rc = check_launch("finish_kernel")) return rc;
-  const int groups_h = std::max(1, (j + kGroupRows - 1) / kGroupRows);
-  hipLaunchKernelGGL((column_kernel<T>), dim3(groups_h, 1, z), dim3(256), 0, s, H, b.ldh, j, fresh ? 1 : 0,
-                     has_w ? 1 : 0, red_r, sink_v, ab, sc, status, stage, breakdown);
-  if (int rc = check_launch("column_kernel")) return rc;
+  const int col_blocks = std::max(1, (j + WAVES - 1) / WAVES);
   if (clock) clock->begin(1, s);
-  hipLaunchKernelGGL((update_kernel<T, S, WAVES>), dim3((b.dim + 63) / 64, 1, z), dim3(64 * WAVES), 0, s, Vw, b.ld, j,
-                     fresh ? 1 : 0, w, ab_r, b.ldh, sc_r, b.dim);
-  if (int rc = check_launch("update_kernel")) return rc;
+  hipLaunchKernelGGL((update_column_kernel<T, S, WAVES>), dim3(col_blocks + (b.dim + 63) / 64, 1, z),
+                     dim3(64 * WAVES), 0, s, Vw, b.ld, j, fresh ? 1 : 0, w, H, b.ldh, red_r, sink_v, sc, status, stage,
+                     breakdown, b.dim, col_blocks);
+  if (int rc = check_launch("update_column_kernel")) return rc;
   if (clock) clock->end(1, count * (col * (j + 1 + (has_w ? 1 : 0) + (fresh ? 0 : 1)) + vec * (has_w ? 1 : 0)), s);
   return C3L_OK;
 }
