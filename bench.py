@@ -278,7 +278,7 @@ def _bench_batch(args) -> int:
     # chunks of consecutive site counts, so that the groups inside a chunk are full
     ids = sorted(range(args.instances), key=lambda i: (config3_modes(i), i))
     timed, setup = 0.0, 0.0
-    solutions, skipped, group_sizes = [], [], []
+    solutions, skipped, group_sizes, tuned = [], [], [], []
     steps = max(1, args.steps)
     for c0 in range(0, len(ids), args.batch_chunk):
         chunk = ids[c0:c0 + args.batch_chunk]
@@ -288,6 +288,7 @@ def _bench_batch(args) -> int:
         setup += prep.setup_seconds
         skipped += prep.skipped
         group_sizes += [(g.n, g.count) for g in prep.groups]
+        tuned += [r for g in prep.groups for r in g.tuning]
         for g in prep.groups:  # warm-up: program images are built at their first launch
             solve_group(g, omega, policy=policy)
         torch.cuda.synchronize()
@@ -331,6 +332,10 @@ def _bench_batch(args) -> int:
             "group_sizes_by_n": sorted(group_sizes),
             "chunk": args.batch_chunk,
             "timed_seconds": timed,
+            "autotuner": {p: {"launches_tuned": sum(1 for r in tuned if r["program"] == p),
+                              "ms_default_sum": round(sum(r["ms_default"] for r in tuned if r["program"] == p), 3),
+                              "ms_tuned_sum": round(sum(r["ms_tuned"] for r in tuned if r["program"] == p), 3)}
+                          for p in ("residual", "jvp", "sigma")},
             "setup_seconds_total": setup,
             "n_sites_range": [min(n_of.values()), max(n_of.values())] if n_of else None,
             "omega_points": int(omega.size),
@@ -554,6 +559,9 @@ def main() -> int:
             "eta": 0.0,
             "charges": list(charges),
             "setup_seconds": pipe.setup_seconds,
+            "autotuner": [{"program": r["program"] + (f"[{r['charge']:+d}]" if "charge" in r else ""),
+                           "us_default": round(1e3 * r["ms_default"], 2), "us_tuned": round(1e3 * r["ms_tuned"], 2),
+                           "shape": r["shape"]} for r in pipe.tuning],
             "newton_iterations": last.newton_iterations,
             "jvp_evaluations": last.jvp_evaluations,
             "residual_evaluations": last.residual_evaluations,

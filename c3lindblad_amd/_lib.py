@@ -39,6 +39,7 @@ SIGNATURES = {
     "c3l_plan_set_precision": ([_P, _I32, _I32, _I32], ctypes.c_int),
     "c3l_plan_set_matrix_cores": ([_P, _I32], ctypes.c_int),
     "c3l_plan_matrix_core_tiles": ([_P, _I32, _P], ctypes.c_int),
+    "c3l_plan_autotune": ([_P, _I32, _P, _P, _P, _I32, _P, _P, _P, _P], ctypes.c_int),
     "c3l_plan_block_count": ([_P], _I32),
     "c3l_plan_block_info": ([_P, _I32, _P, _P, _P, _P, _P], ctypes.c_int),
     "c3l_plan_program_time": ([_P, _I32, _P, _P], ctypes.c_int),

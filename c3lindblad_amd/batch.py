@@ -70,6 +70,7 @@ class InstanceGroup:
     members: list  # BatchInstance
     charges: tuple
     site: int = 0
+    tuning: list = field(default_factory=list)  # autotuner records of the union plans
     _dev: dict = field(default_factory=dict, repr=False)
 
     @property
@@ -112,6 +113,34 @@ class InstanceGroup:
             for plan in (value if isinstance(value, list) else [value]):
                 if isinstance(plan, DevicePlan):
                     plan.lowered = None
+
+    def autotune(self, jvp_bits: int = 32, reps: int = 3) -> list:
+        """Launch-shape autotuner (``tuning.py``) on the group's union plans: residual, JVP in
+        ``jvp_bits`` and every charge's sigma; one tuning per union plan, at set-up."""
+
+        torch = _lib.require_device()
+        dev = _lib.DEVICE
+        plan, diag = self._dev["plan"], self._dev["diag"]
+        t = torch.zeros(self.count * self.D, dtype=torch.complex128, device=dev)
+        out = torch.empty_like(t)
+        plan.load("T", t)
+        records = [plan.autotune("residual", out, diag, t, reps=reps)]
+        if jvp_bits == 32:
+            plan.set_precision("D", "jvp", 32)
+        try:
+            plan.load("D", t)
+            records.append(plan.autotune("jvp", out, diag, t, reps=reps))
+        finally:
+            if jvp_bits == 32:
+                plan.set_precision("D", "jvp", 64)
+        for q in self.charges:
+            sigma_plan = self._dev[("eom", q)][0]
+            x = torch.zeros(self.count * self.P(q), dtype=torch.complex128, device=dev)
+            sigma_plan.load("T", t)
+            sigma_plan.load("X", x)
+            records.append(dict(sigma_plan.autotune("sigma", torch.empty_like(x), self._dev[("eom_diag", q)], x,
+                                                    reps=reps), charge=q))
+        return records
 
     def upload(self) -> None:
         """Union plans (static; per charge two twin copies of the sigma plan), union tilde maps,
@@ -449,10 +478,11 @@ class PreparedBatch:
 
 
 def prepare_batch(models, charges=(1, -1), *, max_group: int = 8, workers: int = 1, indices=None,
-                  release_host: bool = False) -> PreparedBatch:
+                  release_host: bool = False, autotune: bool = True, jvp_bits: int = 32) -> PreparedBatch:
     """Host set-up of every instance (in ``workers`` processes when > 1; no GPU is touched there),
-    grouping by site count and upload of the groups' union plans.  Instances whose full-HF reference
-    does not converge are skipped and listed.  ``indices`` names the instances (default 0..)."""
+    grouping by site count, upload of the groups' union plans and (``autotune``) the launch-shape
+    autotuner on them, with the JVP in ``jvp_bits``.  Instances whose full-HF reference does not
+    converge are skipped and listed.  ``indices`` names the instances (default 0..)."""
 
     started = perf_counter()
     names = range(len(models)) if indices is None else indices
@@ -471,6 +501,8 @@ def prepare_batch(models, charges=(1, -1), *, max_group: int = 8, workers: int =
     groups = [InstanceGroup(members, tuple(charges)) for members in group_by_size(instances, max_group)]
     for g in groups:
         g.upload()
+        if autotune:
+            g.tuning = g.autotune(jvp_bits)
         if release_host:
             g.release_host()
     return PreparedBatch(groups, skipped, perf_counter() - started)

@@ -1056,8 +1056,26 @@ struct PlanBlock {
 
 // Device image of one program: every block's arrays concatenated, the block
 // descriptors and the tile lists of the two kernel variants.
+// Launch shape of one program image.  The defaults are the measured heuristics; the
+// autotuner (c3l_plan_autotune) times alternatives on the plan itself and keeps the fastest
+// for every plan with the same digest, program and precision.
+struct ProgramShape {
+  int32_t lanes_max = 2;       // most lanes per coordinate, log2 (small blocks split their entries)
+  int32_t lanes_target = 512;  // workgroups per block below which a block splits its entries
+  int32_t reps_max = 2;        // coordinates per thread of large two-factor blocks (<= kMaxReps)
+  int32_t reps_target = 512;   // workgroups per block below which a block halves its coordinates per thread
+  int32_t xcd = 1;             // tiles of a block sliced over the eight XCDs
+  int32_t mfma_min = kTileMinEntries;  // FP32: fewest entries of a block on the matrix cores (INT_MAX: none)
+  bool operator==(const ProgramShape& o) const {
+    return lanes_max == o.lanes_max && lanes_target == o.lanes_target && reps_max == o.reps_max &&
+           reps_target == o.reps_target && xcd == o.xcd && mfma_min == o.mfma_min;
+  }
+};
+
 struct ProgramExec {
   bool built = false;
+  ProgramShape shape;      // the shape the current images were built with
+  int64_t tune_gen = -1;   // tuning generation at which `shape` was looked up
   int64_t max_dest = -1;
   int32_t tiles_two = 0;
   int32_t tiles_general = 0;
@@ -1079,6 +1097,7 @@ struct ProgramExec {
   int2* mfma_tile_list = nullptr;
   int32_t tiles32_two = 0, tiles32_general = 0;
   int2* tiles32 = nullptr;
+  std::vector<void*> owned_mfma;  // device arrays of the matrix-core image and tiles32
 };
 
 struct PendingTiming {
@@ -1295,8 +1314,10 @@ int c3l_plan_destroy(c3l_plan* plan) {
     (void)hipEventDestroy(p.start);
     (void)hipEventDestroy(p.stop);
   }
-  for (ProgramExec& pe : plan->programs)
+  for (ProgramExec& pe : plan->programs) {
     for (void* p : pe.owned) (void)hipFree(p);
+    for (void* p : pe.owned_mfma) (void)hipFree(p);
+  }
   for (KindExec& ke : plan->kinds)
     for (void* p : ke.owned) (void)hipFree(p);
   for (void* p : plan->owned) (void)hipFree(p);
@@ -1656,15 +1677,92 @@ namespace {
 
 // Lanes per coordinate: split the entries of small blocks over up to four waves
 // so that a program launch keeps at least ~2 workgroups per CU busy.
-int lanes_log2_for(const PlanBlock& b) {
+int lanes_log2_for(const PlanBlock& b, const ProgramShape& sh) {
   int lg = 0;
-  while (lg < 2 && (b.count + (256 >> lg) - 1) / (256 >> lg) < 512 && b.n_entries >= (2 << lg)) ++lg;
+  while (lg < sh.lanes_max && (b.count + (256 >> lg) - 1) / (256 >> lg) < sh.lanes_target && b.n_entries >= (2 << lg))
+    ++lg;
   return lg;
+}
+
+int build_program(c3l_plan* plan, int32_t program);
+
+// Tuned shapes by (plan digest, program, precision bits), shared by every plan built from the
+// same arrays (twins, repeated instances); g_tune_gen changes whenever an entry does.
+std::mutex g_tune_mutex;
+std::map<std::tuple<uint64_t, int32_t, int32_t>, ProgramShape> g_tuned;
+int64_t g_tune_gen = 0;
+
+ProgramShape wanted_shape(c3l_plan* plan, int32_t program) {
+  ProgramExec& pe = plan->programs[program];
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  if (pe.tune_gen == g_tune_gen) return pe.shape;
+  const auto it = g_tuned.find(std::make_tuple(plan->digest, program, plan->program_bits[program]));
+  return it == g_tuned.end() ? ProgramShape{} : it->second;
+}
+
+void set_tuned(c3l_plan* plan, int32_t program, const ProgramShape& sh) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  g_tuned[std::make_tuple(plan->digest, program, plan->program_bits[program])] = sh;
+  ++g_tune_gen;
+}
+
+// Frees a program's images (after the device has finished every launch that may read them).
+int release_mfma_image(ProgramExec& pe) {
+  if (!pe.owned_mfma.empty()) {
+    C3L_HIP_CHECK(hipDeviceSynchronize());
+    for (void* p : pe.owned_mfma) C3L_HIP_CHECK(hipFree(p));
+    pe.owned_mfma.clear();
+  }
+  pe.mfma_built = false;
+  pe.mfma_tiles = 0;
+  pe.tiles32_two = pe.tiles32_general = 0;
+  pe.tiles32 = nullptr;
+  return C3L_OK;
+}
+
+int release_program_image(ProgramExec& pe) {
+  int rc = release_mfma_image(pe);
+  if (rc != C3L_OK) return rc;
+  if (!pe.owned.empty()) {
+    C3L_HIP_CHECK(hipDeviceSynchronize());
+    for (void* p : pe.owned) C3L_HIP_CHECK(hipFree(p));
+    pe.owned.clear();
+  }
+  pe.built = false;
+  pe.max_dest = -1;
+  pe.tiles_two = pe.tiles_general = 0;
+  pe.descs = nullptr;
+  pe.tiles = nullptr;
+  pe.desc_block.clear();
+  pe.host_tiles.clear();
+  return C3L_OK;
+}
+
+// Builds the program's gather image with its current (tuned or default) shape, rebuilding
+// it when the tuned shape has changed since it was built.
+int ensure_program(c3l_plan* plan, int32_t program) {
+  ProgramExec& pe = plan->programs[program];
+  const ProgramShape want = wanted_shape(plan, program);
+  if (!(want == pe.shape)) {
+    const ProgramShape had = pe.shape;
+    const bool gather_same = had.lanes_max == want.lanes_max && had.lanes_target == want.lanes_target &&
+                             had.reps_max == want.reps_max && had.reps_target == want.reps_target &&
+                             had.xcd == want.xcd;
+    int rc = gather_same ? release_mfma_image(pe) : release_program_image(pe);
+    if (rc != C3L_OK) return rc;
+    pe.shape = want;
+  }
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    pe.tune_gen = g_tune_gen;
+  }
+  return build_program(plan, program);
 }
 
 int build_program(c3l_plan* plan, int32_t program) {
   ProgramExec& pe = plan->programs[program];
   if (pe.built) return C3L_OK;
+  const ProgramShape& sh = pe.shape;
   std::vector<size_t> members;
   for (size_t i = 0; i < plan->blocks.size(); ++i)
     if (plan->blocks[i].program == program && plan->blocks[i].count > 0) members.push_back(i);
@@ -1672,7 +1770,7 @@ int build_program(c3l_plan* plan, int32_t program) {
   std::stable_sort(members.begin(), members.end(), [&](size_t a, size_t b) {
     const PlanBlock& A = plan->blocks[a];
     const PlanBlock& B = plan->blocks[b];
-    return (A.n_entries >> lanes_log2_for(A)) > (B.n_entries >> lanes_log2_for(B));
+    return (A.n_entries >> lanes_log2_for(A, sh)) > (B.n_entries >> lanes_log2_for(B, sh));
   });
   std::vector<WedgeBlockDesc> descs;
   std::vector<int2> two, general;
@@ -1681,13 +1779,13 @@ int build_program(c3l_plan* plan, int32_t program) {
     WedgeBlockDesc d;
     d.count = b.count;
     d.n_entries = b.n_entries;
-    d.lanes_log2 = lanes_log2_for(b);
+    d.lanes_log2 = lanes_log2_for(b, sh);
     d.reps = 1;
     if (b.two_factors && d.lanes_log2 == 0) {
       // coordinates per thread for large two-factor blocks (more independent gathers in
       // flight; kMaxReps = 2 measured best on MI355X for the n = 25 JVP)
-      int reps = kMaxReps;
-      while (reps > 1 && (b.count + 256 * reps - 1) / (256 * reps) < 512) reps >>= 1;
+      int reps = std::min(sh.reps_max, kMaxReps);
+      while (reps > 1 && (b.count + 256 * reps - 1) / (256 * reps) < sh.reps_target) reps >>= 1;
       d.reps = reps;
     }
     d.dense_n = b.dense ? plan->n : 0;
@@ -1744,8 +1842,10 @@ int build_program(c3l_plan* plan, int32_t program) {
     }
     list.swap(out);
   };
-  xcd_order(two);
-  xcd_order(general);
+  if (sh.xcd) {
+    xcd_order(two);
+    xcd_order(general);
+  }
   pe.tiles_two = (int32_t)two.size();
   pe.tiles_general = (int32_t)general.size();
   std::vector<int2> tiles(two);
@@ -1847,7 +1947,7 @@ int build_program_mfma(c3l_plan* plan, int32_t program) {
   for (size_t di = 0; fits && di < pe.desc_block.size(); ++di) {
     const PlanBlock& b = plan->blocks[(size_t)pe.desc_block[di]];
     if (b.dense || b.deg != 4 || !b.two_factors || b.n_entries == 0 || b.max_dest >= (int64_t(1) << 31)) continue;
-    if (plan->mfma == 1 && b.n_entries < kTileMinEntries) continue;
+    if (plan->mfma == 1 && b.n_entries < pe.shape.mfma_min) continue;
     std::vector<TileRule> rules;
     std::vector<TileEntry> entries;
     std::vector<float2> coeffs;
@@ -1871,7 +1971,7 @@ int build_program_mfma(c3l_plan* plan, int32_t program) {
     }
     if (4 * b.count < volume) continue;  // too sparse for whole tiles
     int32_t* dst = nullptr;
-    int rc = upload_owned(pe.owned, &dst, dest.data(), dest.size());
+    int rc = upload_owned(pe.owned_mfma, &dst, dest.data(), dest.size());
     if (rc != C3L_OK) return rc;
     d.n = n;
     d.per_dim = T;
@@ -1910,13 +2010,13 @@ int build_program_mfma(c3l_plan* plan, int32_t program) {
   pe.tiles32_two = (int32_t)two.size();
   pe.tiles32_general = (int32_t)general.size();
   two.insert(two.end(), general.begin(), general.end());
-  int rc = upload_owned(pe.owned, &pe.tiles32, two.data(), two.size());
-  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_descs, descs.data(), descs.size());
-  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_tile_list, tile_list.data(), tile_list.size());
-  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_rules, all_rules.data(), all_rules.size());
-  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_entries, all_entries.data(), all_entries.size());
-  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_coeff_im, all_im.data(), all_im.size());
-  if (rc == C3L_OK) rc = upload_owned(pe.owned, &pe.mfma_scalars, all_scalars.data(), all_scalars.size());
+  int rc = upload_owned(pe.owned_mfma, &pe.tiles32, two.data(), two.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned_mfma, &pe.mfma_descs, descs.data(), descs.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned_mfma, &pe.mfma_tile_list, tile_list.data(), tile_list.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned_mfma, &pe.mfma_rules, all_rules.data(), all_rules.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned_mfma, &pe.mfma_entries, all_entries.data(), all_entries.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned_mfma, &pe.mfma_coeff_im, all_im.data(), all_im.size());
+  if (rc == C3L_OK) rc = upload_owned(pe.owned_mfma, &pe.mfma_scalars, all_scalars.data(), all_scalars.size());
   if (rc != C3L_OK) return rc;
   pe.mfma_tiles = (int32_t)tile_list.size();
   pe.mfma_real = true;
@@ -2019,7 +2119,89 @@ int c3l_plan_set_matrix_cores(c3l_plan* plan, int32_t mode) {
   if (plan == nullptr || mode < 0 || mode > 2) return fail(C3L_EINVAL, "c3l_plan_set_matrix_cores: invalid argument");
   if (mode != plan->mfma) {
     plan->mfma = mode;
-    for (ProgramExec& pe : plan->programs) pe.mfma_built = false;  // rebuilt at the next FP32 launch
+    for (ProgramExec& pe : plan->programs) {  // rebuilt at the next FP32 launch
+      int rc = release_mfma_image(pe);
+      if (rc != C3L_OK) return rc;
+    }
+  }
+  return C3L_OK;
+}
+
+// Autotuner of the program images' launch shapes: times `reps` launches per candidate on x
+// into out (after two untimed ones) by coordinate descent over, in order, the matrix-core
+// threshold of an FP32 program, lanes per coordinate and the workgroup target of the lane split,
+// coordinates per thread and its workgroup target, and the XCD slicing, starting from the default
+// heuristics.  Knobs that change the summation order of a coordinate's entries (matrix cores,
+// lane split) replace the best only at a 10 % gain, so that timing noise does not change results;
+// the others at 2 %.  The winner is kept for every plan with this plan's digest, program and
+// precision (images are rebuilt at their next launch).
+int c3l_plan_autotune(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out, int32_t reps,
+                      void* stream, double* ms_default, double* ms_best, int32_t* shape_out) {
+  if (plan == nullptr || program < 0 || program >= kPrograms || out == nullptr || reps < 1 || reps > 1000 ||
+      (diag == nullptr) != (x == nullptr) || ms_default == nullptr || ms_best == nullptr)
+    return fail(C3L_EINVAL, "c3l_plan_autotune: invalid argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  C3L_HIP_CHECK(hipEventCreate(&e0));
+  C3L_HIP_CHECK(hipEventCreate(&e1));
+  auto measure = [&](const ProgramShape& sh, double* ms) -> int {
+    set_tuned(plan, program, sh);
+    for (int w = 0; w < 2; ++w) {
+      const int rc = c3l_internal_plan_apply_many(&plan, 1, program, diag, &x, &out, stream);
+      if (rc != C3L_OK) return rc;
+    }
+    C3L_HIP_CHECK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) {
+      const int rc = c3l_internal_plan_apply_many(&plan, 1, program, diag, &x, &out, stream);
+      if (rc != C3L_OK) return rc;
+    }
+    C3L_HIP_CHECK(hipEventRecord(e1, s));
+    C3L_HIP_CHECK(hipEventSynchronize(e1));
+    float elapsed = 0.f;
+    C3L_HIP_CHECK(hipEventElapsedTime(&elapsed, e0, e1));
+    *ms = (double)elapsed / reps;
+    return C3L_OK;
+  };
+  ProgramShape best{};
+  double t_best = 0.0;
+  int rc = measure(best, &t_best);
+  *ms_default = t_best;
+  auto descend = [&](int32_t ProgramShape::*knob, std::initializer_list<int32_t> values, double gain) -> int {
+    const int32_t start = best.*knob;
+    for (int32_t v : values) {
+      if (v == start) continue;
+      ProgramShape cand = best;
+      cand.*knob = v;
+      double t = 0.0;
+      const int r = measure(cand, &t);
+      if (r != C3L_OK) return r;
+      if (t < (1.0 - gain) * t_best) {
+        best = cand;
+        t_best = t;
+      }
+    }
+    return C3L_OK;
+  };
+  constexpr double kReorders = 0.10, kKeeps = 0.02;
+  if (rc == C3L_OK && plan->program_bits[program] == 32 && plan->mfma == 1)
+    rc = descend(&ProgramShape::mfma_min, {kTileMinEntries, INT_MAX, 32, 1}, kReorders);
+  if (rc == C3L_OK) rc = descend(&ProgramShape::lanes_max, {2, 1, 0}, kReorders);
+  if (rc == C3L_OK) rc = descend(&ProgramShape::lanes_target, {512, 256, 1024, 2048}, kReorders);
+  if (rc == C3L_OK) rc = descend(&ProgramShape::reps_max, {2, 1}, kKeeps);
+  if (rc == C3L_OK) rc = descend(&ProgramShape::reps_target, {512, 256, 1024, 2048}, kKeeps);
+  if (rc == C3L_OK) rc = descend(&ProgramShape::xcd, {1, 0}, kKeeps);
+  set_tuned(plan, program, best);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != C3L_OK) return rc;
+  *ms_best = t_best;
+  if (shape_out != nullptr) {
+    shape_out[0] = best.lanes_max;
+    shape_out[1] = best.lanes_target;
+    shape_out[2] = best.reps_max;
+    shape_out[3] = best.reps_target;
+    shape_out[4] = best.xcd;
+    shape_out[5] = best.mfma_min;
   }
   return C3L_OK;
 }
@@ -2083,7 +2265,7 @@ int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t 
       return fail(C3L_EINVAL, "c3l_plan_apply: plans applied together must be FP64 twins with external outputs");
   }
   c3l_plan* plan = plans[0];
-  int rc = build_program(plan, program);
+  int rc = ensure_program(plan, program);
   if (rc != C3L_OK) return rc;
   const ProgramExec& pe = plan->programs[program];
   if (pe.tiles_two + pe.tiles_general == 0) return C3L_OK;

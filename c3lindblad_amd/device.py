@@ -137,6 +137,23 @@ class DevicePlan:
         _lib.call("c3l_plan_matrix_core_tiles", self.handle, PROGRAMS[program], ctypes.byref(tiles))
         return int(tiles.value)
 
+    def autotune(self, program: str, out, diag=None, x=None, reps: int = 5) -> dict:
+        """Choose the launch shape of ``program`` by timing candidates on this plan
+        (``c3l_plan_autotune``); the choice applies to every plan with the same digest.
+        Returns the default and tuned time per launch (ms) and the chosen shape."""
+
+        ms_default, ms_best = ctypes.c_double(), ctypes.c_double()
+        shape = (ctypes.c_int32 * 6)()
+        _lib.call(
+            "c3l_plan_autotune", self.handle, PROGRAMS[program], None if diag is None else _lib.dptr(diag),
+            None if x is None else _lib.dptr(x), _lib.dptr(out), int(reps), _lib.stream_ptr(self.torch),
+            ctypes.byref(ms_default), ctypes.byref(ms_best), shape,
+        )
+        names = ("lanes_max_log2", "lane_split_workgroups", "coordinates_per_thread", "coordinate_workgroups",
+                 "xcd_slicing", "matrix_core_min_entries")
+        return {"program": program, "ms_default": ms_default.value, "ms_tuned": ms_best.value,
+                "shape": dict(zip(names, (int(v) for v in shape)))}
+
     def set_timing(self, enable: bool) -> None:
         _lib.call("c3l_plan_set_timing", self.handle, 1 if enable else 0)
 

@@ -47,9 +47,13 @@ class SolvePipeline:
     rtol: float = 1e-10
     concurrent: bool = True
     eom_streams: int | None = None  # streams for the EOM resolvent tasks (None: one per task)
+    tuning: list = field(default_factory=list)  # autotuner records of the set-up
 
     @classmethod
-    def prepare(cls, model, omega, eta=0.0, charges=(1, -1), policy=None, rtol=1e-10):
+    def prepare(cls, model, omega, eta=0.0, charges=(1, -1), policy=None, rtol=1e-10, autotune=True):
+        """Host set-up and upload; ``autotune`` runs the launch-shape autotuner on the residual,
+        JVP (in the policy's precision) and sigma programs (``tuning.py``), inside the set-up time."""
+
         started = perf_counter()
         problem = factorized_cc_problem(model)
         static = ComponentStaticProblem.compile(problem)
@@ -61,8 +65,15 @@ class SolvePipeline:
             b.device_plan(1)
             b.source_device(model.impurity_site)
             b.sink(model.impurity_site)
+        policy = policy or NewtonKrylovPolicy()
+        tuning = []
+        if autotune:
+            from .tuning import autotune_sigma, autotune_static
+
+            tuning = autotune_static(static, 32 if policy.jacobian_precision == "fp32" else 64)
+            tuning += [dict(autotune_sigma(b), charge=q) for q, b in blocks.items()]
         return cls(static, blocks, np.asarray(omega, dtype=np.float64), float(eta), perf_counter() - started,
-                   policy or NewtonKrylovPolicy(), rtol)
+                   policy, rtol, tuning=tuning)
 
     def solve(self) -> SolveResult:
         t0 = perf_counter()

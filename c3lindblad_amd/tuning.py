@@ -1,0 +1,52 @@
+"""Set-up step of a solve: the launch-shape autotuner on every program a solve launches.
+
+The reference leaves its schedules to XLA (``cc/component_jax.py:248-462``).  Here each
+program image (the coordinate tiles of ``wedge_program_kernel`` and, for FP32 programs, the
+matrix-core tiles of ``tile_program_mfma_kernel``) has a launch shape: lanes per coordinate,
+coordinates per thread, the workgroup target behind both, the XCD slicing of a block's tiles
+and the matrix-core entry threshold.  ``c3l_plan_autotune`` times candidates on the plan itself
+(coordinate descent from the default heuristics) and keeps the fastest for every plan with the
+same digest, so twin plans and repeated instances share one tuning.  Tuning runs once per plan
+at set-up, outside any timed solve; programs are timed on zero inputs (their cost does not
+depend on the values).
+"""
+
+from __future__ import annotations
+
+from . import _lib
+
+
+def autotune_static(static, jvp_bits: int = 64, reps: int = 5) -> list[dict]:
+    """Tune the residual program and the JVP program (in the precision the Newton solve uses)
+    of a compiled static problem."""
+
+    dev = static.device
+    dev.t.zero_()
+    dev.dt.zero_()
+    dev.load_amplitudes(dev.t)
+    out = [dev.plan.autotune("residual", dev.out, dev.diag, dev.t, reps=reps)]
+    if jvp_bits == 32:
+        dev.plan.set_precision("D", "jvp", 32)
+    try:
+        dev.plan.load("D", dev.dt)
+        out.append(dev.plan.autotune("jvp", dev.out, dev.diag, dev.dt, reps=reps))
+    finally:
+        if jvp_bits == 32:
+            dev.plan.set_precision("D", "jvp", 64)
+    return out
+
+
+def autotune_sigma(block, reps: int = 5) -> dict:
+    """Tune the sigma program of a compiled EOM charge block (shared by its twin slots)."""
+
+    torch = _lib.require_device()
+    plan = block.device_plan(0)
+    D = block.static.oracle.num_amplitudes
+    P = block.dimension
+    t = torch.zeros(D, dtype=torch.complex128, device=_lib.DEVICE)
+    x = torch.zeros(P, dtype=torch.complex128, device=_lib.DEVICE)
+    out = torch.empty_like(x)
+    diag = torch.from_numpy(block.diagonal).to(_lib.DEVICE)
+    plan.load("T", t)
+    plan.load("X", x)
+    return plan.autotune("sigma", out, diag, x, reps=reps)

@@ -123,6 +123,23 @@ int c3l_plan_set_matrix_cores(c3l_plan* plan, int32_t mode);
  * qualifies). */
 int c3l_plan_matrix_core_tiles(c3l_plan* plan, int32_t program, int32_t* tiles);
 
+/* Launch-shape autotuner (no counterpart in the reference, whose XLA compiler picks its own
+ * schedules, cc/component_jax.py:248-462): times `reps` launches of `program` on x into out
+ * (diag and x as for c3l_plan_apply; out must not be NULL) for each candidate image shape, by
+ * coordinate descent from the default heuristics over the matrix-core threshold of an FP32
+ * program (128 / none / 32 / 1 entries), lanes per coordinate (4 / 2 / 1) and the workgroup
+ * target of that split (512 / 256 / 1024 / 2048 per block), coordinates per thread (2 / 1) and
+ * its workgroup target, and the XCD slicing of each block's tiles (on / off), and keeps the
+ * fastest for every plan with this plan's digest, program and precision (images are rebuilt at
+ * their next launch).  The matrix-core threshold and the lane split change the summation order
+ * of a coordinate's entries and are taken only at a 10 % gain, the other knobs (results bit for
+ * bit unchanged) at 2 %.  ms_default / ms_best: time per launch of the default and the chosen
+ * shape; shape_out (6 ints, nullable): lanes_max (log2), lane-split workgroup target,
+ * coordinates per thread, its workgroup target, XCD slicing, matrix-core threshold (INT32_MAX:
+ * none).  Synchronises the device. */
+int c3l_plan_autotune(c3l_plan* plan, int32_t program, const double* diag, const double* x, double* out,
+                      int32_t reps, void* stream, double* ms_default, double* ms_best, int32_t* shape_out);
+
 /* Measurement hooks (no counterpart in the reference, which times with perf_counter,
  * cc/component_jax.py:53-65): when enabled, every program launch of c3l_plan_apply is
  * bracketed by HIP events on its stream; c3l_plan_program_time waits for them and

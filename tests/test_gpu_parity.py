@@ -469,6 +469,62 @@ def test_fp32_matrix_core_jvp(torch_gpu, case):
     assert e_gather < 2e-6 and e_cores < 2e-6 and e_pair < 2e-6
 
 
+@pytest.mark.parametrize("n,seed", [(7, 23), (17, 29)])
+def test_autotuned_program_shapes_keep_the_results(torch_gpu, n, seed):
+    """Launch-shape autotuner: residual, FP64 and FP32 JVP and sigma after tuning against the
+    same programs on the default shapes (FP64 within 1e-13 relative: only the summation order of
+    a coordinate's entries may change, and bit for bit equal when the tuner kept the default lane
+    split; FP32 within 2e-6 of the FP64 JVP), every tuned time at most the default one, and every
+    chosen shape one of the candidates."""
+
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, component_eom_sigma, eom_ccsd_problem
+    from c3lindblad_amd.tuning import autotune_sigma, autotune_static
+
+    print(f"case autotune n={n}", flush=True)
+    p, sp = _static(n, 1.2, seed)  # a fresh digest: nothing tuned for it yet
+    dev = sp.device
+    torch = torch_gpu
+    rng = np.random.default_rng(seed)
+    x = torch.from_numpy(0.05 * rng.normal(size=p.num_variables)).cuda()
+    dx = torch.from_numpy(rng.normal(size=p.num_variables)).cuda()
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    v = rng.normal(size=ep.dimension) + 1j * rng.normal(size=ep.dimension)
+
+    def evaluate():
+        res, j64, j32 = (torch.empty_like(x) for _ in range(3))
+        dev.residual_real(x, res)
+        dev.jvp_real(dx, j64)
+        dev.plan.set_precision("D", "jvp", 32)
+        try:
+            dev.jvp_real(dx, j32)
+        finally:
+            dev.plan.set_precision("D", "jvp", 64)
+        sigma = component_eom_sigma(component_eom_context(ep, t), v).retained
+        return [a.cpu().numpy() for a in (res, j64, j32)] + [sigma]
+
+    before = evaluate()
+    records = autotune_static(sp, 64) + autotune_static(sp, 32) + [autotune_sigma(ep)]
+    after = evaluate()
+    for r in records:
+        print(f"  {r['program']}: {r['ms_default'] * 1e3:.1f} -> {r['ms_tuned'] * 1e3:.1f} us {r['shape']}", flush=True)
+        assert r["ms_tuned"] <= r["ms_default"]
+        sh = r["shape"]
+        assert sh["lanes_max_log2"] in (0, 1, 2) and sh["coordinates_per_thread"] in (1, 2)
+        assert sh["lane_split_workgroups"] in (256, 512, 1024, 2048) and sh["xcd_slicing"] in (0, 1)
+        assert sh["coordinate_workgroups"] in (256, 512, 1024, 2048)
+        assert sh["matrix_core_min_entries"] in (1, 32, 128, 2**31 - 1)
+    res0, j640, j320, s0 = before
+    res1, j641, j321, s1 = after
+    lanes = records[0]["shape"]  # residual, FP64
+    if lanes["lanes_max_log2"] == 2 and lanes["lane_split_workgroups"] == 512:
+        assert np.array_equal(res1, res0)  # order-preserving knobs only
+    assert _rel(res1, res0) < 1e-13
+    assert _rel(j641, j640) < 1e-13
+    assert _rel(s1, s0) < 1e-13
+    assert _rel(j321, j640) < 2e-6 and _rel(j320, j640) < 2e-6
+
+
 def test_mixed_precision_newton_reaches_fp64_root(torch_gpu):
     from c3lindblad_amd.solver import NewtonKrylovPolicy, solve_component_ccsd
 
