@@ -559,7 +559,9 @@ def main() -> int:
             "eta": 0.0,
             "charges": list(charges),
             "setup_seconds": pipe.setup_seconds,
-            "autotuner": [{"program": r["program"] + (f"[{r['charge']:+d}]" if "charge" in r else ""),
+            "autotuner": [r if r["program"] == "residual_path" else
+                          {"program": r["program"] + (f"[{r['charge']:+d}]" if "charge" in r else "")
+                           + (f"[fp{r['bits']}]" if "bits" in r else ""),
                            "us_default": round(1e3 * r["ms_default"], 2), "us_tuned": round(1e3 * r["ms_tuned"], 2),
                            "shape": r["shape"]} for r in pipe.tuning],
             "newton_iterations": last.newton_iterations,

@@ -125,6 +125,12 @@ class DevicePlan:
         """Build ``kind`` atoms and evaluate ``program`` in FP32 (``bits=32``) or FP64."""
 
         _lib.call("c3l_plan_set_precision", self.handle, KINDS[kind], PROGRAMS[program], int(bits))
+        self.__dict__.setdefault("_bits", {})[program] = int(bits)
+
+    def precision(self, program: str) -> int:
+        """Arithmetic bits of ``program`` (64 unless set otherwise)."""
+
+        return self.__dict__.get("_bits", {}).get(program, 64)
 
     def set_matrix_cores(self, mode) -> None:
         """FP32 programs: rank-4 two-atom blocks on the matrix cores -- ``False``/0 never,

@@ -44,6 +44,7 @@ class ComponentStaticProblem:
     lowered: LoweredPlan
     diagonal: np.ndarray  # -i Λ_μ
     compile_seconds: float
+    residual_degree: int = 4  # highest power of the amplitudes in any residual term
     _device: "StaticDevice | None" = field(default=None, repr=False)
 
     @classmethod
@@ -70,7 +71,8 @@ class ComponentStaticProblem:
             {"residual": coords, "reference": coords, "jvp": coords},
         )
         diagonal = -1j * problem.rapidity_sums()
-        return cls(problem, lowered, diagonal, perf_counter() - started)
+        degree = max((sum(1 for atom in key if getattr(atom, "base", None) == "T") for key in residual), default=0)
+        return cls(problem, lowered, diagonal, perf_counter() - started, max(int(degree), 1))
 
     @property
     def coordinates(self):
@@ -104,6 +106,12 @@ class StaticDevice:
         self.out = torch.zeros(dim, dtype=torch.complex128, device=dev)
         self.ref = torch.zeros(1, dtype=torch.complex128, device=dev)
         self.t_token = 0
+        # residual evaluation: "program" (the residual program) or "quadrature" (FP64 JVPs at
+        # Gauss-Legendre points); the set-up tuner picks the faster one for the problem
+        self.residual_path = "program"
+        self._r0 = None  # R(0), the constant part, for the quadrature path
+        self._ts = None
+        self._qout = None
 
     # complex-vector actions ------------------------------------------------
     def load_amplitudes(self, t) -> None:
@@ -115,9 +123,52 @@ class StaticDevice:
         """Residual of the currently loaded amplitudes into ``out`` (complex)."""
 
         out = self.out if out is None else out
-        self.plan.apply("residual", out, self.diag, self.t)
+        if self.residual_path == "quadrature":
+            self._residual_by_quadrature(out)
+        else:
+            self.plan.apply("residual", out, self.diag, self.t)
         self.plan.apply("reference", self.ref)
         return out
+
+    def quadrature_nodes(self) -> tuple[np.ndarray, np.ndarray]:
+        """Gauss-Legendre nodes and weights on [0, 1] that integrate ``s -> J(s t) t`` exactly: a
+        residual of degree d makes it a polynomial of degree d - 1, and k nodes are exact up to
+        degree 2k - 1."""
+
+        k = max(1, (self.problem.residual_degree + 1) // 2)
+        x, w = np.polynomial.legendre.leggauss(k)
+        return 0.5 * (x + 1.0), 0.5 * w
+
+    def _residual_by_quadrature(self, out) -> None:
+        """``R(t) = R(0) + int_0^1 J(s t) t ds``: the residual's multi-factor terms (the costly
+        gathers of the residual program) become FP64 JVPs, whose amplitude-only factors are the
+        creation tensors materialised at each node.  Exact for the polynomial residual (to
+        rounding); the amplitude atoms of ``t`` are loaded again at the end."""
+
+        torch = self.torch
+        if self._ts is None:
+            self._ts = torch.empty_like(self.t)
+            self._qout = torch.empty_like(self.t)
+        if self._r0 is None:
+            self._ts.zero_()
+            self.plan.load("T", self._ts)
+            self._r0 = torch.empty_like(self.t)
+            self.plan.apply("residual", self._r0, self.diag, self._ts)
+        bits = self.plan.precision("jvp")
+        if bits != 64:
+            self.plan.set_precision("D", "jvp", 64)
+        try:
+            out.copy_(self._r0)
+            for s, w in zip(*self.quadrature_nodes()):
+                torch.mul(self.t, float(s), out=self._ts)
+                self.plan.load("T", self._ts)
+                self.plan.load("D", self.t)
+                self.plan.apply("jvp", self._qout, self.diag, self.t)
+                out.add_(self._qout, alpha=float(w))
+        finally:
+            if bits != 64:
+                self.plan.set_precision("D", "jvp", bits)
+            self.t_token = self.plan.load("T", self.t)
 
     def jvp_complex(self, dt, out=None):
         out = self.out if out is None else out

@@ -8,32 +8,67 @@ and the matrix-core entry threshold.  ``c3l_plan_autotune`` times candidates on 
 (coordinate descent from the default heuristics) and keeps the fastest for every plan with the
 same digest, so twin plans and repeated instances share one tuning.  Tuning runs once per plan
 at set-up, outside any timed solve; programs are timed on zero inputs (their cost does not
-depend on the values).
+depend on the values).  The set-up also picks, per static problem, whether the residual is
+evaluated by its program or by Gauss-Legendre quadrature of FP64 JVPs (whichever is faster).
 """
 
 from __future__ import annotations
+
+import numpy as np
 
 from . import _lib
 
 
 def autotune_static(static, jvp_bits: int = 64, reps: int = 5) -> list[dict]:
     """Tune the residual program and the JVP program (in the precision the Newton solve uses)
-    of a compiled static problem."""
+    of a compiled static problem, then choose how the residual is evaluated
+    (``choose_residual_path``)."""
 
     dev = static.device
     dev.t.zero_()
     dev.dt.zero_()
     dev.load_amplitudes(dev.t)
     out = [dev.plan.autotune("residual", dev.out, dev.diag, dev.t, reps=reps)]
+    dev.plan.load("D", dev.dt)
+    out.append(dev.plan.autotune("jvp", dev.out, dev.diag, dev.dt, reps=reps))  # FP64: the quadrature path
     if jvp_bits == 32:
         dev.plan.set_precision("D", "jvp", 32)
-    try:
-        dev.plan.load("D", dev.dt)
-        out.append(dev.plan.autotune("jvp", dev.out, dev.diag, dev.dt, reps=reps))
-    finally:
-        if jvp_bits == 32:
+        try:
+            dev.plan.load("D", dev.dt)
+            out.append(dict(dev.plan.autotune("jvp", dev.out, dev.diag, dev.dt, reps=reps), bits=32))
+        finally:
             dev.plan.set_precision("D", "jvp", 64)
+    out.append(choose_residual_path(static))
     return out
+
+
+def choose_residual_path(static, reps: int = 3) -> dict:
+    """Time one residual evaluation (amplitude load with its creation tensors, then the residual)
+    by the residual program and by Gauss-Legendre quadrature of FP64 JVPs
+    (``StaticDevice._residual_by_quadrature``) and keep the faster for this problem."""
+
+    torch = _lib.require_device()
+    dev = static.device
+    rng = np.random.default_rng(0)
+    t = 0.01 * (rng.normal(size=dev.dim) + 1j * rng.normal(size=dev.dim))
+    dev.t.copy_(torch.from_numpy(t).to(_lib.DEVICE))
+    times = {}
+    for path in ("program", "quadrature"):
+        dev.residual_path = path
+        dev.load_amplitudes(dev.t)
+        dev.residual_complex(dev.out)  # untimed: first use builds the constant part R(0)
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            dev.load_amplitudes(dev.t)
+            dev.residual_complex(dev.out)
+        stop.record()
+        stop.synchronize()
+        times[path] = start.elapsed_time(stop) / reps
+    dev.residual_path = min(times, key=times.get)
+    return {"program": "residual_path", "ms_program": times["program"], "ms_quadrature": times["quadrature"],
+            "choice": dev.residual_path, "quadrature_nodes": int(len(dev.quadrature_nodes()[0])),
+            "residual_degree": int(static.residual_degree)}
 
 
 def autotune_sigma(block, reps: int = 5) -> dict:

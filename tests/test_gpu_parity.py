@@ -506,6 +506,10 @@ def test_autotuned_program_shapes_keep_the_results(torch_gpu, n, seed):
     before = evaluate()
     records = autotune_static(sp, 64) + autotune_static(sp, 32) + [autotune_sigma(ep)]
     after = evaluate()
+    paths = [r for r in records if r["program"] == "residual_path"]
+    records = [r for r in records if r["program"] != "residual_path"]
+    assert len(paths) == 2 and all(r["choice"] in ("program", "quadrature") for r in paths)
+    print(f"  residual path: {paths[-1]}", flush=True)
     for r in records:
         print(f"  {r['program']}: {r['ms_default'] * 1e3:.1f} -> {r['ms_tuned'] * 1e3:.1f} us {r['shape']}", flush=True)
         assert r["ms_tuned"] <= r["ms_default"]
@@ -517,12 +521,76 @@ def test_autotuned_program_shapes_keep_the_results(torch_gpu, n, seed):
     res0, j640, j320, s0 = before
     res1, j641, j321, s1 = after
     lanes = records[0]["shape"]  # residual, FP64
-    if lanes["lanes_max_log2"] == 2 and lanes["lane_split_workgroups"] == 512:
+    if lanes["lanes_max_log2"] == 2 and lanes["lane_split_workgroups"] == 512 and sp.device.residual_path == "program":
         assert np.array_equal(res1, res0)  # order-preserving knobs only
     assert _rel(res1, res0) < 1e-13
     assert _rel(j641, j640) < 1e-13
     assert _rel(s1, s0) < 1e-13
     assert _rel(j321, j640) < 2e-6 and _rel(j320, j640) < 2e-6
+
+
+@pytest.mark.parametrize("case", ["random_3", "random_7", "kanamori_3x2"])
+def test_residual_by_quadrature_matches_the_residual_program(torch_gpu, case):
+    """Residual as R(0) + sum_i w_i J(s_i t) t at the Gauss-Legendre nodes of the residual's
+    degree (two nodes for degree 4) against the residual program (1e-13 relative), against the
+    polynomial oracle at n = 3 (1e-12), entered with the JVP in FP32 (the FP32 state must be back
+    afterwards), and a mixed-precision Newton solve on the quadrature path reaching the root of
+    the program path (packed residual <= 1e-10, amplitudes within 1e-8)."""
+
+    from c3lindblad_amd.solver import NewtonKrylovPolicy, solve_component_ccsd
+    from oracle import polynomial as poly
+
+    print(f"case quadrature residual {case}", flush=True)
+    p, sp = _matrix_core_case(case)
+    dev = sp.device
+    torch = torch_gpu
+    assert sp.residual_degree == 4 and len(dev.quadrature_nodes()[0]) == 2
+    rng = np.random.default_rng(6)
+    x = torch.from_numpy(0.05 * rng.normal(size=p.num_variables)).cuda()
+    got = {}
+    for path in ("program", "quadrature", "quadrature"):  # the second quadrature call reuses R(0)
+        dev.residual_path = path
+        r = torch.empty_like(x)
+        dev.residual_real(x, r)
+        got[path] = (r.cpu().numpy(), dev.out.cpu().numpy(), complex(dev.ref.item()))
+    assert _rel(got["quadrature"][0], got["program"][0]) < 1e-13
+    assert abs(got["quadrature"][2] - got["program"][2]) <= 1e-13 * max(1.0, abs(got["program"][2]))
+    if case == "random_3":
+        want, _ = poly.residual(p, p.tilde.unpack(x.cpu().numpy()))
+        assert _rel(got["quadrature"][1], want) < 1e-12
+    # entered while the JVP program is FP32: quadrature JVPs run in FP64, FP32 comes back
+    dx = torch.from_numpy(rng.normal(size=p.num_variables)).cuda()
+    j64 = torch.empty_like(x)
+    dev.residual_path = "program"
+    dev.residual_real(x, torch.empty_like(x))
+    dev.jvp_real(dx, j64)
+    dev.plan.set_precision("D", "jvp", 32)
+    try:
+        dev.residual_path = "quadrature"
+        r = torch.empty_like(x)
+        dev.residual_real(x, r)
+        assert dev.plan.precision("jvp") == 32
+        j32 = torch.empty_like(x)
+        dev.jvp_real(dx, j32)
+    finally:
+        dev.plan.set_precision("D", "jvp", 64)
+    assert _rel(r.cpu().numpy(), got["program"][0]) < 1e-13
+    e = _rel(j32.cpu().numpy(), j64.cpu().numpy())
+    assert 1e-12 < e < 2e-6  # still FP32 after the quadrature residual, at the same amplitudes
+    policy = NewtonKrylovPolicy.mixed_precision(forcing_strategy="eisenstat_walker", gmres_restart=150)
+    dev.residual_path = "program"
+    ref = solve_component_ccsd(sp, policy=policy)
+    dev.residual_path = "quadrature"
+    try:
+        sol = solve_component_ccsd(sp, policy=policy)
+    finally:
+        dev.residual_path = "program"
+    print(f"  program path: converged={ref.converged} |pack R|={ref.packed_residual_norm:.2e} jvps={ref.jvp_evaluations}; "
+          f"quadrature path: converged={sol.converged} |pack R|={sol.packed_residual_norm:.2e} "
+          f"jvps={sol.jvp_evaluations}", flush=True)
+    assert ref.converged and sol.converged and sol.packed_residual_norm <= 1e-10
+    scale = max(1.0, np.max(np.abs(ref.amplitudes)))
+    assert np.max(np.abs(sol.amplitudes - ref.amplitudes)) < 1e-8 * scale
 
 
 def test_mixed_precision_newton_reaches_fp64_root(torch_gpu):
