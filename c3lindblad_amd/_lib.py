@@ -75,6 +75,7 @@ SIGNATURES = {
                      ctypes.c_int),
     "c3l_to_bf16": ([_P, _P, _I64, _P], ctypes.c_int),
     "c3l_krylov_workspace": ([_I32, _I64, _I32, _I32, _P, _P], ctypes.c_int),
+    "c3l_krylov_basis_ld": ([_I64], _I64),
     "c3l_krylov_set_timing": ([_I32], ctypes.c_int),
     "c3l_krylov_timing": ([_I32, _P, _P, _P], ctypes.c_int),
     "c3l_arnoldi": (
@@ -165,6 +166,12 @@ def host_ptr(array: np.ndarray) -> int:
     if not array.flags.c_contiguous:
         raise ValueError("host array must be contiguous")
     return array.ctypes.data
+
+
+def krylov_basis_ld(rows: int) -> int:
+    """Leading dimension (entries) of a c3l_gmres basis of ``rows`` real coordinates."""
+
+    return int(load_library().c3l_krylov_basis_ld(int(rows)))
 
 
 def krylov_workspace(complex_basis: bool, dim: int, max_columns: int, n_shifts: int = 0) -> tuple[int, int]:

@@ -497,6 +497,7 @@ struct Gmres {
   void* stream;
   int64_t rd;
   int32_t* jvp_calls;
+  int64_t ldv = 0;  // basis leading dimension (entries)
 
   int apply(const double* v, double* out) {
     D_CALL(c3l_tilde_unpack(v, t_work, tm->k0, tm->k1, tm->ab, tm->dim, stream));
@@ -513,10 +514,10 @@ struct Gmres {
   int store_scaled(const double* x, float* col, double a) { return c3l_internal_narrow_scale(x, col, a, rd, stream); }
   // x += V y
   int add_combination(const double* V, int used, const double* y, double* x) {
-    return c3l_rmultiaxpy(V, rd, used, y, 1.0, x, rd, stream);
+    return c3l_rmultiaxpy(V, ldv > 0 ? ldv : rd, used, y, 1.0, x, rd, stream);
   }
   int add_combination(const float* V, int used, const double* y, double* x) {
-    return c3l_internal_rmultiaxpy_f32(V, rd, used, y, 1.0, x, rd, stream);
+    return c3l_internal_rmultiaxpy_f32(V, ldv > 0 ? ldv : rd, used, y, 1.0, x, rd, stream);
   }
   // out = M^{-1} J v with the preconditioner fused into the pack of J v
   template <typename S>
@@ -568,7 +569,8 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
     return dfail(C3L_EINVAL, "c3l_gmres: invalid argument");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int64_t rd = tm->real_dim;
-  Gmres g{plan, tm, jvp_diag, t_work, out_work, scratch, coef, stream, rd, jvp_calls};
+  const int64_t ldv = c3l_krylov_basis_ld(rd);
+  Gmres g{plan, tm, jvp_diag, t_work, out_work, scratch, coef, stream, rd, jvp_calls, ldv};
   hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(rd)), dim3(256), 0, s, x, 0.0, rd);
   D_CHECK(hipGetLastError());
   double bnrm = 0.0;
@@ -599,7 +601,7 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   }
   const int R = restart;
   const Layout L = gmres_layout(rd, R);
-  c3l_ortho::Basis<double, Store> basis{V, rd, rd, coef + L.H, R + 1, nullptr, nullptr, scratch + L.partial,
+  c3l_ortho::Basis<double, Store> basis{V, ldv, rd, coef + L.H, R + 1, nullptr, nullptr, scratch + L.partial,
                                  coef + L.red, coef + L.sc, nullptr, coef + L.stage};
   DriverClock clock(2);
   HostColumns pinned;
@@ -661,7 +663,7 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
     int absorbed = 0;
     for (int j = 0; j <= R && !stop; ++j) {
       const bool has_w = j < R;
-      if (has_w) D_CALL(g.apply_preconditioned(V + (int64_t)j * rd, w));
+      if (has_w) D_CALL(g.apply_preconditioned(V + (int64_t)j * ldv, w));
       D_CALL(c3l_ortho::step(basis, j, j == 0, has_w ? w : nullptr, 0.0, s, &clock));
       if (j == 0) continue;
       const int c = j - 1;
@@ -1306,6 +1308,8 @@ int c3l_arnoldi_group_begin(c3l_plan* const* slot_plans, int32_t n_slots, int32_
   *out = r;
   return C3L_OK;
 }
+
+int64_t c3l_krylov_basis_ld(int64_t rows) { return rows <= 0 ? 0 : (rows + 63) / 64 * 64; }
 
 int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
               const double* b, double* x, void* V, int32_t basis_bits, double* w, double* tmp, double* scratch,

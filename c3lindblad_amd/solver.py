@@ -187,7 +187,8 @@ class _DeviceNewton:
         # FP32 Krylov basis when the Jacobian products are FP32 (dots and updates stay FP64)
         self.basis_bits = int(basis_bits)
         basis_dtype = torch.float32 if self.basis_bits == 32 else f64
-        self.V = torch.zeros(restart + 1, D, dtype=basis_dtype, device=_lib.DEVICE)
+        # columns padded to c3l_krylov_basis_ld(D) entries: every column starts on a cache line
+        self.V = torch.zeros(restart + 1, _lib.krylov_basis_ld(D), dtype=basis_dtype, device=_lib.DEVICE)
         self.w = torch.empty(D, dtype=f64, device=_lib.DEVICE)
         self.tmp = torch.empty(D, dtype=f64, device=_lib.DEVICE)
         self.r = torch.empty(D, dtype=f64, device=_lib.DEVICE)

@@ -229,14 +229,21 @@ typedef struct c3l_tilde_maps {
  * (initial ||M b|| min(1, target/||b||), rescaled after each restart).  info = 0 when the
  * target is met, else `cycles`.  On return w holds the true residual b - J x.
  * The plan's T atoms must be loaded (linearisation point).  Workspace: t_work, out_work
- * complex [dim]; V [(restart+1) x real_dim] of doubles (basis_bits = 64) or floats
- * (basis_bits = 32: basis vectors stored in FP32, every dot, coefficient and update
- * computed in FP64; meant for FP32 Jacobian products); w, tmp [real_dim]; scratch and coef at least
+ * complex [dim]; V [(restart+1) x ld] with ld = c3l_krylov_basis_ld(real_dim) (each column
+ * starts on a 256-byte boundary; entries past real_dim are never read) of doubles
+ * (basis_bits = 64) or floats (basis_bits = 32: basis vectors stored in FP32, every dot,
+ * coefficient and update computed in FP64; meant for FP32 Jacobian products); w, tmp
+ * [real_dim]; scratch and coef at least
  * the sizes c3l_krylov_workspace(0, real_dim, restart, 0, ...) returns.  Orthogonalisation:
  * classical Gram-Schmidt with delayed reorthogonalisation (one reduction and one update pass
  * per iteration); each iteration applies J to the once-projected basis vector, so a cycle
  * that stops on the inner test has applied J once more than it has iterations.
  * x is overwritten with the step. */
+/* Leading dimension (entries) of the c3l_gmres basis for `rows` real coordinates: rows rounded up
+ * to a multiple of 64, so that every column starts on a cache-line boundary in FP32 and FP64
+ * (no counterpart in the reference, whose basis is a list of NumPy vectors). */
+int64_t c3l_krylov_basis_ld(int64_t rows);
+
 int c3l_gmres(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, double* t_work, double* out_work,
               const double* b, double* x, void* V, int32_t basis_bits, double* w, double* tmp, double* scratch,
               double* coef, double rtol, double atol, int32_t restart, int32_t cycles, int32_t* info,
