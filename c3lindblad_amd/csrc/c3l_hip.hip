@@ -1014,6 +1014,11 @@ struct KindExec {
   bool built = false;
   std::vector<BaseRun> runs;
   std::vector<void*> owned;
+  // inverse of the scatter (c3l_internal_plan_scatter_map): per canonical index two codes
+  // (padded offset << 1 | negative), -1 for none; built at its first request
+  bool inverse_built = false;
+  int64_t inverse_dim = 0;
+  int64_t* inverse = nullptr;
   // levels before tree_level run one launch each; the jobs of the later levels run as
   // contraction trees in one launch (tree_level = number of levels: no trees)
   int32_t tree_level = 0;
@@ -1579,8 +1584,66 @@ int c3l_plan_load(c3l_plan* plan, int32_t kind, const double* v, void* stream) {
 
 // Loads vector v[i] into twin plan plans[i] for i < count with one launch per kernel
 // (blockIdx.z = instance).  Twins are checked by digest; several plans load in FP64 only.
+int plan_load_many(c3l_plan* const* plans, int32_t count, int32_t kind, const double* const* v, void* stream,
+                   bool scatter);
+
 int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t kind, const double* const* v,
                                 void* stream) {
+  return plan_load_many(plans, count, kind, v, stream, true);
+}
+
+// c3l_internal_plan_load_many without the scatter of the input: the padded base blocks of
+// `kind` already hold the input (written by the producer of the vector through the map of
+// c3l_internal_plan_scatter_map); only the contraction levels and trees run.
+int c3l_internal_plan_load_prepared_many(c3l_plan* const* plans, int32_t count, int32_t kind, void* stream) {
+  const double* none[kMaxTwins] = {nullptr, nullptr, nullptr, nullptr};
+  double dummy = 0.0;
+  for (int32_t i = 0; i < count && i < kMaxTwins; ++i) none[i] = &dummy;  // never read
+  return plan_load_many(plans, count, kind, none, stream, false);
+}
+
+// Inverse of the scatter of kind `kind` for canonical vectors of `dim` entries: map[2 e + k]
+// (k = 0, 1) = (workspace offset << 1) | (1 when the value enters negated) of the padded
+// elements that take entry e, -1 for none.  Only kinds whose entries each fill at most two
+// padded elements have one (*map = NULL otherwise).  The workspace offsets are relative to
+// the plan's workspace (twin plans share the map).
+int c3l_internal_plan_scatter_map(c3l_plan* plan, int32_t kind, int64_t dim, const int64_t** map) {
+  if (plan == nullptr || map == nullptr || kind < 0 || kind >= kKinds || dim <= 0)
+    return fail(C3L_EINVAL, "c3l_internal_plan_scatter_map: invalid argument");
+  KindExec& ke = plan->kinds[kind];
+  *map = nullptr;
+  if (ke.inverse_built) {
+    if (ke.inverse_dim == dim) *map = ke.inverse;
+    return C3L_OK;
+  }
+  ke.inverse_built = true;
+  ke.inverse_dim = dim;
+  std::vector<int64_t> inv((size_t)(2 * dim), -1);
+  for (const PlanBase& b : plan->bases) {
+    if (b.kind != kind) continue;
+    for (int64_t e = 0; e < b.count; ++e) {
+      const int64_t src = b.src[(size_t)e];
+      if (src < 0) continue;
+      if (src >= dim) return C3L_OK;  // not a vector of this size: no map
+      const int64_t code = ((b.offset + e) << 1) | (b.sign[(size_t)e] < 0 ? 1 : 0);
+      if (inv[(size_t)(2 * src)] < 0)
+        inv[(size_t)(2 * src)] = code;
+      else if (inv[(size_t)(2 * src + 1)] < 0)
+        inv[(size_t)(2 * src + 1)] = code;
+      else
+        return C3L_OK;  // more than two padded elements per entry: no map
+    }
+  }
+  int rc = upload_owned(ke.owned, &ke.inverse, inv.data(), inv.size());
+  if (rc != C3L_OK) return rc;
+  *map = ke.inverse;
+  return C3L_OK;
+}
+
+double* c3l_internal_plan_workspace(c3l_plan* plan) { return plan ? reinterpret_cast<double*>(plan->ws) : nullptr; }
+
+int plan_load_many(c3l_plan* const* plans, int32_t count, int32_t kind, const double* const* v, void* stream,
+                   bool scatter) {
   if (plans == nullptr || v == nullptr || count < 1 || count > kMaxTwins)
     return fail(C3L_EINVAL, "c3l_plan_load: invalid argument");
   if (kind < 0 || kind >= kKinds) return fail(C3L_EINVAL, "c3l_plan_load: invalid kind");
@@ -1629,7 +1692,9 @@ int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t k
     ws32.p[i] = plans[i]->ws32;
   }
   const dim3 inst(1, 1, (unsigned)count);
+  if (!scatter && narrow) return fail(C3L_EINVAL, "c3l_plan_load: prepared loads are FP64 only");
   for (const BaseRun& r : ke.runs) {
+    if (!scatter) break;
     Twins<double2*> pad = ws;
     Twins<float2*> pad32 = ws32;
     for (int32_t i = 0; i < count; ++i) {

@@ -45,6 +45,9 @@ extern "C" int c3l_internal_rmultiaxpy_f32(const float* V, int64_t ld, int32_t n
                                            double scale, double* w, int64_t dim, void* stream);
 // Twin plans (equal digests) loaded / applied together, one launch per kernel.
 extern "C" int c3l_internal_plans_are_twins(c3l_plan* const* plans, int32_t count, int32_t kind, int32_t program);
+extern "C" int c3l_internal_plan_load_prepared_many(c3l_plan* const* plans, int32_t count, int32_t kind, void* stream);
+extern "C" int c3l_internal_plan_scatter_map(c3l_plan* plan, int32_t kind, int64_t dim, const int64_t** map);
+extern "C" double* c3l_internal_plan_workspace(c3l_plan* plan);
 extern "C" int c3l_internal_plan_load_many(c3l_plan* const* plans, int32_t count, int32_t kind,
                                            const double* const* v, void* stream);
 extern "C" int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t program,
@@ -994,6 +997,10 @@ struct c3l_arnoldi_run {
   double tolerance = 0.0, breakdown = 0.0;
   hipStream_t stream = nullptr;
   int32_t j = 0, c0 = 0;
+  // the update pass of step j scatters q_{j+1} into each plan's padded input blocks, so the
+  // sigma of step j + 1 skips its scatter (plans whose input entries fill at most two
+  // padded elements each)
+  bool fused_scatter = false;
   double* pinned = nullptr;  // two slots of 4 * count doubles
   hipEvent_t copied[2] = {nullptr, nullptr};
   int32_t batches = 0;     // status copies enqueued so far
@@ -1117,7 +1124,10 @@ struct c3l_arnoldi_run {
       ls.p[u] = state[(size_t)i];
     }
     if (has_w) {
-      D_CALL(c3l_internal_plan_load_many(lp, k, 2, vj, stream));
+      if (fused_scatter && j > 0)
+        D_CALL(c3l_internal_plan_load_prepared_many(lp, k, 2, stream));
+      else
+        D_CALL(c3l_internal_plan_load_many(lp, k, 2, vj, stream));
       D_CALL(c3l_internal_plan_apply_many(lp, k, 3, diag, vj, wj, stream));
     }
     D_CALL(c3l_ortho::step_many(lb, k, j, j == 0, wz, breakdown, stream, &clock));
@@ -1202,6 +1212,16 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
     hipLaunchKernelGGL(init_shift_state_kernel, dim3(blocks_for(n_shifts)), dim3(256), 0, s, r->state.back().gamma,
                        r->state.back().green, beta[i], n_shifts, status + 4 * i);
     D_CHECK(hipGetLastError());
+  }
+  // twins share their padded-block layout, so one map serves every basis
+  const int64_t* map = nullptr;
+  D_CALL(c3l_internal_plan_scatter_map(plans[0], 2, dim, &map));
+  if (map != nullptr) {
+    r->fused_scatter = true;
+    for (int32_t i = 0; i < count; ++i) {
+      r->basis[(size_t)i].pad = reinterpret_cast<double2*>(c3l_internal_plan_workspace(plans[i]));
+      r->basis[(size_t)i].pad_map = map;
+    }
   }
   D_CHECK(hipHostMalloc(reinterpret_cast<void**>(&r->pinned), 2 * 4 * sizeof(double) * (size_t)count));
   for (hipEvent_t& e : r->copied) D_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));

@@ -35,6 +35,11 @@ struct Basis {
   T* sc;           // step scalars, two alternating blocks of 8 (16 entries)
   double* status;  // status[0] = final H[j, j-1] when set (nullable)
   T* stage;        // when set: final column j-1 rows 0..j, then its h0 (j+2 entries; nullable)
+  // When set, the update pass also scatters q_{j+1} into the padded input blocks of the
+  // operator's plan: entry i goes to pad[m >> 1] (negated when m & 1) for each code m >= 0 of
+  // pad_map[2 i], pad_map[2 i + 1], so the next operator application skips its own scatter.
+  T* pad = nullptr;
+  const int64_t* pad_map = nullptr;
 };
 
 // Scratch sizes in units of T (entries) for a basis of `dim` rows and at most

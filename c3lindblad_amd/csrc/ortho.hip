@@ -388,7 +388,8 @@ __device__ __forceinline__ void column_part(int blk, T* __restrict__ H, int64_t 
 template <typename T, typename S, int WAVES>
 __device__ __forceinline__ void update_part(int blk, S* __restrict__ V, int64_t ld, int j, int fresh,
                                             const T* __restrict__ w, const T* __restrict__ red, int64_t dim,
-                                            const T* __restrict__ sc_prev, double breakdown) {
+                                            const T* __restrict__ sc_prev, double breakdown, T* __restrict__ pad,
+                                            const int64_t* __restrict__ pad_map) {
   using N = Num<T>;
   __shared__ T pa[WAVES][64];
   __shared__ T pb[WAVES][64];
@@ -445,8 +446,16 @@ __device__ __forceinline__ void update_part(int blk, S* __restrict__ V, int64_t 
       if (has_w) {
         const T ci = N::scale(o.c, o.ir);
         const T b = N::scale(N::sub(B, N::mul(ci, A)), o.ig);
-        V[(int64_t)(j + 1) * ld + i] =
-            narrow_to<S>(N::sub(N::sub(N::scale(w[i], o.ig), N::mul(N::scale(ci, o.ig), qi)), b));
+        const T next = N::sub(N::sub(N::scale(w[i], o.ig), N::mul(N::scale(ci, o.ig), qi)), b);
+        V[(int64_t)(j + 1) * ld + i] = narrow_to<S>(next);
+        if (pad != nullptr) {  // the next operator input, scattered into its padded blocks
+          const T stored = widen(narrow_to<S>(next));
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            const int64_t m = pad_map[2 * i + k];
+            if (m >= 0) pad[m >> 1] = (m & 1) ? N::sub(N::zero(), stored) : stored;
+          }
+        }
       }
     }
   }
@@ -460,7 +469,8 @@ __global__ __launch_bounds__(64 * WAVES) void update_column_kernel(Many<S*> V, i
                                                                    Many<const T*> w, Many<T*> H, int64_t ldh,
                                                                    Many<const T*> red, Many<T*> sink_v, Many<T*> sc,
                                                                    Many<double*> status, Many<T*> stage,
-                                                                   double breakdown, int64_t dim, int col_blocks) {
+                                                                   double breakdown, int64_t dim, int col_blocks,
+                                                                   Many<T*> pad, Many<const int64_t*> pad_map) {
   const int i = blockIdx.z;
   const int has_w = w.p[i] != nullptr ? 1 : 0;
   T* sc_prev = sc.p[i] + 8 * ((j + 1) & 1);
@@ -472,7 +482,7 @@ __global__ __launch_bounds__(64 * WAVES) void update_column_kernel(Many<S*> V, i
                           stage.p[i], o);
   } else {
     update_part<T, S, WAVES>((int)blockIdx.x - col_blocks, V.p[i], ld, j, fresh, w.p[i], red.p[i], dim, sc_prev,
-                             breakdown);
+                             breakdown, pad.p[i], pad_map.p[i]);
   }
 }
 
@@ -518,8 +528,14 @@ int step_many(const Basis<T, S>* bs, int count, int j, bool fresh, const T* cons
   Many<const T*> w{}, sink{}, partial_r{}, red_r{};
   Many<T*> partial{}, red{}, H{}, sink_v{}, sc{}, stage{};
   Many<double*> status{};
+  Many<T*> pad{};
+  Many<const int64_t*> pad_map{};
   for (int i = 0; i < count; ++i) {
     const Basis<T, S>& o = bs[i];
+    if ((o.pad == nullptr) != (o.pad_map == nullptr) || (o.pad != nullptr) != (bs[0].pad != nullptr))
+      return fail("Krylov orthogonalisation step: invalid argument");
+    pad.p[i] = o.pad;
+    pad_map.p[i] = o.pad_map;
     Vr.p[i] = o.V;
     Vw.p[i] = o.V;
     w.p[i] = ws[i];
@@ -554,7 +570,7 @@ int step_many(const Basis<T, S>* bs, int count, int j, bool fresh, const T* cons
   if (clock) clock->begin(1, s);
   hipLaunchKernelGGL((update_column_kernel<T, S, WAVES>), dim3(col_blocks + (b.dim + 63) / 64, 1, z),
                      dim3(64 * WAVES), 0, s, Vw, b.ld, j, fresh ? 1 : 0, w, H, b.ldh, red_r, sink_v, sc, status, stage,
-                     breakdown, b.dim, col_blocks);
+                     breakdown, b.dim, col_blocks, pad, pad_map);
   if (int rc = check_launch("update_column_kernel")) return rc;
   if (clock) clock->end(1, count * (col * (j + 1 + (has_w ? 1 : 0) + (fresh ? 0 : 1)) + vec * (has_w ? 1 : 0)), s);
   return C3L_OK;
