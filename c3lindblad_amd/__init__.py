@@ -46,6 +46,7 @@ from .response import (
 )
 from .matsubara import MatsubaraGrid, component_matsubara_green, matsubara_grid
 from .checkpoint import CheckpointedArnoldiResult, checkpointed_arnoldi_resolvent, eom_checkpoint_identity
+from .s135 import CompiledFullS135Action, FullS135ResponseOperator, full_s135_problem, full_s135_sources
 from .observables import (
     ComponentObservableProblem,
     contact_currents,
@@ -56,6 +57,10 @@ from .observables import (
 __all__ = [
     "ArnoldiResolvent",
     "CheckpointedArnoldiResult",
+    "CompiledFullS135Action",
+    "FullS135ResponseOperator",
+    "full_s135_problem",
+    "full_s135_sources",
     "checkpointed_arnoldi_resolvent",
     "eom_checkpoint_identity",
     "ComponentCCSolution",

@@ -44,6 +44,7 @@ import numpy as np
 from .lowering import STATIC_KEYS, BlockKey, FactorizedModeProduct, LinearModeOperator
 
 N_GROUPS = 4
+MAX_RANK = 5  # highest rank of an atom or output block (the rank-five shell of the S135 response)
 # U: constant creator vector, T: amplitude block, C: creation tensor (a merged
 # sum of wedges of U/T atoms), D: tangent block, X: EOM input block.
 BASE_ORDER = {"U": 0, "T": 1, "C": 2, "D": 3, "X": 4}
@@ -380,7 +381,7 @@ class BlockProgram:
     coefficients: np.ndarray  # complex [E]
     factor_start: np.ndarray  # int32 [E+1]
     factor_atom: np.ndarray  # int32 [F] atom id
-    factor_positions: np.ndarray  # int32 [F, 4] output positions per slot (-1 unused)
+    factor_positions: np.ndarray  # int32 [F, MAX_RANK] output positions per slot (-1 unused)
     target_atom: int = -1
     scalar_index: int = -1
 
@@ -416,14 +417,14 @@ def _block_program(term_list, key, index, target_atom=-1, scalar_index=-1) -> Bl
             coefficients.append(sign * coefficient)
             for a, ps in zip(term_atoms, positions):
                 f_atom.append(index[a])
-                f_pos.append(ps + [-1] * (4 - len(ps)))
+                f_pos.append(ps + [-1] * (MAX_RANK - len(ps)))
             starts.append(len(f_atom))
     return BlockProgram(
         key,
         np.asarray(coefficients, dtype=np.complex128),
         np.asarray(starts, dtype=np.int32),
         np.asarray(f_atom, dtype=np.int32),
-        np.asarray(f_pos, dtype=np.int32).reshape(-1, 4),
+        np.asarray(f_pos, dtype=np.int32).reshape(-1, MAX_RANK),
         target_atom,
         scalar_index,
     )

@@ -223,3 +223,23 @@ def config5_model(instance: int = 0) -> FermionicLindbladModel:
 
 def config5_grid() -> np.ndarray:
     return np.linspace(-8.0, 8.0, 8001)
+
+
+def s135_validation_model() -> FermionicLindbladModel:
+    """The compact interacting impurity of the reference's full-S135 tests
+    (``tests/s135_fixture.py:8-28``): impurity level -U/2 with U = 1.2, two bath sites at -1.2 and
+    1.2 coupled by 0.28 + 0.04 (1 - |e| / max|e|), losses 0.06 + 0.42 s and gains 0.48 - 0.42 s on
+    the bath (s the bath level scaled to [0, 1])."""
+
+    U = 1.2
+    bath = np.linspace(-1.2, 1.2, 2)
+    E = np.diag(np.concatenate(([-U / 2.0], bath))).astype(np.complex128)
+    coupling = 0.28 + 0.04 * (1.0 - np.abs(bath) / np.max(np.abs(bath)))
+    E[0, 1:] = coupling
+    E[1:, 0] = coupling
+    scaled = (bath - bath[0]) / (bath[-1] - bath[0])
+    loss = np.zeros(3)
+    gain = np.zeros(3)
+    loss[1:] = 0.06 + 0.42 * scaled
+    gain[1:] = 0.48 - 0.42 * scaled
+    return FermionicLindbladModel(E, np.diag(loss), np.diag(gain), U=U)

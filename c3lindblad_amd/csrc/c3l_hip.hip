@@ -292,11 +292,12 @@ __global__ __launch_bounds__(256) void contract_trees_kernel(Twins<C*> wss, cons
 // Factor f of an entry reads ws[offset_f + sum_k idx_k * stride_f[k]]: the
 // padded-tensor flat index as a dot product of the coordinate with strides
 // prepared on the host (0 for positions the factor does not use).
-// Coordinates are packed as four 8-bit indices (n <= 255).
+// Coordinates are packed as four 8-bit indices (n <= 255) up to rank four and as five 6-bit
+// indices (n <= 63) at rank five (the S135 response shell).
 
 struct WedgeFactor {
   int64_t offset;
-  int32_t stride[4];
+  int32_t stride[5];
 };
 
 struct WedgeEntry {
@@ -315,6 +316,7 @@ struct WedgeBlockDesc {
   int32_t n_entries;
   int32_t lanes_log2;
   int32_t reps;     // coordinates per thread (two-factor kernel, lanes_log2 == 0 only)
+  int32_t coord_bits;  // bits per index of the packed coordinates: 8 up to rank four, 6 at rank five
   int32_t dense_n;  // dense block: coordinate e is the index tuple of flat index e (first index
                     // fastest, base dense_n) and writes position dense_base + e
   int64_t dense_base;
@@ -323,11 +325,16 @@ struct WedgeBlockDesc {
 constexpr int kMaxReps = 2;
 constexpr int kStagedEntries = 64;
 
-__device__ __forceinline__ int64_t factor_index(const WedgeFactor& f, int i0, int i1, int i2, int i3) {
-  return f.offset + (int64_t)(i0 * f.stride[0] + i1 * f.stride[1] + i2 * f.stride[2] + i3 * f.stride[3]);
+template <bool kFive>
+__device__ __forceinline__ int64_t factor_index(const WedgeFactor& f, int i0, int i1, int i2, int i3, int i4) {
+  int flat = i0 * f.stride[0] + i1 * f.stride[1] + i2 * f.stride[2] + i3 * f.stride[3];
+  if constexpr (kFive) flat += i4 * f.stride[4];
+  return f.offset + (int64_t)flat;
 }
 
-template <bool kTwoFactors, typename C>
+// kFive: the program has rank-five output blocks (a fifth index per coordinate); the other
+// programs use the four-index form, which needs fewer registers.
+template <bool kTwoFactors, typename C, bool kFive>
 __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss, Twins<double2*> outs,
                                                             const WedgeBlockDesc* __restrict__ descs,
                                                             const int2* __restrict__ tiles,
@@ -354,7 +361,7 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
   const int group = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> (8 - lg)));
   const int step = 1 << lg;
   constexpr int R = kTwoFactors ? kMaxReps : 1;
-  int i0[R], i1[R], i2[R], i3[R];
+  int i0[R], i1[R], i2[R], i3[R], i4[R];
   bool act[R];
   int64_t e[R];
   C acc[R];
@@ -362,14 +369,17 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
   for (int r = 0; r < R; ++r) {
     e[r] = (int64_t)tile.y + local + (int64_t)r * per;
     act[r] = r < d.reps && e[r] < d.count;
-    i0[r] = i1[r] = i2[r] = i3[r] = 0;
+    i0[r] = i1[r] = i2[r] = i3[r] = i4[r] = 0;
     if (act[r]) {
       if (d.coord != nullptr) {
         const uint32_t c = d.coord[e[r]];
-        i0[r] = c & 255u;
-        i1[r] = (c >> 8) & 255u;
-        i2[r] = (c >> 16) & 255u;
-        i3[r] = c >> 24;
+        const int bits = d.coord_bits;
+        const uint32_t m = (1u << bits) - 1u;
+        i0[r] = c & m;
+        i1[r] = (c >> bits) & m;
+        i2[r] = (c >> (2 * bits)) & m;
+        i3[r] = (c >> (3 * bits)) & m;
+        if constexpr (kFive) i4[r] = bits < 8 ? (c >> (4 * bits)) & m : 0u;  // rank-five blocks pack a fifth index
       } else {  // dense: e < dense_n^deg, so the unused higher indices come out 0
         const int n = d.dense_n;
         int f = (int)e[r];
@@ -378,7 +388,13 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
         i1[r] = f % n;
         f /= n;
         i2[r] = f % n;
-        i3[r] = f / n;
+        f /= n;
+        if constexpr (kFive) {
+          i3[r] = f % n;
+          i4[r] = f / n;
+        } else {
+          i3[r] = f;
+        }
       }
     }
     acc[r] = czero<C>();
@@ -394,8 +410,8 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
         const WedgeFactor fb = s_fac[2 * t + 1];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const C a = ws[factor_index(fa, i0[r], i1[r], i2[r], i3[r])];
-          const C b = ws[factor_index(fb, i0[r], i1[r], i2[r], i3[r])];
+          const C a = ws[factor_index<kFive>(fa, i0[r], i1[r], i2[r], i3[r], i4[r])];
+          const C b = ws[factor_index<kFive>(fb, i0[r], i1[r], i2[r], i3[r], i4[r])];
           acc[r] = cadd(acc[r], cmul(c, cmul(a, b)));
         }
       }
@@ -407,8 +423,8 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
         const WedgeFactor fb = d.factors[2 * t + 1];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const C a = ws[factor_index(fa, i0[r], i1[r], i2[r], i3[r])];
-          const C b = ws[factor_index(fb, i0[r], i1[r], i2[r], i3[r])];
+          const C a = ws[factor_index<kFive>(fa, i0[r], i1[r], i2[r], i3[r], i4[r])];
+          const C b = ws[factor_index<kFive>(fb, i0[r], i1[r], i2[r], i3[r], i4[r])];
           acc[r] = cadd(acc[r], cmul(c, cmul(a, b)));
         }
       }
@@ -418,7 +434,7 @@ __global__ __launch_bounds__(256) void wedge_program_kernel(Twins<const C*> wss,
       const WedgeEntry en = d.entries[t];
       C prod = from_d2<C>(en.coeff);
       for (int f = en.first; f < en.first + en.nf; ++f)
-        prod = cmul(prod, ws[factor_index(d.factors[f], i0[0], i1[0], i2[0], i3[0])]);
+        prod = cmul(prod, ws[factor_index<kFive>(d.factors[f], i0[0], i1[0], i2[0], i3[0], i4[0])]);
       acc[0] = cadd(acc[0], prod);
     }
   }
@@ -1079,6 +1095,7 @@ struct ProgramShape {
 
 struct ProgramExec {
   bool built = false;
+  bool five = false;       // the program has rank-five output blocks
   ProgramShape shape;      // the shape the current images were built with
   int64_t tune_gen = -1;   // tuning generation at which `shape` was looked up
   int64_t max_dest = -1;
@@ -1376,7 +1393,7 @@ int32_t c3l_plan_block_count(c3l_plan* plan) { return plan == nullptr ? 0 : (int
 
 int c3l_plan_add_base(c3l_plan* plan, int32_t kind, int64_t offset, int32_t deg, const int64_t* src,
                       const int8_t* sign) {
-  if (plan == nullptr || kind < 0 || kind >= kKinds || deg < 0 || deg > 4 || offset < 0)
+  if (plan == nullptr || kind < 0 || kind >= kKinds || deg < 0 || deg > 5 || offset < 0)
     return fail(C3L_EINVAL, "c3l_plan_add_base: invalid argument");
   PlanBase b;
   b.kind = kind;
@@ -1406,7 +1423,7 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
   std::vector<ContractJob> jobs((size_t)n_jobs);
   int64_t max_out = 1;  // threads the level's largest job needs
   for (int32_t j = 0; j < n_jobs; ++j) {
-    if (par_rank[j] < 1 || par_rank[j] > 4 || slot[j] < 0 || slot[j] >= par_rank[j] || vec[j] < 0 ||
+    if (par_rank[j] < 1 || par_rank[j] > 5 || slot[j] < 0 || slot[j] >= par_rank[j] || vec[j] < 0 ||
         vec[j] >= plan->n_vectors)
       return fail(C3L_EINVAL, "c3l_plan_add_level: job out of range");
     const int64_t outsz = ipow(plan->n, par_rank[j] - 1);
@@ -1451,13 +1468,13 @@ int parse_block_entries(c3l_plan* plan, PlanBlock& b, int32_t deg, int32_t n_ent
   b.factors.resize((size_t)n_factors);
   for (int32_t f = 0; f < n_factors; ++f) {
     const int rank = fcode[f] & 7;
-    if (rank > 4 || foffset[f] < 0 || foffset[f] + ipow(plan->n, rank) > plan->workspace)
+    if (rank > 5 || foffset[f] < 0 || foffset[f] + ipow(plan->n, rank) > plan->workspace)
       return fail(C3L_EINVAL, "c3l_plan_add_block: factor exceeds workspace");
     WedgeFactor wf;
     wf.offset = foffset[f];
-    for (int k = 0; k < 4; ++k) wf.stride[k] = 0;
+    for (int k = 0; k < 5; ++k) wf.stride[k] = 0;
     for (int q = 0; q < rank; ++q) {
-      const int pos = (fcode[f] >> (3 + 2 * q)) & 3;
+      const int pos = (fcode[f] >> (3 + 3 * q)) & 7;
       if (pos >= deg) return fail(C3L_EINVAL, "c3l_plan_add_block: factor position out of range");
       wf.stride[pos] += (int32_t)ipow(plan->n, q);  // atoms store their first index fastest
     }
@@ -1492,10 +1509,14 @@ extern "C" {
 int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
                        const int64_t* dest, int32_t n_entries, const double* coeff, const int32_t* fstart,
                        int32_t n_factors, const int64_t* foffset, const int32_t* fcode) {
-  if (plan == nullptr || program < 0 || program >= kPrograms || deg < 0 || deg > 4 || count < 0 ||
+  if (plan == nullptr || program < 0 || program >= kPrograms || deg < 0 || deg > 5 || count < 0 ||
       count > INT32_MAX || n_entries < 0 || n_factors < 0)
     return fail(C3L_EINVAL, "c3l_plan_add_block: invalid argument");
-  if (plan->n > 255) return fail(C3L_EINVAL, "c3l_plan_add_block: more than 255 sites per group");
+  // packed coordinates: four 8-bit indices up to rank four, five 6-bit indices at rank five
+  const int bits = deg <= 4 ? 8 : 6;
+  if (plan->n > (1 << bits) - 1)
+    return fail(C3L_EINVAL, deg <= 4 ? "c3l_plan_add_block: more than 255 sites per group"
+                                     : "c3l_plan_add_block: rank-five blocks take at most 63 sites per group");
   PlanBlock b;
   b.program = program;
   b.deg = deg;
@@ -1511,7 +1532,7 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
     for (int q = 0; q < deg; ++q) {
       const int32_t c = coord[k * deg + q];
       if (c < 0 || c >= plan->n) return fail(C3L_EINVAL, "c3l_plan_add_block: coordinate out of range");
-      packed |= (uint32_t)c << (8 * q);
+      packed |= (uint32_t)c << (bits * q);
     }
     b.coord[(size_t)k] = packed;
   }
@@ -1531,7 +1552,7 @@ int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t cou
 int c3l_plan_add_dense_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t base, int32_t n_entries,
                              const double* coeff, const int32_t* fstart, int32_t n_factors, const int64_t* foffset,
                              const int32_t* fcode) {
-  if (plan == nullptr || program < 0 || program >= kPrograms || deg < 0 || deg > 4 || base < 0 || n_entries < 0 ||
+  if (plan == nullptr || program < 0 || program >= kPrograms || deg < 0 || deg > 5 || base < 0 || n_entries < 0 ||
       n_factors < 0)
     return fail(C3L_EINVAL, "c3l_plan_add_dense_block: invalid argument");
   if (plan->n > 255) return fail(C3L_EINVAL, "c3l_plan_add_dense_block: more than 255 sites per group");
@@ -1794,6 +1815,7 @@ int release_program_image(ProgramExec& pe) {
     pe.owned.clear();
   }
   pe.built = false;
+  pe.five = false;
   pe.max_dest = -1;
   pe.tiles_two = pe.tiles_general = 0;
   pe.descs = nullptr;
@@ -1854,6 +1876,8 @@ int build_program(c3l_plan* plan, int32_t program) {
       d.reps = reps;
     }
     d.dense_n = b.dense ? plan->n : 0;
+    d.coord_bits = b.deg <= 4 ? 8 : 6;
+    if (b.deg == 5) pe.five = true;
     d.dense_base = b.dense ? b.dense_base : 0;
     uint32_t* coord = nullptr;
     int64_t* dest = nullptr;
@@ -2372,24 +2396,40 @@ int c3l_internal_plan_apply_many(c3l_plan* const* plans, int32_t count, int32_t 
       C3L_HIP_CHECK(hipGetLastError());
     }
     if (pe.tiles32_two > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<true, float2>), dim3(pe.tiles32_two), dim3(kThreads), 0, s, ws32, o,
-                         pe.descs, pe.tiles32, dg, xx);
+      if (pe.five)
+        hipLaunchKernelGGL((wedge_program_kernel<true, float2, true>), dim3(pe.tiles32_two), dim3(kThreads), 0, s,
+                           ws32, o, pe.descs, pe.tiles32, dg, xx);
+      else
+        hipLaunchKernelGGL((wedge_program_kernel<true, float2, false>), dim3(pe.tiles32_two), dim3(kThreads), 0, s,
+                           ws32, o, pe.descs, pe.tiles32, dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
     if (pe.tiles32_general > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<false, float2>), dim3(pe.tiles32_general), dim3(kThreads), 0, s, ws32,
-                         o, pe.descs, pe.tiles32 + pe.tiles32_two, dg, xx);
+      if (pe.five)
+        hipLaunchKernelGGL((wedge_program_kernel<false, float2, true>), dim3(pe.tiles32_general), dim3(kThreads), 0,
+                           s, ws32, o, pe.descs, pe.tiles32 + pe.tiles32_two, dg, xx);
+      else
+        hipLaunchKernelGGL((wedge_program_kernel<false, float2, false>), dim3(pe.tiles32_general), dim3(kThreads), 0,
+                           s, ws32, o, pe.descs, pe.tiles32 + pe.tiles32_two, dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
   } else {
     if (pe.tiles_two > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<true, double2>), two, dim3(kThreads), 0, s, ws, o, pe.descs, pe.tiles,
-                         dg, xx);
+      if (pe.five)
+        hipLaunchKernelGGL((wedge_program_kernel<true, double2, true>), two, dim3(kThreads), 0, s, ws, o, pe.descs,
+                           pe.tiles, dg, xx);
+      else
+        hipLaunchKernelGGL((wedge_program_kernel<true, double2, false>), two, dim3(kThreads), 0, s, ws, o, pe.descs,
+                           pe.tiles, dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
     if (pe.tiles_general > 0) {
-      hipLaunchKernelGGL((wedge_program_kernel<false, double2>), general, dim3(kThreads), 0, s, ws, o, pe.descs,
-                         pe.tiles + pe.tiles_two, dg, xx);
+      if (pe.five)
+        hipLaunchKernelGGL((wedge_program_kernel<false, double2, true>), general, dim3(kThreads), 0, s, ws, o,
+                           pe.descs, pe.tiles + pe.tiles_two, dg, xx);
+      else
+        hipLaunchKernelGGL((wedge_program_kernel<false, double2, false>), general, dim3(kThreads), 0, s, ws, o,
+                           pe.descs, pe.tiles + pe.tiles_two, dg, xx);
       C3L_HIP_CHECK(hipGetLastError());
     }
     if (internal) ++plan->fixed_version;

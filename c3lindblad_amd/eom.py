@@ -32,19 +32,31 @@ from .static import ComponentStaticProblem
 class EOMCCSDProblem:
     cc: FactorizedCCProblem
     response_spin: int = 0
+    space: str = "S13"  # "S135": S1 + S3 + the whole rank-five shell
 
     @property
     def dimension(self) -> int:
+        """Response coordinates of both charge blocks."""
+
         n = self.cc.model.n_sites
-        return 2 * (n + n * (n * (n - 1) // 2) + n**3)
+        return sum(response_coordinates(n, q, self.space).dimension for q in (1, -1))
+
+    @property
+    def max_rank(self) -> int:
+        return 5 if self.space == "S135" else 3
 
 
-def eom_ccsd_problem(cc: FactorizedCCProblem, *, response_spin: int = 0) -> EOMCCSDProblem:
+def eom_ccsd_problem(cc: FactorizedCCProblem, *, response_spin: int = 0, space: str = "S13") -> EOMCCSDProblem:
+    """``space="S135"`` adds the rank-five shell (the reference's full S1 + S3 + S5 response,
+    ``cc/s135_full.py:101-361``) to the S13 space of ``cc/eom.py:93-135``."""
+
     if not isinstance(cc, FactorizedCCProblem):
         raise TypeError("cc must be a factorized CC problem")
     if response_spin != 0:
         raise ValueError("this build compiles response_spin=0 (the reference default)")
-    return EOMCCSDProblem(cc, response_spin)
+    if space not in ("S13", "S135"):
+        raise ValueError("response space must be 'S13' or 'S135'")
+    return EOMCCSDProblem(cc, response_spin, space)
 
 
 @dataclass
@@ -64,10 +76,10 @@ class ComponentEOMProblem:
         if charge not in (-1, 1):
             raise ValueError("response charge must be +1 or -1")
         n = problem.cc.model.n_sites
-        coords = response_coordinates(n, charge)
+        coords = response_coordinates(n, charge, problem.space)
         keys = tuple(b.key for b in coords.blocks)
         compiler = TermCompiler(n, problem.cc.products)
-        terms = {k: v for k, v in compiler.response_terms(keys, 3).items() if term_key(k) in keys}
+        terms = {k: v for k, v in compiler.response_terms(keys, problem.max_rank).items() if term_key(k) in keys}
         factored, creations = factor_linear_terms(terms, "X")
         plan = emit_plan({"sigma": (factored, keys)}, n, compiler.table.array(), creations)
         lowered = lower_plan(plan, {"T": (static.oracle.coordinates, STATIC_KEYS), "X": (coords, keys)}, {"sigma": coords})
@@ -95,7 +107,7 @@ class ComponentEOMProblem:
             actions = {}
             for name in ("addition", "removal"):
                 op, coefficient = ops[name]
-                terms = compiler.apply_factor({(): complex(coefficient)}, compiler._lin(op), 3)
+                terms = compiler.apply_factor({(): complex(coefficient)}, compiler._lin(op), self.oracle.max_rank)
                 actions["source_" + name] = ({k: v for k, v in terms.items() if term_key(k) in self.keys}, self.keys)
             plan = emit_plan(actions, cc.model.n_sites, compiler.table.array())
             plans[site] = lower_plan(

@@ -300,24 +300,34 @@ def static_coordinates(n: int) -> CoordinateSet:
     return _frozen_set(coordinate_set(n, [[(1, 1, 0, 0), (0, 0, 1, 1)], [(2, 2, 0, 0), (1, 1, 1, 1), (0, 0, 2, 2)]]))
 
 
-def response_keys(charge: int) -> list[list[BlockKey]]:
-    """S13 keys of one charge block for response spin 0 (``ci/basis.py:181-234``)."""
+RESPONSE_SPACES = ("S13", "S135")
 
+
+def response_keys(charge: int, space: str = "S13") -> list[list[BlockKey]]:
+    """Keys of one charge block for response spin 0: S13 (ranks 1 and 3, ``ci/basis.py:181-234``)
+    or S135, which adds the whole rank-five shell (``cc/eom.py:1-40``, ``cc/s135_full.py:101-361``):
+    every key of rank five with the block's charge and no net spin flip."""
+
+    if space not in RESPONSE_SPACES:
+        raise ValueError("response space must be 'S13' or 'S135'")
     if charge == 1:
-        return [[(1, 0, 0, 0)], [(2, 1, 0, 0), (1, 0, 1, 1)]]
-    if charge == -1:
-        return [[(0, 1, 0, 0)], [(1, 2, 0, 0), (0, 1, 1, 1)]]
-    raise ValueError("response charge must be +1 or -1")
+        keys = [[(1, 0, 0, 0)], [(2, 1, 0, 0), (1, 0, 1, 1)]]
+        five = [(3, 2, 0, 0), (2, 1, 1, 1), (1, 0, 2, 2)]
+    elif charge == -1:
+        keys = [[(0, 1, 0, 0)], [(1, 2, 0, 0), (0, 1, 1, 1)]]
+        five = [(2, 3, 0, 0), (1, 2, 1, 1), (0, 1, 2, 2)]
+    else:
+        raise ValueError("response charge must be +1 or -1")
+    return keys + [five] if space == "S135" else keys
 
 
 @lru_cache(maxsize=128)
-def response_coordinates(n: int, charge: int) -> CoordinateSet:
-    """S13 strings of one charge block (built once per site count and charge, read-only)."""
+def response_coordinates(n: int, charge: int, space: str = "S13") -> CoordinateSet:
+    """Strings of one charge block (built once per site count, charge and space, read-only):
+    rank by rank, keys whose group counts exceed n hold no string and are left out."""
 
-    keys = response_keys(charge)
-    if n < 2:
-        keys = [keys[0], [k for k in keys[1] if max(k) <= n]]
-    return _frozen_set(coordinate_set(n, keys))
+    keys = [[k for k in rank if max(k) <= n] for rank in response_keys(charge, space)]
+    return _frozen_set(coordinate_set(n, [rank for rank in keys if rank]))
 
 
 # ----------------------------------------------------------------------------- amplitude tilde map

@@ -10,7 +10,7 @@ workspace and converts the symbolic plan of ``algebra.emit_plan`` into:
   ``component_tensor.py:302-337``);
 * ``levels`` -- contraction jobs per kind and depth;
 * ``blocks`` -- per program and output block: coordinates, destinations and
-  the wedge entries (``fcode = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9``);
+  the wedge entries (``fcode = rank | pos0<<3 | pos1<<6 | ... | pos4<<15``);
   creation programs write every padded index tuple of a ``C`` atom into the
   workspace (destinations are workspace offsets).
 
@@ -66,12 +66,12 @@ class DenseDest:
 
 
 def encode_factors(ranks: np.ndarray, positions: np.ndarray) -> np.ndarray:
-    """``fcode = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9`` (rank <= 4, positions < 4)."""
+    """``fcode = rank | pos0<<3 | pos1<<6 | pos2<<9 | pos3<<12 | pos4<<15`` (rank <= 5, positions < 5)."""
 
     code = ranks.astype(np.int32).copy()
     for s in range(positions.shape[1] if positions.size else 0):
         pos = np.where(positions[:, s] >= 0, positions[:, s], 0).astype(np.int32)
-        code |= (pos << (3 + 2 * s)) * (s < ranks)
+        code |= (pos << (3 + 3 * s)) * (s < ranks)
     return code
 
 

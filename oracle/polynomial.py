@@ -218,8 +218,9 @@ def vacuum_polynomial(problem, amplitudes, masks, max_rank: int = 4) -> dict:
     return vac
 
 
-def sigma(problem, amplitudes, response_masks, x, static_masks=None) -> np.ndarray:
-    """EOM-CCSD sigma on one charge block: ``Λ∘X + P[K̄ X|0> - X ∧ K̄|0>]``."""
+def sigma(problem, amplitudes, response_masks, x, static_masks=None, max_rank: int = 3) -> np.ndarray:
+    """EOM-CCSD sigma on one charge block: ``Λ∘X + P[K̄ X|0> - X ∧ K̄|0>]`` through ``max_rank``
+    (3: S13; 5: S135, the reference's full rank-five response, ``cc/s135_full.py:101-361``)."""
 
     static_masks = coordinate_masks(problem.coordinates) if static_masks is None else static_masks
     index = {m: i for i, m in enumerate(response_masks)}
@@ -229,11 +230,11 @@ def sigma(problem, amplitudes, response_masks, x, static_masks=None) -> np.ndarr
     xpoly = {m: complex(v) for m, v in zip(response_masks, x) if v != 0.0}
     out = _diagonal(response_masks, problem.rapidities, x)
     for record in problem.products:
-        for mask, value in product_action(record, shifts, 3, initial=xpoly).items():
+        for mask, value in product_action(record, shifts, max_rank, initial=xpoly).items():
             i = index.get(mask)
             if i is not None:
                 out[i] += value
-    vac = vacuum_polynomial(problem, t, static_masks, max_rank=2)
+    vac = vacuum_polynomial(problem, t, static_masks, max_rank=max_rank - 1)
     for xm, xv in xpoly.items():
         for vm, vv in vac.items():
             if xm & vm:

@@ -68,7 +68,7 @@ class PortPlan:
                     code = int(fcode[f])
                     flat = np.zeros(rows.size, dtype=np.int64)
                     for slot in range((code & 7) - 1, -1, -1):
-                        flat = flat * n + sub[:, (code >> (3 + 2 * slot)) & 3]
+                        flat = flat * n + sub[:, (code >> (3 + 3 * slot)) & 7]
                     prod *= self.ws[foff[f] + flat]
                 acc += prod
             dests.append(dest[rows])
@@ -94,7 +94,7 @@ class PortPlan:
                     rank = code & 7
                     flat = np.zeros(count, dtype=np.int64)
                     for s in range(rank - 1, -1, -1):  # slot s has stride n**s
-                        flat = flat * n + coord[:, (code >> (3 + 2 * s)) & 3]
+                        flat = flat * n + coord[:, (code >> (3 + 3 * s)) & 7]
                     prod *= self.ws[foff[f] + flat]
                 acc += prod
             if diag is not None:

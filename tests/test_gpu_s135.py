@@ -1,0 +1,122 @@
+"""Full S1 + S3 + S5 response on the GPU (rank-five shell), against the CPU oracle.
+
+* The S135 sigma (``FullS135ResponseOperator.matvec``) against the polynomial oracle through rank
+  five, whose S135 sigma is pinned by the Fock-space brute force at n = 2, 3
+  (``tests/test_oracle_pins.py``): random models at n = 2, 3, 4, both charges, 1e-12 relative.
+* The reference's compact S135 model (``tests/s135_fixture.py``, n = 3): the GPU S135 retarded
+  Green function from the shared Krylov basis against dense solves of the oracle's S135 matrix at
+  the device-converged amplitudes (1e-8), S135 sources equal to the S13 sources embedded with
+  zeros, and the S135 result differing from S13 by more than the tolerance (the rank-five shell
+  matters at U = 1.2).
+* A noninteracting two-site model: S135 G^R equals the physical Lindblad ED (1e-8), as in the
+  reference's ``test_charge_soc_noninteracting_full_s135_matches_physical_ed``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("n", [2, 3, 4])
+@pytest.mark.parametrize("charge", [1, -1])
+def test_s135_sigma_matches_polynomial_oracle(torch_gpu, n, charge):
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, eom_ccsd_problem
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.s135 import CompiledFullS135Action, FullS135ResponseOperator
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from tests.conftest import random_model
+
+    print(f"case s135 sigma n={n} charge={charge}", flush=True)
+    p = factorized_cc_problem(random_model(n, 1.4, 60 + n))
+    sp = ComponentStaticProblem.compile(p)
+    rng = np.random.default_rng(n + 3)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    ctx = component_eom_context(ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, charge), t)
+    op = FullS135ResponseOperator(CompiledFullS135Action(ctx))
+    assert op.p_dimension == ctx.problem.dimension and op.s5_dimension > 0
+    x = rng.normal(size=op.dimension) + 1j * rng.normal(size=op.dimension)
+    want = poly.sigma(p, t, poly.coordinate_masks(op.coordinates), x, max_rank=5)
+    assert _rel(op.matvec(x), want) < 1e-12
+    # the S13 rows and columns of the S135 operator are the S13 sigma's
+    xs = np.zeros(op.dimension, complex)
+    xs[: op.p_dimension] = x[: op.p_dimension]
+    s13 = poly.sigma(p, t, poly.coordinate_masks(ctx.problem.coordinates), x[: op.p_dimension])
+    assert _rel(op.matvec(xs)[: op.p_dimension], s13) < 1e-12
+
+
+def test_s135_green_on_the_reference_validation_model(torch_gpu):
+    from c3lindblad_amd.configs import s135_validation_model
+    from c3lindblad_amd.eom import (
+        ComponentEOMProblem,
+        ComponentSigmaOperator,
+        component_eom_context,
+        component_eom_sources,
+        eom_ccsd_problem,
+    )
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.response import component_retarded_green
+    from c3lindblad_amd.s135 import CompiledFullS135Action, FullS135ResponseOperator, full_s135_sources
+    from c3lindblad_amd.solver import solve_component_ccsd
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    print("case s135 green validation model", flush=True)
+    p = factorized_cc_problem(s135_validation_model())
+    sp = ComponentStaticProblem.compile(p)
+    sol = solve_component_ccsd(sp)
+    assert sol.converged and sol.packed_residual_norm <= 1e-10
+    omega = np.linspace(-3.0, 3.0, 41)
+    for charge in (1, -1):
+        ctx = component_eom_context(ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, charge), sol.amplitudes)
+        op = FullS135ResponseOperator(CompiledFullS135Action(ctx, materialization_batch_size=4), s5_tile_size=7)
+        src = full_s135_sources(op)
+        s13 = component_eom_sources(ctx)
+        for name in ("addition", "removal", "sink"):
+            embedded = np.zeros(op.dimension, complex)
+            embedded[: op.p_dimension] = getattr(s13, name)
+            assert np.max(np.abs(getattr(src, name) - embedded)) < 1e-14 * max(1.0, np.max(np.abs(embedded)))
+        got = component_retarded_green(op, src, omega, 0.0)
+        masks = poly.coordinate_masks(op.coordinates)
+        A = solvers.dense_matrix(lambda v: poly.sigma(p, sol.amplitudes, masks, v, max_rank=5), op.dimension)
+        add, rem = solvers.retarded_green_dense(A, src.addition, src.removal, src.sink, omega, 0.0)
+        err = _rel(got.retarded, add + rem)
+        s13_green = component_retarded_green(ComponentSigmaOperator(ctx), s13, omega, 0.0).retarded
+        shift = _rel(s13_green, add + rem)
+        print(f"  charge {charge:+d}: P={op.p_dimension} Q5={op.s5_dimension} |dG|/|G| vs dense {err:.2e}, "
+              f"S13 vs S135 {shift:.2e}", flush=True)
+        assert err < 1e-8
+        assert shift > 1e-6
+
+
+def test_s135_noninteracting_matches_lindblad_ed(torch_gpu):
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, eom_ccsd_problem
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.model import FermionicLindbladModel
+    from c3lindblad_amd.response import component_retarded_green
+    from c3lindblad_amd.s135 import CompiledFullS135Action, FullS135ResponseOperator, full_s135_sources
+    from c3lindblad_amd.solver import solve_component_ccsd
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import fock
+
+    print("case s135 noninteracting vs ED", flush=True)
+    E = np.array([[-0.4, 0.3], [0.3, 0.7]], dtype=np.complex128)
+    model = FermionicLindbladModel(E, np.diag([0.0, 0.35]), np.diag([0.0, 0.2]), U=0.0)
+    p = factorized_cc_problem(model)
+    sp = ComponentStaticProblem.compile(p)
+    sol = solve_component_ccsd(sp)
+    assert sol.converged
+    ctx = component_eom_context(ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1), sol.amplitudes)
+    op = FullS135ResponseOperator(CompiledFullS135Action(ctx))
+    omega = np.linspace(-3.0, 3.0, 31)
+    got = component_retarded_green(op, full_s135_sources(op), omega, 0.0)
+    want = fock.exact_retarded_green(model, omega, 0.0)
+    assert _rel(got.retarded, want) < 1e-8

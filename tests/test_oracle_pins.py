@@ -68,6 +68,47 @@ def test_polynomial_sigma_equals_fock_brute_force(n, charge):
     assert _rel(poly.sigma(p, t, poly.coordinate_masks(rc), x), want) < 1e-13
 
 
+@pytest.mark.parametrize("n,charge", [(2, 1), (2, -1), (3, 1), (3, -1)])
+def test_polynomial_s135_sigma_equals_fock_brute_force(n, charge):
+    """The full S1 + S3 + S5 sigma (rank-five shell, ``cc/s135_full.py:101-361``) of the
+    polynomial oracle against the Fock-space brute force of ``[K̄, X]|0>`` projected on the S135
+    strings (n = 2: 22 strings per charge, n = 3: 150)."""
+
+    from c3lindblad_amd.lowering import response_coordinates
+    from oracle import polynomial as poly
+    from oracle.fock import PHFockSpace
+
+    p, _ = _setup(n, 0.8, 50 + n)
+    rng = np.random.default_rng(n + 11)
+    t = p.tilde.unpack(0.1 * rng.normal(size=p.num_variables))
+    rc = response_coordinates(p.model.n_sites, charge, "S135")
+    assert any(sum(b.key) == 5 for b in rc.blocks)
+    x = rng.normal(size=rc.dimension) + 1j * rng.normal(size=rc.dimension)
+    want = PHFockSpace(p).sigma(p.coordinates, t, rc, x)
+    assert _rel(poly.sigma(p, t, poly.coordinate_masks(rc), x, max_rank=5), want) < 1e-13
+
+
+@pytest.mark.parametrize("n,charge", [(2, 1), (3, -1), (4, 1)])
+def test_compiled_s135_sigma_equals_polynomial_oracle(n, charge):
+    """The S135 plan (rank-five input, output and creation tensors) executed by the NumPy port
+    against the polynomial oracle through rank five."""
+
+    from c3lindblad_amd.eom import ComponentEOMProblem, eom_ccsd_problem
+    from oracle import polynomial as poly
+    from oracle.port import PortPlan
+
+    p, sp = _setup(n, 1.7, 70 + n)
+    rng = np.random.default_rng(n + 13)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p, space="S135"), sp, charge)
+    x = rng.normal(size=ep.dimension) + 1j * rng.normal(size=ep.dimension)
+    port = PortPlan(ep.lowered)
+    port.load(0, t)
+    port.load(2, x)
+    got = port.apply(3, np.zeros(ep.dimension, complex), ep.diagonal, x)
+    assert _rel(got, poly.sigma(p, t, poly.coordinate_masks(ep.coordinates), x, max_rank=5)) < 1e-13
+
+
 def test_polynomial_jvp_equals_finite_difference():
     from oracle import polynomial as poly
 
