@@ -359,7 +359,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="siam24", choices=("siam24", "readme", "kanamori39", "batch512"))
+    ap.add_argument("--config", default="siam24", choices=("siam24", "readme", "kanamori39", "batch512", "siam12"))
+    ap.add_argument("--space", default="S13", choices=("S13", "S135"),
+                    help="EOM response space: S13, or S135 (S1 + S3 + the rank-five shell)")
     ap.add_argument("--instances", type=int, default=512, help="batch512: instances of the uneven-size batch")
     ap.add_argument("--batch-chunk", type=int, default=64, help="batch512: instances set up and solved per chunk")
     ap.add_argument("--max-group", type=int, default=8, help="batch512: instances per lockstep group")
@@ -416,6 +418,11 @@ def main() -> int:
     if args.config == "siam24":
         model, grid, charges = config2_model(), config2_grid(), (1, -1)
         workload = "siam_24_pseudomodes_U4_ccsd_eom_s13_gr_4001w"
+    elif args.config == "siam12":
+        from c3lindblad_amd.configs import siam_model
+
+        model, grid, charges = siam_model(11, 4.0, seed=0), config2_grid(), (1, -1)
+        workload = "siam_11_pseudomodes_U4_ccsd_eom_" + args.space.lower() + "_gr_4001w"
     elif args.config == "kanamori39":
         from c3lindblad_amd.configs import config5_grid, config5_model
 
@@ -430,7 +437,9 @@ def main() -> int:
         policy = NewtonKrylovPolicy.mixed_precision(forcing_strategy=args.forcing, gmres_restart=args.restart)
     else:
         policy = NewtonKrylovPolicy(forcing_strategy=args.forcing, gmres_restart=args.restart)
-    pipe = SolvePipeline.prepare(model, grid, 0.0, charges=charges, policy=policy)
+    pipe = SolvePipeline.prepare(model, grid, 0.0, charges=charges, policy=policy, space=args.space)
+    if args.space == "S135" and args.config == "siam24":
+        workload = "siam_24_pseudomodes_U4_ccsd_eom_s135_gr_4001w"
     pipe.eom_streams = args.eom_streams
     print(f"[bench] setup {pipe.setup_seconds:.2f}s n={model.n_sites} D={pipe.static.oracle.num_amplitudes}", file=sys.stderr, flush=True)
     extra = []

@@ -50,14 +50,15 @@ class SolvePipeline:
     tuning: list = field(default_factory=list)  # autotuner records of the set-up
 
     @classmethod
-    def prepare(cls, model, omega, eta=0.0, charges=(1, -1), policy=None, rtol=1e-10, autotune=True):
+    def prepare(cls, model, omega, eta=0.0, charges=(1, -1), policy=None, rtol=1e-10, autotune=True, space="S13"):
         """Host set-up and upload; ``autotune`` runs the launch-shape autotuner on the residual,
-        JVP (in the policy's precision) and sigma programs (``tuning.py``), inside the set-up time."""
+        JVP (in the policy's precision) and sigma programs (``tuning.py``), inside the set-up time.
+        ``space="S135"`` solves the response on S1 + S3 + the rank-five shell."""
 
         started = perf_counter()
         problem = factorized_cc_problem(model)
         static = ComponentStaticProblem.compile(problem)
-        eom = eom_ccsd_problem(problem)
+        eom = eom_ccsd_problem(problem, space=space)
         blocks = {q: ComponentEOMProblem.compile(eom, static, q) for q in charges}
         static.device  # upload
         for b in blocks.values():

@@ -120,3 +120,64 @@ def test_s135_noninteracting_matches_lindblad_ed(torch_gpu):
     got = component_retarded_green(op, full_s135_sources(op), omega, 0.0)
     want = fock.exact_retarded_green(model, omega, 0.0)
     assert _rel(got.retarded, want) < 1e-8
+
+
+def test_s135_siam8_green_certified_by_the_port(torch_gpu):
+    """A 7-mode SIAM (n = 8, P = 744, 22,176 rank-five strings per charge) on 1001 frequencies
+    with both charge blocks' S135 resolvents in lockstep pairs: every shift's GMRES residual
+    <= 1e-10, and at sampled frequencies the device solution's true residual measured with the
+    NumPy port's S135 sigma <= 1e-9, its sink projection equal to the device G^R value."""
+
+    from c3lindblad_amd.configs import siam_model
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, eom_ccsd_problem
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.response import retarded_green_blocks
+    from c3lindblad_amd.s135 import CompiledFullS135Action, FullS135ResponseOperator, full_s135_sources
+    from c3lindblad_amd.solver import NewtonKrylovPolicy, solve_component_ccsd
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle.port import PortPlan
+
+    print("case s135 siam8 n=8, 1001 frequencies, charges +-1", flush=True)
+    p = factorized_cc_problem(siam_model(7, 4.0, seed=2))
+    sp = ComponentStaticProblem.compile(p)
+    policy = NewtonKrylovPolicy.mixed_precision(forcing_strategy="eisenstat_walker", gmres_restart=150)
+    sol = solve_component_ccsd(sp, policy=policy)
+    assert sol.converged and sol.packed_residual_norm <= 1e-10
+    t = sol.amplitudes
+    omega = np.linspace(-8.0, 8.0, 1001)
+    pairs = []
+    for q in (1, -1):
+        ctx = component_eom_context(ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, q), t)
+        op = FullS135ResponseOperator(CompiledFullS135Action(ctx))
+        pairs.append((op, full_s135_sources(op)))
+    results = retarded_green_blocks(pairs, omega, 0.0, rtol=1e-10)
+    for (op, src), g in zip(pairs, results):
+        problem = op.context.problem
+        port = PortPlan(problem.lowered)
+        port.load(0, t)
+
+        def port_sigma(x):
+            port.load(2, x)
+            return port.apply(3, np.zeros(op.dimension, complex), problem.diagonal, x)
+
+        worst = max(float(np.max(g.addition_relative_residuals)), float(np.max(g.removal_relative_residuals)))
+        dims = (g.addition_projection.dimension, g.removal_projection.dimension)
+        print(f"  dimension {op.dimension}: Krylov {dims}, worst shift residual {worst:.2e}", flush=True)
+        assert worst <= 1e-10
+        scale = float(np.max(np.abs(g.retarded)))
+        for name in ("addition", "removal"):
+            proj = getattr(g, f"{name}_projection")
+            b = getattr(src, name)
+            m = proj.dimension
+            Hbar = proj.hessenberg[: m + 1, :m]
+            for k in (0, 333, 500, 1000):
+                z = complex(omega[k])
+                A = -Hbar.copy()
+                A[np.arange(m), np.arange(m)] += z
+                rhs = np.zeros(m + 1, dtype=complex)
+                rhs[0] = proj.source_norm
+                y = np.linalg.lstsq(A, rhs, rcond=None)[0]
+                xk = proj._device_combination(y[None, :], m).cpu().numpy()[0]
+                true = np.linalg.norm(b - (z * xk - port_sigma(xk))) / np.linalg.norm(b)
+                assert true <= 1e-9, (name, k, true)
+                assert abs(src.sink @ xk - getattr(g, name)[k]) <= 1e-9 * scale, (name, k)
