@@ -200,6 +200,15 @@ def dptr(tensor) -> int:
     return tensor.data_ptr()
 
 
+def dptr_rows(tensor) -> int:
+    """Pointer of a 2-D GPU tensor whose rows are contiguous (any row stride, passed separately as
+    the leading dimension), e.g. a basis view of padded columns."""
+
+    if not tensor.is_cuda or tensor.dim() != 2 or tensor.stride(1) != 1 or tensor.stride(0) < tensor.shape[1]:
+        raise ValueError("device tensor must be a GPU matrix with contiguous rows")
+    return tensor.data_ptr()
+
+
 def stream_ptr(torch) -> int:
     return torch.cuda.current_stream().cuda_stream
 

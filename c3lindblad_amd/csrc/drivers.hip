@@ -993,6 +993,7 @@ struct c3l_arnoldi_run {
   const double2* shifts = nullptr;
   double* status = nullptr;
   int64_t dim = 0;
+  int64_t ld = 0;  // basis leading dimension (complex entries), c3l_krylov_basis_ld(dim)
   int32_t count = 0, m_max = 0, n_shifts = 0, min_dim = 0;
   double tolerance = 0.0, breakdown = 0.0;
   hipStream_t stream = nullptr;
@@ -1117,7 +1118,7 @@ struct c3l_arnoldi_run {
     for (int u = 0; u < k; ++u) {
       const int32_t i = live[(size_t)u];
       lp[u] = plans[(size_t)i];
-      vj[u] = V[(size_t)i] + (int64_t)2 * j * dim;
+      vj[u] = V[(size_t)i] + (int64_t)2 * j * ld;
       wj[u] = w[(size_t)i];
       wz[u] = has_w ? reinterpret_cast<const double2*>(wj[u]) : nullptr;
       lb[u] = basis[(size_t)i];
@@ -1173,8 +1174,10 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
     return dfail(C3L_EINVAL, "c3l_arnoldi: the bases' plans must be FP64 twins (built from the same arrays)");
   hipStream_t s = static_cast<hipStream_t>(stream);
   const Layout L = arnoldi_layout(dim, m_max, n_shifts);
-  // per-basis strides (doubles) of the caller's consecutive instance blocks
-  const int64_t sV = 2 * ((int64_t)m_max + 1) * dim, sH = 2 * (int64_t)m_max * ((int64_t)m_max + 1);
+  // per-basis strides (doubles) of the caller's consecutive instance blocks; basis columns are
+  // padded to c3l_krylov_basis_ld(dim) complex entries so that each starts on a cache line
+  const int64_t ld = c3l_krylov_basis_ld(dim);
+  const int64_t sV = 2 * ((int64_t)m_max + 1) * ld, sH = 2 * (int64_t)m_max * ((int64_t)m_max + 1);
   const int64_t sRot = (int64_t)n_shifts * m_max, sShift = n_shifts;
   r->plans.assign(plans, plans + count);
   r->count = count;
@@ -1182,6 +1185,7 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
   r->shifts = reinterpret_cast<const double2*>(shifts);
   r->status = status;
   r->dim = dim;
+  r->ld = ld;
   r->m_max = m_max;
   r->n_shifts = n_shifts;
   r->min_dim = min_dim;
@@ -1193,7 +1197,7 @@ int arnoldi_setup(c3l_arnoldi_run* r, c3l_plan* const* plans, int32_t count, con
     double* cf = coef + i * L.coef_total;
     double2* Hz = reinterpret_cast<double2*>(H + i * sH);
     double2* sink_v = reinterpret_cast<double2*>(cf + L.sink_v);
-    r->basis.push_back(c3l_ortho::Basis<double2>{reinterpret_cast<double2*>(V + i * sV), dim, dim, Hz,
+    r->basis.push_back(c3l_ortho::Basis<double2>{reinterpret_cast<double2*>(V + i * sV), ld, dim, Hz,
                                                  (int64_t)m_max + 1, sink_v,
                                                  reinterpret_cast<const double2*>(sink + 2 * i * dim),
                                                  reinterpret_cast<double2*>(sc + L.partial),

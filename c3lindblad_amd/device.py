@@ -315,12 +315,12 @@ class Reducer:
         ev = KernelTimer.start()
         if self.complex:
             _lib.call(
-                "c3l_zmultidot", _lib.dptr(V), ld, ncols, _lib.dptr(w), self.dim, 1 if conjugate else 0,
+                "c3l_zmultidot", _lib.dptr_rows(V), ld, ncols, _lib.dptr(w), self.dim, 1 if conjugate else 0,
                 _lib.dptr(self.scratch), _lib.dptr(out), _lib.stream_ptr(self.torch),
             )
         else:
             _lib.call(
-                "c3l_rmultidot", _lib.dptr(V), ld, ncols, _lib.dptr(w), self.dim,
+                "c3l_rmultidot", _lib.dptr_rows(V), ld, ncols, _lib.dptr(w), self.dim,
                 _lib.dptr(self.scratch), _lib.dptr(out), _lib.stream_ptr(self.torch),
             )
         elem = 16 if self.complex else 8

@@ -329,8 +329,9 @@ def arnoldi_resolvent(
     dev = _lib.DEVICE
     c128, f64 = torch.complex128, torch.float64
     ns = z.size
-    V = torch.empty(m_max + 1, n, dtype=c128, device=dev)  # columns are written before they are read
-    V[0].copy_(torch.from_numpy(b / beta))
+    # columns padded to c3l_krylov_basis_ld(n) entries (each starts on a cache line), written before read
+    V = torch.empty(m_max + 1, _lib.krylov_basis_ld(n), dtype=c128, device=dev)
+    V[0, :n].copy_(torch.from_numpy(b / beta))
     H = torch.zeros(m_max, m_max + 1, dtype=c128, device=dev)  # row j = column j of H
     zs = torch.from_numpy(z).to(dev)
     sink_d = torch.from_numpy(np.ascontiguousarray(sink_h)).to(dev)
@@ -360,7 +361,7 @@ def arnoldi_resolvent(
     operator.action_calls += m
     residuals = resid.cpu().numpy()
     return ArnoldiResolvent(
-        basis=V[: m + 1],
+        basis=V[: m + 1, :n],
         hessenberg=H[:m, : m + 1].cpu().numpy().T.copy(),
         source_norm=beta,
         breakdown=bool(broke.value),
@@ -414,8 +415,10 @@ class _ArnoldiRun:
         dev = _lib.DEVICE
         c128, f64 = torch.complex128, torch.float64
         ns = z.size
-        self.V = torch.empty(count, m_max + 1, n, dtype=c128, device=dev)  # columns written before read
-        self.V[:, 0].copy_(torch.from_numpy(b / beta[:, None]))
+        # columns padded to c3l_krylov_basis_ld(n) entries (each starts on a cache line), written before read
+        self.n = n
+        self.V = torch.empty(count, m_max + 1, _lib.krylov_basis_ld(n), dtype=c128, device=dev)
+        self.V[:, 0, :n].copy_(torch.from_numpy(b / beta[:, None]))
         self.H = torch.zeros(count, m_max, m_max + 1, dtype=c128, device=dev)
         zs = torch.from_numpy(z).to(dev)
         sink_d = torch.from_numpy(np.ascontiguousarray(snk)).to(dev)
@@ -462,7 +465,7 @@ class _ArnoldiRun:
             m = int(m_out[i])
             op.action_calls += m
             out.append(ArnoldiResolvent(
-                basis=self.V[i, : m + 1],
+                basis=self.V[i, : m + 1, : self.n],
                 hessenberg=self.H[i, :m, : m + 1].cpu().numpy().T.copy(),
                 source_norm=float(self.beta[i]),
                 breakdown=bool(broke[i]),

@@ -272,7 +272,8 @@ int c3l_gmres_group(c3l_plan* plan, const c3l_tilde_maps* tm, int32_t count, con
  * resolvent arnoldi_resolvent, src/c3l/green/krylov.py:284-417).  One Arnoldi basis (classical
  * Gram-Schmidt with delayed reorthogonalisation) serves every shift; each shift runs GMRES on
  * it with its sink projection
- * green[s] = sink . x_m(z_s) updated incrementally.  V [(m_max+1) x dim] complex with
+ * green[s] = sink . x_m(z_s) updated incrementally.  V [(m_max+1) x c3l_krylov_basis_ld(dim)]
+ * complex (columns padded to a cache-line multiple, padding never read) with
  * V[0] = b/beta on entry; H complex, column j at H + j*(m_max+1); rot_c [n_shifts*m_max]
  * doubles, rot_s and pi [n_shifts*m_max] complex, gamma and green [n_shifts] complex, resid
  * [n_shifts] doubles (relative GMRES residual); scratch and coef at least the sizes

@@ -750,7 +750,7 @@ def test_batched_arnoldi_rejects_plans_that_are_not_twins(torch_gpu):
     m, ns, count = 4, 3, 2
     scratch_size, coef_size = _lib.krylov_workspace(True, n, m, ns)
     dev = "cuda"
-    V = torch.zeros(count, m + 1, n, dtype=torch.complex128, device=dev)
+    V = torch.zeros(count, m + 1, _lib.krylov_basis_ld(n), dtype=torch.complex128, device=dev)
     V[:, 0, 0] = 1.0
     H = torch.zeros(count, m, m + 1, dtype=torch.complex128, device=dev)
     zs = torch.full((ns,), 1j, dtype=torch.complex128, device=dev)
