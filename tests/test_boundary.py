@@ -190,3 +190,51 @@ def test_batch_skips_instances_without_a_full_hf_reference():
     assert inst is None and why[0] == 7 and "full-HF" in why[1]
     inst, why = _compile_job((3, siam_model(3, 2.0, seed=1), (1, -1)))
     assert why is None and inst.index == 3 and set(inst.eom) == {1, -1}
+
+
+def test_s135_space_keys_and_layout():
+    """S135 adds the rank-five keys of the charge (``cc/eom.py:1-40``) after the S13 strings, which
+    keep their S13 order, so S13 vectors embed as a prefix; unknown spaces are rejected."""
+
+    from math import comb
+
+    from c3lindblad_amd.eom import eom_ccsd_problem
+    from c3lindblad_amd.lowering import factorized_cc_problem, response_coordinates, response_keys
+
+    assert response_keys(1, "S135")[2] == [(3, 2, 0, 0), (2, 1, 1, 1), (1, 0, 2, 2)]
+    assert response_keys(-1, "S135")[2] == [(2, 3, 0, 0), (1, 2, 1, 1), (0, 1, 2, 2)]
+    for n in (2, 3, 5):
+        for q in (1, -1):
+            s13 = response_coordinates(n, q)
+            s135 = response_coordinates(n, q, "S135")
+            shell = comb(n, 3) * comb(n, 2) + comb(n, 2) * n**3 + n * comb(n, 2) ** 2
+            assert s135.dimension == s13.dimension + shell
+            width = s13.modes.shape[1]
+            assert np.array_equal(s135.modes[: s13.dimension, :width], s13.modes)
+            assert np.all(s135.modes[: s13.dimension, width:] == -1)
+    with pytest.raises(ValueError, match="space"):
+        response_keys(1, "S7")
+    cc = _readme_static().oracle
+    with pytest.raises(ValueError, match="space"):
+        eom_ccsd_problem(cc, space="S1357")
+    problem = eom_ccsd_problem(cc, space="S135")
+    assert problem.max_rank == 5 and problem.dimension == sum(
+        response_coordinates(3, q, "S135").dimension for q in (1, -1))
+
+
+def test_s135_reference_knobs_are_validated_before_the_device():
+    from types import SimpleNamespace
+
+    from c3lindblad_amd.s135 import CompiledFullS135Action, FullS135ResponseOperator
+
+    with pytest.raises(ValueError, match="materialization_batch_size"):
+        CompiledFullS135Action(SimpleNamespace(), materialization_batch_size=0)
+    with pytest.raises(ValueError, match="s5_tile_size"):
+        FullS135ResponseOperator(SimpleNamespace(), s5_tile_size=0)
+
+
+def test_residual_degree_is_four():
+    """The residual is a quartic polynomial of the amplitudes (two-body products), which the
+    two-node Gauss-Legendre residual path relies on."""
+
+    assert _readme_static().residual_degree == 4
