@@ -262,16 +262,17 @@ def tangent_terms(terms: dict) -> dict:
     return {k: v for k, v in out.items() if v != 0.0}
 
 
-def factor_linear_terms(terms: dict, linear_base: str) -> tuple[dict, list[tuple[BlockKey, dict]]]:
+def factor_linear_terms(terms: dict, linear_base: str, first_index: int = 0) -> tuple[dict, list[tuple[BlockKey, dict]]]:
     """Rewrite terms linear in one ``linear_base`` atom as ``C_g ∧ L_g``.
 
     Every term contains exactly one atom ``L`` of ``linear_base``; moving it
     to the right end costs ``(-1)^(rank L * rank of the atoms after it)``.
     Terms are grouped by ``(L, block key of the rest)``; the rest of each
-    group is summed into one creation tensor ``C_g`` (a new ``C`` atom),
-    which depends only on the fixed atoms and is materialised once per
-    linearisation point.  Returns the two-factor terms and, per creation
-    atom, its key and the terms that define it.
+    group is summed into one creation tensor ``C_g`` (a new ``C`` atom,
+    numbered from ``first_index`` so that several factorings share one
+    creation program), which depends only on the fixed atoms and is
+    materialised once per linearisation point.  Returns the two-factor terms
+    and, per creation atom, its key and the terms that define it.
     """
 
     groups: dict = defaultdict(lambda: defaultdict(complex))
@@ -291,7 +292,7 @@ def factor_linear_terms(terms: dict, linear_base: str) -> tuple[dict, list[tuple
         rest_terms = {k: v for k, v in rest_terms.items() if v != 0.0}
         if not rest_terms:
             continue
-        c = Atom("C", len(creations), key_slots(key))
+        c = Atom("C", first_index + len(creations), key_slots(key))
         creations.append((key, rest_terms))
         factored[(c, lin)] = 1.0 + 0.0j
     return factored, creations

@@ -151,7 +151,8 @@ class InstanceGroup:
         G, D = self.count, self.D
         statics = [m.static for m in self.members]
         static_plan = union_plan([s.lowered for s in statics], {KINDS["T"]: D, KINDS["D"]: D},
-                                 {PROGRAMS["residual"]: D, PROGRAMS["reference"]: 1, PROGRAMS["jvp"]: D})
+                                 {PROGRAMS["residual"]: D, PROGRAMS["reference"]: 1, PROGRAMS["jvp"]: D,
+                                  PROGRAMS["residual_low"]: D})
         self._dev["plan"] = DevicePlan(static_plan)
         self._dev["tilde"] = DeviceTilde(union_tilde([s.oracle.tilde for s in statics]))
         self._dev["diag"] = torch.from_numpy(np.concatenate([s.diagonal for s in statics])).to(dev)

@@ -1129,7 +1129,7 @@ struct PendingTiming {
   hipEvent_t stop;
 };
 
-constexpr int kPrograms = 8;
+constexpr int kPrograms = 9;
 
 struct c3l_plan {
   int32_t n = 0;
@@ -1147,7 +1147,7 @@ struct c3l_plan {
   // a float2 mirror of the workspace; FP64-produced atoms they read are narrowed into it
   // whenever the FP64 side changed (fixed_version) since the last narrowing.
   int32_t kind_bits[kKinds] = {64, 64, 64};
-  int32_t program_bits[kPrograms] = {64, 64, 64, 64, 64, 64, 64, 64};
+  int32_t program_bits[kPrograms] = {64, 64, 64, 64, 64, 64, 64, 64, 64};
   float2* ws32 = nullptr;
   int64_t ws32_tail = 0;  // zeroed elements allocated past the FP32 workspace
   float2* vectors32 = nullptr;

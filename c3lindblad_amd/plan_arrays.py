@@ -42,6 +42,7 @@ PROGRAMS = {
     "observable": 5,
     "source_addition": 6,
     "source_removal": 7,
+    "residual_low": 8,  # degree-0 terms plus 1/6 of the degree-1 terms (Simpson residual)
 }
 
 

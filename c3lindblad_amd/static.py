@@ -29,6 +29,12 @@ from .lowering import STATIC_KEYS, FactorizedCCProblem
 from .plan_arrays import LoweredPlan, lower_plan
 
 
+def _t_degree(term_key) -> int:
+    """Amplitude atoms of a term: its degree in the amplitudes."""
+
+    return sum(1 for atom in term_key if getattr(atom, "base", None) == "T")
+
+
 @dataclass
 class ComponentStaticAction:
     retained: np.ndarray
@@ -54,24 +60,35 @@ class ComponentStaticProblem:
         residual = compiler.static_terms(4)
         jvp = {k: -1j * v for k, v in tangent_terms(residual).items() if term_key(k) in STATIC_KEYS}
         jvp_factored, creations = factor_linear_terms(jvp, "D")
+        # R(0) + J(0) t / 6 without the diagonal term, the end point s = 0 of Simpson's rule: the
+        # terms with no amplitude atom, and a sixth of those with one, factored as constant
+        # creation tensors (products of the generator's atoms alone) times one amplitude atom
+        low = {k: -1j * v for k, v in residual.items() if _t_degree(k) == 0 and term_key(k) in STATIC_KEYS}
+        linear = {k: -1j * v / 6.0 for k, v in residual.items() if _t_degree(k) == 1 and term_key(k) in STATIC_KEYS}
+        low_factored, low_creations = factor_linear_terms(linear, "T", first_index=len(creations))
+        overlap = set(low) & set(low_factored)
+        if overlap:
+            raise AssertionError("constant and factored linear residual terms collide")
+        low.update(low_factored)
         plan = emit_plan(
             {
                 "residual": ({k: -1j * v for k, v in residual.items()}, STATIC_KEYS),
                 "reference": ({k: -1j * v for k, v in residual.items()}, ((0, 0, 0, 0),)),
                 "jvp": (jvp_factored, STATIC_KEYS),
+                "residual_low": (low, STATIC_KEYS),
             },
             problem.model.n_sites,
             compiler.table.array(),
-            creations,
+            creations + low_creations,
         )
         coords = problem.coordinates
         lowered = lower_plan(
             plan,
             {"T": (coords, STATIC_KEYS), "D": (coords, STATIC_KEYS)},
-            {"residual": coords, "reference": coords, "jvp": coords},
+            {"residual": coords, "reference": coords, "jvp": coords, "residual_low": coords},
         )
         diagonal = -1j * problem.rapidity_sums()
-        degree = max((sum(1 for atom in key if getattr(atom, "base", None) == "T") for key in residual), default=0)
+        degree = max((_t_degree(key) for key in residual), default=0)
         return cls(problem, lowered, diagonal, perf_counter() - started, max(int(degree), 1))
 
     @property
@@ -106,12 +123,14 @@ class StaticDevice:
         self.out = torch.zeros(dim, dtype=torch.complex128, device=dev)
         self.ref = torch.zeros(1, dtype=torch.complex128, device=dev)
         self.t_token = 0
-        # residual evaluation: "program" (the residual program) or "quadrature" (FP64 JVPs at
-        # Gauss-Legendre points); the set-up tuner picks the faster one for the problem
+        # residual evaluation: "program" (the residual program), "quadrature" (FP64 JVPs at
+        # Gauss-Legendre points) or "simpson" (FP64 JVPs at s = 1/2 and 1 plus the low-degree
+        # program); the set-up tuner picks the fastest for the problem
         self.residual_path = "program"
         self._r0 = None  # R(0), the constant part, for the quadrature path
         self._ts = None
         self._qout = None
+        self._diag6 = None  # diag / 6, the diagonal's share of the Simpson end point s = 0
 
     # complex-vector actions ------------------------------------------------
     def load_amplitudes(self, t) -> None:
@@ -119,12 +138,25 @@ class StaticDevice:
             self.t.copy_(t)
         self.t_token = self.plan.load("T", self.t)
 
+    def residual_at(self, t, out=None):
+        """Residual at the amplitudes ``t`` (complex, on the device) into ``out``; the amplitude
+        atoms of ``t`` are loaded afterwards.  The quadrature and Simpson paths load the atoms at
+        their own nodes and end with ``t``, so only the program path loads ``t`` first."""
+
+        if t.data_ptr() != self.t.data_ptr():
+            self.t.copy_(t)
+        if self.residual_path == "program":
+            self.t_token = self.plan.load("T", self.t)
+        return self.residual_complex(out)
+
     def residual_complex(self, out=None):
         """Residual of the currently loaded amplitudes into ``out`` (complex)."""
 
         out = self.out if out is None else out
         if self.residual_path == "quadrature":
             self._residual_by_quadrature(out)
+        elif self.residual_path == "simpson":
+            self._residual_by_simpson(out)
         else:
             self.plan.apply("residual", out, self.diag, self.t)
         self.plan.apply("reference", self.ref)
@@ -170,6 +202,42 @@ class StaticDevice:
                 self.plan.set_precision("D", "jvp", bits)
             self.t_token = self.plan.load("T", self.t)
 
+    def _residual_by_simpson(self, out) -> None:
+        """Simpson's rule on ``R(t) = R(0) + int_0^1 f(s) ds`` with ``f(s) = J(s t) t``, a cubic in
+        ``s`` for a residual of degree at most four, so the rule is exact (to rounding):
+        ``R(t) = R(0) + f(0)/6 + 4 f(1/2)/6 + f(1)/6``.  ``f(1/2)`` and ``f(1)`` are FP64 JVPs with
+        the amplitude atoms loaded at ``t/2`` and then at ``t``; ``R(0) + f(0)/6`` is the
+        ``residual_low`` program (terms of degree zero, and a sixth of the linear terms as constant
+        creation tensors times one amplitude atom) with a sixth of the diagonal, launched while
+        ``t`` is loaded.  Two amplitude loads with their creation tensors instead of the three of
+        two-node Gauss-Legendre, and the amplitudes end loaded at ``t`` as every residual path
+        leaves them."""
+
+        if self.problem.residual_degree > 4:
+            raise ValueError(f"Simpson's rule is exact up to residual degree 4, not {self.problem.residual_degree}")
+        torch = self.torch
+        if self._ts is None:
+            self._ts = torch.empty_like(self.t)
+            self._qout = torch.empty_like(self.t)
+        if self._diag6 is None:
+            self._diag6 = self.diag / 6.0
+        bits = self.plan.precision("jvp")
+        if bits != 64:
+            self.plan.set_precision("D", "jvp", 64)
+        try:
+            torch.mul(self.t, 0.5, out=self._ts)
+            self.plan.load("T", self._ts)
+            self.plan.load("D", self.t)
+            self.plan.apply("jvp", self._qout, self.diag, self.t)  # f(1/2)
+            self.t_token = self.plan.load("T", self.t)
+            self.plan.apply("jvp", out, self.diag, self.t)  # f(1)
+            out.add_(self._qout, alpha=4.0)
+            self.plan.apply("residual_low", self._qout, self._diag6, self.t)  # R(0) + f(0)/6
+            out.div_(6.0).add_(self._qout)
+        finally:
+            if bits != 64:
+                self.plan.set_precision("D", "jvp", bits)
+
     def jvp_complex(self, dt, out=None):
         out = self.out if out is None else out
         if dt.data_ptr() != self.dt.data_ptr():
@@ -181,8 +249,7 @@ class StaticDevice:
     # real tilde-coordinate actions -----------------------------------------
     def residual_real(self, x, r) -> None:
         self.tilde.unpack(x, self.t)
-        self.load_amplitudes(self.t)
-        self.residual_complex(self.out)
+        self.residual_at(self.t, self.out)
         self.tilde.pack(self.out, r)
 
     def jvp_real(self, dx, jdx) -> None:
@@ -208,8 +275,7 @@ def component_static_residual(problem: ComponentStaticProblem, amplitudes) -> Co
     t = _validate(problem, amplitudes, "amplitudes")
     dev = problem.device
     torch = dev.torch
-    dev.load_amplitudes(_to_device(t, torch))
-    out = dev.residual_complex()
+    out = dev.residual_at(_to_device(t, torch))
     retained = out.cpu().numpy().copy()
     reference = complex(dev.ref.cpu().numpy()[0])
     return ComponentStaticAction(retained, reference, perf_counter() - started)

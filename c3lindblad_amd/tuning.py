@@ -8,8 +8,9 @@ and the matrix-core entry threshold.  ``c3l_plan_autotune`` times candidates on 
 (coordinate descent from the default heuristics) and keeps the fastest for every plan with the
 same digest, so twin plans and repeated instances share one tuning.  Tuning runs once per plan
 at set-up, outside any timed solve; programs are timed on zero inputs (their cost does not
-depend on the values).  The set-up also picks, per static problem, whether the residual is
-evaluated by its program or by Gauss-Legendre quadrature of FP64 JVPs (whichever is faster).
+depend on the values).  The set-up also picks, per static problem, how the residual is
+evaluated: by its program, by Gauss-Legendre quadrature of FP64 JVPs or by Simpson's rule
+(two FP64 JVPs and the low-degree program), whichever is fastest.
 """
 
 from __future__ import annotations
@@ -43,9 +44,10 @@ def autotune_static(static, jvp_bits: int = 64, reps: int = 5) -> list[dict]:
 
 
 def choose_residual_path(static, reps: int = 3) -> dict:
-    """Time one residual evaluation (amplitude load with its creation tensors, then the residual)
-    by the residual program and by Gauss-Legendre quadrature of FP64 JVPs
-    (``StaticDevice._residual_by_quadrature``) and keep the faster for this problem."""
+    """Time one residual evaluation (``StaticDevice.residual_at``: the amplitude loads with their
+    creation tensors that the path needs, then the residual) by the residual program, by Gauss-Legendre quadrature of FP64 JVPs
+    (``StaticDevice._residual_by_quadrature``) and, for residuals of degree at most four, by
+    Simpson's rule (``StaticDevice._residual_by_simpson``), and keep the fastest."""
 
     torch = _lib.require_device()
     dev = static.device
@@ -53,21 +55,20 @@ def choose_residual_path(static, reps: int = 3) -> dict:
     t = 0.01 * (rng.normal(size=dev.dim) + 1j * rng.normal(size=dev.dim))
     dev.t.copy_(torch.from_numpy(t).to(_lib.DEVICE))
     times = {}
-    for path in ("program", "quadrature"):
+    paths = ("program", "quadrature", "simpson") if static.residual_degree <= 4 else ("program", "quadrature")
+    for path in paths:
         dev.residual_path = path
-        dev.load_amplitudes(dev.t)
-        dev.residual_complex(dev.out)  # untimed: first use builds the constant part R(0)
+        dev.residual_at(dev.t, dev.out)  # untimed: first use builds the constant part R(0)
         start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         start.record()
         for _ in range(reps):
-            dev.load_amplitudes(dev.t)
-            dev.residual_complex(dev.out)
+            dev.residual_at(dev.t, dev.out)
         stop.record()
         stop.synchronize()
         times[path] = start.elapsed_time(stop) / reps
     dev.residual_path = min(times, key=times.get)
     return {"program": "residual_path", "ms_program": times["program"], "ms_quadrature": times["quadrature"],
-            "choice": dev.residual_path, "quadrature_nodes": int(len(dev.quadrature_nodes()[0])),
+            "ms_simpson": times.get("simpson"), "choice": dev.residual_path, "quadrature_nodes": int(len(dev.quadrature_nodes()[0])),
             "residual_degree": int(static.residual_degree)}
 
 

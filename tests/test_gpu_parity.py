@@ -508,7 +508,7 @@ def test_autotuned_program_shapes_keep_the_results(torch_gpu, n, seed):
     after = evaluate()
     paths = [r for r in records if r["program"] == "residual_path"]
     records = [r for r in records if r["program"] != "residual_path"]
-    assert len(paths) == 2 and all(r["choice"] in ("program", "quadrature") for r in paths)
+    assert len(paths) == 2 and all(r["choice"] in ("program", "quadrature", "simpson") for r in paths)
     print(f"  residual path: {paths[-1]}", flush=True)
     for r in records:
         print(f"  {r['program']}: {r['ms_default'] * 1e3:.1f} -> {r['ms_tuned'] * 1e3:.1f} us {r['shape']}", flush=True)
@@ -529,18 +529,21 @@ def test_autotuned_program_shapes_keep_the_results(torch_gpu, n, seed):
     assert _rel(j321, j640) < 2e-6 and _rel(j320, j640) < 2e-6
 
 
-@pytest.mark.parametrize("case", ["random_3", "random_7", "kanamori_3x2"])
-def test_residual_by_quadrature_matches_the_residual_program(torch_gpu, case):
+@pytest.mark.parametrize("case,path", [("random_3", "quadrature"), ("random_7", "quadrature"),
+                                       ("kanamori_3x2", "quadrature"), ("random_3", "simpson"),
+                                       ("random_7", "simpson"), ("kanamori_3x2", "simpson")])
+def test_residual_by_quadrature_matches_the_residual_program(torch_gpu, case, path):
     """Residual as R(0) + sum_i w_i J(s_i t) t at the Gauss-Legendre nodes of the residual's
-    degree (two nodes for degree 4) against the residual program (1e-13 relative), against the
+    degree (two nodes for degree 4), or by Simpson's rule (JVPs at s = 1/2 and 1 plus the
+    residual_low program), against the residual program (1e-13 relative), against the
     polynomial oracle at n = 3 (1e-12), entered with the JVP in FP32 (the FP32 state must be back
-    afterwards), and a mixed-precision Newton solve on the quadrature path reaching the root of
+    afterwards), and a mixed-precision Newton solve on that path reaching the root of
     the program path (packed residual <= 1e-10, amplitudes within 1e-8)."""
 
     from c3lindblad_amd.solver import NewtonKrylovPolicy, solve_component_ccsd
     from oracle import polynomial as poly
 
-    print(f"case quadrature residual {case}", flush=True)
+    print(f"case {path} residual {case}", flush=True)
     p, sp = _matrix_core_case(case)
     dev = sp.device
     torch = torch_gpu
@@ -548,16 +551,16 @@ def test_residual_by_quadrature_matches_the_residual_program(torch_gpu, case):
     rng = np.random.default_rng(6)
     x = torch.from_numpy(0.05 * rng.normal(size=p.num_variables)).cuda()
     got = {}
-    for path in ("program", "quadrature", "quadrature"):  # the second quadrature call reuses R(0)
-        dev.residual_path = path
+    for which in ("program", path, path):  # the second call reuses the cached constant parts
+        dev.residual_path = which
         r = torch.empty_like(x)
         dev.residual_real(x, r)
-        got[path] = (r.cpu().numpy(), dev.out.cpu().numpy(), complex(dev.ref.item()))
-    assert _rel(got["quadrature"][0], got["program"][0]) < 1e-13
-    assert abs(got["quadrature"][2] - got["program"][2]) <= 1e-13 * max(1.0, abs(got["program"][2]))
+        got[which] = (r.cpu().numpy(), dev.out.cpu().numpy(), complex(dev.ref.item()))
+    assert _rel(got[path][0], got["program"][0]) < 1e-13
+    assert abs(got[path][2] - got["program"][2]) <= 1e-13 * max(1.0, abs(got["program"][2]))
     if case == "random_3":
         want, _ = poly.residual(p, p.tilde.unpack(x.cpu().numpy()))
-        assert _rel(got["quadrature"][1], want) < 1e-12
+        assert _rel(got[path][1], want) < 1e-12
     # entered while the JVP program is FP32: quadrature JVPs run in FP64, FP32 comes back
     dx = torch.from_numpy(rng.normal(size=p.num_variables)).cuda()
     j64 = torch.empty_like(x)
@@ -566,7 +569,7 @@ def test_residual_by_quadrature_matches_the_residual_program(torch_gpu, case):
     dev.jvp_real(dx, j64)
     dev.plan.set_precision("D", "jvp", 32)
     try:
-        dev.residual_path = "quadrature"
+        dev.residual_path = path
         r = torch.empty_like(x)
         dev.residual_real(x, r)
         assert dev.plan.precision("jvp") == 32
@@ -576,17 +579,17 @@ def test_residual_by_quadrature_matches_the_residual_program(torch_gpu, case):
         dev.plan.set_precision("D", "jvp", 64)
     assert _rel(r.cpu().numpy(), got["program"][0]) < 1e-13
     e = _rel(j32.cpu().numpy(), j64.cpu().numpy())
-    assert 1e-12 < e < 2e-6  # still FP32 after the quadrature residual, at the same amplitudes
+    assert 1e-12 < e < 2e-6  # still FP32 after the residual, at the same amplitudes
     policy = NewtonKrylovPolicy.mixed_precision(forcing_strategy="eisenstat_walker", gmres_restart=150)
     dev.residual_path = "program"
     ref = solve_component_ccsd(sp, policy=policy)
-    dev.residual_path = "quadrature"
+    dev.residual_path = path
     try:
         sol = solve_component_ccsd(sp, policy=policy)
     finally:
         dev.residual_path = "program"
     print(f"  program path: converged={ref.converged} |pack R|={ref.packed_residual_norm:.2e} jvps={ref.jvp_evaluations}; "
-          f"quadrature path: converged={sol.converged} |pack R|={sol.packed_residual_norm:.2e} "
+          f"{path} path: converged={sol.converged} |pack R|={sol.packed_residual_norm:.2e} "
           f"jvps={sol.jvp_evaluations}", flush=True)
     assert ref.converged and sol.converged and sol.packed_residual_norm <= 1e-10
     scale = max(1.0, np.max(np.abs(ref.amplitudes)))

@@ -143,6 +143,39 @@ def test_compiled_plan_equals_polynomial_oracle(n, U, seed):
     assert p.tilde.defect(R) < 1e-13 * max(1.0, np.max(np.abs(R)))
 
 
+@pytest.mark.parametrize("n,U,seed", [(2, 0.9, 2), (3, 1.2, 3), ("kanamori3", 0, 3)])
+def test_simpson_residual_equals_polynomial_oracle(n, U, seed):
+    """The residual is a polynomial of degree four in the amplitudes, so ``f(s) = J(s t) t`` is a
+    cubic and Simpson's rule integrates it exactly: ``R(t) = R(0) + (f(0) + 4 f(1/2) + f(1))/6``
+    with the oracle's own residual and JVP.  The compiled ``residual_low`` program (degree-zero
+    terms, one sixth of the linear ones, a sixth of the diagonal) equals ``R(0) + f(0)/6`` of the
+    oracle, and the three compiled programs assembled by the rule give the oracle's residual."""
+
+    from oracle import polynomial as poly
+    from oracle.port import PortPlan
+    from c3lindblad_amd.plan_arrays import PROGRAMS
+
+    p, sp = _setup(n, U, seed)
+    assert sp.residual_degree == 4
+    rng = np.random.default_rng(seed + 300)
+    t = p.tilde.unpack(0.05 * rng.normal(size=p.num_variables))
+    R, _ = poly.residual(p, t)
+    zero = np.zeros_like(t)
+    f = [poly.jvp(p, s * t, t) for s in (0.0, 0.5, 1.0)]
+    assert _rel(poly.residual(p, zero)[0] + (f[0] + 4 * f[1] + f[2]) / 6, R) < 1e-13
+    # the trapezoid rule is not exact here: the identity above is not satisfied by accident
+    assert _rel(poly.residual(p, zero)[0] + (f[0] + f[2]) / 2, R) > 1e-6
+    port = PortPlan(sp.lowered)
+    port.load(0, t)
+    low = port.apply(PROGRAMS["residual_low"], np.zeros(p.num_amplitudes, complex), sp.diagonal / 6, t)
+    assert _rel(low, poly.residual(p, zero)[0] + f[0] / 6) < 1e-13
+    port.load(1, t)
+    f1 = port.apply(PROGRAMS["jvp"], np.zeros(p.num_amplitudes, complex), sp.diagonal, t)
+    port.load(0, 0.5 * t)
+    fh = port.apply(PROGRAMS["jvp"], np.zeros(p.num_amplitudes, complex), sp.diagonal, t)
+    assert _rel(low + (4 * fh + f1) / 6, R) < 1e-13
+
+
 @pytest.mark.parametrize("n,charge", [(2, 1), (3, -1), (4, 1), ("kanamori3", 1), ("kanamori4", -1)])
 def test_compiled_sigma_and_sources_equal_polynomial_oracle(n, charge):
     from c3lindblad_amd.eom import ComponentEOMProblem, _sink, channel_operators, eom_ccsd_problem
@@ -302,7 +335,7 @@ def test_union_plan_equals_its_parts():
     rng = np.random.default_rng(4)
     ts = [p.tilde.unpack(0.05 * rng.normal(size=D)) for p, _ in parts]
     dts = [p.tilde.unpack(rng.normal(size=D)) for p, _ in parts]
-    u = union_plan([sp.lowered for _, sp in parts], {0: D, 1: D}, {0: D, 1: 1, 2: D})
+    u = union_plan([sp.lowered for _, sp in parts], {0: D, 1: D}, {0: D, 1: 1, 2: D, 8: D})
     port = PortPlan(u)
     port.load(0, np.concatenate(ts))
     port.load(1, np.concatenate(dts))
