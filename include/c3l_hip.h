@@ -73,11 +73,12 @@ int c3l_plan_add_level(c3l_plan* plan, int32_t kind, int32_t n_jobs, const int64
                        const int64_t* par_offset, const int32_t* par_rank, const int32_t* slot,
                        const int32_t* vec, const int32_t* sign);
 
-/* Register one output block of program `program` (0 <= program < 8): `count` coordinates
- * of degree `deg` (coord[count*deg] local indices < n <= 255, dest[count] output positions)
- * and `n_entries` wedge entries: entry t = coeff[t] * prod_{f in [fstart[t], fstart[t+1])}
- * ws[foffset[f] + flat_f], flat_f = sum_s coord[pos_s] n^s (first index fastest) with
- * fcode[f] = rank | pos0<<3 | pos1<<5 | pos2<<7 | pos3<<9 (rank <= 4).  All blocks of a
+/* Register one output block of program `program` (0 <= program < 9: residual, reference,
+ * jvp, sigma, creation, observable, source_addition, source_removal, residual_low): `count`
+ * coordinates of degree `deg` <= 5 (coord[count*deg] local indices < n <= 255, dest[count]
+ * output positions) and `n_entries` wedge entries: entry t = coeff[t] * prod_{f in [fstart[t],
+ * fstart[t+1])} ws[foffset[f] + flat_f], flat_f = sum_s coord[pos_s] n^s (first index fastest)
+ * with fcode[f] = rank | pos0<<3 | pos1<<6 | pos2<<9 | pos3<<12 | pos4<<15 (rank <= 5).  All blocks of a
  * program are evaluated by one launch (blocks of a program must not read each other's
  * destinations); registering a block invalidates that program's device image. */
 int c3l_plan_add_block(c3l_plan* plan, int32_t program, int32_t deg, int64_t count, const int32_t* coord,
