@@ -546,11 +546,22 @@ struct Gmres {
 };
 
 // Two pinned host slots and their events: the host reads Hessenberg column c from slot
-// c % 2 while the GPU already works on the next step.
+// c % 2 while the GPU already works on the next step.  The slots are mapped, coherent host
+// memory: the update pass of step c + 1 writes the column straight into its slot (no copy
+// launch), and the event recorded after that pass tells the host the slot is complete.
 struct HostColumns {
   double* host = nullptr;
+  double* dev = nullptr;  // the same slots as the device addresses them
   hipEvent_t done[2] = {nullptr, nullptr};
   int64_t stride = 0;
+  int allocate(size_t doubles) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(double) * doubles,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+      return dfail(C3L_EHIP, "hipHostMalloc of the Hessenberg column slots failed");
+    if (hipHostGetDevicePointer(reinterpret_cast<void**>(&dev), host, 0) != hipSuccess)
+      return dfail(C3L_EHIP, "hipHostGetDevicePointer of the Hessenberg column slots failed");
+    return C3L_OK;
+  }
   ~HostColumns() {
     for (hipEvent_t e : done)
       if (e != nullptr) {
@@ -609,7 +620,7 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
   DriverClock clock(2);
   HostColumns pinned;
   pinned.stride = R + 2;
-  D_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned.host), sizeof(double) * 2 * pinned.stride, 0));
+  D_CALL(pinned.allocate((size_t)(2 * pinned.stride)));
   for (hipEvent_t& e : pinned.done) D_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   std::vector<double> H((size_t)(R + 1) * R), cs(R), sn(R), S(R + 1), y(R);
   auto h = [&](int i, int j) -> double& { return H[(size_t)i * R + j]; };
@@ -661,17 +672,17 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
       return C3L_OK;
     };
     // Step j applies the operator to the once-projected v_j (none at j = R) and completes
-    // column j-1, which is copied to pinned slot (j-1) % 2; the host absorbs column j-2
-    // meanwhile, so it never waits for the step it has just queued.
+    // column j-1, which its update pass writes into pinned slot (j-1) % 2; the host absorbs
+    // column j-2 meanwhile, so it never waits for the step it has just queued (and has read
+    // slot (j-2) % 2 before step j+1, which rewrites it, is queued).
     int absorbed = 0;
     for (int j = 0; j <= R && !stop; ++j) {
       const bool has_w = j < R;
       if (has_w) D_CALL(g.apply_preconditioned(V + (int64_t)j * ldv, w));
+      const int c = j - 1;
+      basis.stage = j == 0 ? nullptr : pinned.dev + (int64_t)(c % 2) * pinned.stride;
       D_CALL(c3l_ortho::step(basis, j, j == 0, has_w ? w : nullptr, 0.0, s, &clock));
       if (j == 0) continue;
-      const int c = j - 1;
-      D_CHECK(hipMemcpyAsync(pinned.host + (int64_t)(c % 2) * pinned.stride, coef + L.stage,
-                             sizeof(double) * (c + 3), hipMemcpyDeviceToHost, s));
       D_CHECK(hipEventRecord(pinned.done[c % 2], s));
       if (c >= 1) {
         D_CALL(absorb(c - 1));
@@ -816,7 +827,7 @@ int gmres_group_run(c3l_plan* plan, const c3l_tilde_maps* tm, int32_t count, con
   // pinned Hessenberg columns: two slots per system
   HostColumns pinned;
   pinned.stride = R + 2;
-  D_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pinned.host), sizeof(double) * 2 * pinned.stride * count, 0));
+  D_CALL(pinned.allocate((size_t)(2 * pinned.stride * count)));
   std::vector<hipEvent_t> events((size_t)2 * count, nullptr);
   struct EventGuard {
     std::vector<hipEvent_t>& e;
@@ -830,6 +841,7 @@ int gmres_group_run(c3l_plan* plan, const c3l_tilde_maps* tm, int32_t count, con
   } guard{events};
   for (hipEvent_t& e : events) D_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   auto slot = [&](int32_t i, int c) { return pinned.host + ((int64_t)i * 2 + (c % 2)) * pinned.stride; };
+  auto slot_dev = [&](int32_t i, int c) { return pinned.dev + ((int64_t)i * 2 + (c % 2)) * pinned.stride; };
   auto absorb = [&](int32_t i, int c) -> int {
     GroupSystem& q = sys[(size_t)i];
     const double* col = slot(i, c);
@@ -912,14 +924,15 @@ int gmres_group_run(c3l_plan* plan, const c3l_tilde_maps* tm, int32_t count, con
         D_CALL(g.apply_preconditioned(V + (int64_t)j * ldv, w));
         for (int32_t i : iterating) ++jvp_calls[i];
       }
+      const int c = j - 1;
+      for (int32_t i : iterating) {  // the update pass writes column c into its pinned slot
+        GroupSystem& q = sys[(size_t)i];
+        q.b32.stage = j == 0 ? nullptr : slot_dev(i, c);
+        q.b64.stage = q.b32.stage;
+      }
       D_CALL(step_over(iterating, j, has_w));
       if (j == 0) continue;
-      const int c = j - 1;
-      for (int32_t i : iterating) {
-        D_CHECK(hipMemcpyAsync(slot(i, c), sys[(size_t)i].coef + L.stage, sizeof(double) * (c + 3),
-                               hipMemcpyDeviceToHost, s));
-        D_CHECK(hipEventRecord(events[(size_t)i * 2 + (c % 2)], s));
-      }
+      for (int32_t i : iterating) D_CHECK(hipEventRecord(events[(size_t)i * 2 + (c % 2)], s));
       if (c >= 1)
         for (int32_t i : iterating) {
           D_CALL(absorb(i, c - 1));

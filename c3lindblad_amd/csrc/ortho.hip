@@ -106,20 +106,24 @@ __device__ __forceinline__ S narrow_to(T v) {
 }
 
 // Rows per reduction chunk (8 per lane: q and w stay in registers at full
-// occupancy) and waves per update workgroup.  Complex bases are short (tens of
-// thousands of rows): 8-wave update groups keep the GPU full.  Real bases (the
-// CCSD Newton correction) are long.
+// occupancy), waves per update workgroup and basis rows per update lane.  Complex
+// bases are short (tens of thousands of rows): 8-wave update groups of one row per
+// lane keep the GPU full.  Real bases (the CCSD Newton correction) are long: four
+// rows per lane cut the update grid fourfold, which at a few columns is bound by
+// the number of workgroups rather than by the bytes.
 template <typename T>
 struct Shape;
 template <>
 struct Shape<double> {
   static constexpr int kRows = 512;
   static constexpr int kWaves = 4;
+  static constexpr int kRowsPerLane = 4;
 };
 template <>
 struct Shape<double2> {
   static constexpr int kRows = 512;
   static constexpr int kWaves = 8;
+  static constexpr int kRowsPerLane = 1;
 };
 
 // Grid (chunks, column groups), 4 waves per group; wave `wave` of group g takes
@@ -141,12 +145,14 @@ __device__ __forceinline__ void reduce_body(const S* __restrict__ V, int64_t ld,
   if (kbase > nv) return;  // whole wave
   const int64_t r0 = blockIdx.x * (int64_t)ROWS + lane;
   const S* __restrict__ q = V + (int64_t)(nv - 1) * ld;
-  // rows past the end read the last row (always valid) against zero q and w entries
-  T qv[kPer], wv[kPer];
+  // rows past the end read the last row (always valid) against zero q and w entries; q and
+  // the basis columns stay in their storage type in registers and widen at the multiply
+  S qv[kPer];
+  T wv[kPer];
 #pragma unroll
   for (int t = 0; t < kPer; ++t) {
     const int64_t i = r0 + t * 64;
-    qv[t] = i < dim ? T(widen(q[i])) : N::zero();
+    qv[t] = i < dim ? q[i] : S{};
     wv[t] = (w != nullptr && i < dim) ? w[i] : N::zero();
   }
   // basis columns, two per iteration so that 2 kPer loads are in flight
@@ -156,20 +162,21 @@ __device__ __forceinline__ void reduce_body(const S* __restrict__ V, int64_t ld,
     const bool two = c + 1 < cpw && k1 < nv;
     const S* __restrict__ col0 = V + (int64_t)k0 * ld;
     const S* __restrict__ col1 = V + (int64_t)(two ? k1 : k0) * ld;
-    T v0[kPer], v1[kPer];
+    S v0[kPer], v1[kPer];
 #pragma unroll
     for (int t = 0; t < kPer; ++t) {
       const int64_t i = min(r0 + t * 64, dim - 1);
-      v0[t] = widen(col0[i]);
-      v1[t] = widen(col1[i]);
+      v0[t] = col0[i];
+      v1[t] = col1[i];
     }
     T as0 = N::zero(), az0 = N::zero(), as1 = N::zero(), az1 = N::zero();
 #pragma unroll
     for (int t = 0; t < kPer; ++t) {
-      as0 = N::cfma(v0[t], qv[t], as0);
-      az0 = N::cfma(v0[t], wv[t], az0);
-      as1 = N::cfma(v1[t], qv[t], as1);
-      az1 = N::cfma(v1[t], wv[t], az1);
+      const T a0 = widen(v0[t]), a1 = widen(v1[t]), qt = widen(qv[t]);
+      as0 = N::cfma(a0, qt, as0);
+      az0 = N::cfma(a0, wv[t], az0);
+      as1 = N::cfma(a1, qt, as1);
+      az1 = N::cfma(a1, wv[t], az1);
     }
     as0 = N::wave_total(as0);
     az0 = N::wave_total(az0);
@@ -191,7 +198,7 @@ __device__ __forceinline__ void reduce_body(const S* __restrict__ V, int64_t ld,
 #pragma unroll
     for (int t = 0; t < kPer; ++t) {
       const int64_t i = r0 + t * 64;
-      if (sink != nullptr && i < dim) as = N::fma(sink[i], qv[t], as);
+      if (sink != nullptr && i < dim) as = N::fma(sink[i], widen(qv[t]), as);
       az = N::cfma(wv[t], wv[t], az);
     }
     as = N::wave_total(as);
@@ -379,82 +386,101 @@ __device__ __forceinline__ void column_part(int blk, T* __restrict__ H, int64_t 
   }
 }
 
-// 64 rows per workgroup, WAVES waves splitting the columns, partial sums met in LDS:
+// 64 RPL rows per workgroup (RPL per lane, 64 apart), WAVES waves splitting the columns,
+// partial sums met in LDS:
 // with A' = V s and Z' = V z over columns 0..j-1,
 //   v_j = q/r - A'/r,  q_{j+1} = w/g - (c/(r g)) q - (Z' - (c/r) A')/g.
 // The scalars r, c, g need |s|^2, |z|^2 and s^H z: each wave sums them over its own
 // columns while it streams them (the coefficients are wave-uniform), and they meet in LDS
-// with the row sums, so the pass starts its loads at once and has one barrier.
-template <typename T, typename S, int WAVES>
+// with the row sums, so the pass starts its loads at once and has one barrier.  After it,
+// wave w finishes the rows of slots t = w, w + WAVES, ... of every lane.
+template <typename T, typename S, int WAVES, int RPL>
 __device__ __forceinline__ void update_part(int blk, S* __restrict__ V, int64_t ld, int j, int fresh,
                                             const T* __restrict__ w, const T* __restrict__ red, int64_t dim,
                                             const T* __restrict__ sc_prev, double breakdown, T* __restrict__ pad,
                                             const int64_t* __restrict__ pad_map) {
   using N = Num<T>;
-  __shared__ T pa[WAVES][64];
-  __shared__ T pb[WAVES][64];
+  __shared__ T pa[WAVES][64 * RPL];
+  __shared__ T pb[WAVES][64 * RPL];
   __shared__ double ps[WAVES][4];
   const T* Sv = red;
   const T* Zv = red + j + 2;
   const int has_w = w != nullptr ? 1 : 0;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  const int64_t i = blk * (int64_t)64 + lane;
-  const int64_t ic = i < dim ? i : dim - 1;  // rows past the end read the last row, discarded
-  T A = N::zero(), B = N::zero(), sz = N::zero();
+  const int64_t base = blk * (int64_t)(64 * RPL) + lane;
+  int64_t ic[RPL];
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int64_t i = base + 64 * t;
+    ic[t] = i < dim ? i : dim - 1;  // rows past the end read the last row, discarded
+  }
+  T A[RPL], B[RPL];
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) A[t] = B[t] = N::zero();
+  T sz = N::zero();
   double ss = 0.0, zz = 0.0;
 #pragma unroll 4
   for (int k = wave; k < j; k += WAVES) {
-    const T v = widen(V[(int64_t)k * ld + ic]);
+    const S* __restrict__ col = V + (int64_t)k * ld;
     const T sk = fresh ? N::zero() : Sv[k];
     const T zk = has_w ? Zv[k] : N::zero();
-    A = N::fma(sk, v, A);
-    B = N::fma(zk, v, B);
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const T v = widen(col[ic[t]]);
+      A[t] = N::fma(sk, v, A[t]);
+      B[t] = N::fma(zk, v, B[t]);
+    }
     ss += N::abs2(sk);
     zz += N::abs2(zk);
     sz = N::cfma(sk, zk, sz);
   }
-  pa[wave][lane] = A;
-  pb[wave][lane] = B;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    pa[wave][t * 64 + lane] = A[t];
+    pb[wave][t * 64 + lane] = B[t];
+  }
   if (lane == 0) {
-    double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    put(t, 2, sz);
+    double tt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    put(tt, 2, sz);
     ps[wave][0] = ss;
     ps[wave][1] = zz;
-    ps[wave][2] = t[2];
-    ps[wave][3] = t[3];
+    ps[wave][2] = tt[2];
+    ps[wave][3] = tt[3];
   }
   __syncthreads();
-  if (wave == 0) {
-    double tss = 0.0, tzz = 0.0, t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (wave >= RPL) return;  // no row slot left for this wave (RPL < WAVES)
+  double tss = 0.0, tzz = 0.0, tt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int u = 0; u < WAVES; ++u) {
+    tss += ps[u][0];
+    tzz += ps[u][1];
+    tt[2] += ps[u][2];
+    tt[3] += ps[u][3];
+  }
+  const StepScalars<T> o =
+      finish_scalars<T>(tss, tzz, take(tt, 2, T{}), N::zero(), red, j, fresh, has_w, sc_prev, breakdown);
+  for (int t = wave; t < RPL; t += WAVES) {
+    const int64_t i = base + 64 * t;
+    if (i >= dim) continue;
+    T a = N::zero(), bsum = N::zero();
     for (int u = 0; u < WAVES; ++u) {
-      tss += ps[u][0];
-      tzz += ps[u][1];
-      t[2] += ps[u][2];
-      t[3] += ps[u][3];
+      a = N::add(a, pa[u][t * 64 + lane]);
+      bsum = N::add(bsum, pb[u][t * 64 + lane]);
     }
-    const StepScalars<T> o =
-        finish_scalars<T>(tss, tzz, take(t, 2, T{}), N::zero(), red, j, fresh, has_w, sc_prev, breakdown);
-    if (i < dim) {
-      for (int u = 1; u < WAVES; ++u) {
-        A = N::add(A, pa[u][lane]);
-        B = N::add(B, pb[u][lane]);
-      }
-      S* q = V + (int64_t)j * ld;
-      const T qi = widen(q[i]);
-      if (!fresh) q[i] = narrow_to<S>(N::sub(N::scale(qi, o.ir), N::scale(A, o.ir)));
-      if (has_w) {
-        const T ci = N::scale(o.c, o.ir);
-        const T b = N::scale(N::sub(B, N::mul(ci, A)), o.ig);
-        const T next = N::sub(N::sub(N::scale(w[i], o.ig), N::mul(N::scale(ci, o.ig), qi)), b);
-        V[(int64_t)(j + 1) * ld + i] = narrow_to<S>(next);
-        if (pad != nullptr) {  // the next operator input, scattered into its padded blocks
-          const T stored = widen(narrow_to<S>(next));
+    S* q = V + (int64_t)j * ld;
+    const T qi = widen(q[i]);
+    if (!fresh) q[i] = narrow_to<S>(N::sub(N::scale(qi, o.ir), N::scale(a, o.ir)));
+    if (has_w) {
+      const T ci = N::scale(o.c, o.ir);
+      const T b = N::scale(N::sub(bsum, N::mul(ci, a)), o.ig);
+      const T next = N::sub(N::sub(N::scale(w[i], o.ig), N::mul(N::scale(ci, o.ig), qi)), b);
+      V[(int64_t)(j + 1) * ld + i] = narrow_to<S>(next);
+      if (pad != nullptr) {  // the next operator input, scattered into its padded blocks
+        const T stored = widen(narrow_to<S>(next));
 #pragma unroll
-          for (int k = 0; k < 2; ++k) {
-            const int64_t m = pad_map[2 * i + k];
-            if (m >= 0) pad[m >> 1] = (m & 1) ? N::sub(N::zero(), stored) : stored;
-          }
+        for (int k = 0; k < 2; ++k) {
+          const int64_t m = pad_map[2 * i + k];
+          if (m >= 0) pad[m >> 1] = (m & 1) ? N::sub(N::zero(), stored) : stored;
         }
       }
     }
@@ -464,7 +490,7 @@ __device__ __forceinline__ void update_part(int blk, S* __restrict__ V, int64_t 
 // The step's second pass: workgroups [0, col_blocks) write the Hessenberg columns and the
 // scalar block, the others update 64 basis rows each.  The scalar block alternates with the
 // step parity (sc_prev: step j-1, only read; sc_cur: step j, written by workgroup 0).
-template <typename T, typename S, int WAVES>
+template <typename T, typename S, int WAVES, int RPL>
 __global__ __launch_bounds__(64 * WAVES) void update_column_kernel(Many<S*> V, int64_t ld, int j, int fresh,
                                                                    Many<const T*> w, Many<T*> H, int64_t ldh,
                                                                    Many<const T*> red, Many<T*> sink_v, Many<T*> sc,
@@ -481,7 +507,7 @@ __global__ __launch_bounds__(64 * WAVES) void update_column_kernel(Many<S*> V, i
     column_part<T, WAVES>((int)blockIdx.x, H.p[i], ldh, j, fresh, has_w, red.p[i], sink_v.p[i], sc_cur, status.p[i],
                           stage.p[i], o);
   } else {
-    update_part<T, S, WAVES>((int)blockIdx.x - col_blocks, V.p[i], ld, j, fresh, w.p[i], red.p[i], dim, sc_prev,
+    update_part<T, S, WAVES, RPL>((int)blockIdx.x - col_blocks, V.p[i], ld, j, fresh, w.p[i], red.p[i], dim, sc_prev,
                              breakdown, pad.p[i], pad_map.p[i]);
   }
 }
@@ -568,7 +594,8 @@ int step_many(const Basis<T, S>* bs, int count, int j, bool fresh, const T* cons
   if (int rc = check_launch("finish_kernel")) return rc;
   const int col_blocks = std::max(1, (j + WAVES - 1) / WAVES);
   if (clock) clock->begin(1, s);
-  hipLaunchKernelGGL((update_column_kernel<T, S, WAVES>), dim3(col_blocks + (b.dim + 63) / 64, 1, z),
+  constexpr int RPL = Shape<T>::kRowsPerLane;
+  hipLaunchKernelGGL((update_column_kernel<T, S, WAVES, RPL>), dim3(col_blocks + (b.dim + 64 * RPL - 1) / (64 * RPL), 1, z),
                      dim3(64 * WAVES), 0, s, Vw, b.ld, j, fresh ? 1 : 0, w, H, b.ldh, red_r, sink_v, sc, status, stage,
                      breakdown, b.dim, col_blocks, pad, pad_map);
   if (int rc = check_launch("update_column_kernel")) return rc;
