@@ -7,8 +7,9 @@
 // reduction plus one update pass per step complete the previous Hessenberg
 // column and project the new vector.  Column j-1 is therefore final after step j.
 //
-// GMRES: left-preconditioned restarted GMRES; the final Hessenberg column is
-// copied to the host once per iteration for the Givens update (SciPy's tolerance
+// GMRES: left-preconditioned restarted GMRES; the update pass writes the final
+// Hessenberg column into mapped host memory once per iteration for the Givens
+// update on the host (SciPy's tolerance
 // control: target max(atol, rtol ||b||) on the true residual, checked after every
 // restart, with an adaptively rescaled inner tolerance on the preconditioned
 // residual).  The last column of a cycle is completed by a step without operator.
@@ -565,7 +566,7 @@ struct HostColumns {
   ~HostColumns() {
     for (hipEvent_t e : done)
       if (e != nullptr) {
-        (void)hipEventSynchronize(e);  // no copy may still target the slots when they are freed
+        (void)hipEventSynchronize(e);  // no update pass may still write the slots when they are freed
         (void)hipEventDestroy(e);
       }
     if (host != nullptr) (void)hipHostFree(host);
@@ -638,7 +639,7 @@ int gmres_run(c3l_plan* plan, const c3l_tilde_maps* tm, const double* jvp_diag, 
     int used = 0;
     bool breakdown = false, stop = false;
     double presid = 0.0;
-    // Givens update of completed column c from its pinned copy; true when the cycle stops.
+    // Givens update of completed column c from its pinned slot; true when the cycle stops.
     auto absorb = [&](int c) -> int {
       const double* col = pinned.host + (int64_t)(c % 2) * pinned.stride;
       D_CHECK(hipEventSynchronize(pinned.done[c % 2]));
