@@ -364,7 +364,7 @@ def main() -> int:
                     help="EOM response space: S13, or S135 (S1 + S3 + the rank-five shell)")
     ap.add_argument("--instances", type=int, default=512, help="batch512: instances of the uneven-size batch")
     ap.add_argument("--batch-chunk", type=int, default=64, help="batch512: instances set up and solved per chunk")
-    ap.add_argument("--max-group", type=int, default=8, help="batch512: instances per lockstep group")
+    ap.add_argument("--max-group", type=int, default=2, help="batch512: instances per lockstep group")
     ap.add_argument("--setup-workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="batch512: host processes for the instances' set-up (outside the timed region)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
