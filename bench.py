@@ -270,6 +270,7 @@ def _bench_batch(args) -> int:
     import torch
 
     from c3lindblad_amd.batch import prepare_batch, solve_group
+    from c3lindblad_amd.response import run_concurrently
     from c3lindblad_amd.configs import config3_grid, config3_model, config3_modes
     from c3lindblad_amd.solver import NewtonKrylovPolicy
 
@@ -295,8 +296,13 @@ def _bench_batch(args) -> int:
         t0 = time.perf_counter()
         for _ in range(steps):
             chunk_solutions = []
-            for g in prep.groups:
-                chunk_solutions += solve_group(g, omega, policy=policy)
+            if args.group_streams > 1:  # groups on concurrent streams, one host thread each
+                tasks = [lambda g=g: solve_group(g, omega, policy=policy) for g in prep.groups]
+                for sols in run_concurrently(tasks, streams=args.group_streams):
+                    chunk_solutions += sols
+            else:
+                for g in prep.groups:
+                    chunk_solutions += solve_group(g, omega, policy=policy)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         timed += dt
@@ -365,6 +371,8 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=512, help="batch512: instances of the uneven-size batch")
     ap.add_argument("--batch-chunk", type=int, default=64, help="batch512: instances set up and solved per chunk")
     ap.add_argument("--max-group", type=int, default=2, help="batch512: instances per lockstep group")
+    ap.add_argument("--group-streams", type=int, default=4,
+                    help="batch512: groups solved at once, each on its own stream and host thread")
     ap.add_argument("--setup-workers", type=int, default=min(16, os.cpu_count() or 1),
                     help="batch512: host processes for the instances' set-up (outside the timed region)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -123,3 +123,34 @@ def test_group_gmres_equals_single_gmres(torch_gpu):
         assert (info[i], its[i]) == (s_info, s_its)
         assert abs(true[i] - s_true) <= 1e-12
         assert _rel(got[i * D:(i + 1) * D], s_step) < 1e-12
+
+
+def test_groups_on_concurrent_streams_equal_sequential_solves(torch_gpu):
+    """Four two-instance groups solved at once on four streams, one host thread each
+    (``run_concurrently``, as ``bench.py --config batch512`` does), against the same groups solved
+    one after the other: every instance's Newton and JVP counts equal, amplitudes and G^R equal
+    bit for bit (the streams share nothing but the device)."""
+
+    from c3lindblad_amd.batch import InstanceGroup, compile_instance, solve_group
+    from c3lindblad_amd.configs import siam_model
+    from c3lindblad_amd.response import run_concurrently
+    from c3lindblad_amd.solver import NewtonKrylovPolicy
+
+    print("case four groups of two n=6 instances on concurrent streams, 512 frequencies", flush=True)
+    omega = np.linspace(-8.0, 8.0, 512)
+    policy = NewtonKrylovPolicy.mixed_precision(forcing_strategy="eisenstat_walker", gmres_restart=150)
+    models = [siam_model(5, 1.0 + 0.5 * k, seed=40 + k) for k in range(8)]
+    members = [compile_instance(i, m) for i, m in enumerate(models)]
+    groups = [InstanceGroup(members[2 * g:2 * g + 2], (1, -1)) for g in range(4)]
+    for g in groups:
+        g.upload()
+    sequential = [solve_group(g, omega, policy=policy) for g in groups]
+    tasks = [lambda g=g: solve_group(g, omega, policy=policy) for g in groups]
+    concurrent = run_concurrently(tasks, streams=4)
+    for seq, con in zip(sequential, concurrent):
+        for a, b in zip(seq, con):
+            assert a.converged and b.converged
+            assert (a.newton_iterations, a.jvp_evaluations) == (b.newton_iterations, b.jvp_evaluations)
+            assert np.array_equal(a.amplitudes, b.amplitudes)
+            for q in (1, -1):
+                assert np.array_equal(a.green[q], b.green[q]), q
