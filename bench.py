@@ -336,6 +336,8 @@ def _bench_batch(args) -> int:
             "instances_skipped_no_full_hf": [i for i, _ in skipped],
             "groups": len(group_sizes),
             "group_sizes_by_n": sorted(group_sizes),
+            "max_group": args.max_group,
+            "groups_at_once": args.group_streams,
             "chunk": args.batch_chunk,
             "timed_seconds": timed,
             "autotuner": {p: {"launches_tuned": sum(1 for r in tuned if r["program"] == p),
