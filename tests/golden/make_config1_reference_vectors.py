@@ -1,0 +1,64 @@
+"""Writes ``config1_reference_vectors.npz`` from the reference's own NumPy path.
+
+Run in the build container only (the reference is not on the GPU box and never travels): it
+imports the reference from ``/root/reference/src`` and solves the README quick-start model
+(3 orbitals, spin 1/2, U = 1.2 on orbital 0; CCSD steady state, then the charge +1 EOM-CCSD
+retarded Green function on 81 frequencies in [-4, 4], eta = 0) exactly as the reference README
+does.  The fixture pins what the brute-force and closed-form checks cannot: the converged
+amplitudes in the reference's own gauge (full-HF reference, normal-mode canonicalisation) and
+coordinate order, keyed by their particle-hole mode masks.  Stored: omega, the Green function
+(both spin channels), the amplitudes, their masks, the rapidities and the packed residual norm.
+
+    python tests/golden/make_config1_reference_vectors.py
+"""
+
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+REFERENCE_SRC = Path("/root/reference/src")
+OUT = Path(__file__).with_name("config1_reference_vectors.npz")
+
+
+def main() -> None:
+    sys.path.insert(0, str(REFERENCE_SRC))
+    from c3l import FermionicLindbladModel
+    from c3l.cc.component_response import ComponentSigmaOperator, component_retarded_green
+    from c3l.cc.component_solver import solve_component_ccsd
+    from c3l.cc.component_tensor import (ComponentEOMProblem, ComponentStaticProblem, component_eom_context,
+                                         component_eom_sources)
+    from c3l.cc.eom import eom_ccsd_problem
+    from c3l.cc.factorized import factorized_cc_problem
+
+    model = FermionicLindbladModel(
+        E=np.array([[-0.6, 0.28, 0.28], [0.28, -1.2, 0.0], [0.28, 0.0, 1.2]]),
+        gamma_loss=np.diag([0.0, 0.06, 0.48]),
+        gamma_gain=np.diag([0.0, 0.48, 0.06]),
+        U=1.2,
+    )
+    oracle = factorized_cc_problem(model, "CCSD")
+    problem = ComponentStaticProblem.compile(oracle)
+    solution = solve_component_ccsd(problem)
+    eom = eom_ccsd_problem(problem.oracle)
+    context = component_eom_context(ComponentEOMProblem.compile(eom, problem, charge=1), solution.amplitudes)
+    omega = np.linspace(-4.0, 4.0, 81)
+    response = component_retarded_green(ComponentSigmaOperator(context), component_eom_sources(context), omega,
+                                        eta=0.0)
+    np.savez(
+        OUT,
+        omega=omega,
+        retarded=np.asarray(response.retarded, dtype=np.complex128),
+        amplitudes=np.asarray(solution.amplitudes, dtype=np.complex128),
+        masks=np.asarray(oracle.cluster_basis.masks, dtype=np.int64),
+        rapidities=np.asarray(oracle.rapidities, dtype=np.complex128),
+        packed_residual_norm=np.float64(solution.packed_residual_norm),
+    )
+    print(f"wrote {OUT}: {len(oracle.cluster_basis.masks)} amplitudes, packed residual "
+          f"{solution.packed_residual_norm:.2e}, G^R shape {np.shape(response.retarded)}")
+
+
+if __name__ == "__main__":
+    main()

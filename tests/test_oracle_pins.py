@@ -362,3 +362,41 @@ def test_union_plan_equals_its_parts():
         single.load(2, xs[i])
         assert np.array_equal(S[i * P:(i + 1) * P], single.apply(3, np.zeros(P, complex)))
     assert len(parts[1][1].lowered.blocks[0][4]) < len(parts[0][1].lowered.blocks[0][4])  # U = 0: fewer entries
+
+
+def test_config1_matches_the_reference_golden_vectors():
+    """The README quick-start model solved by the reference's own NumPy path
+    (``tests/golden/make_config1_reference_vectors.py``, run in the build container) against the
+    oracle: the same amplitude masks in the same order, the same rapidities, the same converged
+    amplitudes in the reference's gauge (full-HF reference and normal-mode canonicalisation,
+    which no brute-force check fixes) and the same charge +1 G^R on 81 frequencies."""
+
+    from pathlib import Path
+
+    from c3lindblad_amd.configs import readme_model
+    from c3lindblad_amd.eom import ComponentEOMProblem, channel_operators, eom_ccsd_problem
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from oracle import solvers
+
+    g = np.load(Path(__file__).parent / "golden" / "config1_reference_vectors.npz")
+    model = readme_model()
+    p = factorized_cc_problem(model)
+    masks = poly.coordinate_masks(p.coordinates)
+    assert masks == [int(m) for m in g["masks"]]
+    assert np.max(np.abs(np.asarray(p.rapidities) - g["rapidities"])) < 1e-12
+    _, t, norm, _, _ = solvers.solve_ccsd(p.tilde, p.rapidity_sums(), lambda a: poly.residual(p, a)[0],
+                                          lambda a: (lambda d: poly.jvp(p, a, d)))
+    assert norm <= 1e-10 and float(g["packed_residual_norm"]) <= 1e-10
+    assert np.max(np.abs(t - g["amplitudes"])) < 1e-12 * np.max(np.abs(g["amplitudes"]))
+    sp = ComponentStaticProblem.compile(p)
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
+    rmasks = poly.coordinate_masks(ep.coordinates)
+    ops = channel_operators(p.reference, model.impurity_site, 0, 1)
+    add = ops["addition"][1] * poly.source(p, t, rmasks, ops["addition"][0].components)
+    rem = ops["removal"][1] * poly.source(p, t, rmasks, ops["removal"][0].components)
+    sink = poly.sink(rmasks, ops["sink"][0].components)
+    A = solvers.dense_matrix(lambda v: poly.sigma(p, t, rmasks, v, masks), ep.dimension)
+    ga, gr = solvers.retarded_green_dense(A, add, rem, sink, g["omega"], 0.0)
+    assert _rel(ga + gr, g["retarded"]) < 1e-11
