@@ -1,4 +1,5 @@
-"""Writes ``config1_reference_vectors.npz`` from the reference's own NumPy path.
+"""Writes ``config1_reference_vectors.npz`` (and two n = 6 companions) from the reference's own
+NumPy path.
 
 Run in the build container only (the reference is not on the GPU box and never travels): it
 imports the reference from ``/root/reference/src`` and solves the README quick-start model
@@ -58,6 +59,35 @@ def main() -> None:
     )
     print(f"wrote {OUT}: {len(oracle.cluster_basis.masks)} amplitudes, packed residual "
           f"{solution.packed_residual_norm:.2e}, G^R shape {np.shape(response.retarded)}")
+
+    # Two n = 6 models of the benchmark families, built by this repository's generators and
+    # handed to the reference through its own model record (to_dict / from_dict): a SIAM with
+    # five pseudomodes at U = 4, and two Kanamori orbitals with V_pqrs and two pseudomodes each.
+    sys.path.insert(0, str(Path(__file__).resolve().parents[2]))
+    from c3lindblad_amd.configs import multiorbital_model, siam_model
+
+    for name, ours in (("siam6", siam_model(5, 4.0, seed=3)), ("multiorbital6", multiorbital_model(2, 2, seed=3))):
+        model = FermionicLindbladModel.from_dict(ours.to_dict())
+        oracle = factorized_cc_problem(model, "CCSD")
+        problem = ComponentStaticProblem.compile(oracle)
+        solution = solve_component_ccsd(problem)
+        eom = eom_ccsd_problem(problem.oracle)
+        context = component_eom_context(ComponentEOMProblem.compile(eom, problem, charge=1), solution.amplitudes)
+        omega = np.linspace(-6.0, 6.0, 61)
+        response = component_retarded_green(ComponentSigmaOperator(context), component_eom_sources(context), omega,
+                                            eta=0.0)
+        out = OUT.with_name(f"{name}_reference_vectors.npz")
+        np.savez(
+            out,
+            omega=omega,
+            retarded=np.asarray(response.retarded, dtype=np.complex128),
+            amplitudes=np.asarray(solution.amplitudes, dtype=np.complex128),
+            masks=np.asarray(oracle.cluster_basis.masks, dtype=np.int64),
+            rapidities=np.asarray(oracle.rapidities, dtype=np.complex128),
+            packed_residual_norm=np.float64(solution.packed_residual_norm),
+        )
+        print(f"wrote {out}: {len(oracle.cluster_basis.masks)} amplitudes, packed residual "
+              f"{solution.packed_residual_norm:.2e}, G^R shape {np.shape(response.retarded)}")
 
 
 if __name__ == "__main__":
