@@ -330,6 +330,39 @@ def response_coordinates(n: int, charge: int, space: str = "S13") -> CoordinateS
     return _frozen_set(coordinate_set(n, [rank for rank in keys if rank]))
 
 
+def reference_response_order(coords: CoordinateSet) -> np.ndarray:
+    """Positions of ``coords`` (a response set built by ``response_coordinates``) in the
+    reference's vector order: the S1 and S3 strings in the same order as here (both sort them by
+    mask value), then the rank-five shell as the reference enumerates it
+    (``component_tensor.py:384-409,439-470``): its keys in ascending tuple order and, within a
+    key, the lexicographic index combinations of every slot group raveled with the last group
+    fastest.  ``vector_here[order]`` is the vector in the reference's order."""
+
+    from itertools import combinations
+    from math import comb
+
+    n = coords.n
+    blocks = sorted((b for b in coords.blocks if sum(b.key) == 5), key=lambda b: tuple(b.key))
+    head = np.sort(np.concatenate([b.positions for b in coords.blocks if sum(b.key) != 5] or
+                                  [np.zeros(0, np.int64)]))
+    parts = [head]
+    for b in blocks:
+        counts = [int(c) for c in b.key]
+        dims = [comb(n, c) for c in counts]
+        rank_of = [{cmb: r for r, cmb in enumerate(combinations(range(n), c))} for c in counts]
+        flat = np.zeros(b.positions.size, dtype=np.int64)
+        col = 0
+        for g, c in enumerate(counts):  # block indices hold group 0's indices first, each group sorted
+            ranks = np.array([rank_of[g][tuple(int(v) for v in row)] for row in b.indices[:, col:col + c]],
+                             dtype=np.int64) if c else np.zeros(b.positions.size, dtype=np.int64)
+            flat = flat * dims[g] + ranks
+            col += c
+        order = np.empty(b.positions.size, dtype=np.int64)
+        order[flat] = b.positions
+        parts.append(order)
+    return np.concatenate(parts)
+
+
 # ----------------------------------------------------------------------------- amplitude tilde map
 
 

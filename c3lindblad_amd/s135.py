@@ -20,6 +20,7 @@ from time import perf_counter
 
 import numpy as np
 
+from .lowering import reference_response_order
 from .eom import (
     ComponentEOMContext,
     ComponentEOMProblem,
@@ -85,6 +86,9 @@ class FullS135ResponseOperator(ComponentSigmaOperator):
         self.p_dimension = action.p_dimension
         self.s5_dimension = self.dimension - self.p_dimension
         self.coordinates = action.context.problem.coordinates
+        # the reference orders the rank-five shell by key and index combination, this build by
+        # mask value: vector[reference_order] is a vector of this operator in the reference's order
+        self.reference_order = reference_response_order(self.coordinates)
 
     def on_slot(self, slot: int) -> "FullS135ResponseOperator":
         return FullS135ResponseOperator(self.action, s5_tile_size=self.s5_tile_size, slot=slot)

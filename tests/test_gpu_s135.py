@@ -97,6 +97,33 @@ def test_s135_green_on_the_reference_validation_model(torch_gpu):
         assert shift > 1e-6
 
 
+def test_s135_operator_matches_the_reference_golden_action(torch_gpu):
+    """``FullS135ResponseOperator.matvec`` on the GPU against the reference's own full S135 action
+    (``tests/golden/s135_action_reference_vectors.npz``: its compact model, amplitudes
+    6e-4 cos(i + 0.19), charge +1, a seeded input), vectors exchanged through
+    ``operator.reference_order``: within 1e-12 relative."""
+
+    from pathlib import Path
+
+    from c3lindblad_amd.configs import s135_validation_model
+    from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, eom_ccsd_problem
+    from c3lindblad_amd.lowering import factorized_cc_problem
+    from c3lindblad_amd.s135 import CompiledFullS135Action, FullS135ResponseOperator
+    from c3lindblad_amd.static import ComponentStaticProblem
+
+    print("case s135 operator vs reference golden action", flush=True)
+    g = np.load(Path(__file__).parent / "golden" / "s135_action_reference_vectors.npz")
+    p = factorized_cc_problem(s135_validation_model())
+    sp = ComponentStaticProblem.compile(p)
+    ctx = component_eom_context(ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1), g["amplitudes"])
+    op = FullS135ResponseOperator(CompiledFullS135Action(ctx, materialization_batch_size=4), s5_tile_size=7)
+    assert op.p_dimension == int(g["p_dimension"]) and op.dimension == g["x"].size
+    x = np.empty(op.dimension, dtype=np.complex128)
+    x[op.reference_order] = g["x"]
+    y = op.matvec(x)
+    assert _rel(y[op.reference_order], g["y"]) < 1e-12
+
+
 def test_s135_noninteracting_matches_lindblad_ed(torch_gpu):
     from c3lindblad_amd.eom import ComponentEOMProblem, component_eom_context, eom_ccsd_problem
     from c3lindblad_amd.lowering import factorized_cc_problem

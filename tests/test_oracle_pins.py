@@ -437,3 +437,38 @@ def test_reference_roots_solve_the_compiled_equations(name):
     port.load(0, t_bad, creation=False)
     R_bad = port.apply(0, np.zeros(p.num_amplitudes, complex), sp.diagonal, t_bad)
     assert np.linalg.norm(p.tilde.pack(R_bad, check=False)) > 1e-6
+
+
+def test_s135_action_matches_the_reference_golden_action():
+    """The full S1 + S3 + S5 action of the reference's own S135 test (its compact model,
+    amplitudes 6e-4 cos(i + 0.19), charge +1, a seeded input; written by
+    ``tests/golden/make_config1_reference_vectors.py``) against the compiled S135 sigma program
+    run by the NumPy port: ``reference_response_order`` reproduces the reference's coordinate
+    order exactly (the S13 part as here, the rank-five shell by key and index combination), and
+    the action agrees to 1e-13 relative in that order."""
+
+    from pathlib import Path
+
+    from c3lindblad_amd.configs import s135_validation_model
+    from c3lindblad_amd.eom import ComponentEOMProblem, eom_ccsd_problem
+    from c3lindblad_amd.lowering import factorized_cc_problem, reference_response_order
+    from c3lindblad_amd.static import ComponentStaticProblem
+    from oracle import polynomial as poly
+    from oracle.port import PortPlan
+
+    g = np.load(Path(__file__).parent / "golden" / "s135_action_reference_vectors.npz")
+    p = factorized_cc_problem(s135_validation_model())
+    assert poly.coordinate_masks(p.coordinates) == [int(m) for m in g["static_masks"]]
+    sp = ComponentStaticProblem.compile(p)
+    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p, space="S135"), sp, 1)
+    order = reference_response_order(ep.coordinates)
+    masks = np.asarray(poly.coordinate_masks(ep.coordinates), dtype=np.int64)
+    assert np.array_equal(masks[order], g["response_masks"])
+    assert np.array_equal(order[: int(g["p_dimension"])], np.arange(int(g["p_dimension"])))
+    x = np.empty(ep.dimension, dtype=np.complex128)
+    x[order] = g["x"]
+    port = PortPlan(ep.lowered)
+    port.load(0, g["amplitudes"])
+    port.load(2, x)
+    y = port.apply(3, np.zeros(ep.dimension, complex), ep.diagonal, x)
+    assert _rel(y[order], g["y"]) < 1e-13
