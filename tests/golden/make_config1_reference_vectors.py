@@ -8,8 +8,8 @@ mode masks, and the reference's full S1 + S3 + S5 action in its own coordinate o
 
 * ``config1``: the README quick-start model (3 orbitals, spin 1/2, U = 1.2 on orbital 0; CCSD
   steady state, then the charge +1 EOM-CCSD retarded Green function on 81 frequencies in
-  [-4, 4], eta = 0) solved exactly as the reference README does ->
-  ``config1_reference_vectors.npz``;
+  [-4, 4], eta = 0) solved exactly as the reference README does, plus the charge -1 Green
+  function and the occupations (covariance) -> ``config1_reference_vectors.npz``;
 * ``families``: two n = 6 models of the benchmark families, built by this repository's
   generators and handed to the reference through its own model record (to_dict / from_dict):
   a SIAM with five pseudomodes at U = 4 and two Kanamori orbitals with V_pqrs and two
@@ -26,6 +26,7 @@ mode masks, and the reference's full S1 + S3 + S5 action in its own coordinate o
 from __future__ import annotations
 
 import sys
+import types
 from pathlib import Path
 
 import numpy as np
@@ -35,18 +36,27 @@ HERE = Path(__file__).resolve().parent
 CASES = ("config1", "families", "s135")
 
 
-def _solve_and_save(ref, model, omega, out: Path) -> None:
-    """CCSD root and charge +1 G^R of one model by the reference; saves the fixture."""
+def _solve_and_save(ref, model, omega, out: Path, removal_charge: bool = False) -> None:
+    """CCSD root, occupations (covariance) and charge +1 G^R of one model by the reference (and
+    the charge -1 G^R when asked); saves the fixture."""
 
     oracle = ref.factorized_cc_problem(model, "CCSD")
     problem = ref.ComponentStaticProblem.compile(oracle)
     solution = ref.solve_component_ccsd(problem)
     eom = ref.eom_ccsd_problem(problem.oracle)
-    context = ref.component_eom_context(ref.ComponentEOMProblem.compile(eom, problem, charge=1), solution.amplitudes)
-    response = ref.component_retarded_green(ref.ComponentSigmaOperator(context), ref.component_eom_sources(context),
-                                            omega, eta=0.0)
+    green = {}
+    for charge in ((1, -1) if removal_charge else (1,)):
+        context = ref.component_eom_context(ref.ComponentEOMProblem.compile(eom, problem, charge=charge),
+                                            solution.amplitudes)
+        green[charge] = ref.component_retarded_green(ref.ComponentSigmaOperator(context),
+                                                     ref.component_eom_sources(context), omega, eta=0.0).retarded
+    covariance = ref.factorized_ccsd_covariance(oracle.reference, oracle.cluster_basis, solution.amplitudes)
+    extra = {"retarded_minus": np.asarray(green[-1], dtype=np.complex128)} if removal_charge else {}
+    response = types.SimpleNamespace(retarded=green[1])
     np.savez(
         out,
+        **extra,
+        covariance=np.asarray(covariance, dtype=np.complex128),
         omega=omega,
         retarded=np.asarray(response.retarded, dtype=np.complex128),
         amplitudes=np.asarray(solution.amplitudes, dtype=np.complex128),
@@ -61,7 +71,6 @@ def _solve_and_save(ref, model, omega, out: Path) -> None:
 def main(cases) -> None:
     sys.path.insert(0, str(REFERENCE_SRC))
     sys.path.insert(1, str(HERE.parents[1]))  # this repository, for its model generators
-    import types
 
     from c3l import FermionicLindbladModel
     from c3l.cc.component_response import ComponentSigmaOperator, component_retarded_green
@@ -70,8 +79,10 @@ def main(cases) -> None:
                                          component_eom_sources)
     from c3l.cc.eom import eom_ccsd_problem
     from c3l.cc.factorized import factorized_cc_problem
+    from c3l.cc.observables import factorized_ccsd_covariance
 
     ref = types.SimpleNamespace(
+        factorized_ccsd_covariance=factorized_ccsd_covariance,
         factorized_cc_problem=factorized_cc_problem, ComponentStaticProblem=ComponentStaticProblem,
         solve_component_ccsd=solve_component_ccsd, eom_ccsd_problem=eom_ccsd_problem,
         component_eom_context=component_eom_context, ComponentEOMProblem=ComponentEOMProblem,
@@ -85,7 +96,8 @@ def main(cases) -> None:
             gamma_gain=np.diag([0.0, 0.48, 0.06]),
             U=1.2,
         )
-        _solve_and_save(ref, model, np.linspace(-4.0, 4.0, 81), HERE / "config1_reference_vectors.npz")
+        _solve_and_save(ref, model, np.linspace(-4.0, 4.0, 81), HERE / "config1_reference_vectors.npz",
+                        removal_charge=True)
 
     if "families" in cases:
         from c3lindblad_amd.configs import multiorbital_model, siam_model

@@ -224,7 +224,8 @@ def test_readme_solve_matches_the_reference_golden_vectors(torch_gpu):
     """The device path (Newton-Krylov CCSD and the shared-basis Arnoldi resolvent) on the README
     quick-start model against the reference's own NumPy results stored in
     ``tests/golden/config1_reference_vectors.npz``: amplitudes in the reference's gauge and order
-    within 1e-10, G^R on the 81 frequencies within 1e-8 relative."""
+    within 1e-10, charge +1 and -1 G^R on the 81 frequencies and the occupations within 1e-8
+    relative."""
 
     from pathlib import Path
 
@@ -248,10 +249,14 @@ def test_readme_solve_matches_the_reference_golden_vectors(torch_gpu):
     sol = solve_component_ccsd(sp)
     assert sol.converged and sol.packed_residual_norm <= 1e-10
     assert np.max(np.abs(sol.amplitudes - g["amplitudes"])) < 1e-10 * np.max(np.abs(g["amplitudes"]))
-    ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, 1)
-    ctx = component_eom_context(ep, sol.amplitudes)
-    result = component_retarded_green(ComponentSigmaOperator(ctx), component_eom_sources(ctx), g["omega"], 0.0)
-    assert _rel(result.retarded, g["retarded"]) < 1e-8
+    for charge, key in ((1, "retarded"), (-1, "retarded_minus")):
+        ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, charge)
+        ctx = component_eom_context(ep, sol.amplitudes)
+        result = component_retarded_green(ComponentSigmaOperator(ctx), component_eom_sources(ctx), g["omega"], 0.0)
+        assert _rel(result.retarded, g[key]) < 1e-8, charge
+    from c3lindblad_amd.observables import factorized_ccsd_covariance
+
+    assert _rel(factorized_ccsd_covariance(sp, sol.amplitudes), g["covariance"]) < 1e-8
 
 
 @pytest.mark.parametrize("name", ["siam6", "multiorbital6"])

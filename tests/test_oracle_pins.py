@@ -369,7 +369,8 @@ def test_config1_matches_the_reference_golden_vectors():
     (``tests/golden/make_config1_reference_vectors.py``, run in the build container) against the
     oracle: the same amplitude masks in the same order, the same rapidities, the same converged
     amplitudes in the reference's gauge (full-HF reference and normal-mode canonicalisation,
-    which no brute-force check fixes) and the same charge +1 G^R on 81 frequencies."""
+    which no brute-force check fixes), the same charge +1 and -1 G^R on 81 frequencies and the
+    same occupations."""
 
     from pathlib import Path
 
@@ -400,6 +401,17 @@ def test_config1_matches_the_reference_golden_vectors():
     A = solvers.dense_matrix(lambda v: poly.sigma(p, t, rmasks, v, masks), ep.dimension)
     ga, gr = solvers.retarded_green_dense(A, add, rem, sink, g["omega"], 0.0)
     assert _rel(ga + gr, g["retarded"]) < 1e-11
+    # the charge -1 block and the occupations of the same root
+    em = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, -1)
+    mmasks = poly.coordinate_masks(em.coordinates)
+    ops = channel_operators(p.reference, model.impurity_site, 0, -1)
+    add = ops["addition"][1] * poly.source(p, t, mmasks, ops["addition"][0].components)
+    rem = ops["removal"][1] * poly.source(p, t, mmasks, ops["removal"][0].components)
+    sink = poly.sink(mmasks, ops["sink"][0].components)
+    A = solvers.dense_matrix(lambda v: poly.sigma(p, t, mmasks, v, masks), em.dimension)
+    ga, gr = solvers.retarded_green_dense(A, add, rem, sink, g["omega"], 0.0)
+    assert _rel(ga + gr, g["retarded_minus"]) < 1e-11
+    assert _rel(poly.covariance(p, t), g["covariance"]) < 1e-13
 
 
 @pytest.mark.parametrize("name", ["siam6", "multiorbital6"])
