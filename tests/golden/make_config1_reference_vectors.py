@@ -49,10 +49,16 @@ def _solve_and_save(ref, model, omega, out: Path, removal_charge: bool = False) 
         context = ref.component_eom_context(ref.ComponentEOMProblem.compile(eom, problem, charge=charge),
                                             solution.amplitudes)
         green[charge] = ref.component_retarded_green(ref.ComponentSigmaOperator(context),
-                                                     ref.component_eom_sources(context), omega, eta=0.0).retarded
+                                                     ref.component_eom_sources(context), omega, eta=0.0)
     covariance = ref.factorized_ccsd_covariance(oracle.reference, oracle.cluster_basis, solution.amplitudes)
-    extra = {"retarded_minus": np.asarray(green[-1], dtype=np.complex128)} if removal_charge else {}
-    response = types.SimpleNamespace(retarded=green[1])
+    extra = {}
+    if removal_charge:  # both charges, each channel on its own
+        extra = {"retarded_minus": np.asarray(green[-1].retarded, dtype=np.complex128),
+                 "addition_plus": np.asarray(green[1].addition, dtype=np.complex128),
+                 "removal_plus": np.asarray(green[1].removal, dtype=np.complex128),
+                 "addition_minus": np.asarray(green[-1].addition, dtype=np.complex128),
+                 "removal_minus": np.asarray(green[-1].removal, dtype=np.complex128)}
+    response = types.SimpleNamespace(retarded=green[1].retarded)
     np.savez(
         out,
         **extra,

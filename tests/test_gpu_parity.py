@@ -249,11 +249,13 @@ def test_readme_solve_matches_the_reference_golden_vectors(torch_gpu):
     sol = solve_component_ccsd(sp)
     assert sol.converged and sol.packed_residual_norm <= 1e-10
     assert np.max(np.abs(sol.amplitudes - g["amplitudes"])) < 1e-10 * np.max(np.abs(g["amplitudes"]))
-    for charge, key in ((1, "retarded"), (-1, "retarded_minus")):
+    for charge, key, tag in ((1, "retarded", "plus"), (-1, "retarded_minus", "minus")):
         ep = ComponentEOMProblem.compile(eom_ccsd_problem(p), sp, charge)
         ctx = component_eom_context(ep, sol.amplitudes)
         result = component_retarded_green(ComponentSigmaOperator(ctx), component_eom_sources(ctx), g["omega"], 0.0)
         assert _rel(result.retarded, g[key]) < 1e-8, charge
+        assert _rel(result.addition, g[f"addition_{tag}"]) < 1e-8, charge  # each channel, not only the sum
+        assert _rel(result.removal, g[f"removal_{tag}"]) < 1e-8, charge
     from c3lindblad_amd.observables import factorized_ccsd_covariance
 
     assert _rel(factorized_ccsd_covariance(sp, sol.amplitudes), g["covariance"]) < 1e-8
